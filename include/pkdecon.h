@@ -1,0 +1,126 @@
+/*
+ * pkdecon.h -- C-ABI of the MI355X (gfx950) deconvolution engine.
+ *
+ * Drop-in boundary for PetaKit5D's RL / OMW deconvolution hot path.  Each
+ * entry point names the reference interface it replaces (paths relative to
+ * the PetaKit5D tree).  A MATLAB MEX gateway, a ctypes loader or a C/C++
+ * program binds these symbols; no C++ or torch types cross the boundary.
+ *
+ * Conventions
+ *  - Volumes are MATLAB column-major: dims[0] (n1) is the contiguous axis.
+ *  - 64-bit sizes everywhere (a 2048x2048x512 volume has 2^31 voxels).
+ *  - Every call returns a pk_status; on failure pk_last_error(ctx) holds a
+ *    message.  Calls never abort the process.
+ *  - Input buffers are borrowed read-only; outputs are caller-allocated.
+ *  - Calls on one context are serialised by the context.
+ *  - The GPU transforms support 2/3/5-smooth sizes on every axis (pad with
+ *    pk_find_good_factor_number(n, 1, 0, 5) first); other sizes return
+ *    PK_ERR_UNSUPPORTED_SIZE.
+ */
+#ifndef PKDECON_H
+#define PKDECON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PK_OK = 0,
+    PK_ERR_INVALID_ARG = 1,
+    PK_ERR_OUT_SIZE = 2,          /* 'outSizeIsSmallerThanPsfSize' (decon_psf2otf.m:11-13) */
+    PK_ERR_UNSUPPORTED_SIZE = 3,  /* an axis is not 2/3/5-smooth */
+    PK_ERR_UNSUPPORTED_TYPE = 4,  /* input class other than uint16 / single */
+    PK_ERR_HIP = 5,               /* HIP runtime error */
+    PK_ERR_NO_MEMORY = 6,         /* device allocation failed */
+    PK_ERR_BUDGET = 7,            /* request exceeds the context's HBM budget */
+    PK_ERR_EMPTY_PSF = 8,         /* 'A valid nonzero PSF needs to be provided!' (omw_backprojector_generation.m:25-27) */
+    PK_ERR_EMPTY_MASK = 9,        /* 'The OTF mask is empty ...' (omw_backprojector_generation.m:50-52) */
+    PK_ERR_NO_DEVICE = 10,        /* no HIP device */
+    PK_ERR_BUSY = 11,             /* serving queue full / job not finished */
+    PK_ERR_CANCELLED = 12
+} pk_status;
+
+typedef enum { PK_UINT16 = 1, PK_SINGLE = 2 } pk_dtype;
+typedef enum { PK_MEM_HOST = 0, PK_MEM_DEVICE = 1 } pk_memspace;
+
+/* A volume: data pointer (host or device), class, MATLAB size. */
+typedef struct {
+    void *data;
+    int32_t dtype;   /* pk_dtype */
+    int32_t mem;     /* pk_memspace */
+    int64_t dims[3]; /* [n1 n2 n3], column-major */
+} pk_vol;
+
+/* Name-value options of decon_lucy_function / decon_lucy_omw_function
+ * (decon_lucy_function.m:21-36; decon_lucy_omw_function.m:12-28). */
+typedef struct {
+    double background;     /* 'Background'  ([] -> 0) */
+    double damp_factor;    /* 'dampFactor'  (1) */
+    double scale_factor;   /* 'scaleFactor' (1) */
+    double decon_offset;   /* 'deconOffset' (0) */
+    int32_t edge_erosion;  /* 'EdgeErosion' (0) */
+    int32_t save16bit;     /* 'save16bit'   (true): output class uint16, else single */
+    int32_t has_bbox;      /* 'bbox' given */
+    int32_t reserved;
+    int64_t bbox[6];       /* 'bbox' [s1 s2 s3 e1 e2 e3], 1-based inclusive (decon_lucy_function.m:199-203) */
+} pk_opts;
+
+typedef struct pk_ctx pk_ctx;
+
+/* Defaults of the reference's inputParser. */
+void pk_opts_default(pk_opts *o);
+
+/* Context: one device, its stream, device workspace and the OTF cache (the
+ * reference's `persistent H` / `persistent OTF_f OTF_b`,
+ * decon_lucy_function.m:61-77, decon_lucy_omw_function.m:53-72).
+ * hbm_budget_bytes = 0: no limit beyond the device. */
+pk_status pk_ctx_create(int device, uint64_t hbm_budget_bytes, pk_ctx **out);
+void pk_ctx_destroy(pk_ctx *ctx);
+/* Drops cached OTFs and workspace (mirrors reset(gpuDevice),
+ * demos/demo_RL_deconvolution.m:165-167). */
+pk_status pk_reset(pk_ctx *ctx);
+/* Use the caller's HIP stream (hipStream_t) for all work; NULL restores the
+ * context's own stream. */
+pk_status pk_set_stream(pk_ctx *ctx, void *hip_stream);
+const char *pk_last_error(const pk_ctx *ctx);
+/* Bytes of device memory the context holds (workspace + OTF cache). */
+uint64_t pk_device_bytes(const pk_ctx *ctx);
+
+/* [J, err_mat, k] = decon_lucy_function(I, PSF, NUMIT, ...)
+ * (microscopeDataProcessing/decon/decon_lucy_function.m:1-215): Biggs-
+ * accelerated Richardson-Lucy.  psf: host single, psf_dims.  out->dims must
+ * equal the bbox size (or I's size); out->dtype = save16bit ? UINT16 : SINGLE.
+ * err_mat (NUMIT x 4, column-major, may be NULL) receives zeros (debug off). */
+pk_status pk_decon_lucy(pk_ctx *ctx, const pk_vol *I, const float *psf, const int64_t psf_dims[3], int32_t numit,
+                        const pk_opts *opts, pk_vol *out, double *err_mat);
+
+/* [J, err_mat] = decon_lucy_omw_function(I, PSF_f, PSF_b, NUMIT, ...)
+ * (microscopeDataProcessing/decon/decon_lucy_omw_function.m:1-178).  The OTF
+ * cache is keyed on both PSFs (the reference keys on PSF_b only). */
+pk_status pk_decon_lucy_omw(pk_ctx *ctx, const pk_vol *I, const float *psf_f, const int64_t f_dims[3],
+                            const float *psf_b, const int64_t b_dims[3], int32_t numit, const pk_opts *opts,
+                            pk_vol *out, double *err_mat);
+
+/* otf = decon_psf2otf(psf, outSize) (microscopeDataProcessing/decon/decon_psf2otf.m:1-46).
+ * otf: host buffer of prod(out_dims) complex single values, interleaved
+ * (re, im), the full spectrum as MATLAB returns it. */
+pk_status pk_psf2otf(pk_ctx *ctx, const float *psf, const int64_t psf_dims[3], const int64_t out_dims[3],
+                     float *otf);
+
+/* Transform diagnostics (MATLAB fftn / ifftn of real data, decon_lucy_function.m:136,142).
+ * Device pointers.  r2c: real N floats -> half spectrum (floor(n1/2)+1)*n2*n3
+ * complex (interleaved); c2r: half spectrum -> real, unnormalised (N * ifftn). */
+pk_status pk_fft_r2c(pk_ctx *ctx, const float *d_real, float *d_spec, const int64_t dims[3]);
+pk_status pk_fft_c2r(pk_ctx *ctx, const float *d_spec, float *d_real, const int64_t dims[3]);
+
+/* findGoodFactorNumber(givenNum, directionFlag, allowOddSeven) (utils/findGoodFactorNumber.m:1-26).
+ * max_prime = 7 is the reference rule; 5 gives the 2/3/5-smooth sizes the GPU transforms need. */
+int64_t pk_find_good_factor_number(int64_t n, int32_t direction, int32_t allow_odd_seven, int32_t max_prime);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PKDECON_H */

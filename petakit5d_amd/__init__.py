@@ -1,0 +1,30 @@
+"""petakit5d_amd -- MI355X (gfx950) engine for PetaKit5D's RL / OMW deconvolution hot path.
+
+The compute lives in libpkdecon.so (hand-written HIP kernels behind the C-ABI
+declared in include/pkdecon.h); this package is its Python front end with the
+reference's MATLAB call signatures.
+"""
+from ._lib import Context, PkDeconError, default_context, load  # noqa: F401
+from .decon import (  # noqa: F401
+    crop_psf_to_data,
+    decon_lucy_function,
+    decon_lucy_omw_function,
+    decon_psf2otf,
+    fft_c2r,
+    fft_r2c,
+    findGoodFactorNumber,
+)
+
+__all__ = [
+    "Context",
+    "PkDeconError",
+    "default_context",
+    "load",
+    "decon_lucy_function",
+    "decon_lucy_omw_function",
+    "decon_psf2otf",
+    "findGoodFactorNumber",
+    "crop_psf_to_data",
+    "fft_r2c",
+    "fft_c2r",
+]

@@ -1,0 +1,199 @@
+// pk_capi.hip -- the extern "C" boundary (include/pkdecon.h).  Every entry
+// point validates its arguments, serialises on the context, converts engine
+// exceptions into a pk_status plus pk_last_error() text, and never aborts.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/pkdecon.h"
+#include "pk_engine.hpp"
+
+struct pk_ctx : pk::pk_ctx_impl {};
+
+namespace {
+template <class Fn>
+pk_status guarded(pk_ctx* c, Fn&& fn) {
+    if (!c) return PK_ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    try {
+        PK_HIP(hipSetDevice(c->device));
+        fn();
+        c->last_error.clear();
+        return PK_OK;
+    } catch (const pk::Error& e) {
+        c->last_error = e.msg;
+        return e.st;
+    } catch (const std::bad_alloc&) {
+        c->last_error = "host allocation failed";
+        return PK_ERR_NO_MEMORY;
+    } catch (...) {
+        c->last_error = "unexpected exception";
+        return PK_ERR_INVALID_ARG;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void pk_opts_default(pk_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->background = 0.0;
+    o->damp_factor = 1.0;
+    o->scale_factor = 1.0;
+    o->decon_offset = 0.0;
+    o->edge_erosion = 0;
+    o->save16bit = 1;
+    o->has_bbox = 0;
+}
+
+pk_status pk_ctx_create(int device, uint64_t hbm_budget_bytes, pk_ctx** out) {
+    if (!out) return PK_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return PK_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) return PK_ERR_INVALID_ARG;
+    pk_ctx* c = new (std::nothrow) pk_ctx();
+    if (!c) return PK_ERR_NO_MEMORY;
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return PK_ERR_HIP;
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return PK_OK;
+}
+
+void pk_ctx_destroy(pk_ctx* c) {
+    if (!c) return;
+    {
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        try {
+            c->reset();
+        } catch (...) {
+        }
+        if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    }
+    delete c;
+}
+
+pk_status pk_reset(pk_ctx* c) {
+    return guarded(c, [&] { c->reset(); });
+}
+
+pk_status pk_set_stream(pk_ctx* c, void* s) {
+    return guarded(c, [&] {
+        PK_HIP(hipStreamSynchronize(c->stream));
+        c->stream = s ? static_cast<hipStream_t>(s) : c->own_stream;
+    });
+}
+
+const char* pk_last_error(const pk_ctx* c) { return c ? c->last_error.c_str() : "null context"; }
+
+uint64_t pk_device_bytes(const pk_ctx* c) { return c ? c->live_bytes : 0; }
+
+pk_status pk_decon_lucy(pk_ctx* c, const pk_vol* I, const float* psf, const int64_t psf_dims[3], int32_t numit,
+                        const pk_opts* opts, pk_vol* out, double* err_mat) {
+    return guarded(c, [&] {
+        if (!psf_dims) pk::fail(PK_ERR_INVALID_ARG, "psf_dims is null");
+        c->decon(0, I, psf, psf_dims, nullptr, nullptr, numit, opts, out, err_mat);
+    });
+}
+
+pk_status pk_decon_lucy_omw(pk_ctx* c, const pk_vol* I, const float* psf_f, const int64_t f_dims[3],
+                            const float* psf_b, const int64_t b_dims[3], int32_t numit, const pk_opts* opts,
+                            pk_vol* out, double* err_mat) {
+    return guarded(c, [&] {
+        if (!f_dims || !b_dims) pk::fail(PK_ERR_INVALID_ARG, "PSF dims are null");
+        c->decon(1, I, psf_f, f_dims, psf_b, b_dims, numit, opts, out, err_mat);
+    });
+}
+
+pk_status pk_psf2otf(pk_ctx* c, const float* psf, const int64_t pd[3], const int64_t od[3], float* otf) {
+    return guarded(c, [&] {
+        if (!psf || !pd || !od || !otf) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        for (int d = 0; d < 3; ++d) {
+            if (pd[d] < 1 || od[d] < 1) pk::fail(PK_ERR_INVALID_ARG, "empty size");
+            if (od[d] < pd[d]) pk::fail(PK_ERR_OUT_SIZE, "outSizeIsSmallerThanPsfSize");
+        }
+        const pk::VolPlan& vp = c->plan(od[0], od[1], od[2]);
+        pk::DevBuf spec;
+        c->alloc(spec, sizeof(float2) * (size_t)vp.S);
+        c->build_otf(vp, psf, pd, spec.as<float2>());
+        std::vector<float2> h((size_t)vp.S);
+        PK_HIP(hipMemcpy(h.data(), spec.p, sizeof(float2) * (size_t)vp.S, hipMemcpyDeviceToHost));
+        // expand the half spectrum with Hermitian symmetry into MATLAB's full OTF
+        const int64_t n1 = vp.n1, n2 = vp.n2, n3 = vp.n3, h1 = vp.h1;
+        for (int64_t k3 = 0; k3 < n3; ++k3)
+            for (int64_t k2 = 0; k2 < n2; ++k2)
+                for (int64_t k1 = 0; k1 < n1; ++k1) {
+                    float2 v;
+                    if (k1 < h1) {
+                        v = h[(size_t)(k1 + h1 * (k2 + n2 * k3))];
+                    } else {
+                        const int64_t m1 = n1 - k1, m2 = (n2 - k2) % n2, m3 = (n3 - k3) % n3;
+                        v = h[(size_t)(m1 + h1 * (m2 + n2 * m3))];
+                        v.y = -v.y;
+                    }
+                    const size_t o = (size_t)(k1 + n1 * (k2 + n2 * k3));
+                    otf[2 * o] = v.x;
+                    otf[2 * o + 1] = v.y;
+                }
+    });
+}
+
+pk_status pk_fft_r2c(pk_ctx* c, const float* d_real, float* d_spec, const int64_t dims[3]) {
+    return guarded(c, [&] {
+        if (!d_real || !d_spec || !dims) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        const pk::VolPlan& vp = c->plan(dims[0], dims[1], dims[2]);
+        c->r2c3d(vp, d_real, reinterpret_cast<float2*>(d_spec));
+        PK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+pk_status pk_fft_c2r(pk_ctx* c, const float* d_spec, float* d_real, const int64_t dims[3]) {
+    return guarded(c, [&] {
+        if (!d_real || !d_spec || !dims) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        const pk::VolPlan& vp = c->plan(dims[0], dims[1], dims[2]);
+        c->ensure(c->ws.A, sizeof(float2) * (size_t)vp.S);
+        PK_HIP(hipMemcpyAsync(c->ws.A.p, d_spec, sizeof(float2) * (size_t)vp.S, hipMemcpyDeviceToDevice, c->stream));
+        c->c2r3d(vp, c->ws.A.as<float2>(), d_real);
+        PK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int64_t pk_find_good_factor_number(int64_t n, int32_t dir, int32_t allow_odd_seven, int32_t max_prime) {
+    if (dir == 0) dir = 1;
+    for (int guard = 0; guard < (1 << 24); ++guard) {
+        if (n <= 1) return n;  // factor(1) = 1 and factor(0) = 0 end the search
+        int64_t m = n, pmin = 0, pmax = 1;
+        for (int64_t p = 2; p * p <= m; ++p)
+            while (m % p == 0) {
+                if (!pmin) pmin = p;
+                pmax = p;
+                m /= p;
+            }
+        if (m > 1) {
+            if (!pmin) pmin = m;
+            if (m > pmax) pmax = m;
+        }
+        bool bad = pmax > max_prime;
+        if (max_prime >= 7 && !allow_odd_seven && pmin > 2 && pmax == 7) bad = true;
+        if (!bad) return n;
+        n += dir;
+    }
+    return -1;
+}
+
+}  // extern "C"

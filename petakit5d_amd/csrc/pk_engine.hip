@@ -1,0 +1,606 @@
+// pk_engine.hip -- host side of the gfx950 deconvolution engine.
+//
+// Iteration schedules (one launch per line; x = dim-1 row pass, y / z =
+// strided passes; every elementwise step of the reference rides inside the
+// first or last pass of a transform):
+//
+//   OMW (decon_lucy_omw_function.m:321-329), per iteration:
+//     [x PREP or previous x UPDATE]  J -> A = R2C_x(J)
+//     y FWD;  z FWD*OTF_f*INV;  y INV
+//     x RATIO      c = C2R_x(A); r = I ./ max(c, eps); A = R2C_x(r)
+//     y FWD;  z FWD*OTF_b*INV;  y INV
+//     x UPDATE     t = C2R_x(A); J = max(t .* J, 0); A = R2C_x(J) unless last
+//
+//   Biggs RL (decon_lucy_function.m:121-143), per iteration k:
+//     [k >= 3] lambda from the block partials of the previous update
+//     x BIGGS_Y    Y = max(J2 + lambda (J2 - J3), 0); A = R2C_x(Y)
+//     y FWD;  z FWD*H*INV;  y INV
+//     x RATIO
+//     y FWD;  z FWD*conj(H)*INV;  y INV
+//     x BIGGS_UPDATE  Jn = max(Y .* C2R_x(A), 0) (into J3's buffer);
+//                     J4 = Jn - Y, with the two dot products of the next
+//                     step length accumulated before J4 is overwritten
+#include "pk_engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pk_launch.hpp"
+#include "pk_misc.hpp"
+
+namespace pk {
+
+static const float kEps = 2.220446049250313080847e-16f;  // MATLAB eps (2^-52), exact in single
+
+void fail(pk_status st, const std::string& msg) { throw Error{st, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        pk_status st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? PK_ERR_NO_MEMORY : PK_ERR_HIP;
+        fail(st, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+bool smooth235(int64_t n) {
+    if (n < 1) return false;
+    for (int p : {2, 3, 5})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+// ------------------------------------------------------------------ DevBuf
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
+    if (p) {
+        (void)hipFree(p);
+        if (owner) owner->live_bytes -= bytes;
+    }
+    p = nullptr;
+    bytes = 0;
+}
+
+void pk_ctx_impl::alloc(DevBuf& b, size_t bytes) {
+    b.release();
+    if (bytes == 0) return;
+    if (budget && live_bytes + bytes > budget)
+        fail(PK_ERR_BUDGET, "device allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
+                                std::to_string(live_bytes) + " of " + std::to_string(budget) + " bytes in use)");
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(PK_ERR_NO_MEMORY, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+    }
+    b.p = p;
+    b.bytes = bytes;
+    b.owner = this;
+    live_bytes += bytes;
+}
+
+void pk_ctx_impl::ensure(DevBuf& b, size_t bytes) {
+    if (b.bytes < bytes) alloc(b, bytes);
+}
+
+// ------------------------------------------------------------------ planning
+static const int kEs[] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 27, 30, 32};
+static const int kRadices[] = {32, 30, 27, 25, 24, 20, 18, 16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+
+// Factor L into radices that all divide E; E is a multiple of every prime of L.
+static FftPlan plan_1d(int L, int target_e, int max_threads) {
+    FftPlan best{};
+    best.L = L;
+    if (L == 1) {
+        best.P = 1;
+        best.E = 1;
+        best.nst = 0;
+        return best;
+    }
+    int best_score = 1 << 30;
+    bool found = false;
+    for (int E : kEs) {
+        if (L % E) continue;
+        if (L / E > max_threads) continue;
+        bool ok = true;
+        for (int p : {2, 3, 5})
+            if (L % p == 0 && E % p != 0) ok = false;
+        if (!ok || !e_supported(E)) continue;
+        FftPlan c{};
+        c.L = L;
+        c.E = E;
+        c.P = L / E;
+        int rem = L, ns = 1;
+        while (rem > 1) {
+            int r = 0;
+            for (int cand : kRadices)
+                if (E % cand == 0 && rem % cand == 0) {
+                    r = cand;
+                    break;
+                }
+            if (!r || c.nst >= kMaxStages) {
+                ok = false;
+                break;
+            }
+            c.R[c.nst] = r;
+            c.Ns[c.nst] = ns;
+            c.mag[c.nst] = ns > 1 ? (uint32_t)((1ull << 32) / (uint64_t)ns + 1ull) : 0u;
+            c.tws[c.nst] = L / (ns * r);
+            c.nst++;
+            ns *= r;
+            rem /= r;
+        }
+        if (!ok) continue;
+        const int score = c.nst * 100 + std::abs(E - target_e);
+        if (score < best_score) {
+            best_score = score;
+            best = c;
+            found = true;
+        }
+    }
+    if (!found) fail(PK_ERR_UNSUPPORTED_SIZE, "no transform plan for length " + std::to_string(L));
+    return best;
+}
+
+static void fill_twiddles(std::vector<float2>& t, int L) {
+    t.resize(L);
+    for (int e = 0; e < L; ++e) {
+        // exact at quarter turns; long double elsewhere, rounded once
+        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)L;
+        float c = (float)cosl(a), s = (float)(-sinl(a));
+        if (4 * e == L) { c = 0.f; s = -1.f; }
+        if (2 * e == L) { c = -1.f; s = 0.f; }
+        if (4 * e == 3 * L) { c = 0.f; s = 1.f; }
+        t[e] = make_float2(c, s);
+    }
+}
+
+const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
+    std::vector<int64_t> key{n1, n2, n3};
+    auto it = plans.find(key);
+    if (it != plans.end()) return it->second;
+    for (int64_t n : {n1, n2, n3})
+        if (!smooth235(n))
+            fail(PK_ERR_UNSUPPORTED_SIZE, "axis length " + std::to_string(n) +
+                                              " is not 2/3/5-smooth; pad it with findGoodFactorNumber(n, 1, false, 5)");
+    if (n1 > 16384 || n2 > 8192 || n3 > 8192) fail(PK_ERR_UNSUPPORTED_SIZE, "axis longer than the supported maximum");
+    VolPlan vp;
+    vp.n1 = n1;
+    vp.n2 = n2;
+    vp.n3 = n3;
+    vp.h1 = n1 / 2 + 1;
+    vp.N = n1 * n2 * n3;
+    vp.S = vp.h1 * n2 * n3;
+    vp.packed = (n1 % 2 == 0);
+    const int Lx = (int)(vp.packed ? n1 / 2 : n1);
+
+    auto table = [&](int L) -> const float2* {
+        auto t = tw.find(L);
+        if (t != tw.end()) return t->second->as<float2>();
+        std::vector<float2> h;
+        fill_twiddles(h, L);
+        auto b = std::make_unique<DevBuf>();
+        alloc(*b, sizeof(float2) * (size_t)L);
+        PK_HIP(hipMemcpy(b->p, h.data(), sizeof(float2) * (size_t)L, hipMemcpyHostToDevice));
+        const float2* r = b->as<float2>();
+        tw[L] = std::move(b);
+        return r;
+    };
+
+    // x (rows)
+    {
+        AxisLaunch& ax = vp.x;
+        ax.fp = plan_1d(Lx, 16, kMaxThreads);
+        ax.fp.tw = table(Lx);
+        ax.lpad = Lx;
+        int rt = std::max(1, kMaxThreads / ax.fp.P);
+        while (rt > 1 && (size_t)rt * ax.lpad * sizeof(float2) > 65536) rt--;
+        ax.tile = rt;
+        ax.block = dim3(rt * ax.fp.P);
+        ax.lds = std::max<size_t>((size_t)rt * ax.lpad * sizeof(float2), 2 * sizeof(double) * kMaxThreads);
+    }
+    // y, z (columns)
+    auto colplan = [&](AxisLaunch& ax, int L) {
+        ax.fp = plan_1d(L, 16, kMaxThreads);
+        ax.fp.tw = table(L);
+        int T = std::max(1, kMaxThreads / ax.fp.P);
+        while (T > 1 && (size_t)T * L * sizeof(float2) > 65536) T--;
+        ax.tile = T;
+        ax.block = dim3(T * ax.fp.P);
+        ax.lds = std::max<size_t>((size_t)T * L * sizeof(float2), 16);
+    };
+    colplan(vp.y, (int)n2);
+    colplan(vp.z, (int)n3);
+
+    if (vp.packed) {
+        auto r = rtw.find(n1);
+        if (r == rtw.end()) {
+            const int M = (int)(n1 / 2);
+            std::vector<float2> h(M + 1);
+            for (int k = 0; k <= M; ++k) {
+                long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n1;
+                float c = (float)cosl(a), s = (float)(-sinl(a));
+                if (4 * k == n1) { c = 0.f; s = -1.f; }
+                if (2 * k == n1) { c = -1.f; s = 0.f; }
+                h[k] = make_float2(c, s);
+            }
+            auto b = std::make_unique<DevBuf>();
+            alloc(*b, sizeof(float2) * (size_t)(M + 1));
+            PK_HIP(hipMemcpy(b->p, h.data(), sizeof(float2) * (size_t)(M + 1), hipMemcpyHostToDevice));
+            r = rtw.emplace(n1, std::move(b)).first;
+        }
+        vp.rtw = r->second->as<float2>();
+    }
+    return plans.emplace(key, vp).first->second;
+}
+
+// ------------------------------------------------------------------ passes
+void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a) {
+    a.n1 = (int)vp.n1;
+    a.h1 = (int)vp.h1;
+    a.packed = vp.packed;
+    a.RT = vp.x.tile;
+    a.lpad = vp.x.lpad;
+    a.nrows = vp.n2 * vp.n3;
+    a.rtw = vp.rtw;
+    a.eps = kEps;
+    const int64_t nblk = (a.nrows + a.RT - 1) / a.RT;
+    PK_HIP(launch_xpass(vp.x.fp, dim3((unsigned)nblk), vp.x.block, vp.x.lds, stream, a));
+}
+
+static int64_t xpass_blocks(const VolPlan& vp) { return (vp.n2 * vp.n3 + vp.x.tile - 1) / vp.x.tile; }
+
+void pk_ctx_impl::run_col(const VolPlan& vp, int axis, float2* data, int mode, const float2* otf, float scale) {
+    const AxisLaunch& ax = axis == 2 ? vp.y : vp.z;
+    if (ax.fp.L == 1 && (mode == CM_FWD || mode == CM_INV)) return;  // length-1 transform: identity
+    ColArgs a{};
+    a.data = data;
+    a.otf = otf;
+    a.T = ax.tile;
+    a.mode = mode;
+    a.scale = scale;
+    dim3 grid;
+    if (axis == 2) {
+        a.acount = vp.h1;
+        a.bstride = vp.h1 * vp.n2;
+        a.stride = vp.h1;
+        grid = dim3((unsigned)((vp.h1 + a.T - 1) / a.T), (unsigned)vp.n3);
+    } else {
+        a.acount = vp.h1 * vp.n2;
+        a.bstride = 0;
+        a.stride = vp.h1 * vp.n2;
+        grid = dim3((unsigned)((a.acount + a.T - 1) / a.T), 1);
+    }
+    PK_HIP(launch_colpass(ax.fp, grid, ax.block, ax.lds, stream, a));
+}
+
+void pk_ctx_impl::r2c3d(const VolPlan& vp, const float* real, float2* spec) {
+    XArgs a{};
+    a.mode = XM_R2C;
+    a.scale = 1.f;
+    a.src = real;
+    a.spec = spec;
+    run_x(vp, a);
+    run_col(vp, 2, spec, CM_FWD, nullptr, 1.f);
+    run_col(vp, 3, spec, CM_FWD, nullptr, 1.f);
+}
+
+void pk_ctx_impl::c2r3d(const VolPlan& vp, float2* spec, float* real) {
+    run_col(vp, 3, spec, CM_INV, nullptr, 1.f);
+    run_col(vp, 2, spec, CM_INV, nullptr, 1.f);
+    XArgs a{};
+    a.mode = XM_C2R;
+    a.scale = 1.f;
+    a.spec = spec;
+    a.dst = real;
+    run_x(vp, a);
+}
+
+void pk_ctx_impl::spectral_conv(const VolPlan& vp, float2* A, const float2* H, bool conj) {
+    const float scale = (float)(1.0 / (double)vp.N);  // ifftn's 1/N
+    run_col(vp, 2, A, CM_FWD, nullptr, 1.f);
+    run_col(vp, 3, A, conj ? CM_CONVJ : CM_CONV, H, scale);
+    run_col(vp, 2, A, CM_INV, nullptr, 1.f);
+}
+
+static int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 65536); }
+
+// ------------------------------------------------------------------ OTF cache
+static uint64_t fnv1a(const void* p, size_t n, uint64_t h = 1469598103934665603ull) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= c[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+// OTF of PSF ./ sum(PSF(:)) at the volume size (decon_lucy_function.m:76,
+// decon_lucy_omw_function.m:286-287), half spectrum, cached on the device.
+const float2* pk_ctx_impl::otf_for(const VolPlan& vp, const float* psf, const int64_t pd[3]) {
+    const int64_t P = pd[0] * pd[1] * pd[2];
+    const uint64_t h = fnv1a(psf, sizeof(float) * (size_t)P, fnv1a(pd, sizeof(int64_t) * 3));
+    for (auto& e : otf) {
+        if (e.dims[0] == vp.n1 && e.dims[1] == vp.n2 && e.dims[2] == vp.n3 && e.hash == h &&
+            e.pdims[0] == pd[0] && e.pdims[1] == pd[1] && e.pdims[2] == pd[2]) {
+            e.tick = ++tick;
+            return e.spec->as<float2>();
+        }
+    }
+    if (pd[0] > vp.n1 || pd[1] > vp.n2 || pd[2] > vp.n3)
+        fail(PK_ERR_OUT_SIZE, "outSizeIsSmallerThanPsfSize: PSF larger than the volume");
+    // evict least recently used
+    while (otf.size() >= otf_max_entries) {
+        auto lru = std::min_element(otf.begin(), otf.end(),
+                                    [](const OtfEntry& a, const OtfEntry& b) { return a.tick < b.tick; });
+        otf.erase(lru);
+    }
+    // PSF ./ sum(PSF(:)): sum accumulated in double, divided in single
+    std::vector<float> q((size_t)P);
+    double s = 0;
+    for (int64_t i = 0; i < P; ++i) s += (double)psf[i];
+    const float sf = (float)s;
+    for (int64_t i = 0; i < P; ++i) q[(size_t)i] = psf[i] / sf;
+
+    OtfEntry ent{};
+    ent.dims[0] = vp.n1;
+    ent.dims[1] = vp.n2;
+    ent.dims[2] = vp.n3;
+    std::memcpy(ent.pdims, pd, sizeof(int64_t) * 3);
+    ent.hash = h;
+    ent.tick = ++tick;
+    ent.spec = std::make_unique<DevBuf>();
+    alloc(*ent.spec, sizeof(float2) * (size_t)vp.S);
+    build_otf(vp, q.data(), pd, ent.spec->as<float2>());
+    otf.push_back(std::move(ent));
+    return otf.back().spec->as<float2>();
+}
+
+void pk_ctx_impl::build_otf(const VolPlan& vp, const float* q, const int64_t pd[3], float2* dst) {
+    const int64_t P = pd[0] * pd[1] * pd[2];
+    ensure(ws.tmp, sizeof(float) * (size_t)std::max<int64_t>(vp.N, P));
+    DevBuf dq;
+    alloc(dq, sizeof(float) * (size_t)P);
+    PK_HIP(hipMemcpyAsync(dq.p, q, sizeof(float) * (size_t)P, hipMemcpyHostToDevice, stream));
+    float* real = ws.tmp.as<float>();
+    hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(vp.N)), dim3(256), 0, stream, real, vp.N, 0.f);
+    PK_HIP(hipGetLastError());
+    bool allzero = true;
+    for (int64_t i = 0; i < P && allzero; ++i) allzero = (q[i] == 0.f);
+    if (!allzero) {  // an all-zero PSF gives a zero OTF (decon_psf2otf.m:42-44)
+        hipLaunchKernelGGL(k_scatter_psf, dim3(grid_for(P)), dim3(256), 0, stream, dq.as<float>(), (int)pd[0],
+                           (int)pd[1], (int)pd[2], real, vp.n1, vp.n2, vp.n3);
+        PK_HIP(hipGetLastError());
+    }
+    r2c3d(vp, real, dst);
+    PK_HIP(hipStreamSynchronize(stream));  // dq is freed on return
+}
+
+void pk_ctx_impl::reset() {
+    PK_HIP(hipStreamSynchronize(stream));
+    otf.clear();
+    plans.clear();
+    tw.clear();
+    rtw.clear();
+    ws.I.release(); ws.J.release(); ws.A.release(); ws.J3.release(); ws.Y.release(); ws.J4.release();
+    ws.partials.release(); ws.lambda.release(); ws.raw.release(); ws.out.release(); ws.mask.release();
+    ws.mask2.release(); ws.count.release(); ws.tmp.release();
+}
+
+// ------------------------------------------------------------------ decon driver
+static double matlab_u16(double x) {
+    if (!(x > 0)) return 0;
+    if (x >= 65535.0) return 65535;
+    return std::round(x);
+}
+
+void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const int64_t fd[3], const float* psf_b,
+                        const int64_t bd[3], int numit, const pk_opts* o, pk_vol* out, double* err_mat) {
+    if (!Iv || !out || !o || !psf_f || numit < 0) fail(PK_ERR_INVALID_ARG, "null argument or negative NUMIT");
+    if (method == 1 && !psf_b) fail(PK_ERR_INVALID_ARG, "PSF_b missing");
+    if (Iv->dtype != PK_UINT16 && Iv->dtype != PK_SINGLE)
+        fail(PK_ERR_UNSUPPORTED_TYPE, "input class must be uint16 or single");
+    const int64_t n1 = Iv->dims[0], n2 = Iv->dims[1], n3 = Iv->dims[2];
+    if (n1 < 1 || n2 < 1 || n3 < 1) fail(PK_ERR_INVALID_ARG, "empty volume");
+    for (int d = 0; d < 3; ++d) {
+        if (fd[d] < 1 || (method == 1 && bd[d] < 1)) fail(PK_ERR_INVALID_ARG, "empty PSF");
+        if (fd[d] > Iv->dims[d] || (method == 1 && bd[d] > Iv->dims[d]))
+            fail(PK_ERR_OUT_SIZE, "outSizeIsSmallerThanPsfSize: PSF larger than the volume");
+    }
+    // output geometry (bbox, 1-based inclusive)
+    int64_t b0[3] = {0, 0, 0}, od[3] = {n1, n2, n3};
+    if (o->has_bbox) {
+        for (int d = 0; d < 3; ++d) {
+            const int64_t s = o->bbox[d], e = o->bbox[d + 3];
+            if (s < 1 || e > Iv->dims[d] || e < s) fail(PK_ERR_INVALID_ARG, "bbox outside the volume");
+            b0[d] = s - 1;
+            od[d] = e - s + 1;
+        }
+    }
+    if (out->dims[0] != od[0] || out->dims[1] != od[1] || out->dims[2] != od[2])
+        fail(PK_ERR_INVALID_ARG, "output dims do not match the (bbox) result size");
+    if (out->dtype != (o->save16bit ? PK_UINT16 : PK_SINGLE))
+        fail(PK_ERR_INVALID_ARG, "output class must be uint16 when save16bit, single otherwise");
+    if (!out->data || !Iv->data) fail(PK_ERR_INVALID_ARG, "null data pointer");
+
+    const VolPlan& vp = plan(n1, n2, n3);
+    const int64_t N = vp.N;
+    const float2* Hf = otf_for(vp, psf_f, fd);
+    const float2* Hb = method == 1 ? otf_for(vp, psf_b, bd) : nullptr;
+
+    // workspace
+    ensure(ws.I, sizeof(float) * N);
+    ensure(ws.J, sizeof(float) * N);
+    ensure(ws.A, sizeof(float2) * vp.S);
+    const int64_t nxb = xpass_blocks(vp);
+    if (method == 0) {
+        ensure(ws.J3, sizeof(float) * N);
+        ensure(ws.Y, sizeof(float) * N);
+        ensure(ws.J4, sizeof(float) * N);
+        ensure(ws.partials, sizeof(double) * 2 * (size_t)nxb);
+        ensure(ws.lambda, sizeof(float) * 4);
+    }
+    const size_t esz = Iv->dtype == PK_UINT16 ? 2 : 4;
+    const void* raw = Iv->data;
+    if (Iv->mem == PK_MEM_HOST) {
+        ensure(ws.raw, esz * N);
+        PK_HIP(hipMemcpyAsync(ws.raw.p, Iv->data, esz * N, hipMemcpyHostToDevice, stream));
+        raw = ws.raw.p;
+    }
+    const uint16_t* raw16 = Iv->dtype == PK_UINT16 ? static_cast<const uint16_t*>(raw) : nullptr;
+    const float* rawf = Iv->dtype == PK_SINGLE ? static_cast<const float*>(raw) : nullptr;
+
+    float* I = ws.I.as<float>();
+    float* J = ws.J.as<float>();
+    float2* A = ws.A.as<float2>();
+
+    // prep (decon_lucy_function.m:85-100, GPU branch): uint16 I - uint16(Background)
+    // saturates at 0; single I gives max(I - Background, 0)
+    {
+        XArgs a{};
+        a.scale = 1.f;
+        a.spec = A;
+        a.I = I;
+        a.J = J;
+        if (raw16) {
+            a.mode = XM_PREP_U16;
+            a.raw16 = raw16;
+            a.bg = (float)matlab_u16(o->background);
+        } else {
+            a.mode = XM_PREP_F32;
+            a.rawf = rawf;
+            a.bg = (float)o->background;
+        }
+        run_x(vp, a);
+    }
+
+    if (method == 1) {
+        for (int k = 1; k <= numit; ++k) {
+            spectral_conv(vp, A, Hf, false);
+            XArgs r{};
+            r.mode = XM_RATIO;
+            r.scale = 1.f;
+            r.spec = A;
+            r.I = I;
+            run_x(vp, r);
+            spectral_conv(vp, A, Hb, false);
+            XArgs u{};
+            u.mode = XM_OMW_UPDATE;
+            u.scale = 1.f;
+            u.spec = A;
+            u.J = J;
+            u.r2c_out = k < numit;
+            run_x(vp, u);
+        }
+    } else {
+        float* J2 = J;
+        float* J3 = ws.J3.as<float>();
+        float* Y = ws.Y.as<float>();
+        float* J4 = ws.J4.as<float>();
+        float* lam = ws.lambda.as<float>();
+        double* part = ws.partials.as<double>();
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N)), dim3(256), 0, stream, J3, N, 0.f);
+        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, stream, lam, (int64_t)1, 0.f);
+        PK_HIP(hipGetLastError());
+        for (int k = 1; k <= numit; ++k) {
+            if (k > 2) {
+                hipLaunchKernelGGL(k_lambda, dim3(1), dim3(256), 0, stream, part, (int)nxb, kEps, lam);
+                PK_HIP(hipGetLastError());
+            }
+            XArgs y{};
+            y.mode = XM_BIGGS_Y;
+            y.scale = 1.f;
+            y.spec = A;
+            y.J = J2;
+            y.J3 = J3;
+            y.Y = Y;
+            y.lambda = lam;
+            run_x(vp, y);
+            spectral_conv(vp, A, Hf, false);
+            XArgs r{};
+            r.mode = XM_RATIO;
+            r.scale = 1.f;
+            r.spec = A;
+            r.I = I;
+            run_x(vp, r);
+            spectral_conv(vp, A, Hf, true);
+            XArgs u{};
+            u.mode = XM_BIGGS_UPDATE;
+            u.scale = 1.f;
+            u.spec = A;
+            u.Y = Y;
+            u.Jn = J3;  // x_{k-2} is dead once Y is formed
+            u.J4 = J4;
+            u.acc = (k >= 2 && k < numit);
+            u.partials = part;
+            u.r2c_out = 0;
+            run_x(vp, u);
+            std::swap(J2, J3);
+        }
+        J = J2;
+    }
+
+    // epilogue (decon_lucy_function.m:177-203)
+    const bool need_mask = o->decon_offset != 0 || o->edge_erosion > 0;
+    const uint8_t* mask = nullptr;
+    if (o->edge_erosion > 0) {
+        ensure(ws.mask, (size_t)N);
+        ensure(ws.mask2, (size_t)N);
+        ensure(ws.count, 16);
+        PK_HIP(hipMemsetAsync(ws.count.p, 0, 16, stream));
+        auto* cnt = ws.count.as<unsigned long long>();
+        uint8_t* m1 = ws.mask.as<uint8_t>();
+        uint8_t* m2 = ws.mask2.as<uint8_t>();
+        const int is3d = n3 > 1;
+        hipLaunchKernelGGL(k_mask_init, dim3(grid_for(N)), dim3(256), 0, stream, raw16, rawf, N, m1, cnt);
+        hipLaunchKernelGGL(k_mask_faces, dim3(grid_for(N)), dim3(256), 0, stream, m1, n1, n2, n3, is3d, cnt,
+                           (int)o->edge_erosion);
+        hipLaunchKernelGGL(k_mask_erode_disk, dim3(grid_for(N)), dim3(256), 0, stream, m1, m2, n1, n2, n3,
+                           (int)o->edge_erosion - 1, cnt);
+        mask = m2;
+        if (is3d) {
+            hipLaunchKernelGGL(k_mask_erode_z, dim3(grid_for(N)), dim3(256), 0, stream, m2, m1, n1, n2, n3,
+                               (int)o->edge_erosion - 1, cnt);
+            mask = m1;
+        }
+        PK_HIP(hipGetLastError());
+    }
+    const int64_t on = od[0] * od[1] * od[2];
+    const size_t osz = o->save16bit ? 2 : 4;
+    void* dout = out->data;
+    if (out->mem == PK_MEM_HOST) {
+        ensure(ws.out, osz * on);
+        dout = ws.out.p;
+    }
+    EpiArgs e{};
+    e.J = J;
+    e.I = I;
+    e.mask = mask;
+    e.raw16 = raw16;
+    e.rawf = rawf;
+    e.out16 = o->save16bit ? static_cast<uint16_t*>(dout) : nullptr;
+    e.outf = o->save16bit ? nullptr : static_cast<float*>(dout);
+    e.n1 = n1;
+    e.n2 = n2;
+    e.o1 = od[0];
+    e.o2 = od[1];
+    e.o3 = od[2];
+    e.b1 = b0[0];
+    e.b2 = b0[1];
+    e.b3 = b0[2];
+    e.damp = (float)o->damp_factor;
+    e.scale = (float)o->scale_factor;
+    e.offset = (float)o->decon_offset;
+    e.apply_scale = (o->scale_factor != 1 || o->decon_offset != 0 || o->edge_erosion > 0);
+    e.use_mask = need_mask;
+    if (!(o->damp_factor > 1)) e.damp = 0.f;
+    hipLaunchKernelGGL(k_epilogue, dim3(grid_for(on)), dim3(256), 0, stream, e);
+    PK_HIP(hipGetLastError());
+    if (out->mem == PK_MEM_HOST)
+        PK_HIP(hipMemcpyAsync(out->data, dout, osz * on, hipMemcpyDeviceToHost, stream));
+    PK_HIP(hipStreamSynchronize(stream));
+    if (err_mat) std::memset(err_mat, 0, sizeof(double) * 4 * (size_t)numit);
+}
+
+}  // namespace pk

@@ -1,0 +1,111 @@
+// pk_engine.hpp -- host side of the deconvolution engine: transform plans,
+// device workspace, the OTF cache and the RL / OMW iteration drivers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pkdecon.h"
+#include "pk_fft_dev.hpp"
+
+namespace pk {
+
+struct Error {
+    pk_status st;
+    std::string msg;
+};
+
+[[noreturn]] void fail(pk_status st, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define PK_HIP(x) ::pk::hip_check((x), #x)
+
+bool smooth235(int64_t n);
+
+// Device allocation owned by a context (counted against its budget).
+struct pk_ctx_impl;
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    pk_ctx_impl* owner = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf();
+    void release();
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct AxisLaunch {
+    FftPlan fp{};
+    int tile = 1;      // rows (x) or columns (y, z) per block
+    int lpad = 0;      // x: LDS row pitch
+    dim3 block{1, 1, 1};
+    size_t lds = 0;
+};
+
+struct VolPlan {
+    int64_t n1 = 0, n2 = 0, n3 = 0, h1 = 0, N = 0, S = 0;
+    int packed = 0;
+    AxisLaunch x, y, z;
+    const float2* rtw = nullptr;  // W_{n1}^k, k in [0, n1/2]
+};
+
+struct OtfEntry {
+    int64_t dims[3];
+    int64_t pdims[3];
+    uint64_t hash;
+    uint64_t tick;
+    std::unique_ptr<DevBuf> spec;
+};
+
+struct Workspace {
+    DevBuf I, J, A, J3, Y, J4, partials, lambda, raw, out, mask, mask2, count, tmp;
+};
+
+struct pk_ctx_impl {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t live_bytes = 0;
+    std::recursive_mutex mu;
+    std::string last_error;
+    std::map<int, std::unique_ptr<DevBuf>> tw;    // per transform length
+    std::map<int64_t, std::unique_ptr<DevBuf>> rtw;  // per even n1
+    std::map<std::vector<int64_t>, VolPlan> plans;
+    std::vector<OtfEntry> otf;
+    uint64_t tick = 0;
+    size_t otf_max_entries = 8;
+    Workspace ws;
+
+    // device memory helpers
+    void alloc(DevBuf& b, size_t bytes);
+    void ensure(DevBuf& b, size_t bytes);
+
+    const VolPlan& plan(int64_t n1, int64_t n2, int64_t n3);
+    const float2* otf_for(const VolPlan& vp, const float* psf, const int64_t pdims[3]);
+    // R2C of the zero-padded, circularly shifted PSF q (decon_psf2otf.m:19-27) into dst
+    void build_otf(const VolPlan& vp, const float* q, const int64_t pdims[3], float2* dst);
+
+    // transforms (device pointers)
+    void r2c3d(const VolPlan& vp, const float* real, float2* spec);
+    void c2r3d(const VolPlan& vp, float2* spec, float* real);
+    void run_x(const VolPlan& vp, XArgs a);
+    void run_col(const VolPlan& vp, int axis, float2* data, int mode, const float2* otf, float scale);
+    // convolution of the spectrum in ws.A: forward y, z-conv, inverse y
+    void spectral_conv(const VolPlan& vp, float2* A, const float2* H, bool conj);
+
+    void decon(int method, const pk_vol* I, const float* psf_f, const int64_t fd[3], const float* psf_b,
+               const int64_t bd[3], int numit, const pk_opts* o, pk_vol* out, double* err_mat);
+    void reset();
+};
+
+}  // namespace pk

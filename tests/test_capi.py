@@ -1,0 +1,79 @@
+"""CPU checks of the C-ABI boundary: libpkdecon.so loads, exports every
+symbol include/pkdecon.h declares, and its host-only logic behaves; plus the
+Python front end's argument handling (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import petakit5d_amd as pk
+from petakit5d_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    txt = open(os.path.join(ROOT, "include", "pkdecon.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pk_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_matches_binding_list():
+    assert _header_functions() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    for name in _header_functions():
+        assert hasattr(L, name), name
+        assert ctypes.cast(getattr(L, name), ctypes.c_void_p).value
+
+
+def test_find_good_factor_number_matches_oracle(oracle):
+    for n in range(1, 1500, 7):
+        for mp, odd in ((7, False), (7, True), (5, False)):
+            assert _lib.find_good_factor_number(n, 1, odd, mp) == oracle.findGoodFactorNumber(n, 1, odd, mp)
+            assert _lib.find_good_factor_number(n, -1, odd, mp) == oracle.findGoodFactorNumber(n, -1, odd, mp)
+
+
+def test_opts_default():
+    o = _lib.PkOpts()
+    _lib.load().pk_opts_default(ctypes.byref(o))
+    assert (o.background, o.damp_factor, o.scale_factor, o.decon_offset, o.edge_erosion, o.save16bit,
+            o.has_bbox) == (0.0, 1.0, 1.0, 0.0, 0, 1, 0)
+
+
+def test_null_arguments_do_not_crash():
+    L = _lib.load()
+    assert L.pk_reset(None) == 1
+    assert L.pk_decon_lucy(None, None, None, None, 1, None, None, None) == 1
+    assert L.pk_last_error(None) == b"null context"
+    assert L.pk_ctx_create(0, 0, None) == 1
+
+
+def test_no_device_is_reported():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    h = ctypes.c_void_p()
+    assert _lib.load().pk_ctx_create(0, 0, ctypes.byref(h)) == 10  # PK_ERR_NO_DEVICE
+
+
+def test_option_parsing():
+    from petakit5d_amd.decon import _parse
+    o = _parse({"background": 100, "SAVE16BIT": False, "dampfactor": 2})
+    assert o["Background"] == 100 and o["save16bit"] is False and o["dampFactor"] == 2
+    with pytest.raises(TypeError):
+        _parse({"Backgroundd": 1})
+    with pytest.raises(ValueError, match="useGPU"):
+        pk.decon_lucy_function(np.zeros((4, 4, 4), np.uint16), np.ones((1, 1, 1)), 1, useGPU=False)
+
+
+def test_crop_psf_to_data():
+    p = np.arange(5 * 6 * 7, dtype=np.float32).reshape(5, 6, 7)
+    c = pk.crop_psf_to_data(p, (3, 6, 4))
+    # s = max(0, floor((p - im)/2)) + 1 (1-based), t = s + min(p, im) - 1
+    np.testing.assert_array_equal(c, p[1:4, 0:6, 1:5])
+    assert pk.crop_psf_to_data(p, (8, 8, 8)) is p
