@@ -116,6 +116,21 @@ pk_status pk_psf2otf(pk_ctx *ctx, const float *psf, const int64_t psf_dims[3], c
 pk_status pk_fft_r2c(pk_ctx *ctx, const float *d_real, float *d_spec, const int64_t dims[3]);
 pk_status pk_fft_c2r(pk_ctx *ctx, const float *d_spec, float *d_real, const int64_t dims[3]);
 
+/* Per-pass timing (no counterpart in the reference, which only has tic/toc):
+ * HIP events recorded around every kernel launch on the context's stream.
+ * enable=1 clears the table and starts recording, 0 stops.  Entries are
+ * aggregated by pass name with the pass's algorithmic bytes per launch
+ * (every operand read or written once). */
+typedef struct {
+    char name[32];
+    int64_t launches;
+    double total_ms;
+    double bytes_per_launch;
+} pk_prof_entry;
+pk_status pk_profile_enable(pk_ctx *ctx, int32_t enable);
+/* Copies up to max entries into out; returns the number of entries (-1: null ctx). */
+int32_t pk_profile_read(pk_ctx *ctx, pk_prof_entry *out, int32_t max);
+
 /* findGoodFactorNumber(givenNum, directionFlag, allowOddSeven) (utils/findGoodFactorNumber.m:1-26).
  * max_prime = 7 is the reference rule; 5 gives the 2/3/5-smooth sizes the GPU transforms need. */
 int64_t pk_find_good_factor_number(int64_t n, int32_t direction, int32_t allow_odd_seven, int32_t max_prime);

@@ -46,6 +46,8 @@ EXPORTS = [
     "pk_psf2otf",
     "pk_fft_r2c",
     "pk_fft_c2r",
+    "pk_profile_enable",
+    "pk_profile_read",
     "pk_find_good_factor_number",
 ]
 
@@ -70,6 +72,15 @@ class PkOpts(ctypes.Structure):
         ("has_bbox", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("bbox", ctypes.c_int64 * 6),
+    ]
+
+
+class PkProfEntry(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 32),
+        ("launches", ctypes.c_int64),
+        ("total_ms", ctypes.c_double),
+        ("bytes_per_launch", ctypes.c_double),
     ]
 
 
@@ -125,11 +136,14 @@ def load():
         L.pk_psf2otf.argtypes = [vp, vp, i64p, i64p, vp]
         L.pk_fft_r2c.argtypes = [vp, vp, vp, i64p]
         L.pk_fft_c2r.argtypes = [vp, vp, vp, i64p]
+        L.pk_profile_enable.argtypes = [vp, ctypes.c_int32]
+        L.pk_profile_read.argtypes = [vp, ctypes.POINTER(PkProfEntry), ctypes.c_int32]
+        L.pk_profile_read.restype = ctypes.c_int32
         L.pk_find_good_factor_number.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.pk_find_good_factor_number.restype = ctypes.c_int64
         for name in EXPORTS:
             if name not in ("pk_opts_default", "pk_ctx_destroy", "pk_last_error", "pk_device_bytes",
-                            "pk_find_good_factor_number"):
+                            "pk_find_good_factor_number", "pk_profile_read"):
                 getattr(L, name).restype = ctypes.c_int
         _lib = L
         return _lib
@@ -161,6 +175,18 @@ class Context:
 
     def set_stream(self, stream_ptr: int | None):
         self.check(self._L.pk_set_stream(self.handle, ctypes.c_void_p(stream_ptr or 0)))
+
+    def profile(self, enable: bool):
+        """Start (clearing) or stop per-pass HIP-event timing."""
+        self.check(self._L.pk_profile_enable(self.handle, 1 if enable else 0))
+
+    def profile_read(self) -> dict:
+        """{pass name: (launches, total_ms, algorithmic bytes per launch)}."""
+        n = self._L.pk_profile_read(self.handle, None, 0)
+        buf = (PkProfEntry * max(n, 1))()
+        n = self._L.pk_profile_read(self.handle, buf, n)
+        return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms), float(buf[i].bytes_per_launch))
+                for i in range(n)}
 
     def device_bytes(self) -> int:
         return int(self._L.pk_device_bytes(self.handle))

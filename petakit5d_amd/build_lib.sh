@@ -1,5 +1,6 @@
 #!/usr/bin/env bash
 # Builds petakit5d_amd/libpkdecon.so (HIP, gfx950) from petakit5d_amd/csrc.
+# Translation units compile in parallel (at most $PK_JOBS at a time, default 8).
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 SRC="$HERE/csrc"
@@ -7,14 +8,12 @@ OBJ="$HERE/csrc/obj"
 mkdir -p "$OBJ"
 FLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Wall -Wno-unused-function
        -Xclang -target-feature -Xclang -packed-fp32-ops)
-pids=()
-for f in pk_kern_x pk_kern_col pk_misc pk_engine pk_capi; do
-  hipcc "${FLAGS[@]}" -c "$SRC/$f.hip" -o "$OBJ/$f.o" 2> >(grep -v "packed-fp32-ops" >&2) &
-  pids+=($!)
-done
-rc=0
-for p in "${pids[@]}"; do wait "$p" || rc=1; done
-[ $rc -eq 0 ] || { echo "compile failed" >&2; exit 1; }
-hipcc --offload-arch=gfx950 -shared -fPIC -o "$HERE/libpkdecon.so.tmp" "$OBJ"/pk_kern_x.o "$OBJ"/pk_kern_col.o \
-  "$OBJ"/pk_misc.o "$OBJ"/pk_engine.o "$OBJ"/pk_capi.o
+JOBS="${PK_JOBS:-8}"
+units=()
+for f in "$SRC"/*.hip; do units+=("$(basename "$f" .hip)"); done
+printf '%s\n' "${units[@]}" | xargs -P "$JOBS" -I{} bash -c \
+  'hipcc "$@" -c "'"$SRC"'/{}.hip" -o "'"$OBJ"'/{}.o" 2> >(grep -v "packed-fp32-ops" >&2)' _ "${FLAGS[@]}"
+objs=()
+for u in "${units[@]}"; do objs+=("$OBJ/$u.o"); done
+hipcc --offload-arch=gfx950 -shared -fPIC -o "$HERE/libpkdecon.so.tmp" "${objs[@]}"
 mv -f "$HERE/libpkdecon.so.tmp" "$HERE/libpkdecon.so"

@@ -134,16 +134,16 @@ pk_status pk_psf2otf(pk_ctx* c, const float* psf, const int64_t pd[3], const int
         std::vector<float2> h((size_t)vp.S);
         PK_HIP(hipMemcpy(h.data(), spec.p, sizeof(float2) * (size_t)vp.S, hipMemcpyDeviceToHost));
         // expand the half spectrum with Hermitian symmetry into MATLAB's full OTF
-        const int64_t n1 = vp.n1, n2 = vp.n2, n3 = vp.n3, h1 = vp.h1;
+        const int64_t n1 = vp.n1, n2 = vp.n2, n3 = vp.n3, h1 = vp.h1, hp = vp.hp;
         for (int64_t k3 = 0; k3 < n3; ++k3)
             for (int64_t k2 = 0; k2 < n2; ++k2)
                 for (int64_t k1 = 0; k1 < n1; ++k1) {
                     float2 v;
                     if (k1 < h1) {
-                        v = h[(size_t)(k1 + h1 * (k2 + n2 * k3))];
+                        v = h[(size_t)(k1 + hp * (k2 + n2 * k3))];
                     } else {
                         const int64_t m1 = n1 - k1, m2 = (n2 - k2) % n2, m3 = (n3 - k3) % n3;
-                        v = h[(size_t)(m1 + h1 * (m2 + n2 * m3))];
+                        v = h[(size_t)(m1 + hp * (m2 + n2 * m3))];
                         v.y = -v.y;
                     }
                     const size_t o = (size_t)(k1 + n1 * (k2 + n2 * k3));
@@ -157,7 +157,11 @@ pk_status pk_fft_r2c(pk_ctx* c, const float* d_real, float* d_spec, const int64_
     return guarded(c, [&] {
         if (!d_real || !d_spec || !dims) pk::fail(PK_ERR_INVALID_ARG, "null argument");
         const pk::VolPlan& vp = c->plan(dims[0], dims[1], dims[2]);
-        c->r2c3d(vp, d_real, reinterpret_cast<float2*>(d_spec));
+        c->ensure(c->ws.A, sizeof(float2) * (size_t)vp.S);
+        c->r2c3d(vp, d_real, c->ws.A.as<float2>());
+        // pitched (hp) working spectrum -> the caller's dense (h1) layout
+        PK_HIP(hipMemcpy2DAsync(d_spec, sizeof(float2) * vp.h1, c->ws.A.p, sizeof(float2) * vp.hp,
+                                sizeof(float2) * vp.h1, (size_t)(vp.n2 * vp.n3), hipMemcpyDeviceToDevice, c->stream));
         PK_HIP(hipStreamSynchronize(c->stream));
     });
 }
@@ -167,10 +171,38 @@ pk_status pk_fft_c2r(pk_ctx* c, const float* d_spec, float* d_real, const int64_
         if (!d_real || !d_spec || !dims) pk::fail(PK_ERR_INVALID_ARG, "null argument");
         const pk::VolPlan& vp = c->plan(dims[0], dims[1], dims[2]);
         c->ensure(c->ws.A, sizeof(float2) * (size_t)vp.S);
-        PK_HIP(hipMemcpyAsync(c->ws.A.p, d_spec, sizeof(float2) * (size_t)vp.S, hipMemcpyDeviceToDevice, c->stream));
+        PK_HIP(hipMemcpy2DAsync(c->ws.A.p, sizeof(float2) * vp.hp, d_spec, sizeof(float2) * vp.h1,
+                                sizeof(float2) * vp.h1, (size_t)(vp.n2 * vp.n3), hipMemcpyDeviceToDevice, c->stream));
         c->c2r3d(vp, c->ws.A.as<float2>(), d_real);
         PK_HIP(hipStreamSynchronize(c->stream));
     });
+}
+
+pk_status pk_profile_enable(pk_ctx* c, int32_t enable) {
+    return guarded(c, [&] {
+        PK_HIP(hipStreamSynchronize(c->stream));
+        c->prof_flush();
+        c->prof.clear();
+        c->prof_on = enable != 0;
+    });
+}
+
+int32_t pk_profile_read(pk_ctx* c, pk_prof_entry* out, int32_t max) {
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    int32_t n = 0;
+    for (const auto& kv : c->prof) {
+        if (out && n < max) {
+            pk_prof_entry& e = out[n];
+            std::memset(&e, 0, sizeof(e));
+            std::strncpy(e.name, kv.first.c_str(), sizeof(e.name) - 1);
+            e.launches = kv.second.launches;
+            e.total_ms = kv.second.total_ms;
+            e.bytes_per_launch = kv.second.bytes;
+        }
+        ++n;
+    }
+    return n;
 }
 
 int64_t pk_find_good_factor_number(int64_t n, int32_t dir, int32_t allow_odd_seven, int32_t max_prime) {

@@ -84,6 +84,41 @@ void pk_ctx_impl::ensure(DevBuf& b, size_t bytes) {
     if (b.bytes < bytes) alloc(b, bytes);
 }
 
+// ------------------------------------------------------------------ profiling
+hipEvent_t pk_ctx_impl::prof_mark() {
+    if (!prof_on) return nullptr;
+    hipEvent_t e;
+    if (!ev_pool.empty()) {
+        e = ev_pool.back();
+        ev_pool.pop_back();
+    } else {
+        PK_HIP(hipEventCreate(&e));
+    }
+    PK_HIP(hipEventRecord(e, stream));
+    return e;
+}
+
+void pk_ctx_impl::prof_done(const char* name, double bytes, hipEvent_t e0) {
+    if (!prof_on || !e0) return;
+    hipEvent_t e1 = prof_mark();
+    prof_pending.push_back(ProfPending{name, bytes, e0, e1});
+}
+
+void pk_ctx_impl::prof_flush() {
+    for (auto& p : prof_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.e0, p.e1) == hipSuccess) {
+            ProfAgg& a = prof[p.name];
+            a.launches++;
+            a.total_ms += ms;
+            a.bytes = p.bytes;
+        }
+        ev_pool.push_back(p.e0);
+        ev_pool.push_back(p.e1);
+    }
+    prof_pending.clear();
+}
+
 // ------------------------------------------------------------------ planning
 static const int kEs[] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 27, 30, 32};
 static const int kRadices[] = {32, 30, 27, 25, 24, 20, 18, 16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
@@ -170,8 +205,10 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
     vp.n2 = n2;
     vp.n3 = n3;
     vp.h1 = n1 / 2 + 1;
+    vp.hp = (vp.h1 + 15) / 16 * 16;
     vp.N = n1 * n2 * n3;
-    vp.S = vp.h1 * n2 * n3;
+    vp.S = vp.hp * n2 * n3;
+    vp.Sa = vp.h1 * n2 * n3;
     vp.packed = (n1 % 2 == 0);
     const int Lx = (int)(vp.packed ? n1 / 2 : n1);
 
@@ -193,7 +230,7 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
         AxisLaunch& ax = vp.x;
         ax.fp = plan_1d(Lx, 16, kMaxThreads);
         ax.fp.tw = table(Lx);
-        ax.lpad = Lx;
+        ax.lpad = row_pitch(Lx);
         int rt = std::max(1, kMaxThreads / ax.fp.P);
         while (rt > 1 && (size_t)rt * ax.lpad * sizeof(float2) > 65536) rt--;
         ax.tile = rt;
@@ -239,6 +276,7 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
 void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a) {
     a.n1 = (int)vp.n1;
     a.h1 = (int)vp.h1;
+    a.hp = (int)vp.hp;
     a.packed = vp.packed;
     a.RT = vp.x.tile;
     a.lpad = vp.x.lpad;
@@ -246,7 +284,25 @@ void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a) {
     a.rtw = vp.rtw;
     a.eps = kEps;
     const int64_t nblk = (a.nrows + a.RT - 1) / a.RT;
+    hipEvent_t e0 = prof_mark();
     PK_HIP(launch_xpass(vp.x.fp, dim3((unsigned)nblk), vp.x.block, vp.x.lds, stream, a));
+    if (prof_on) {
+        // algorithmic bytes: each operand read or written once
+        const double N = (double)vp.N, S = (double)vp.Sa;
+        const char* name = "x_pass";
+        double b = 0;
+        switch (a.mode) {
+            case XM_R2C: name = "x_r2c"; b = 4 * N + 8 * S; break;
+            case XM_C2R: name = "x_c2r"; b = 8 * S + 4 * N; break;
+            case XM_PREP_U16: name = "x_prep"; b = 2 * N + 8 * N + 8 * S; break;
+            case XM_PREP_F32: name = "x_prep"; b = 4 * N + 8 * N + 8 * S; break;
+            case XM_RATIO: name = "x_ratio"; b = 16 * S + 4 * N; break;
+            case XM_OMW_UPDATE: name = a.r2c_out ? "x_update_r2c" : "x_update"; b = 8 * S + 8 * N + (a.r2c_out ? 8 * S : 0); break;
+            case XM_BIGGS_Y: name = "x_biggs_y"; b = 12 * N + 8 * S; break;
+            case XM_BIGGS_UPDATE: name = "x_biggs_update"; b = 8 * S + 12 * N + (a.acc ? 4 * N : 0) + (a.r2c_out ? 8 * S : 0); break;
+        }
+        prof_done(name, b, e0);
+    }
 }
 
 static int64_t xpass_blocks(const VolPlan& vp) { return (vp.n2 * vp.n3 + vp.x.tile - 1) / vp.x.tile; }
@@ -263,23 +319,31 @@ void pk_ctx_impl::run_col(const VolPlan& vp, int axis, float2* data, int mode, c
     dim3 grid;
     if (axis == 2) {
         a.acount = vp.h1;
-        a.bstride = vp.h1 * vp.n2;
-        a.stride = vp.h1;
+        a.bstride = vp.hp * vp.n2;
+        a.stride = vp.hp;
         grid = dim3((unsigned)((vp.h1 + a.T - 1) / a.T), (unsigned)vp.n3);
     } else {
-        a.acount = vp.h1 * vp.n2;
+        a.acount = vp.hp * vp.n2;
+        a.amod = vp.hp;
+        a.alim = vp.h1;
         a.bstride = 0;
-        a.stride = vp.h1 * vp.n2;
+        a.stride = vp.hp * vp.n2;
         grid = dim3((unsigned)((a.acount + a.T - 1) / a.T), 1);
     }
+    hipEvent_t e0 = prof_mark();
     PK_HIP(launch_colpass(ax.fp, grid, ax.block, ax.lds, stream, a));
+    if (prof_on) {
+        const double S = (double)vp.Sa;
+        const char* name = axis == 2 ? (mode == CM_INV ? "y_inv" : mode == CM_FWD ? "y_fwd" : "y_conv")
+                                     : (mode == CM_INV ? "z_inv" : mode == CM_FWD ? "z_fwd" : "z_conv");
+        prof_done(name, mode >= CM_CONV ? 24 * S : 16 * S, e0);
+    }
 }
 
 void pk_ctx_impl::r2c3d(const VolPlan& vp, const float* real, float2* spec) {
     XArgs a{};
     a.mode = XM_R2C;
-    a.scale = 1.f;
-    a.src = real;
+        a.src = real;
     a.spec = spec;
     run_x(vp, a);
     run_col(vp, 2, spec, CM_FWD, nullptr, 1.f);
@@ -291,8 +355,7 @@ void pk_ctx_impl::c2r3d(const VolPlan& vp, float2* spec, float* real) {
     run_col(vp, 2, spec, CM_INV, nullptr, 1.f);
     XArgs a{};
     a.mode = XM_C2R;
-    a.scale = 1.f;
-    a.spec = spec;
+        a.spec = spec;
     a.dst = real;
     run_x(vp, a);
 }
@@ -307,20 +370,40 @@ void pk_ctx_impl::spectral_conv(const VolPlan& vp, float2* A, const float2* H, b
 static int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 65536); }
 
 // ------------------------------------------------------------------ OTF cache
-static uint64_t fnv1a(const void* p, size_t n, uint64_t h = 1469598103934665603ull) {
-    const unsigned char* c = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n; ++i) {
-        h ^= c[i];
-        h *= 1099511628211ull;
-    }
+// 64-bit content hash of a PSF (cache key): four independent multiply-xorshift
+// lanes over 8-byte words, ~1 ns per 32 bytes, then a final avalanche.
+static inline uint64_t mix64(uint64_t h) {
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
     return h;
+}
+static uint64_t content_hash(const void* p, size_t n, uint64_t seed) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    uint64_t h[4] = {seed ^ 0x9e3779b97f4a7c15ull, seed + 0x632be59bd9b4e019ull, seed ^ 0x85ebca77c2b2ae63ull,
+                     seed + 0x27d4eb2f165667c5ull};
+    size_t i = 0;
+    for (; i + 32 <= n; i += 32) {
+        uint64_t w[4];
+        std::memcpy(w, c + i, 32);
+        for (int l = 0; l < 4; ++l) {
+            h[l] = (h[l] ^ w[l]) * 0x9e3779b97f4a7c15ull;
+            h[l] ^= h[l] >> 29;
+        }
+    }
+    uint64_t t = 0;
+    for (; i < n; ++i) t = (t << 8) ^ c[i] ^ (t >> 56);
+    return mix64(h[0] ^ mix64(h[1] ^ mix64(h[2] ^ mix64(h[3] ^ t ^ n))));
 }
 
 // OTF of PSF ./ sum(PSF(:)) at the volume size (decon_lucy_function.m:76,
 // decon_lucy_omw_function.m:286-287), half spectrum, cached on the device.
 const float2* pk_ctx_impl::otf_for(const VolPlan& vp, const float* psf, const int64_t pd[3]) {
     const int64_t P = pd[0] * pd[1] * pd[2];
-    const uint64_t h = fnv1a(psf, sizeof(float) * (size_t)P, fnv1a(pd, sizeof(int64_t) * 3));
+    const uint64_t h = content_hash(psf, sizeof(float) * (size_t)P,
+                                    content_hash(pd, sizeof(int64_t) * 3, 0x5eed));
     for (auto& e : otf) {
         if (e.dims[0] == vp.n1 && e.dims[1] == vp.n2 && e.dims[2] == vp.n3 && e.hash == h &&
             e.pdims[0] == pd[0] && e.pdims[1] == pd[1] && e.pdims[2] == pd[2]) {
@@ -375,10 +458,14 @@ void pk_ctx_impl::build_otf(const VolPlan& vp, const float* q, const int64_t pd[
     }
     r2c3d(vp, real, dst);
     PK_HIP(hipStreamSynchronize(stream));  // dq is freed on return
+    prof_flush();
 }
 
 void pk_ctx_impl::reset() {
     PK_HIP(hipStreamSynchronize(stream));
+    prof_flush();
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    ev_pool.clear();
     otf.clear();
     plans.clear();
     tw.clear();
@@ -459,8 +546,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
     // saturates at 0; single I gives max(I - Background, 0)
     {
         XArgs a{};
-        a.scale = 1.f;
-        a.spec = A;
+                a.spec = A;
         a.I = I;
         a.J = J;
         if (raw16) {
@@ -480,14 +566,12 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
             spectral_conv(vp, A, Hf, false);
             XArgs r{};
             r.mode = XM_RATIO;
-            r.scale = 1.f;
             r.spec = A;
             r.I = I;
             run_x(vp, r);
             spectral_conv(vp, A, Hb, false);
             XArgs u{};
             u.mode = XM_OMW_UPDATE;
-            u.scale = 1.f;
             u.spec = A;
             u.J = J;
             u.r2c_out = k < numit;
@@ -510,7 +594,6 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
             }
             XArgs y{};
             y.mode = XM_BIGGS_Y;
-            y.scale = 1.f;
             y.spec = A;
             y.J = J2;
             y.J3 = J3;
@@ -520,19 +603,18 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
             spectral_conv(vp, A, Hf, false);
             XArgs r{};
             r.mode = XM_RATIO;
-            r.scale = 1.f;
             r.spec = A;
             r.I = I;
             run_x(vp, r);
             spectral_conv(vp, A, Hf, true);
             XArgs u{};
             u.mode = XM_BIGGS_UPDATE;
-            u.scale = 1.f;
             u.spec = A;
             u.Y = Y;
             u.Jn = J3;  // x_{k-2} is dead once Y is formed
             u.J4 = J4;
             u.acc = (k >= 2 && k < numit);
+            u.write_j4 = (k < numit);
             u.partials = part;
             u.r2c_out = 0;
             run_x(vp, u);
@@ -595,11 +677,14 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
     e.apply_scale = (o->scale_factor != 1 || o->decon_offset != 0 || o->edge_erosion > 0);
     e.use_mask = need_mask;
     if (!(o->damp_factor > 1)) e.damp = 0.f;
+    hipEvent_t ee = prof_mark();
     hipLaunchKernelGGL(k_epilogue, dim3(grid_for(on)), dim3(256), 0, stream, e);
     PK_HIP(hipGetLastError());
+    prof_done("epilogue", (double)on * (4.0 + (e.damp > 1.f ? 4.0 : 0.0) + (double)osz), ee);
     if (out->mem == PK_MEM_HOST)
         PK_HIP(hipMemcpyAsync(out->data, dout, osz * on, hipMemcpyDeviceToHost, stream));
     PK_HIP(hipStreamSynchronize(stream));
+    prof_flush();
     if (err_mat) std::memset(err_mat, 0, sizeof(double) * 4 * (size_t)numit);
 }
 

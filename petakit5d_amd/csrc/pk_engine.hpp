@@ -52,7 +52,9 @@ struct AxisLaunch {
 };
 
 struct VolPlan {
-    int64_t n1 = 0, n2 = 0, n3 = 0, h1 = 0, N = 0, S = 0;
+    // S: float2 elements of a (pitched) spectrum buffer; Sa = h1*n2*n3, the
+    // algorithmic half-spectrum size used for byte accounting
+    int64_t n1 = 0, n2 = 0, n3 = 0, h1 = 0, hp = 0, N = 0, S = 0, Sa = 0;
     int packed = 0;
     AxisLaunch x, y, z;
     const float2* rtw = nullptr;  // W_{n1}^k, k in [0, n1/2]
@@ -70,6 +72,18 @@ struct Workspace {
     DevBuf I, J, A, J3, Y, J4, partials, lambda, raw, out, mask, mask2, count, tmp;
 };
 
+// Per-pass timing with HIP events recorded on the engine's stream.
+struct ProfAgg {
+    int64_t launches = 0;
+    double total_ms = 0;
+    double bytes = 0;  // algorithmic bytes per launch
+};
+struct ProfPending {
+    std::string name;
+    double bytes;
+    hipEvent_t e0, e1;
+};
+
 struct pk_ctx_impl {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -85,6 +99,14 @@ struct pk_ctx_impl {
     uint64_t tick = 0;
     size_t otf_max_entries = 8;
     Workspace ws;
+    bool prof_on = false;
+    std::vector<ProfPending> prof_pending;
+    std::vector<hipEvent_t> ev_pool;
+    std::map<std::string, ProfAgg> prof;
+
+    hipEvent_t prof_mark();                       // records an event (profiling on) or returns null
+    void prof_done(const char* name, double bytes, hipEvent_t e0);
+    void prof_flush();                            // after a stream sync: fold event pairs into prof
 
     // device memory helpers
     void alloc(DevBuf& b, size_t bytes);

@@ -2,7 +2,10 @@
 //
 // Volumes are MATLAB column-major: dim 1 (n1) contiguous.  Real volumes hold
 // N = n1*n2*n3 floats; spectra hold the half spectrum along dim 1,
-// h1 = floor(n1/2)+1 complex values per row, S = h1*n2*n3 float2.
+// h1 = floor(n1/2)+1 complex values per row stored at a row pitch hp (h1
+// rounded up to 16 float2 = 128 bytes, so every row and every 16-column tile
+// of the strided passes starts on a cache-line boundary); the pad columns
+// are never read or written.
 //
 // One 1-D transform of length L is executed by P threads that each hold E
 // values in registers (L = P*E).  Thread p always owns the "home" positions
@@ -36,8 +39,19 @@ struct FftPlan {
     const float2* tw;           // W_L^e = exp(-2 pi i e / L), e in [0, L)
 };
 
+// LDS slot of position pos.  Column tiles (ROWS=false) interleave T columns:
+// base + pos*T.  Row tiles (ROWS=true) keep a row contiguous with one pad
+// slot after every 16 positions, so the stride-R writes of the first stage
+// (lane stride R*8 bytes) and the home-position reads spread over all banks.
+template <bool ROWS>
+__device__ __forceinline__ int lds_slot(int lbase, int pos, int lstride) {
+    if constexpr (ROWS) return lbase + pos + (pos >> 4);
+    else return lbase + pos * lstride;
+}
+__host__ __device__ constexpr int row_pitch(int L) { return L + (L >> 4) + 1; }
+
 // ------------------------------------------------------------------ one stage
-template <int E, int R, bool INV>
+template <int E, int R, bool INV, bool ROWS>
 __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s, int p, float2* lds, int lbase,
                                         int lstride, bool last) {
     constexpr int Q = E / R;  // butterflies per thread
@@ -78,7 +92,7 @@ __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s
             const int ob = g * Ns * R + k;
             static_for<0, R>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                lds[lbase + (ob + q * Ns) * lstride] = u[q];
+                lds[lds_slot<ROWS>(lbase, ob + q * Ns, lstride)] = u[q];
             });
         }
     }
@@ -86,12 +100,12 @@ __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s
 
 #define PK_RADIX_CASE(RR)                                                    \
     case RR:                                                                 \
-        if constexpr (E % RR == 0) stage_r<E, RR, INV>(v, pl, s, p, lds, lbase, lstride, last); \
+        if constexpr (E % RR == 0) stage_r<E, RR, INV, ROWS>(v, pl, s, p, lds, lbase, lstride, last); \
         break;
 
 // Full transform of the E home values of every thread (all threads of the
 // block must call it: it contains barriers).
-template <int E, bool INV>
+template <int E, bool INV, bool ROWS>
 __device__ __forceinline__ void fft_run(float2 (&v)[E], const FftPlan& pl, int p, float2* lds, int lbase,
                                         int lstride) {
     for (int s = 0; s < pl.nst; ++s) {
@@ -121,7 +135,7 @@ __device__ __forceinline__ void fft_run(float2 (&v)[E], const FftPlan& pl, int p
         if (!last) {
             __syncthreads();
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = lds[lbase + (p + e * pl.P) * lstride];
+            for (int e = 0; e < E; ++e) v[e] = lds[lds_slot<ROWS>(lbase, p + e * pl.P, lstride)];
             __syncthreads();
         }
     }
@@ -134,23 +148,24 @@ enum XMode : int {
     XM_C2R = 1,          // spectrum -> real dst (unnormalised)
     XM_PREP_U16 = 2,     // raw uint16: I = sat(raw - bg16); I -> dst_I, J; spectrum of I
     XM_PREP_F32 = 3,     // raw float: I = max(raw - bg, 0); I -> dst_I, J; spectrum of I
-    XM_RATIO = 4,        // spectrum -> c; r = I / max(c*scale, eps); spectrum of r
-    XM_OMW_UPDATE = 5,   // spectrum -> t; J = max(t*scale * J, 0); [spectrum of J]
+    XM_RATIO = 4,        // spectrum -> c; r = I / max(c, eps); spectrum of r
+    XM_OMW_UPDATE = 5,   // spectrum -> t; J = max(t * J, 0); [spectrum of J]
     XM_BIGGS_Y = 6,      // Y = max(J2 + lambda (J2 - J3), 0) -> Y; spectrum of Y
-    XM_BIGGS_UPDATE = 7, // spectrum -> t; Jn = max(Y * t*scale, 0) -> Jn; J4 = Jn - Y; dot partials
+    XM_BIGGS_UPDATE = 7, // spectrum -> t; Jn = max(Y * t, 0) -> Jn; J4 = Jn - Y; dot partials
 };
 
 struct XArgs {
     int n1;            // real row length
     int h1;            // n1/2 + 1
+    int hp;            // spectrum row pitch (float2)
     int packed;        // n1 even: transform length n1/2 over (x[2m], x[2m+1])
     int RT;            // rows per block
     int lpad;          // LDS row pitch (float2)
     int mode;
     int r2c_out;       // modes 5/7: also produce the spectrum of the new estimate
     int acc;           // mode 7: accumulate the two dot products
+    int write_j4;      // mode 7: store J4 = Jn - Y
     int64_t nrows;     // n2 * n3
-    float scale;       // 1/N folded into the inverse transform
     float bg;          // background (float) / bg16 for uint16
     float eps;         // MATLAB eps as single
     float2* spec;      // half spectrum (in and/or out)
@@ -170,66 +185,33 @@ struct XArgs {
 };
 
 __device__ __forceinline__ float fmax0(float a) { return a > 0.f ? a : 0.f; }  // max(a,0), NaN -> 0
+__device__ __forceinline__ float2 fmax0(float2 a) { return make_float2(fmax0(a.x), fmax0(a.y)); }
 
-// Elementwise step in real space on one value x at real index gi.
-// Returns the value that feeds the forward transform (if any).
-__device__ __forceinline__ float x_elem(const XArgs& a, int64_t gi, float x, double& s_num, double& s_den) {
-    switch (a.mode) {
-        case XM_R2C:
-            return a.src[gi];
-        case XM_C2R:
-            a.dst[gi] = x;
-            return 0.f;
-        case XM_PREP_U16: {
-            const float r = a.raw16[gi];
-            const float v = fmax0(r - a.bg);  // uint16 saturating subtract of an integer
-            a.I[gi] = v;
-            a.J[gi] = v;
-            return v;
-        }
-        case XM_PREP_F32: {
-            const float v = fmax0(a.rawf[gi] - a.bg);
-            a.I[gi] = v;
-            a.J[gi] = v;
-            return v;
-        }
-        case XM_RATIO: {
-            const float c = x * a.scale;
-            const float cm = c > a.eps ? c : a.eps;  // max(CX, eps)
-            return a.I[gi] / cm;
-        }
-        case XM_OMW_UPDATE: {
-            const float v = fmax0((x * a.scale) * a.J[gi]);
-            a.J[gi] = v;
-            return v;
-        }
-        case XM_BIGGS_Y: {
-            const float lam = *a.lambda;
-            const float j2 = a.J[gi];
-            const float y = fmax0(j2 + lam * (j2 - a.J3[gi]));
-            a.Y[gi] = y;
-            return y;
-        }
-        case XM_BIGGS_UPDATE: {
-            const float y = a.Y[gi];
-            const float v = fmax0(y * (x * a.scale));
-            a.Jn[gi] = v;
-            const float d = v - y;
-            if (a.acc) {
-                const float j4 = a.J4[gi];
-                s_num += (double)(d * j4);
-                s_den += (double)(j4 * j4);
-            }
-            a.J4[gi] = d;
-            return v;
-        }
+// Real-space access of element e of a row: packed rows pair (x[2m], x[2m+1])
+// into one float2 (8-byte vector access); odd rows carry x[m] in .x.
+__device__ __forceinline__ float2 ld_real(const float* __restrict__ base, int64_t xrow, int m, bool packed) {
+    if (packed) return reinterpret_cast<const float2*>(base)[(xrow >> 1) + m];
+    return make_float2(base[xrow + m], 0.f);
+}
+__device__ __forceinline__ void st_real(float* __restrict__ base, int64_t xrow, int m, bool packed, float2 v) {
+    if (packed) reinterpret_cast<float2*>(base)[(xrow >> 1) + m] = v;
+    else base[xrow + m] = v.x;
+}
+__device__ __forceinline__ float2 ld_u16(const uint16_t* __restrict__ base, int64_t xrow, int m, bool packed) {
+    if (packed) {
+        const uint32_t w = reinterpret_cast<const uint32_t*>(base)[(xrow >> 1) + m];
+        return make_float2((float)(w & 0xffffu), (float)(w >> 16));
     }
-    return 0.f;
+    return make_float2((float)base[xrow + m], 0.f);
 }
 
-template <int E>
-__global__ void __launch_bounds__(kMaxThreads) k_xpass(XArgs a, FftPlan pl) {
-    extern __shared__ float2 lds[];
+// One x pass, specialised on its mode: [C2R] -> elementwise -> [R2C].
+template <int E, int MODE>
+__device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, float2* lds) {
+    constexpr bool IN = (MODE == XM_C2R || MODE == XM_RATIO || MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
+    constexpr bool OUT_ALWAYS = (MODE == XM_R2C || MODE == XM_PREP_U16 || MODE == XM_PREP_F32 ||
+                                 MODE == XM_RATIO || MODE == XM_BIGGS_Y);
+    constexpr bool OUT_MAYBE = (MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
     const int t = threadIdx.x;
     const int P = pl.P;
     const int rr = t / P;
@@ -238,19 +220,37 @@ __global__ void __launch_bounds__(kMaxThreads) k_xpass(XArgs a, FftPlan pl) {
     const bool valid = (rr < a.RT) && (row < a.nrows);
     const int lbase = rr * a.lpad;
     const int L = pl.L;
-    const bool in_spec = (a.mode == XM_C2R || a.mode == XM_RATIO || a.mode == XM_OMW_UPDATE ||
-                          a.mode == XM_BIGGS_UPDATE);
-    const bool out_spec = (a.mode == XM_R2C || a.mode == XM_PREP_U16 || a.mode == XM_PREP_F32 ||
-                           a.mode == XM_RATIO || a.mode == XM_BIGGS_Y ||
-                           ((a.mode == XM_OMW_UPDATE || a.mode == XM_BIGGS_UPDATE) && a.r2c_out));
-    const int64_t srow = row * a.h1;      // spectrum row start
-    const int64_t xrow = row * a.n1;      // real row start
-    float2 v[E];
-    double s_num = 0.0, s_den = 0.0;
+    const bool packed = a.packed != 0;
+    const int64_t srow = row * a.hp;  // spectrum row start
+    const int64_t xrow = row * a.n1;  // real row start
 
-    // ---- inverse transform along the row (spectrum -> real)
-    if (in_spec) {
-        if (a.packed) {
+    // ---- prefetch the real-space operands so their latency overlaps the C2R
+    float2 opA[E], opB[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int m = p + e * P;
+        opA[e] = make_float2(0.f, 0.f);
+        opB[e] = make_float2(0.f, 0.f);
+        if (!valid) continue;
+        if constexpr (MODE == XM_R2C) opA[e] = ld_real(a.src, xrow, m, packed);
+        if constexpr (MODE == XM_PREP_U16) opA[e] = ld_u16(a.raw16, xrow, m, packed);
+        if constexpr (MODE == XM_PREP_F32) opA[e] = ld_real(a.rawf, xrow, m, packed);
+        if constexpr (MODE == XM_RATIO) opA[e] = ld_real(a.I, xrow, m, packed);
+        if constexpr (MODE == XM_OMW_UPDATE) opA[e] = ld_real(a.J, xrow, m, packed);
+        if constexpr (MODE == XM_BIGGS_Y) {
+            opA[e] = ld_real(a.J, xrow, m, packed);
+            opB[e] = ld_real(a.J3, xrow, m, packed);
+        }
+        if constexpr (MODE == XM_BIGGS_UPDATE) {
+            opA[e] = ld_real(a.Y, xrow, m, packed);
+            if (a.acc) opB[e] = ld_real(a.J4, xrow, m, packed);
+        }
+    }
+
+    // ---- inverse transform along the row (half spectrum -> real)
+    float2 v[E];
+    if constexpr (IN) {
+        if (packed) {
             const int M = L;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -274,46 +274,75 @@ __global__ void __launch_bounds__(kMaxThreads) k_xpass(XArgs a, FftPlan pl) {
                 v[e] = z;
             }
         }
-        fft_run<E, true>(v, pl, p, lds, lbase, 1);
+        fft_run<E, true, true>(v, pl, p, lds, lbase, 1);
     }
 
-    // ---- elementwise in real space
-    if (a.packed) {
+    // ---- elementwise step in real space
+    double s_num = 0.0, s_den = 0.0;
+    float lam = 0.f;
+    if constexpr (MODE == XM_BIGGS_Y) lam = *a.lambda;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int m = p + e * P;
-            float2 o = make_float2(0.f, 0.f);
-            if (valid) {
-                const int64_t gi = xrow + 2 * m;
-                o.x = x_elem(a, gi, v[e].x, s_num, s_den);
-                o.y = x_elem(a, gi + 1, v[e].y, s_num, s_den);
+    for (int e = 0; e < E; ++e) {
+        const int m = p + e * P;
+        float2 o = make_float2(0.f, 0.f);
+        if (valid) {
+            if constexpr (MODE == XM_R2C) {
+                o = opA[e];
+            } else if constexpr (MODE == XM_C2R) {
+                st_real(a.dst, xrow, m, packed, v[e]);
+            } else if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
+                // uint16: I - uint16(Background) saturates at 0; single: max(I - Background, 0)
+                o = fmax0(make_float2(opA[e].x - a.bg, opA[e].y - a.bg));
+                st_real(a.I, xrow, m, packed, o);
+                st_real(a.J, xrow, m, packed, o);
+            } else if constexpr (MODE == XM_RATIO) {
+                const float2 c = v[e];
+                o.x = opA[e].x / (c.x > a.eps ? c.x : a.eps);  // I ./ max(CX, eps)
+                o.y = opA[e].y / (c.y > a.eps ? c.y : a.eps);
+            } else if constexpr (MODE == XM_OMW_UPDATE) {
+                o = fmax0(make_float2(v[e].x * opA[e].x, v[e].y * opA[e].y));
+                st_real(a.J, xrow, m, packed, o);
+            } else if constexpr (MODE == XM_BIGGS_Y) {
+                const float2 j2 = opA[e], j3 = opB[e];
+                o = fmax0(make_float2(j2.x + lam * (j2.x - j3.x), j2.y + lam * (j2.y - j3.y)));
+                st_real(a.Y, xrow, m, packed, o);
+            } else if constexpr (MODE == XM_BIGGS_UPDATE) {
+                const float2 y = opA[e];
+                o = fmax0(make_float2(y.x * v[e].x, y.y * v[e].y));
+                st_real(a.Jn, xrow, m, packed, o);
+                const float2 d = make_float2(o.x - y.x, o.y - y.y);
+                if (a.acc) {
+                    const float2 j4 = opB[e];
+                    s_num += (double)(d.x * j4.x);
+                    s_den += (double)(j4.x * j4.x);
+                    if (packed) {
+                        s_num += (double)(d.y * j4.y);
+                        s_den += (double)(j4.y * j4.y);
+                    }
+                }
+                if (a.write_j4) st_real(a.J4, xrow, m, packed, d);
             }
-            v[e] = o;
+            if (!packed) o.y = 0.f;  // odd rows: the forward transform input is real
         }
-    } else {
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int m = p + e * P;
-            float2 o = make_float2(0.f, 0.f);
-            if (valid) o.x = x_elem(a, xrow + m, v[e].x, s_num, s_den);
-            v[e] = o;
-        }
+        v[e] = o;
     }
 
     // ---- forward transform along the row (real -> half spectrum)
+    bool out_spec = OUT_ALWAYS;
+    if constexpr (OUT_MAYBE) out_spec = a.r2c_out != 0;
     if (out_spec) {
-        fft_run<E, false>(v, pl, p, lds, lbase, 1);
-        if (a.packed) {
+        fft_run<E, false, true>(v, pl, p, lds, lbase, 1);
+        if (packed) {
             const int M = L;
             // Z at home positions -> LDS, then X[k] = 1/2[(Z_k + Z*_{M-k}) - i W^k (Z_k - Z*_{M-k})]
 #pragma unroll
-            for (int e = 0; e < E; ++e) lds[lbase + p + e * P] = v[e];
+            for (int e = 0; e < E; ++e) lds[lds_slot<true>(lbase, p + e * P, 1)] = v[e];
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = p + e * P;
-                const float2 A = lds[lbase + k];
-                const float2 B = cconj(lds[lbase + (k == 0 ? 0 : M - k)]);
+                const float2 A = lds[lds_slot<true>(lbase, k, 1)];
+                const float2 B = cconj(lds[lds_slot<true>(lbase, k == 0 ? 0 : M - k, 1)]);
                 const float2 s = cadd(A, B);
                 const float2 d = cmul(csub(A, B), a.rtw[k]);
                 // -i * d = (d.y, -d.x)
@@ -334,25 +363,34 @@ __global__ void __launch_bounds__(kMaxThreads) k_xpass(XArgs a, FftPlan pl) {
         }
     }
 
-    // ---- block partial sums for the Biggs step length (fixed order)
-    if (a.mode == XM_BIGGS_UPDATE && a.acc) {
-        __syncthreads();
-        double* red = reinterpret_cast<double*>(lds);
-        red[2 * t] = s_num;
-        red[2 * t + 1] = s_den;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (t < w && t + w < (int)blockDim.x) {
-                red[2 * t] += red[2 * (t + w)];
-                red[2 * t + 1] += red[2 * (t + w) + 1];
-            }
+    // ---- block partial sums of the Biggs step length (fixed order)
+    if constexpr (MODE == XM_BIGGS_UPDATE) {
+        if (a.acc) {
             __syncthreads();
-        }
-        if (t == 0) {
-            a.partials[2 * blockIdx.x] = red[0];
-            a.partials[2 * blockIdx.x + 1] = red[1];
+            double* red = reinterpret_cast<double*>(lds);
+            red[2 * t] = s_num;
+            red[2 * t + 1] = s_den;
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if (t < w && t + w < (int)blockDim.x) {
+                    red[2 * t] += red[2 * (t + w)];
+                    red[2 * t + 1] += red[2 * (t + w) + 1];
+                }
+                __syncthreads();
+            }
+            if (t == 0) {
+                a.partials[2 * blockIdx.x] = red[0];
+                a.partials[2 * blockIdx.x + 1] = red[1];
+            }
         }
     }
+}
+
+// One kernel per (E, mode) so each mode gets its own register allocation.
+template <int E, int MODE>
+__global__ void __launch_bounds__(kMaxThreads) k_xpass(XArgs a, FftPlan pl) {
+    extern __shared__ float2 lds[];
+    xpass_body<E, MODE>(a, pl, lds);
 }
 
 // ------------------------------------------------------------------ strided axes
@@ -362,6 +400,8 @@ struct ColArgs {
     float2* data;
     const float2* otf;   // same layout as data (CM_CONV / CM_CONVJ)
     int64_t acount;      // columns along grid.x (contiguous index)
+    int64_t amod;        // >0: column c is valid only if c % amod < alim (pad columns)
+    int64_t alim;
     int64_t bstride;     // offset between grid.y slices
     int64_t stride;      // element stride along the transform
     int T;               // columns per block
@@ -369,16 +409,15 @@ struct ColArgs {
     float scale;         // applied with the OTF product
 };
 
-template <int E>
-__global__ void __launch_bounds__(kMaxThreads) k_colpass(ColArgs a, FftPlan pl) {
-    extern __shared__ float2 lds[];
+template <int E, int MODE>
+__device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl, float2* lds) {
     const int t = threadIdx.x;
     const int T = a.T;
     const int p = t / T;
     const int c = t - p * T;
     const int P = pl.P;
     const int64_t col = (int64_t)blockIdx.x * T + c;
-    const bool valid = (p < P) && (col < a.acount);
+    const bool valid = (p < P) && (col < a.acount) && (a.amod == 0 || col % a.amod < a.alim);
     // per-lane base pointer plus a block-uniform step: addresses are
     // recomputed from (ptr0, e*step) instead of held live between load and store
     const int64_t step = (int64_t)P * a.stride;
@@ -387,25 +426,34 @@ __global__ void __launch_bounds__(kMaxThreads) k_colpass(ColArgs a, FftPlan pl) 
     float2 v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? ptr0[e * step] : make_float2(0.f, 0.f);
-    if (a.mode == CM_INV) {
-        fft_run<E, true>(v, pl, p, lds, c, T);
+    if constexpr (MODE == CM_INV) {
+        fft_run<E, true, false>(v, pl, p, lds, c, T);
+    } else if constexpr (MODE == CM_FWD) {
+        fft_run<E, false, false>(v, pl, p, lds, c, T);
     } else {
-        fft_run<E, false>(v, pl, p, lds, c, T);
-        if (a.mode >= CM_CONV) {
-            const float2* __restrict__ h0 = a.otf + (valid ? off0 : 0);
+        // OTF values prefetched before the forward transform
+        const float2* __restrict__ h0 = a.otf + (valid ? off0 : 0);
+        float2 h[E];
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                float2 h = valid ? h0[e * step] : make_float2(0.f, 0.f);
-                h = make_float2(h.x * a.scale, h.y * a.scale);
-                v[e] = (a.mode == CM_CONVJ) ? cmulc(v[e], h) : cmul(v[e], h);
-            }
-            fft_run<E, true>(v, pl, p, lds, c, T);
+        for (int e = 0; e < E; ++e) h[e] = valid ? h0[e * step] : make_float2(0.f, 0.f);
+        fft_run<E, false, false>(v, pl, p, lds, c, T);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const float2 hs = make_float2(h[e].x * a.scale, h[e].y * a.scale);
+            v[e] = (MODE == CM_CONVJ) ? cmulc(v[e], hs) : cmul(v[e], hs);
         }
+        fft_run<E, true, false>(v, pl, p, lds, c, T);
     }
     if (valid) {
 #pragma unroll
         for (int e = 0; e < E; ++e) ptr0[e * step] = v[e];
     }
+}
+
+template <int E, int MODE>
+__global__ void __launch_bounds__(kMaxThreads) k_colpass(ColArgs a, FftPlan pl) {
+    extern __shared__ float2 lds[];
+    colpass_body<E, MODE>(a, pl, lds);
 }
 
 }  // namespace pk

@@ -1,0 +1,61 @@
+// pk_launch.hip -- dispatch of a transform plan to the kernel instantiation
+// compiled for its values-per-thread count.
+#include "pk_launch.hpp"
+
+namespace pk {
+bool launch_x_part_a(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_x_part_b(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_x_part_c(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_x_part_d(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_x_part_e(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_x_part_f(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_col_part_a(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
+bool launch_col_part_b(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
+bool launch_col_part_c(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
+
+hipError_t launch_xpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a) {
+    const bool ok = launch_x_part_a(pl, grid, block, lds, s, a) ||
+           launch_x_part_b(pl, grid, block, lds, s, a) ||
+           launch_x_part_c(pl, grid, block, lds, s, a) ||
+           launch_x_part_d(pl, grid, block, lds, s, a) ||
+           launch_x_part_e(pl, grid, block, lds, s, a) ||
+           launch_x_part_f(pl, grid, block, lds, s, a);
+    if (!ok) return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_colpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a) {
+    const bool ok = launch_col_part_a(pl, grid, block, lds, s, a) ||
+           launch_col_part_b(pl, grid, block, lds, s, a) ||
+           launch_col_part_c(pl, grid, block, lds, s, a);
+    if (!ok) return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+bool e_supported(int E) {
+    switch (E) {
+        case 1:
+        case 2:
+        case 3:
+        case 4:
+        case 5:
+        case 6:
+        case 8:
+        case 9:
+        case 10:
+        case 12:
+        case 15:
+        case 16:
+        case 18:
+        case 20:
+        case 24:
+        case 25:
+        case 27:
+        case 30:
+        case 32:
+            return true;
+        default:
+            return false;
+    }
+}
+}  // namespace pk

@@ -4,11 +4,37 @@
 
 #include "pk_fft_dev.hpp"
 
-// Values-per-thread counts compiled into the transform kernels.
+// Values-per-thread counts compiled into the transform kernels (the split
+// across pk_kern_{x,col}_*.hip must cover the same list).
 #define PK_E_LIST(X) \
     X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(9) X(10) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32)
 
 namespace pk {
+template <int E>
+inline void launch_modes_x(dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a, const FftPlan& pl) {
+    switch (a.mode) {
+        case XM_R2C: hipLaunchKernelGGL((k_xpass<E, XM_R2C>), grid, block, lds, s, a, pl); break;
+        case XM_C2R: hipLaunchKernelGGL((k_xpass<E, XM_C2R>), grid, block, lds, s, a, pl); break;
+        case XM_PREP_U16: hipLaunchKernelGGL((k_xpass<E, XM_PREP_U16>), grid, block, lds, s, a, pl); break;
+        case XM_PREP_F32: hipLaunchKernelGGL((k_xpass<E, XM_PREP_F32>), grid, block, lds, s, a, pl); break;
+        case XM_RATIO: hipLaunchKernelGGL((k_xpass<E, XM_RATIO>), grid, block, lds, s, a, pl); break;
+        case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xpass<E, XM_OMW_UPDATE>), grid, block, lds, s, a, pl); break;
+        case XM_BIGGS_Y: hipLaunchKernelGGL((k_xpass<E, XM_BIGGS_Y>), grid, block, lds, s, a, pl); break;
+        case XM_BIGGS_UPDATE: hipLaunchKernelGGL((k_xpass<E, XM_BIGGS_UPDATE>), grid, block, lds, s, a, pl); break;
+        default: break;
+    }
+}
+template <int E>
+inline void launch_modes_col(dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a, const FftPlan& pl) {
+    switch (a.mode) {
+        case CM_FWD: hipLaunchKernelGGL((k_colpass<E, CM_FWD>), grid, block, lds, s, a, pl); break;
+        case CM_INV: hipLaunchKernelGGL((k_colpass<E, CM_INV>), grid, block, lds, s, a, pl); break;
+        case CM_CONV: hipLaunchKernelGGL((k_colpass<E, CM_CONV>), grid, block, lds, s, a, pl); break;
+        case CM_CONVJ: hipLaunchKernelGGL((k_colpass<E, CM_CONVJ>), grid, block, lds, s, a, pl); break;
+        default: break;
+    }
+}
+
 hipError_t launch_xpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
 hipError_t launch_colpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
 bool e_supported(int E);

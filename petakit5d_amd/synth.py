@@ -43,15 +43,21 @@ def gaussian_psf(size, sigma_xy: float, sigma_z: float, shear_deg: float = 0.0) 
 def _fft_convolve_circular(vol: np.ndarray, psf: np.ndarray) -> np.ndarray:
     """Circular convolution of vol with psf centred at floor(p/2) (float64)."""
     d = vol.shape
-    pad = np.zeros(d, dtype=np.float64)
-    p = psf.astype(np.float64)
-    pad[: p.shape[0], : p.shape[1], : p.shape[2]] = p
-    pad = np.roll(pad, shift=tuple(-(s // 2) for s in p.shape), axis=(0, 1, 2))
+    pad = _wrapped_kernel(psf, d)
     if _sfft is not None:
         F = _sfft.rfftn(vol, workers=-1) * _sfft.rfftn(pad, workers=-1)
         return _sfft.irfftn(F, s=d, workers=-1)
     F = np.fft.rfftn(vol) * np.fft.rfftn(pad)
     return np.fft.irfftn(F, s=d)
+
+
+def _wrapped_kernel(psf: np.ndarray, d) -> np.ndarray:
+    """PSF placed with its floor(p/2) element at the origin, wrapped modulo the
+    volume size d (a PSF larger than the volume folds onto it)."""
+    pad = np.zeros(d, dtype=np.float64)
+    ix = np.ix_(*[(np.arange(ps) - ps // 2) % n for ps, n in zip(psf.shape, d)])
+    np.add.at(pad, ix, psf.astype(np.float64))
+    return pad
 
 
 def point_emitters(shape, n: int, amp_range, g: np.random.Generator) -> np.ndarray:
@@ -62,27 +68,33 @@ def point_emitters(shape, n: int, amp_range, g: np.random.Generator) -> np.ndarr
     return vol
 
 
-def gaussian_blobs(shape, n: int, g: np.random.Generator, sigma_range=(0.5, 2.0), lognorm=(6.0, 0.8),
-                   nbins: int = 6) -> np.ndarray:
-    """n isotropic Gaussian blobs with peak amplitude LogNormal(mu, s) and
-    sigma ~ U(sigma_range); sigmas are grouped into nbins bins whose deltas
-    are blurred together."""
-    vol = np.zeros(shape, dtype=np.float64, order="F")
+def _blob_params(shape, n, g, sigma_range=(0.5, 2.0), lognorm=(6.0, 0.8), nbins=6):
     idx = [g.integers(0, s, size=n) for s in shape]
     sig = g.uniform(sigma_range[0], sigma_range[1], size=n)
     amp = g.lognormal(lognorm[0], lognorm[1], size=n)
     edges = np.linspace(sigma_range[0], sigma_range[1], nbins + 1)
     b = np.clip(np.digitize(sig, edges) - 1, 0, nbins - 1)
+    centers = 0.5 * (edges[:-1] + edges[1:])
+    # peak amplitude A <-> delta weight A (2 pi)^(3/2) s^3 under a unit-sum Gaussian
+    w = amp * (2 * np.pi) ** 1.5 * centers[b] ** 3
+    return idx, b, w, centers
+
+
+def gaussian_blobs(shape, n: int, g: np.random.Generator, sigma_range=(0.5, 2.0), lognorm=(6.0, 0.8),
+                   nbins: int = 6) -> np.ndarray:
+    """n isotropic Gaussian blobs with peak amplitude LogNormal(mu, s) and
+    sigma ~ U(sigma_range); sigmas are grouped into nbins bins whose deltas
+    are blurred together (circular boundary)."""
+    idx, b, w, centers = _blob_params(shape, n, g, sigma_range, lognorm, nbins)
+    vol = np.zeros(shape, dtype=np.float64, order="F")
     for k in range(nbins):
         sel = b == k
         if not sel.any():
             continue
-        s = 0.5 * (edges[k] + edges[k + 1])
         d = np.zeros(shape, dtype=np.float64, order="F")
-        # peak amplitude A <-> delta weight A (2 pi)^(3/2) s^3 under a unit-sum Gaussian
-        np.add.at(d, tuple(i[sel] for i in idx), amp[sel] * (2 * np.pi) ** 1.5 * s ** 3)
+        np.add.at(d, tuple(i[sel] for i in idx), w[sel])
         if _ndi is not None:
-            vol += _ndi.gaussian_filter(d, s, mode="wrap", truncate=4.0)
+            vol += _ndi.gaussian_filter(d, centers[k], mode="wrap", truncate=4.0)
         else:  # pragma: no cover
             vol += d
     return vol
@@ -90,16 +102,21 @@ def gaussian_blobs(shape, n: int, g: np.random.Generator, sigma_range=(0.5, 2.0)
 
 def smooth_background(shape, g: np.random.Generator, mean: float = 100.0, n: int = 3) -> np.ndarray:
     """mean + sum of n low-frequency sinusoids (|k| <= 2 cycles/volume, a ~ U[5, 20])."""
-    grids = np.meshgrid(*[np.arange(s, dtype=np.float64) / s for s in shape], indexing="ij")
     bg = np.full(shape, mean, dtype=np.float64)
+    for a, k, ph in _bg_params(g, n):
+        t = [np.arange(s, dtype=np.float64) / s * k[d] for d, s in enumerate(shape)]
+        bg += a * np.sin(2 * np.pi * (t[0][:, None, None] + t[1][None, :, None] + t[2][None, None, :]) + ph)
+    return bg
+
+
+def _bg_params(g, n):
+    out = []
     for _ in range(n):
         k = g.integers(-2, 3, size=3)
         if not k.any():
             k[0] = 1
-        a = g.uniform(5, 20)
-        ph = g.uniform(0, 2 * np.pi)
-        bg += a * np.sin(2 * np.pi * (k[0] * grids[0] + k[1] * grids[1] + k[2] * grids[2]) + ph)
-    return bg
+        out.append((g.uniform(5, 20), k, g.uniform(0, 2 * np.pi)))
+    return out
 
 
 def poisson_u16(lam: np.ndarray, g: np.random.Generator) -> np.ndarray:
@@ -116,8 +133,68 @@ def emitter_volume(shape, psf, n_emitters=200, amp_range=(500, 3000), background
     return poisson_u16(img, g1)
 
 
-def blob_volume(shape, psf, n_blobs=20000, config_id=2, bg_mean=100.0):
-    """Gaussian blobs + smooth sinusoidal background, blurred by the PSF, Poisson noise."""
+def _gpu_available() -> bool:
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except ImportError:
+        return False
+
+
+def _blob_volume_torch(shape, psf, n_blobs, config_id, bg_mean):
+    """GPU rendering of blob_volume: blobs, background and PSF blur in one
+    Fourier-domain product (float64), Poisson noise from a seeded torch
+    generator.  Same model as the host version; different random stream for
+    the noise."""
+    import torch
+    dev = torch.device("cuda")
+    g0, g1, g2 = rng(config_id, 0), rng(config_id, 1), rng(config_id, 2)
+    idx, b, w, centers = _blob_params(shape, n_blobs, g0)
+    n1, n2, n3 = shape
+    # torch tensors are indexed [z, y, x] = MATLAB (x=dim1 contiguous) reversed
+    flat = torch.from_numpy(idx[0] + n1 * (idx[1] + n2 * idx[2])).to(dev)
+    rfft = lambda v: torch.fft.rfftn(v, dim=(0, 1, 2))  # noqa: E731
+    acc = None
+    for k in range(len(centers)):
+        sel = torch.from_numpy(b == k).to(dev)
+        if not bool(sel.any()):
+            continue
+        d = torch.zeros(n1 * n2 * n3, dtype=torch.float64, device=dev)
+        d.index_add_(0, flat[sel], torch.from_numpy(w).to(dev)[sel])
+        s = float(centers[k])
+        # circular Gaussian kernel sampled on the grid, unit sum
+        ax = [torch.arange(m, dtype=torch.float64, device=dev) for m in (n3, n2, n1)]
+        ax = [torch.minimum(a, m - a) for a, m in zip(ax, (n3, n2, n1))]
+        ker = torch.exp(-(ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2)
+                        / (2 * s * s))
+        ker /= ker.sum()
+        term = rfft(d.view(n3, n2, n1)) * rfft(ker)
+        acc = term if acc is None else acc + term
+        del d, ker
+    bg = torch.full((n3, n2, n1), bg_mean, dtype=torch.float64, device=dev)
+    for a, kk, ph in _bg_params(g1, 3):
+        t = [torch.arange(m, dtype=torch.float64, device=dev) / m * kk[dd] for dd, m in enumerate(shape)]
+        bg += a * torch.sin(2 * np.pi * (t[2][:, None, None] + t[1][None, :, None] + t[0][None, None, :]) + ph)
+    acc = acc + rfft(bg)
+    del bg
+    p = torch.from_numpy(np.ascontiguousarray(_wrapped_kernel(psf, shape).transpose(2, 1, 0))).to(dev)
+    img = torch.fft.irfftn(acc * rfft(p), s=(n3, n2, n1), dim=(0, 1, 2)).clamp_(min=0)
+    del acc, p
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(g2.integers(0, 2 ** 62)))
+    counts = torch.poisson(img, generator=gen).clamp_(max=65535).to(torch.int32).cpu().numpy()
+    del img
+    return np.asfortranarray(counts.astype(np.uint16).transpose(2, 1, 0))
+
+
+def blob_volume(shape, psf, n_blobs=20000, config_id=2, bg_mean=100.0, device=None):
+    """Gaussian blobs + smooth sinusoidal background, blurred by the PSF,
+    Poisson noise.  device='cuda' renders on the GPU (much faster for large
+    volumes); the default picks the GPU when one is visible."""
+    if device is None:
+        device = "cuda" if _gpu_available() else "cpu"
+    if device == "cuda":
+        return _blob_volume_torch(tuple(int(s) for s in shape), psf, n_blobs, config_id, bg_mean)
     g0, g1, g2 = rng(config_id, 0), rng(config_id, 1), rng(config_id, 2)
     obj = gaussian_blobs(shape, n_blobs, g0) + smooth_background(shape, g1, bg_mean)
     img = _fft_convolve_circular(obj, psf)
@@ -131,8 +208,13 @@ def config1():
     return emitter_volume((128, 128, 64), psf, 200, (500, 3000), 100.0, 1), psf
 
 
-def config2(shape=(512, 512, 256), n_blobs=20000):
+def config2_psf():
+    """64x64x101 Gaussian PSF, sigma_xy 1.0, sigma_z 2.5, sheared 31.5 deg along dim 2."""
+    return gaussian_psf((64, 64, 101), 1.0, 2.5, 31.5)
+
+
+def config2(shape=(512, 512, 256), n_blobs=20000, device=None):
     """512x512x256 uint16, 20k blobs, smooth background (mean 100), blurred by
-    a 64x64x101 PSF (sigma_xy 1.0, sigma_z 2.5, sheared 31.5 deg), Poisson."""
-    psf = gaussian_psf((64, 64, 101), 1.0, 2.5, 31.5)
-    return blob_volume(shape, psf, n_blobs, 2), psf
+    the config-2 PSF, Poisson."""
+    psf = config2_psf()
+    return blob_volume(shape, psf, n_blobs, 2, device=device), psf
