@@ -104,11 +104,53 @@ pk_status pk_decon_lucy_omw(pk_ctx *ctx, const pk_vol *I, const float *psf_f, co
                             const float *psf_b, const int64_t b_dims[3], int32_t numit, const pk_opts *opts,
                             pk_vol *out, double *err_mat);
 
+/* One chunk of a batch: input and output volumes of the same (any) size. */
+typedef struct {
+    pk_vol in;
+    pk_vol out;
+} pk_batch_item;
+
+/* Batched deconvolution of n chunks of different sizes sharing the PSFs and
+ * options: the per-chunk loop of RLdecon_large_in_memory.m:190-260 /
+ * RLdecon_for_zarr_block.m:101-171 around decon_lucy_function (method 0) or
+ * decon_lucy_omw_function (method 1).  Chunks whose size is not 2/3/5-smooth
+ * are zero-padded to the next smooth size and cropped back; a PSF larger than
+ * a chunk is centre-cropped to it first (RLdecon.m:264-270, 326-332).  Every
+ * pass runs as one launch over all chunks; each chunk's result is bitwise the
+ * same as a single pk_decon_lucy / pk_decon_lucy_omw call on the padded chunk.
+ * bbox and EdgeErosion are per-chunk options and are rejected here. */
+pk_status pk_decon_batch(pk_ctx *ctx, int32_t method, const pk_batch_item *items, int32_t n, const float *psf_f,
+                         const int64_t f_dims[3], const float *psf_b, const int64_t b_dims[3], int32_t numit,
+                         const pk_opts *opts);
+
 /* otf = decon_psf2otf(psf, outSize) (microscopeDataProcessing/decon/decon_psf2otf.m:1-46).
  * otf: host buffer of prod(out_dims) complex single values, interleaved
  * (re, im), the full spectrum as MATLAB returns it. */
 pk_status pk_psf2otf(pk_ctx *ctx, const float *psf, const int64_t psf_dims[3], const int64_t out_dims[3],
                      float *otf);
+
+/* Name-value options of omw_backprojector_generation
+ * (microscopeDataProcessing/decon/omw_backprojector_generation.m:7-15). */
+typedef struct {
+    double otf_cum_thresh;   /* 'OTFCumThresh'  (0.9) */
+    double otf_area_thresh;  /* 'OTFAreaThresh' (100) */
+    double hann_win_lo;      /* 'hannWinBounds' (1) (0.8) */
+    double hann_win_hi;      /* 'hannWinBounds' (2) (1.0) */
+    double min_int_thresh;   /* 'minIntThrsh'   (1e-3) */
+} pk_bp_opts;
+void pk_bp_opts_default(pk_bp_opts *o);
+
+/* [b_omw, OTF_bp_omw, abs_OTF_c, OTF_mask] = omw_backprojector_generation(psf, alpha, skewed, ...)
+ * (microscopeDataProcessing/decon/omw_backprojector_generation.m:1-166).
+ * psf: host single, psf_dims; skewed: -1 = [] (auto-detect), 0 = false,
+ * 1 = true.  Outputs are host buffers of prod(psf_dims) elements: b_omw
+ * (single), otf_bp_omw (complex single, interleaved; may be NULL),
+ * abs_otf_c (single; may be NULL), otf_mask (single 0/1, or 0..3 counts for
+ * skewed data; may be NULL).  skewed_used receives the branch taken.  The
+ * nearest-edge search runs on the context's GPU. */
+pk_status pk_omw_backprojector(pk_ctx *ctx, const float *psf, const int64_t psf_dims[3], double alpha,
+                               int32_t skewed, const pk_bp_opts *opts, float *b_omw, float *otf_bp_omw,
+                               float *abs_otf_c, float *otf_mask, int32_t *skewed_used);
 
 /* Transform diagnostics (MATLAB fftn / ifftn of real data, decon_lucy_function.m:136,142).
  * Device pointers.  r2c: real N floats -> half spectrum (floor(n1/2)+1)*n2*n3

@@ -1,0 +1,241 @@
+"""numpy / scipy restatement of omw_backprojector_generation.
+
+TEST INFRASTRUCTURE ONLY: the checker for the engine's back-projector
+generation.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import it.
+
+Follows microscopeDataProcessing/decon/omw_backprojector_generation.m:1-166
+step by step (line numbers in comments).  MATLAB toolbox semantics restated:
+  * strel('sphere', r): voxels with x^2+y^2+z^2 <= r^2;
+  * imerode pads with true, imdilate with false; imopen = dilate(erode),
+    imclose = erode(dilate);
+  * bwareaopen / bwconncomp / bwlabeln: 26-connectivity in 3-D, 8 in 2-D;
+    labels numbered by the first voxel in column-major scan order;
+  * bwconvhull(BW, 'union'): convex hull of the corners (+-0.5) of all
+    foreground pixels; a pixel is set when its centre lies inside or on it;
+  * knnsearch K=2: exact Euclidean distances in double, ties broken by the
+    lower edge index;
+  * cart2sph / sph2cart: MATLAB's formulas (atan2 / hypot, then cos / sin),
+    evaluated with the C library's functions.
+These toolbox behaviours cannot be executed here (no MATLAB), so parity of
+this function with the reference is unpinned beyond the invariants its tests
+check.
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy import ndimage as ndi
+
+
+def ball(r: int) -> np.ndarray:
+    g = np.arange(-r, r + 1)
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    return (x * x + y * y + z * z) <= r * r
+
+
+def imerode(m, se):
+    return ndi.binary_erosion(m, structure=se, border_value=1)
+
+
+def imdilate(m, se):
+    return ndi.binary_dilation(m, structure=se, border_value=0)
+
+
+def label_colmajor(m, conn_full=True):
+    """Connected components labelled in MATLAB order (first voxel in
+    column-major order gets label 1).  Returns (labels, count, sizes)."""
+    st = np.ones((3,) * m.ndim, bool) if conn_full else None
+    lab, n = ndi.label(m, structure=st)
+    if n == 0:
+        return lab, 0, np.zeros(0, np.int64)
+    flat = lab.reshape(-1, order="F")
+    idx = np.arange(flat.size)
+    first = np.full(n + 1, flat.size, dtype=np.int64)
+    nz = flat > 0
+    np.minimum.at(first, flat[nz], idx[nz])
+    order = np.argsort(first[1:], kind="stable") + 1  # old labels sorted by first voxel
+    remap = np.zeros(n + 1, np.int64)
+    remap[order] = np.arange(1, n + 1)
+    lab = remap[lab]
+    sizes = np.bincount(lab.reshape(-1), minlength=n + 1)[1:]
+    return lab, n, sizes
+
+
+def bwareaopen(m, P):
+    lab, n, sizes = label_colmajor(m)
+    keep = np.concatenate([[False], sizes >= P])
+    return keep[lab]
+
+
+def _hull(points):
+    """Monotone-chain convex hull (counter-clockwise, no collinear points)."""
+    pts = sorted(set(points))
+    if len(pts) <= 2:
+        return pts
+
+    def cross(o, a, b):
+        return (a[0] - o[0]) * (b[1] - o[1]) - (a[1] - o[1]) * (b[0] - o[0])
+
+    lower, upper = [], []
+    for p in pts:
+        while len(lower) >= 2 and cross(lower[-2], lower[-1], p) <= 0:
+            lower.pop()
+        lower.append(p)
+    for p in reversed(pts):
+        while len(upper) >= 2 and cross(upper[-2], upper[-1], p) <= 0:
+            upper.pop()
+        upper.append(p)
+    return lower[:-1] + upper[:-1]
+
+
+def bwconvhull(img2):
+    """bwconvhull(BW, 'union') on a 2-D boolean image (any orientation)."""
+    rr, cc = np.nonzero(img2)
+    if rr.size == 0:
+        return img2.copy()
+    pts = []
+    for r in np.unique(rr):  # extreme pixels of each row suffice for the hull
+        cs = cc[rr == r]
+        for c in (cs.min(), cs.max()):
+            for dr in (-0.5, 0.5):
+                for dc in (-0.5, 0.5):
+                    pts.append((float(r + dr), float(c + dc)))
+    h = _hull(pts)
+    out = np.zeros_like(img2, dtype=bool)
+    R, C = np.meshgrid(np.arange(img2.shape[0]), np.arange(img2.shape[1]), indexing="ij")
+    inside = np.ones(img2.shape, bool)
+    n = len(h)
+    for i in range(n):
+        (r0, c0), (r1, c1) = h[i], h[(i + 1) % n]
+        # counter-clockwise hull: centre inside or on the edge when cross >= 0
+        cr = (r1 - r0) * (C - c0) - (c1 - c0) * (R - r0)
+        inside &= cr >= -1e-9
+    out[inside] = True
+    return out
+
+
+def knn2(edge_pts, query_pts, chunk=4096):
+    """Two nearest edge points per query (Euclidean, double), ties -> lower index."""
+    nq = query_pts.shape[0]
+    inds = np.zeros((nq, 2), np.int64)
+    d = np.zeros((nq, 2), np.float64)
+    for s in range(0, nq, chunk):
+        q = query_pts[s:s + chunk]
+        d2 = ((q[:, None, 0] - edge_pts[None, :, 0]) ** 2 + (q[:, None, 1] - edge_pts[None, :, 1]) ** 2
+              + (q[:, None, 2] - edge_pts[None, :, 2]) ** 2)
+        o = np.argsort(d2, axis=1, kind="stable")[:, :2]
+        inds[s:s + chunk] = o
+        d[s:s + chunk] = np.sqrt(np.take_along_axis(d2, o, axis=1))
+    return inds, d
+
+
+def omw_backprojector_generation(psf, alpha, skewed=None, OTFCumThresh=0.9, OTFAreaThresh=100,
+                                 hannWinBounds=(0.8, 1.0), minIntThrsh=1e-3):
+    """Returns (b_omw, OTF_bp_omw, abs_OTF_c, OTF_mask, skewed_used)."""
+    psf = np.asarray(psf)
+    if psf.size == 0 or not np.any(psf):  # :25-27
+        raise ValueError("A valid nonzero PSF needs to be provided!")
+    psf = psf.astype(np.float32)
+    psf = (psf / np.float32(psf.astype(np.float64).sum())).astype(np.float32)  # :29-30
+    psz = psf.shape
+    # OTF = decon_psf2otf(psf) at the PSF size (:32), single precision result
+    sh = np.roll(psf.astype(np.float64), shift=tuple(-(s // 2) for s in psz), axis=(0, 1, 2))
+    OTF = np.fft.fftn(sh).astype(np.complex64)
+    abs_OTF = np.abs(OTF).astype(np.float32)  # :33
+    abs_OTF_c = np.fft.fftshift(abs_OTF)  # :34
+    vals = np.sort(abs_OTF.astype(np.float64).reshape(-1, order="F"))[::-1]  # :35
+    cs = np.cumsum(vals)
+    tind = int(np.argmax(cs > cs[-1] * OTFCumThresh))  # :38 (sequential sum)
+    dc = np.float32(np.float32(vals[tind]) / abs_OTF[0, 0, 0])  # :39 (single)
+    dc = max(dc, np.float32(minIntThrsh))  # :41
+    mask = abs_OTF_c > abs_OTF[0, 0, 0] * dc  # :43
+    b2 = ball(2)
+    mask = imdilate(imerode(mask, b2), b2)  # :44 imopen
+    mask = bwareaopen(mask, OTFAreaThresh)  # :45
+    mask = imerode(imdilate(mask, b2), b2)  # :46 imclose
+    mask = imdilate(imerode(mask, b2), b2)  # :47 imopen
+    mask = bwareaopen(mask, OTFAreaThresh)  # :48
+    if not mask.any():  # :50-52
+        raise ValueError("The OTF mask is empty, check the PSF and OTF-related parameters!")
+
+    if skewed is None:  # :57-71
+        skewed = False
+        xz = mask.sum(axis=0) > 0  # squeeze(sum(OTF_mask, 1)) > 0 : (n2, n3)
+        _, ncc, _ = label_colmajor(xz)
+        if ncc > 1:
+            line = xz.sum(axis=0)  # sum(OTF_mask_xz, 1): along z
+            pind = int(np.argmax(line)) + 1
+            if np.all(line > 0) and abs(pind - (line.size + 1) / 2) > 1:
+                skewed = True
+
+    if skewed:  # :73-88
+        L, n, sizes = label_colmajor(mask)
+        if n + 1 > 4:
+            top = np.argsort(-sizes, kind="stable")[:3] + 1
+            mask = np.isin(L, top)
+            L, n, sizes = label_colmajor(mask)
+        elif n + 1 == 2:
+            L = (L + 1) * mask
+        mask = np.concatenate([L == 3, L == 2, L == 1], axis=2)
+    else:  # :90-93
+        L, n, sizes = label_colmajor(mask)
+        mask = L == (int(np.argmax(sizes)) + 1)
+    psz3 = mask.shape
+
+    # convex hulls per slice along z, y, x (:98-109); index sets taken first
+    zinds = np.nonzero(mask.sum(axis=(0, 1)))[0]
+    xinds = np.nonzero(mask.sum(axis=(0, 2)))[0]
+    yinds = np.nonzero(mask.sum(axis=(1, 2)))[0]
+    for z in zinds:
+        mask[:, :, z] = bwconvhull(mask[:, :, z])
+    for y in yinds:
+        mask[y, :, :] = bwconvhull(mask[y, :, :])
+    for x in xinds:
+        mask[:, x, :] = bwconvhull(mask[:, x, :])
+
+    edge = mask & ~imerode(mask, ball(1))  # :122
+    c = (np.array(psz3, np.float64) + 1) / 2  # :123
+    Y, X, Z = np.meshgrid(np.arange(1, psz3[0] + 1) - c[0], np.arange(1, psz3[1] + 1) - c[1],
+                          np.arange(1, psz3[2] + 1) - c[2], indexing="ij")
+    # cart2sph / sph2cart (:129-132) with the C library's atan2, hypot, cos and
+    # sin (math module), so exact-distance ties between lattice directions
+    # resolve the same way as in any other glibc-based implementation
+    import math
+    shp = X.shape
+    Xf, Yf, Zf = X.reshape(-1), Y.reshape(-1), Z.reshape(-1)
+    rs = np.empty(Xf.size)
+    Xn = np.empty(Xf.size)
+    Yn = np.empty(Xf.size)
+    Zn = np.empty(Xf.size)
+    for i in range(Xf.size):
+        x, y, z = float(Xf[i]), float(Yf[i]), float(Zf[i])
+        hxy = math.hypot(x, y)
+        rs[i] = math.hypot(hxy, z)
+        el, az = math.atan2(z, hxy), math.atan2(y, x)
+        rc = math.cos(el)
+        Xn[i], Yn[i], Zn[i] = rc * math.cos(az), rc * math.sin(az), math.sin(el)
+    rs, Xn, Yn, Zn = rs.reshape(shp), Xn.reshape(shp), Yn.reshape(shp), Zn.reshape(shp)
+    f = lambda a: a.reshape(-1, order="F")  # noqa: E731
+    ef = f(edge)
+    E = np.stack([f(Yn)[ef], f(Xn)[ef], f(Zn)[ef]], axis=1)
+    Q = np.stack([f(Yn), f(Xn), f(Zn)], axis=1)
+    inds, dmat = knn2(E, Q)  # :135
+    Rn = f(rs)[ef]  # :136
+    d1 = dmat[inds[:, 0], 0]  # :137 (indexes d_mat rows by edge index, as the reference does)
+    d2 = dmat[inds[:, 1], 1]  # :138
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rn = (Rn[inds[:, 0]] * d2 + Rn[inds[:, 1]] * d1) / (d1 + d2)  # :139
+        bw = f(rs) / rn  # :141
+    lo, hi = float(hannWinBounds[0]), float(hannWinBounds[1])
+    with np.errstate(invalid="ignore"):
+        win = (bw < lo).astype(np.float64) + np.cos(np.pi * (bw - lo) / 2 / (hi - lo)) ** 2 * ((bw >= lo) & (bw < hi))
+    m = win.reshape(psz3, order="F")  # :148
+    if skewed:  # :150-153
+        m = m.reshape(psz[0], psz[1], psz[2], 3, order="F").sum(axis=3)
+        mask = mask.reshape(psz[0], psz[1], psz[2], 3, order="F").sum(axis=3)
+    om = np.fft.ifftshift(m)  # :155
+    den = (abs_OTF * abs_OTF + np.float32(alpha)).astype(np.float32)  # :158, single
+    w = (OTF.real / den + 1j * (-OTF.imag / den)).astype(np.complex64)
+    omw = (om.astype(np.float32) * w).astype(np.complex64)  # :161
+    b = np.fft.fftshift(np.real(np.fft.ifftn(omw.astype(np.complex128)))).astype(np.float32)  # :163
+    return b, omw, abs_OTF_c, mask, bool(skewed)

@@ -5,6 +5,8 @@ declared in include/pkdecon.h); this package is its Python front end with the
 reference's MATLAB call signatures.
 """
 from ._lib import Context, PkDeconError, default_context, load  # noqa: F401
+from .backprojector import omw_backprojector_generation  # noqa: F401
+from .batch import decon_batch  # noqa: F401
 from .decon import (  # noqa: F401
     crop_psf_to_data,
     decon_lucy_function,
@@ -24,6 +26,8 @@ __all__ = [
     "decon_lucy_omw_function",
     "decon_psf2otf",
     "findGoodFactorNumber",
+    "omw_backprojector_generation",
+    "decon_batch",
     "crop_psf_to_data",
     "fft_r2c",
     "fft_c2r",

@@ -43,7 +43,10 @@ EXPORTS = [
     "pk_device_bytes",
     "pk_decon_lucy",
     "pk_decon_lucy_omw",
+    "pk_decon_batch",
     "pk_psf2otf",
+    "pk_bp_opts_default",
+    "pk_omw_backprojector",
     "pk_fft_r2c",
     "pk_fft_c2r",
     "pk_profile_enable",
@@ -72,6 +75,16 @@ class PkOpts(ctypes.Structure):
         ("has_bbox", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("bbox", ctypes.c_int64 * 6),
+    ]
+
+
+class PkBpOpts(ctypes.Structure):
+    _fields_ = [
+        ("otf_cum_thresh", ctypes.c_double),
+        ("otf_area_thresh", ctypes.c_double),
+        ("hann_win_lo", ctypes.c_double),
+        ("hann_win_hi", ctypes.c_double),
+        ("min_int_thresh", ctypes.c_double),
     ]
 
 
@@ -134,6 +147,12 @@ def load():
         L.pk_decon_lucy_omw.argtypes = [vp, ctypes.POINTER(PkVol), vp, i64p, vp, i64p, ctypes.c_int32,
                                         ctypes.POINTER(PkOpts), ctypes.POINTER(PkVol), vp]
         L.pk_psf2otf.argtypes = [vp, vp, i64p, i64p, vp]
+        L.pk_decon_batch.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, i64p, vp, i64p, ctypes.c_int32,
+                                     ctypes.POINTER(PkOpts)]
+        L.pk_bp_opts_default.argtypes = [ctypes.POINTER(PkBpOpts)]
+        L.pk_bp_opts_default.restype = None
+        L.pk_omw_backprojector.argtypes = [vp, vp, i64p, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(PkBpOpts),
+                                           vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int32)]
         L.pk_fft_r2c.argtypes = [vp, vp, vp, i64p]
         L.pk_fft_c2r.argtypes = [vp, vp, vp, i64p]
         L.pk_profile_enable.argtypes = [vp, ctypes.c_int32]
@@ -142,7 +161,7 @@ def load():
         L.pk_find_good_factor_number.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.pk_find_good_factor_number.restype = ctypes.c_int64
         for name in EXPORTS:
-            if name not in ("pk_opts_default", "pk_ctx_destroy", "pk_last_error", "pk_device_bytes",
+            if name not in ("pk_opts_default", "pk_bp_opts_default", "pk_ctx_destroy", "pk_last_error", "pk_device_bytes",
                             "pk_find_good_factor_number", "pk_profile_read"):
                 getattr(L, name).restype = ctypes.c_int
         _lib = L

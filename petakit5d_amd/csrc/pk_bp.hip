@@ -1,0 +1,529 @@
+// pk_bp.hip -- OMW back-projector generation
+// (microscopeDataProcessing/decon/omw_backprojector_generation.m:1-166).
+//
+// The per-voxel nearest-edge search (knnsearch K=2 over every voxel of the
+// PSF-sized OTF domain, :135) runs on the GPU as an exact FP64 brute-force
+// kernel.  The small sequential toolbox steps -- OTF at the PSF size (any
+// length, e.g. a prime 101-slice PSF), sphere-SE morphology, 26-connected
+// labelling, per-slice convex hulls -- run on the host in this file.
+//
+// MATLAB semantics restated (steps follow the reference line by line):
+//   strel('sphere', r) = voxels with x^2+y^2+z^2 <= r^2; imerode pads with
+//   true, imdilate with false; bwareaopen / bwconncomp / bwlabeln use
+//   26-connectivity (8 in 2-D) and number components by their first voxel in
+//   column-major order; bwconvhull(BW,'union') = convex hull of the corners of
+//   the foreground pixels, pixel set when its centre is inside or on it;
+//   knnsearch ties go to the lower edge index.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "pk_engine.hpp"
+
+// every product and sum here rounds separately, as MATLAB's elementwise
+// single/double arithmetic does (no fused multiply-add contraction)
+#pragma clang fp contract(off)
+
+namespace pk {
+
+using cd = std::complex<double>;
+
+// ------------------------------------------------------------------ host FFT (any length)
+// Recursive mixed-radix decimation in time; twiddles from a table of the
+// line length's roots of unity (double).
+static void fft_rec(const cd* x, int s, cd* y, int n, int N, const std::vector<cd>& W, std::vector<cd>& tmp) {
+    if (n == 1) {
+        y[0] = x[0];
+        return;
+    }
+    int p = 2;
+    while (n % p) ++p;
+    const int m = n / p;
+    for (int r = 0; r < p; ++r) fft_rec(x + (size_t)r * s, s * p, y + (size_t)r * m, m, N, W, tmp);
+    const int step = N / n;
+    cd t[64];
+    std::vector<cd> big;
+    cd* tt = t;
+    if (p > 64) {
+        big.resize(p);
+        tt = big.data();
+    }
+    for (int k = 0; k < m; ++k) {
+        for (int r = 0; r < p; ++r) tt[r] = y[(size_t)r * m + k] * W[(size_t)((int64_t)r * k * step % N)];
+        for (int q = 0; q < p; ++q) {
+            cd acc = 0;
+            for (int r = 0; r < p; ++r) acc += tt[r] * W[(size_t)(((int64_t)r * q % p) * (N / p))];
+            tmp[(size_t)q * m + k] = acc;
+        }
+    }
+    for (int i = 0; i < n; ++i) y[i] = tmp[i];
+}
+
+// In-place 3-D DFT of a column-major complex volume; inverse includes 1/N.
+static void fft3_host(std::vector<cd>& a, const int64_t d[3], bool inverse) {
+    const int64_t n1 = d[0], n2 = d[1], n3 = d[2];
+    for (int ax = 0; ax < 3; ++ax) {
+        const int n = (int)d[ax];
+        if (n == 1) continue;
+        std::vector<cd> W(n), line(n), out(n), tmp(n);
+        for (int e = 0; e < n; ++e) {
+            const long double ang = 2.0L * 3.14159265358979323846264338327950288L * e / n;
+            W[e] = cd((double)cosl(ang), (double)(inverse ? sinl(ang) : -sinl(ang)));
+        }
+        const int64_t stride = ax == 0 ? 1 : (ax == 1 ? n1 : n1 * n2);
+        const int64_t nlines = n1 * n2 * n3 / n;
+        for (int64_t l = 0; l < nlines; ++l) {
+            int64_t base;
+            if (ax == 0) base = l * n1;
+            else if (ax == 1) base = (l % n1) + (l / n1) * n1 * n2;
+            else base = l;
+            for (int i = 0; i < n; ++i) line[i] = a[(size_t)(base + i * stride)];
+            fft_rec(line.data(), 1, out.data(), n, n, W, tmp);
+            for (int i = 0; i < n; ++i) a[(size_t)(base + i * stride)] = out[i];
+        }
+    }
+    if (inverse) {
+        const double sc = 1.0 / (double)(n1 * n2 * n3);
+        for (auto& v : a) v *= sc;
+    }
+}
+
+// ------------------------------------------------------------------ volume helpers
+struct Vol3 {
+    int64_t d[3];
+    int64_t n() const { return d[0] * d[1] * d[2]; }
+    int64_t at(int64_t i, int64_t j, int64_t k) const { return i + d[0] * (j + d[1] * k); }
+};
+
+template <class T>
+static std::vector<T> shift3(const std::vector<T>& a, const Vol3& v, const int64_t sh[3]) {
+    // out(i) = a(i - sh) (circular): fftshift uses sh = floor(n/2), ifftshift -floor(n/2)
+    std::vector<T> o(a.size());
+    for (int64_t k = 0; k < v.d[2]; ++k)
+        for (int64_t j = 0; j < v.d[1]; ++j)
+            for (int64_t i = 0; i < v.d[0]; ++i) {
+                const int64_t ii = ((i + sh[0]) % v.d[0] + v.d[0]) % v.d[0];
+                const int64_t jj = ((j + sh[1]) % v.d[1] + v.d[1]) % v.d[1];
+                const int64_t kk = ((k + sh[2]) % v.d[2] + v.d[2]) % v.d[2];
+                o[(size_t)v.at(ii, jj, kk)] = a[(size_t)v.at(i, j, k)];
+            }
+    return o;
+}
+
+static std::vector<uint8_t> morph(const std::vector<uint8_t>& m, const Vol3& v, int r, bool erode) {
+    std::vector<int> oi, oj, ok;
+    for (int k = -r; k <= r; ++k)
+        for (int j = -r; j <= r; ++j)
+            for (int i = -r; i <= r; ++i)
+                if (i * i + j * j + k * k <= r * r) {
+                    oi.push_back(i);
+                    oj.push_back(j);
+                    ok.push_back(k);
+                }
+    std::vector<uint8_t> o(m.size());
+    const int ns = (int)oi.size();
+    for (int64_t k = 0; k < v.d[2]; ++k)
+        for (int64_t j = 0; j < v.d[1]; ++j)
+            for (int64_t i = 0; i < v.d[0]; ++i) {
+                bool res = erode;
+                for (int s = 0; s < ns; ++s) {
+                    const int64_t ii = i + oi[s], jj = j + oj[s], kk = k + ok[s];
+                    const bool out = ii < 0 || jj < 0 || kk < 0 || ii >= v.d[0] || jj >= v.d[1] || kk >= v.d[2];
+                    const bool val = out ? erode : m[(size_t)v.at(ii, jj, kk)] != 0;  // erode pads true
+                    if (erode && !val) {
+                        res = false;
+                        break;
+                    }
+                    if (!erode && val) {
+                        res = true;
+                        break;
+                    }
+                }
+                o[(size_t)v.at(i, j, k)] = res;
+            }
+    return o;
+}
+static std::vector<uint8_t> imerode(const std::vector<uint8_t>& m, const Vol3& v, int r) { return morph(m, v, r, true); }
+static std::vector<uint8_t> imdilate(const std::vector<uint8_t>& m, const Vol3& v, int r) { return morph(m, v, r, false); }
+
+// Connected components (26-connectivity in 3-D; 8 in 2-D when d[2] == 1),
+// labels 1..n in order of each component's first voxel in column-major scan.
+static int label_cc(const std::vector<uint8_t>& m, const Vol3& v, std::vector<int>& lab, std::vector<int64_t>& sizes) {
+    lab.assign(m.size(), 0);
+    sizes.clear();
+    int n = 0;
+    std::vector<int64_t> stack;
+    for (int64_t s = 0; s < v.n(); ++s) {
+        if (!m[(size_t)s] || lab[(size_t)s]) continue;
+        ++n;
+        int64_t cnt = 0;
+        stack.push_back(s);
+        lab[(size_t)s] = n;
+        while (!stack.empty()) {
+            const int64_t c = stack.back();
+            stack.pop_back();
+            ++cnt;
+            const int64_t i = c % v.d[0], j = (c / v.d[0]) % v.d[1], k = c / (v.d[0] * v.d[1]);
+            for (int dk = -1; dk <= 1; ++dk)
+                for (int dj = -1; dj <= 1; ++dj)
+                    for (int di = -1; di <= 1; ++di) {
+                        const int64_t ii = i + di, jj = j + dj, kk = k + dk;
+                        if (ii < 0 || jj < 0 || kk < 0 || ii >= v.d[0] || jj >= v.d[1] || kk >= v.d[2]) continue;
+                        const int64_t q = v.at(ii, jj, kk);
+                        if (m[(size_t)q] && !lab[(size_t)q]) {
+                            lab[(size_t)q] = n;
+                            stack.push_back(q);
+                        }
+                    }
+        }
+        sizes.push_back(cnt);
+    }
+    return n;
+}
+
+static void bwareaopen(std::vector<uint8_t>& m, const Vol3& v, double P) {
+    std::vector<int> lab;
+    std::vector<int64_t> sizes;
+    label_cc(m, v, lab, sizes);
+    for (size_t i = 0; i < m.size(); ++i)
+        if (m[i] && (double)sizes[(size_t)lab[i] - 1] < P) m[i] = 0;
+}
+
+// bwconvhull(BW, 'union') of a 2-D image given by an accessor over [0,R) x [0,C).
+template <class Get, class Set>
+static void convhull_fill(int R, int C, Get get, Set set) {
+    std::vector<std::pair<double, double>> pts;
+    for (int r = 0; r < R; ++r) {
+        int cmin = -1, cmax = -1;
+        for (int c = 0; c < C; ++c)
+            if (get(r, c)) {
+                if (cmin < 0) cmin = c;
+                cmax = c;
+            }
+        if (cmin < 0) continue;
+        for (int c : {cmin, cmax})
+            for (double dr : {-0.5, 0.5})
+                for (double dc : {-0.5, 0.5}) pts.emplace_back(r + dr, c + dc);
+    }
+    if (pts.empty()) return;
+    std::sort(pts.begin(), pts.end());
+    pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+    auto cross = [](const std::pair<double, double>& o, const std::pair<double, double>& a,
+                    const std::pair<double, double>& b) {
+        return (a.first - o.first) * (b.second - o.second) - (a.second - o.second) * (b.first - o.first);
+    };
+    std::vector<std::pair<double, double>> h;
+    if (pts.size() <= 2) {
+        h = pts;
+    } else {
+        std::vector<std::pair<double, double>> lo, up;
+        for (auto& p : pts) {
+            while (lo.size() >= 2 && cross(lo[lo.size() - 2], lo.back(), p) <= 0) lo.pop_back();
+            lo.push_back(p);
+        }
+        for (auto it = pts.rbegin(); it != pts.rend(); ++it) {
+            while (up.size() >= 2 && cross(up[up.size() - 2], up.back(), *it) <= 0) up.pop_back();
+            up.push_back(*it);
+        }
+        lo.pop_back();
+        up.pop_back();
+        h = lo;
+        h.insert(h.end(), up.begin(), up.end());
+    }
+    const size_t n = h.size();
+    for (int r = 0; r < R; ++r)
+        for (int c = 0; c < C; ++c) {
+            bool in = true;
+            for (size_t e = 0; e < n && in; ++e) {
+                const auto& a = h[e];
+                const auto& b = h[(e + 1) % n];
+                const double cr = (b.first - a.first) * (c - a.second) - (b.second - a.second) * (r - a.first);
+                in = cr >= -1e-9;
+            }
+            set(r, c, in);
+        }
+}
+
+// ------------------------------------------------------------------ GPU nearest edges
+constexpr int kKnnTile = 1024;
+
+// For every query point, the two nearest edge points (exact FP64 Euclidean
+// distance, non-fused operations so ties resolve identically on any IEEE
+// implementation; equal distances keep the lower edge index first).
+__global__ void k_knn2(const double* __restrict__ E, int nE, const double* __restrict__ Q, int64_t nQ,
+                       int* __restrict__ inds, double* __restrict__ dist) {
+    __shared__ double se[3 * kKnnTile];
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = q < nQ;
+    const double qy = live ? Q[3 * q] : 0.0, qx = live ? Q[3 * q + 1] : 0.0, qz = live ? Q[3 * q + 2] : 0.0;
+    double b1 = 1e300, b2 = 1e300;
+    int i1 = 0, i2 = 0;
+    for (int t0 = 0; t0 < nE; t0 += kKnnTile) {
+        const int nt = min(kKnnTile, nE - t0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < 3 * nt; i += blockDim.x) se[i] = E[3 * (int64_t)t0 + i];
+        __syncthreads();
+        for (int j = 0; j < nt; ++j) {
+            const double dy = __dsub_rn(qy, se[3 * j]);
+            const double dx = __dsub_rn(qx, se[3 * j + 1]);
+            const double dz = __dsub_rn(qz, se[3 * j + 2]);
+            const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dy, dy), __dmul_rn(dx, dx)), __dmul_rn(dz, dz));
+            if (d2 < b1) {
+                b2 = b1;
+                i2 = i1;
+                b1 = d2;
+                i1 = t0 + j;
+            } else if (d2 < b2) {
+                b2 = d2;
+                i2 = t0 + j;
+            }
+        }
+    }
+    if (live) {
+        inds[2 * q] = i1;
+        inds[2 * q + 1] = i2;
+        dist[2 * q] = __dsqrt_rn(b1);
+        dist[2 * q + 1] = __dsqrt_rn(b2);
+    }
+}
+
+// ------------------------------------------------------------------ driver
+void pk_ctx_impl::omw_backprojector(const float* psf_in, const int64_t pd[3], double alpha, int skewed_in,
+                                    const double thr[5], float* b_out, float* otf_bp_out, float* abs_otf_c_out,
+                                    float* mask_out, int* skewed_used) {
+    const double otf_thresh = thr[0], area_thresh = thr[1], lo = thr[2], hi = thr[3], min_int = thr[4];
+    Vol3 v{{pd[0], pd[1], pd[2]}};
+    const int64_t P = v.n();
+    if (P == 0) fail(PK_ERR_EMPTY_PSF, "A valid nonzero PSF needs to be provided!");
+    bool allzero = true;
+    for (int64_t i = 0; i < P && allzero; ++i) allzero = psf_in[i] == 0.f;
+    if (allzero) fail(PK_ERR_EMPTY_PSF, "A valid nonzero PSF needs to be provided!");  // :25-27
+
+    // psf = single(psf) ./ sum(psf, 'all') (:29-30)
+    double s = 0;
+    for (int64_t i = 0; i < P; ++i) s += psf_in[i];
+    const float sf = (float)s;
+    // OTF = decon_psf2otf(psf) at the PSF size, single result (:32)
+    std::vector<cd> a((size_t)P);
+    const int64_t h[3] = {pd[0] / 2, pd[1] / 2, pd[2] / 2};
+    for (int64_t k = 0; k < pd[2]; ++k)
+        for (int64_t j = 0; j < pd[1]; ++j)
+            for (int64_t i = 0; i < pd[0]; ++i) {
+                const int64_t ii = ((i - h[0]) % pd[0] + pd[0]) % pd[0];
+                const int64_t jj = ((j - h[1]) % pd[1] + pd[1]) % pd[1];
+                const int64_t kk = ((k - h[2]) % pd[2] + pd[2]) % pd[2];
+                a[(size_t)v.at(ii, jj, kk)] = cd((double)(psf_in[v.at(i, j, k)] / sf), 0.0);
+            }
+    fft3_host(a, pd, false);
+    std::vector<std::complex<float>> OTF((size_t)P);
+    std::vector<float> absO((size_t)P);
+    for (int64_t i = 0; i < P; ++i) {
+        OTF[(size_t)i] = std::complex<float>((float)a[(size_t)i].real(), (float)a[(size_t)i].imag());
+        absO[(size_t)i] = std::abs(OTF[(size_t)i]);  // :33 (single)
+    }
+    const int64_t fsh[3] = {pd[0] / 2, pd[1] / 2, pd[2] / 2};
+    std::vector<float> absOc = shift3(absO, v, fsh);  // :34 fftshift
+
+    // threshold from the cumulative sorted |OTF| (:35-41)
+    std::vector<double> vals(absO.begin(), absO.end());
+    std::sort(vals.begin(), vals.end(), std::greater<double>());
+    double tot = 0;
+    for (double x : vals) tot += x;
+    double cs = 0;
+    size_t tind = 0;
+    for (size_t i = 0; i < vals.size(); ++i) {
+        cs += vals[i];
+        if (cs > tot * otf_thresh) {
+            tind = i;
+            break;
+        }
+    }
+    float dc = (float)vals[tind] / absO[0];
+    dc = std::max(dc, (float)min_int);
+    std::vector<uint8_t> mask((size_t)P);
+    const float cut = absO[0] * dc;
+    for (int64_t i = 0; i < P; ++i) mask[(size_t)i] = absOc[(size_t)i] > cut;  // :43
+    mask = imdilate(imerode(mask, v, 2), v, 2);  // :44 imopen
+    bwareaopen(mask, v, area_thresh);            // :45
+    mask = imerode(imdilate(mask, v, 2), v, 2);  // :46 imclose
+    mask = imdilate(imerode(mask, v, 2), v, 2);  // :47 imopen
+    bwareaopen(mask, v, area_thresh);            // :48
+    bool any = false;
+    for (auto x : mask) any |= x != 0;
+    if (!any) fail(PK_ERR_EMPTY_MASK, "The OTF mask is empty, check the PSF and OTF-related parameters!");  // :50-52
+
+    // skewed-space auto detection (:57-71)
+    bool skewed = skewed_in > 0;
+    if (skewed_in < 0) {
+        skewed = false;
+        Vol3 v2{{pd[1], pd[2], 1}};
+        std::vector<uint8_t> xz((size_t)(pd[1] * pd[2]), 0);
+        for (int64_t k = 0; k < pd[2]; ++k)
+            for (int64_t j = 0; j < pd[1]; ++j)
+                for (int64_t i = 0; i < pd[0]; ++i)
+                    if (mask[(size_t)v.at(i, j, k)]) xz[(size_t)(j + pd[1] * k)] = 1;
+        std::vector<int> lab;
+        std::vector<int64_t> sizes;
+        if (label_cc(xz, v2, lab, sizes) > 1) {
+            std::vector<int64_t> line((size_t)pd[2], 0);
+            for (int64_t k = 0; k < pd[2]; ++k)
+                for (int64_t j = 0; j < pd[1]; ++j) line[(size_t)k] += xz[(size_t)(j + pd[1] * k)];
+            const int64_t pind = (int64_t)(std::max_element(line.begin(), line.end()) - line.begin()) + 1;
+            bool allpos = true;
+            for (auto x : line) allpos &= x > 0;
+            if (allpos && std::fabs((double)pind - (double)(line.size() + 1) / 2.0) > 1) skewed = true;
+        }
+    }
+
+    Vol3 w = v;  // working domain (3x along z when skewed)
+    if (skewed) {  // :73-88
+        std::vector<int> lab;
+        std::vector<int64_t> sizes;
+        int n = label_cc(mask, v, lab, sizes);
+        if (n + 1 > 4) {
+            std::vector<int> order((size_t)n);
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sizes[(size_t)x] > sizes[(size_t)y]; });
+            std::vector<uint8_t> keep((size_t)n + 1, 0);
+            for (int t = 0; t < 3; ++t) keep[(size_t)order[(size_t)t] + 1] = 1;
+            for (size_t i = 0; i < mask.size(); ++i) mask[i] = keep[(size_t)lab[i]];
+            n = label_cc(mask, v, lab, sizes);
+        } else if (n + 1 == 2) {
+            for (size_t i = 0; i < mask.size(); ++i) lab[i] = (lab[i] + 1) * (mask[i] != 0);
+        }
+        w = Vol3{{pd[0], pd[1], 3 * pd[2]}};
+        std::vector<uint8_t> m3((size_t)w.n(), 0);
+        const int want[3] = {3, 2, 1};  // cat(3, L==3, L==2, L==1)
+        for (int b = 0; b < 3; ++b)
+            for (int64_t i = 0; i < P; ++i) m3[(size_t)(i + b * P)] = lab[(size_t)i] == want[b];
+        mask.swap(m3);
+    } else {  // :90-93
+        std::vector<int> lab;
+        std::vector<int64_t> sizes;
+        label_cc(mask, v, lab, sizes);
+        const int best = (int)(std::max_element(sizes.begin(), sizes.end()) - sizes.begin()) + 1;
+        for (size_t i = 0; i < mask.size(); ++i) mask[i] = lab[i] == best;
+    }
+
+    // convex hulls per slice along z, y, x (:98-109); index sets taken first
+    std::vector<int64_t> zi, yi, xi;
+    {
+        std::vector<uint8_t> hz((size_t)w.d[2], 0), hy((size_t)w.d[0], 0), hx((size_t)w.d[1], 0);
+        for (int64_t k = 0; k < w.d[2]; ++k)
+            for (int64_t j = 0; j < w.d[1]; ++j)
+                for (int64_t i = 0; i < w.d[0]; ++i)
+                    if (mask[(size_t)w.at(i, j, k)]) hz[(size_t)k] = hy[(size_t)i] = hx[(size_t)j] = 1;
+        for (int64_t k = 0; k < w.d[2]; ++k) if (hz[(size_t)k]) zi.push_back(k);
+        for (int64_t i = 0; i < w.d[0]; ++i) if (hy[(size_t)i]) yi.push_back(i);
+        for (int64_t j = 0; j < w.d[1]; ++j) if (hx[(size_t)j]) xi.push_back(j);
+    }
+    for (int64_t z : zi)
+        convhull_fill((int)w.d[0], (int)w.d[1], [&](int r, int c) { return mask[(size_t)w.at(r, c, z)] != 0; },
+                      [&](int r, int c, bool b) { mask[(size_t)w.at(r, c, z)] = b; });
+    for (int64_t y : yi)
+        convhull_fill((int)w.d[1], (int)w.d[2], [&](int r, int c) { return mask[(size_t)w.at(y, r, c)] != 0; },
+                      [&](int r, int c, bool b) { mask[(size_t)w.at(y, r, c)] = b; });
+    for (int64_t x : xi)
+        convhull_fill((int)w.d[0], (int)w.d[2], [&](int r, int c) { return mask[(size_t)w.at(r, x, c)] != 0; },
+                      [&](int r, int c, bool b) { mask[(size_t)w.at(r, x, c)] = b; });
+
+    // edge voxels and unit directions (:122-132)
+    std::vector<uint8_t> er = imerode(mask, w, 1);
+    const int64_t nW = w.n();
+    const double c1 = (w.d[0] + 1) / 2.0, c2 = (w.d[1] + 1) / 2.0, c3 = (w.d[2] + 1) / 2.0;
+    std::vector<double> Q((size_t)(3 * nW)), rs((size_t)nW);
+    std::vector<double> E;
+    std::vector<double> Rn;
+    for (int64_t k = 0; k < w.d[2]; ++k)
+        for (int64_t j = 0; j < w.d[1]; ++j)
+            for (int64_t i = 0; i < w.d[0]; ++i) {
+                const int64_t o = w.at(i, j, k);
+                const double Y = (double)(i + 1) - c1, X = (double)(j + 1) - c2, Z = (double)(k + 1) - c3;
+                const double hxy = std::hypot(X, Y);  // cart2sph
+                const double r = std::hypot(hxy, Z);
+                const double elev = std::atan2(Z, hxy), az = std::atan2(Y, X);
+                const double rc = std::cos(elev);  // sph2cart, r = 1
+                const double xn = rc * std::cos(az), yn = rc * std::sin(az), zn = std::sin(elev);
+                Q[(size_t)(3 * o)] = yn;
+                Q[(size_t)(3 * o + 1)] = xn;
+                Q[(size_t)(3 * o + 2)] = zn;
+                rs[(size_t)o] = r;
+                if (mask[(size_t)o] && !er[(size_t)o]) {
+                    E.push_back(yn);
+                    E.push_back(xn);
+                    E.push_back(zn);
+                    Rn.push_back(r);
+                }
+            }
+    const int nE = (int)Rn.size();
+    if (nE < 2) fail(PK_ERR_EMPTY_MASK, "OTF mask edge has fewer than two voxels");
+
+    // knnsearch(edge, all voxels, 'K', 2) on the GPU (:135)
+    DevBuf dE, dQ, dI, dD;
+    alloc(dE, sizeof(double) * E.size());
+    alloc(dQ, sizeof(double) * Q.size());
+    alloc(dI, sizeof(int) * 2 * (size_t)nW);
+    alloc(dD, sizeof(double) * 2 * (size_t)nW);
+    PK_HIP(hipMemcpyAsync(dE.p, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice, stream));
+    PK_HIP(hipMemcpyAsync(dQ.p, Q.data(), sizeof(double) * Q.size(), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_knn2, dim3((unsigned)((nW + 255) / 256)), dim3(256), 0, stream, dE.as<double>(), nE,
+                       dQ.as<double>(), nW, dI.as<int>(), dD.as<double>());
+    PK_HIP(hipGetLastError());
+    std::vector<int> inds((size_t)(2 * nW));
+    std::vector<double> dm((size_t)(2 * nW));
+    PK_HIP(hipMemcpyAsync(inds.data(), dI.p, sizeof(int) * inds.size(), hipMemcpyDeviceToHost, stream));
+    PK_HIP(hipMemcpyAsync(dm.data(), dD.p, sizeof(double) * dm.size(), hipMemcpyDeviceToHost, stream));
+    PK_HIP(hipStreamSynchronize(stream));
+
+    // rn from the two nearest edges; d_1/d_2 index d_mat rows by edge index
+    // exactly as the reference does (:137-139); apodisation (:145-148)
+    std::vector<double> mk((size_t)nW);
+    const double kPi = 3.14159265358979323846;
+    for (int64_t q = 0; q < nW; ++q) {
+        const int e1 = inds[(size_t)(2 * q)], e2 = inds[(size_t)(2 * q + 1)];
+        const double d1 = dm[(size_t)(2 * (int64_t)e1)], d2 = dm[(size_t)(2 * (int64_t)e2 + 1)];
+        const double rn = (Rn[(size_t)e1] * d2 + Rn[(size_t)e2] * d1) / (d1 + d2);
+        const double x = rs[(size_t)q] / rn;
+        const double cw = std::cos(kPi * (x - lo) / 2 / (hi - lo));
+        mk[(size_t)q] = (double)(x < lo) + cw * cw * (double)(x >= lo && x < hi);
+    }
+    std::vector<double> mfin((size_t)P, 0.0);
+    std::vector<float> mcount((size_t)P, 0.f);
+    const int nb = skewed ? 3 : 1;  // :150-153 sum the three z blocks
+    for (int b = 0; b < nb; ++b)
+        for (int64_t i = 0; i < P; ++i) {
+            mfin[(size_t)i] += mk[(size_t)(i + b * P)];
+            mcount[(size_t)i] += mask[(size_t)(i + b * P)] ? 1.f : 0.f;
+        }
+    // OTF_bp_omw = ifftshift(mask) .* conj(OTF) ./ (|OTF|.^2 + alpha) (:155-161), single
+    const int64_t ish[3] = {-(pd[0] / 2), -(pd[1] / 2), -(pd[2] / 2)};
+    std::vector<double> om = shift3(mfin, v, ish);
+    const float al = (float)alpha;
+    std::vector<cd> bp((size_t)P);
+    std::vector<std::complex<float>> omw((size_t)P);
+    for (int64_t i = 0; i < P; ++i) {
+        const std::complex<float> o = OTF[(size_t)i];
+        const float den = absO[(size_t)i] * absO[(size_t)i] + al;
+        const std::complex<float> wf(o.real() / den, -o.imag() / den);
+        const float mf = (float)om[(size_t)i];
+        omw[(size_t)i] = std::complex<float>(mf * wf.real(), mf * wf.imag());
+        bp[(size_t)i] = cd(omw[(size_t)i].real(), omw[(size_t)i].imag());
+    }
+    // b_omw = fftshift(real(ifftn(OTF_bp_omw))) (:163)
+    fft3_host(bp, pd, true);
+    std::vector<float> br((size_t)P);
+    for (int64_t i = 0; i < P; ++i) br[(size_t)i] = (float)bp[(size_t)i].real();
+    std::vector<float> bo = shift3(br, v, fsh);
+    std::memcpy(b_out, bo.data(), sizeof(float) * (size_t)P);
+    if (otf_bp_out) std::memcpy(otf_bp_out, omw.data(), sizeof(float) * 2 * (size_t)P);
+    if (abs_otf_c_out) std::memcpy(abs_otf_c_out, absOc.data(), sizeof(float) * (size_t)P);
+    if (mask_out) std::memcpy(mask_out, mcount.data(), sizeof(float) * (size_t)P);
+    if (skewed_used) *skewed_used = skewed ? 1 : 0;
+}
+
+}  // namespace pk

@@ -120,6 +120,16 @@ pk_status pk_decon_lucy_omw(pk_ctx* c, const pk_vol* I, const float* psf_f, cons
     });
 }
 
+pk_status pk_decon_batch(pk_ctx* c, int32_t method, const pk_batch_item* items, int32_t n, const float* psf_f,
+                         const int64_t f_dims[3], const float* psf_b, const int64_t b_dims[3], int32_t numit,
+                         const pk_opts* opts) {
+    return guarded(c, [&] {
+        if (!f_dims || (method == 1 && !b_dims)) pk::fail(PK_ERR_INVALID_ARG, "PSF dims are null");
+        if (method != 0 && method != 1) pk::fail(PK_ERR_INVALID_ARG, "method must be 0 (RL) or 1 (OMW)");
+        c->decon_batch(method, items, n, psf_f, f_dims, psf_b, b_dims, numit, opts);
+    });
+}
+
 pk_status pk_psf2otf(pk_ctx* c, const float* psf, const int64_t pd[3], const int64_t od[3], float* otf) {
     return guarded(c, [&] {
         if (!psf || !pd || !od || !otf) pk::fail(PK_ERR_INVALID_ARG, "null argument");
@@ -150,6 +160,33 @@ pk_status pk_psf2otf(pk_ctx* c, const float* psf, const int64_t pd[3], const int
                     otf[2 * o] = v.x;
                     otf[2 * o + 1] = v.y;
                 }
+    });
+}
+
+void pk_bp_opts_default(pk_bp_opts* o) {
+    if (!o) return;
+    o->otf_cum_thresh = 0.9;
+    o->otf_area_thresh = 100;
+    o->hann_win_lo = 0.8;
+    o->hann_win_hi = 1.0;
+    o->min_int_thresh = 1e-3;
+}
+
+pk_status pk_omw_backprojector(pk_ctx* c, const float* psf, const int64_t psf_dims[3], double alpha, int32_t skewed,
+                               const pk_bp_opts* opts, float* b_omw, float* otf_bp_omw, float* abs_otf_c,
+                               float* otf_mask, int32_t* skewed_used) {
+    return guarded(c, [&] {
+        if (!psf || !psf_dims || !b_omw) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        for (int d = 0; d < 3; ++d)
+            if (psf_dims[d] < 1) pk::fail(PK_ERR_EMPTY_PSF, "A valid nonzero PSF needs to be provided!");
+        pk_bp_opts o;
+        pk_bp_opts_default(&o);
+        if (opts) o = *opts;
+        if (!(o.hann_win_hi > o.hann_win_lo)) pk::fail(PK_ERR_INVALID_ARG, "hannWinBounds must be increasing");
+        const double thr[5] = {o.otf_cum_thresh, o.otf_area_thresh, o.hann_win_lo, o.hann_win_hi, o.min_int_thresh};
+        int used = 0;
+        c->omw_backprojector(psf, psf_dims, alpha, skewed, thr, b_omw, otf_bp_omw, abs_otf_c, otf_mask, &used);
+        if (skewed_used) *skewed_used = used;
     });
 }
 

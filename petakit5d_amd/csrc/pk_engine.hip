@@ -472,7 +472,7 @@ void pk_ctx_impl::reset() {
     rtw.clear();
     ws.I.release(); ws.J.release(); ws.A.release(); ws.J3.release(); ws.Y.release(); ws.J4.release();
     ws.partials.release(); ws.lambda.release(); ws.raw.release(); ws.out.release(); ws.mask.release();
-    ws.mask2.release(); ws.count.release(); ws.tmp.release();
+    ws.mask2.release(); ws.count.release(); ws.tmp.release(); ws.batch.release(); ws.batch_tab.release();
 }
 
 // ------------------------------------------------------------------ decon driver

@@ -70,6 +70,7 @@ struct OtfEntry {
 
 struct Workspace {
     DevBuf I, J, A, J3, Y, J4, partials, lambda, raw, out, mask, mask2, count, tmp;
+    DevBuf batch, batch_tab;  // batched decon: per-chunk arena, descriptor tables
 };
 
 // Per-pass timing with HIP events recorded on the engine's stream.
@@ -128,6 +129,13 @@ struct pk_ctx_impl {
     void decon(int method, const pk_vol* I, const float* psf_f, const int64_t fd[3], const float* psf_b,
                const int64_t bd[3], int numit, const pk_opts* o, pk_vol* out, double* err_mat);
     void reset();
+    void decon_batch(int method, const pk_batch_item* items, int n, const float* psf_f, const int64_t fd[3],
+                     const float* psf_b, const int64_t bd[3], int numit, const pk_opts* o);
+
+    // omw_backprojector_generation (host steps + GPU nearest-edge search).
+    // thr = {OTFCumThresh, OTFAreaThresh, hann lo, hann hi, minIntThrsh}.
+    void omw_backprojector(const float* psf, const int64_t pd[3], double alpha, int skewed, const double thr[5],
+                           float* b_omw, float* otf_bp_omw, float* abs_otf_c, float* otf_mask, int* skewed_used);
 };
 
 }  // namespace pk

@@ -53,7 +53,7 @@ __host__ __device__ constexpr int row_pitch(int L) { return L + (L >> 4) + 1; }
 // ------------------------------------------------------------------ one stage
 template <int E, int R, bool INV, bool ROWS>
 __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s, int p, float2* lds, int lbase,
-                                        int lstride, bool last) {
+                                        int lstride, bool last, bool act) {
     constexpr int Q = E / R;  // butterflies per thread
     const int P = pl.P;
     const int Ns = pl.Ns[s];
@@ -88,7 +88,7 @@ __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s
                 constexpr int q = decltype(qc)::value;
                 v[i + q * Q] = u[q];
             });
-        } else {
+        } else if (act) {
             const int ob = g * Ns * R + k;
             static_for<0, R>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
@@ -100,14 +100,16 @@ __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s
 
 #define PK_RADIX_CASE(RR)                                                    \
     case RR:                                                                 \
-        if constexpr (E % RR == 0) stage_r<E, RR, INV, ROWS>(v, pl, s, p, lds, lbase, lstride, last); \
+        if constexpr (E % RR == 0) stage_r<E, RR, INV, ROWS>(v, pl, s, p, lds, lbase, lstride, last, act); \
         break;
 
 // Full transform of the E home values of every thread (all threads of the
-// block must call it: it contains barriers).
+// block must call it: it contains barriers).  act = false marks a thread that
+// owns no tile slot (batched launches with a block larger than the chunk's
+// tile): it joins the barriers but never touches LDS.
 template <int E, bool INV, bool ROWS>
 __device__ __forceinline__ void fft_run(float2 (&v)[E], const FftPlan& pl, int p, float2* lds, int lbase,
-                                        int lstride) {
+                                        int lstride, bool act = true) {
     for (int s = 0; s < pl.nst; ++s) {
         const bool last = (s == pl.nst - 1);
         switch (pl.R[s]) {
@@ -135,7 +137,8 @@ __device__ __forceinline__ void fft_run(float2 (&v)[E], const FftPlan& pl, int p
         if (!last) {
             __syncthreads();
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = lds[lds_slot<ROWS>(lbase, p + e * pl.P, lstride)];
+            for (int e = 0; e < E; ++e)
+                if (act) v[e] = lds[lds_slot<ROWS>(lbase, p + e * pl.P, lstride)];
             __syncthreads();
         }
     }
@@ -207,7 +210,7 @@ __device__ __forceinline__ float2 ld_u16(const uint16_t* __restrict__ base, int6
 
 // One x pass, specialised on its mode: [C2R] -> elementwise -> [R2C].
 template <int E, int MODE>
-__device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, float2* lds) {
+__device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, float2* lds, int64_t blk) {
     constexpr bool IN = (MODE == XM_C2R || MODE == XM_RATIO || MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
     constexpr bool OUT_ALWAYS = (MODE == XM_R2C || MODE == XM_PREP_U16 || MODE == XM_PREP_F32 ||
                                  MODE == XM_RATIO || MODE == XM_BIGGS_Y);
@@ -216,9 +219,10 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
     const int P = pl.P;
     const int rr = t / P;
     const int p = t - rr * P;
-    const int64_t row = (int64_t)blockIdx.x * a.RT + rr;
-    const bool valid = (rr < a.RT) && (row < a.nrows);
-    const int lbase = rr * a.lpad;
+    const int64_t row = blk * a.RT + rr;
+    const bool act = rr < a.RT;  // owns an LDS row slot
+    const bool valid = act && (row < a.nrows);
+    const int lbase = act ? rr * a.lpad : 0;
     const int L = pl.L;
     const bool packed = a.packed != 0;
     const int64_t srow = row * a.hp;  // spectrum row start
@@ -274,7 +278,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
                 v[e] = z;
             }
         }
-        fft_run<E, true, true>(v, pl, p, lds, lbase, 1);
+        fft_run<E, true, true>(v, pl, p, lds, lbase, 1, act);
     }
 
     // ---- elementwise step in real space
@@ -331,16 +335,18 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
     bool out_spec = OUT_ALWAYS;
     if constexpr (OUT_MAYBE) out_spec = a.r2c_out != 0;
     if (out_spec) {
-        fft_run<E, false, true>(v, pl, p, lds, lbase, 1);
+        fft_run<E, false, true>(v, pl, p, lds, lbase, 1, act);
         if (packed) {
             const int M = L;
             // Z at home positions -> LDS, then X[k] = 1/2[(Z_k + Z*_{M-k}) - i W^k (Z_k - Z*_{M-k})]
 #pragma unroll
-            for (int e = 0; e < E; ++e) lds[lds_slot<true>(lbase, p + e * P, 1)] = v[e];
+            for (int e = 0; e < E; ++e)
+                if (act) lds[lds_slot<true>(lbase, p + e * P, 1)] = v[e];
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = p + e * P;
+                if (!act) continue;
                 const float2 A = lds[lds_slot<true>(lbase, k, 1)];
                 const float2 B = cconj(lds[lds_slot<true>(lbase, k == 0 ? 0 : M - k, 1)]);
                 const float2 s = cadd(A, B);
@@ -350,7 +356,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
                 if (valid) a.spec[srow + k] = X;
             }
             if (valid && p == 0) {
-                const float2 z0 = lds[lbase];
+                const float2 z0 = lds[lbase];  // slot of position 0
                 a.spec[srow + M] = make_float2(z0.x - z0.y, 0.f);
             }
             __syncthreads();
@@ -379,8 +385,8 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
                 __syncthreads();
             }
             if (t == 0) {
-                a.partials[2 * blockIdx.x] = red[0];
-                a.partials[2 * blockIdx.x + 1] = red[1];
+                a.partials[2 * blk] = red[0];
+                a.partials[2 * blk + 1] = red[1];
             }
         }
     }
@@ -390,7 +396,36 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
 template <int E, int MODE>
 __global__ void __launch_bounds__(kMaxThreads) k_xpass(XArgs a, FftPlan pl) {
     extern __shared__ float2 lds[];
-    xpass_body<E, MODE>(a, pl, lds);
+    xpass_body<E, MODE>(a, pl, lds, blockIdx.x);
+}
+
+// ------------------------------------------------------------------ batched launches
+// Many volumes ("chunks") of different sizes in one launch: block b belongs to
+// the chunk c with starts[c] <= b < starts[c+1]; its arguments and transform
+// plan come from the chunk's descriptor (block-uniform loads).
+struct XChunk {
+    XArgs a;
+    FftPlan pl;
+};
+
+__device__ __forceinline__ int chunk_of(const int64_t* __restrict__ starts, int n, int64_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int E, int MODE>
+__global__ void __launch_bounds__(kMaxThreads) k_xpass_batch(const XChunk* __restrict__ ch,
+                                                             const int64_t* __restrict__ starts, int n) {
+    extern __shared__ float2 lds[];
+    const int c = chunk_of(starts, n, blockIdx.x);
+    const XArgs a = ch[c].a;
+    const FftPlan pl = ch[c].pl;
+    xpass_body<E, MODE>(a, pl, lds, (int64_t)blockIdx.x - starts[c]);
 }
 
 // ------------------------------------------------------------------ strided axes
@@ -409,40 +444,48 @@ struct ColArgs {
     float scale;         // applied with the OTF product
 };
 
+struct ColChunk {
+    ColArgs a;
+    FftPlan pl;
+    int64_t gx;  // blocks along x of the chunk's own grid
+};
+
 template <int E, int MODE>
-__device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl, float2* lds) {
+__device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl, float2* lds, int64_t bx,
+                                             int64_t by) {
     const int t = threadIdx.x;
     const int T = a.T;
     const int p = t / T;
     const int c = t - p * T;
     const int P = pl.P;
-    const int64_t col = (int64_t)blockIdx.x * T + c;
-    const bool valid = (p < P) && (col < a.acount) && (a.amod == 0 || col % a.amod < a.alim);
+    const bool act = p < P;  // owns an LDS slot
+    const int64_t col = bx * T + c;
+    const bool valid = act && (col < a.acount) && (a.amod == 0 || col % a.amod < a.alim);
     // per-lane base pointer plus a block-uniform step: addresses are
     // recomputed from (ptr0, e*step) instead of held live between load and store
     const int64_t step = (int64_t)P * a.stride;
-    const int64_t off0 = col + (int64_t)blockIdx.y * a.bstride + (int64_t)p * a.stride;
+    const int64_t off0 = col + by * a.bstride + (int64_t)p * a.stride;
     float2* __restrict__ ptr0 = a.data + (valid ? off0 : 0);
     float2 v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? ptr0[e * step] : make_float2(0.f, 0.f);
     if constexpr (MODE == CM_INV) {
-        fft_run<E, true, false>(v, pl, p, lds, c, T);
+        fft_run<E, true, false>(v, pl, p, lds, c, T, act);
     } else if constexpr (MODE == CM_FWD) {
-        fft_run<E, false, false>(v, pl, p, lds, c, T);
+        fft_run<E, false, false>(v, pl, p, lds, c, T, act);
     } else {
         // OTF values prefetched before the forward transform
         const float2* __restrict__ h0 = a.otf + (valid ? off0 : 0);
         float2 h[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) h[e] = valid ? h0[e * step] : make_float2(0.f, 0.f);
-        fft_run<E, false, false>(v, pl, p, lds, c, T);
+        fft_run<E, false, false>(v, pl, p, lds, c, T, act);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const float2 hs = make_float2(h[e].x * a.scale, h[e].y * a.scale);
             v[e] = (MODE == CM_CONVJ) ? cmulc(v[e], hs) : cmul(v[e], hs);
         }
-        fft_run<E, true, false>(v, pl, p, lds, c, T);
+        fft_run<E, true, false>(v, pl, p, lds, c, T, act);
     }
     if (valid) {
 #pragma unroll
@@ -453,7 +496,18 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
 template <int E, int MODE>
 __global__ void __launch_bounds__(kMaxThreads) k_colpass(ColArgs a, FftPlan pl) {
     extern __shared__ float2 lds[];
-    colpass_body<E, MODE>(a, pl, lds);
+    colpass_body<E, MODE>(a, pl, lds, blockIdx.x, blockIdx.y);
+}
+
+template <int E, int MODE>
+__global__ void __launch_bounds__(kMaxThreads) k_colpass_batch(const ColChunk* __restrict__ ch,
+                                                               const int64_t* __restrict__ starts, int n) {
+    extern __shared__ float2 lds[];
+    const int c = chunk_of(starts, n, blockIdx.x);
+    const ColArgs a = ch[c].a;
+    const FftPlan pl = ch[c].pl;
+    const int64_t lb = (int64_t)blockIdx.x - starts[c];
+    colpass_body<E, MODE>(a, pl, lds, lb % ch[c].gx, lb / ch[c].gx);
 }
 
 }  // namespace pk

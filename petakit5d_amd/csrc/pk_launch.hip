@@ -4,14 +4,23 @@
 
 namespace pk {
 bool launch_x_part_a(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_xb_part_a(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch, const int64_t* starts, int n);
 bool launch_x_part_b(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_xb_part_b(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch, const int64_t* starts, int n);
 bool launch_x_part_c(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_xb_part_c(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch, const int64_t* starts, int n);
 bool launch_x_part_d(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_xb_part_d(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch, const int64_t* starts, int n);
 bool launch_x_part_e(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_xb_part_e(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch, const int64_t* starts, int n);
 bool launch_x_part_f(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+bool launch_xb_part_f(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch, const int64_t* starts, int n);
 bool launch_col_part_a(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
+bool launch_colb_part_a(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch, const int64_t* starts, int n);
 bool launch_col_part_b(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
+bool launch_colb_part_b(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch, const int64_t* starts, int n);
 bool launch_col_part_c(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
+bool launch_colb_part_c(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch, const int64_t* starts, int n);
 
 hipError_t launch_xpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a) {
     const bool ok = launch_x_part_a(pl, grid, block, lds, s, a) ||
@@ -28,6 +37,27 @@ hipError_t launch_colpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, 
     const bool ok = launch_col_part_a(pl, grid, block, lds, s, a) ||
            launch_col_part_b(pl, grid, block, lds, s, a) ||
            launch_col_part_c(pl, grid, block, lds, s, a);
+    if (!ok) return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_xpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                              const int64_t* starts, int n) {
+    const bool ok = launch_xb_part_a(E, grid, block, lds, s, mode, ch, starts, n) ||
+           launch_xb_part_b(E, grid, block, lds, s, mode, ch, starts, n) ||
+           launch_xb_part_c(E, grid, block, lds, s, mode, ch, starts, n) ||
+           launch_xb_part_d(E, grid, block, lds, s, mode, ch, starts, n) ||
+           launch_xb_part_e(E, grid, block, lds, s, mode, ch, starts, n) ||
+           launch_xb_part_f(E, grid, block, lds, s, mode, ch, starts, n);
+    if (!ok) return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_colpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch,
+                                const int64_t* starts, int n) {
+    const bool ok = launch_colb_part_a(E, grid, block, lds, s, mode, ch, starts, n) ||
+           launch_colb_part_b(E, grid, block, lds, s, mode, ch, starts, n) ||
+           launch_colb_part_c(E, grid, block, lds, s, mode, ch, starts, n);
     if (!ok) return hipErrorInvalidValue;
     return hipGetLastError();
 }

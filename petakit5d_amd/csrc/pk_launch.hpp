@@ -36,6 +36,35 @@ inline void launch_modes_col(dim3 grid, dim3 block, size_t lds, hipStream_t s, c
 }
 
 hipError_t launch_xpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
+hipError_t launch_xpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                              const int64_t* starts, int n);
+hipError_t launch_colpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch,
+                                const int64_t* starts, int n);
 hipError_t launch_colpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
 bool e_supported(int E);
+template <int E>
+inline void launch_modes_xb(dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                            const int64_t* starts, int n) {
+    switch (mode) {
+        case XM_R2C: hipLaunchKernelGGL((k_xpass_batch<E, XM_R2C>), grid, block, lds, s, ch, starts, n); break;
+        case XM_PREP_U16: hipLaunchKernelGGL((k_xpass_batch<E, XM_PREP_U16>), grid, block, lds, s, ch, starts, n); break;
+        case XM_PREP_F32: hipLaunchKernelGGL((k_xpass_batch<E, XM_PREP_F32>), grid, block, lds, s, ch, starts, n); break;
+        case XM_RATIO: hipLaunchKernelGGL((k_xpass_batch<E, XM_RATIO>), grid, block, lds, s, ch, starts, n); break;
+        case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xpass_batch<E, XM_OMW_UPDATE>), grid, block, lds, s, ch, starts, n); break;
+        case XM_BIGGS_Y: hipLaunchKernelGGL((k_xpass_batch<E, XM_BIGGS_Y>), grid, block, lds, s, ch, starts, n); break;
+        case XM_BIGGS_UPDATE: hipLaunchKernelGGL((k_xpass_batch<E, XM_BIGGS_UPDATE>), grid, block, lds, s, ch, starts, n); break;
+        default: break;
+    }
+}
+template <int E>
+inline void launch_modes_colb(dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch,
+                              const int64_t* starts, int n) {
+    switch (mode) {
+        case CM_FWD: hipLaunchKernelGGL((k_colpass_batch<E, CM_FWD>), grid, block, lds, s, ch, starts, n); break;
+        case CM_INV: hipLaunchKernelGGL((k_colpass_batch<E, CM_INV>), grid, block, lds, s, ch, starts, n); break;
+        case CM_CONV: hipLaunchKernelGGL((k_colpass_batch<E, CM_CONV>), grid, block, lds, s, ch, starts, n); break;
+        case CM_CONVJ: hipLaunchKernelGGL((k_colpass_batch<E, CM_CONVJ>), grid, block, lds, s, ch, starts, n); break;
+        default: break;
+    }
+}
 }  // namespace pk

@@ -1,0 +1,91 @@
+/* Test harness: a tiny in-memory implementation of the mx API declared in
+ * mex.h, plus a driver that calls mexFunction and reports errors. */
+#include <setjmp.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mex.h"
+
+struct mxArray_tag {
+    mxClassID cls;
+    mwSize nd, dims[3];
+    void *data;
+    char *str;
+    const char *fname[8];
+    mxArray *fval[8];
+    int nf;
+};
+
+static jmp_buf g_err;
+static char g_errid[128];
+
+static size_t esz(mxClassID c) { return c == mxDOUBLE_CLASS ? 8 : c == mxSINGLE_CLASS ? 4 : c == mxUINT16_CLASS ? 2 : 1; }
+mwSize mxGetNumberOfDimensions(const mxArray *a) { return a->nd; }
+const mwSize *mxGetDimensions(const mxArray *a) { return a->dims; }
+size_t mxGetNumberOfElements(const mxArray *a) { size_t n = 1; for (mwSize i = 0; i < a->nd; ++i) n *= a->dims[i]; return n; }
+mxClassID mxGetClassID(const mxArray *a) { return a->cls; }
+int mxIsStruct(const mxArray *a) { return a && a->cls == mxSTRUCT_CLASS; }
+int mxIsChar(const mxArray *a) { return a->cls == mxCHAR_CLASS; }
+int mxIsSingle(const mxArray *a) { return a->cls == mxSINGLE_CLASS; }
+int mxIsEmpty(const mxArray *a) { return mxGetNumberOfElements(a) == 0; }
+mxArray *mxGetField(const mxArray *a, size_t i, const char *name) {
+    (void)i;
+    for (int k = 0; k < a->nf; ++k) if (!strcmp(a->fname[k], name)) return a->fval[k];
+    return NULL;
+}
+double mxGetScalar(const mxArray *a) { return a->cls == mxDOUBLE_CLASS ? ((double *)a->data)[0] : 0.0; }
+double *mxGetPr(const mxArray *a) { return (double *)a->data; }
+void *mxGetData(const mxArray *a) { return a->data; }
+char *mxArrayToString(const mxArray *a) { return strdup(a->str); }
+void *mxMalloc(size_t n) { return malloc(n); }
+void mxFree(void *p) { free(p); }
+mxArray *mxCreateNumericArray(mwSize nd, const mwSize *dims, mxClassID c, mxComplexity cx) {
+    mxArray *a = calloc(1, sizeof(*a));
+    a->cls = c; a->nd = nd;
+    for (mwSize i = 0; i < nd; ++i) a->dims[i] = dims[i];
+    a->data = calloc(mxGetNumberOfElements(a) ? mxGetNumberOfElements(a) : 1, esz(c) * (cx == mxCOMPLEX ? 2 : 1));
+    return a;
+}
+mxArray *mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity cx) { mwSize d[2] = {m, n}; return mxCreateNumericArray(2, d, mxDOUBLE_CLASS, cx); }
+mxArray *mxCreateDoubleScalar(double v) { mxArray *a = mxCreateDoubleMatrix(1, 1, mxREAL); ((double *)a->data)[0] = v; return a; }
+void mxDestroyArray(mxArray *a) { if (a) { free(a->data); free(a); } }
+void mexErrMsgIdAndTxt(const char *id, const char *fmt, ...) {
+    (void)fmt;
+    snprintf(g_errid, sizeof(g_errid), "%s", id);
+    longjmp(g_err, 1);
+}
+int mexAtExit(void (*fn)(void)) { (void)fn; return 0; }
+
+void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]);
+
+static mxArray *str(const char *s) { mxArray *a = calloc(1, sizeof(*a)); a->cls = mxCHAR_CLASS; a->nd = 2; a->dims[0] = 1; a->dims[1] = strlen(s); a->str = strdup(s); return a; }
+static mxArray *dbl(double v) { return mxCreateDoubleScalar(v); }
+
+static int call(int nlhs, mxArray **plhs, int nrhs, const mxArray **prhs) {
+    if (setjmp(g_err)) { printf("error %s\n", g_errid); return 1; }
+    mexFunction(nlhs, plhs, nrhs, prhs);
+    return 0;
+}
+
+int main(void) {
+    mxArray *out[4] = {0};
+    const mxArray *a1[] = {str("fgfn"), dbl(101), dbl(1), dbl(0), dbl(7)};
+    if (!call(1, out, 5, a1)) printf("fgfn %g\n", ((double *)out[0]->data)[0]);
+    const mxArray *a2[] = {str("fgfn"), dbl(257), dbl(1), dbl(0), dbl(5)};
+    if (!call(1, out, 5, a2)) printf("fgfn %g\n", ((double *)out[0]->data)[0]);
+    const mxArray *a3[] = {str("nosuchop")};
+    call(0, out, 1, a3);
+    mwSize d[3] = {8, 8, 4};
+    mxArray *I = mxCreateNumericArray(3, d, mxDOUBLE_CLASS, mxREAL);
+    mwSize pdm[3] = {3, 3, 3};
+    mxArray *P = mxCreateNumericArray(3, pdm, mxSINGLE_CLASS, mxREAL);
+    mxArray *s = calloc(1, sizeof(*s)); s->cls = mxSTRUCT_CLASS; s->nd = 2; s->dims[0] = s->dims[1] = 1;
+    const mxArray *a4[] = {str("rl"), I, P, dbl(2), s};
+    call(1, out, 5, a4);  /* double input -> unsupportedType */
+    mxArray *I16 = mxCreateNumericArray(3, d, mxUINT16_CLASS, mxREAL);
+    const mxArray *a5[] = {str("rl"), I16, P, dbl(2), s};
+    if (!call(2, out, 5, a5)) printf("rl ok %zu %zu %zu\n", out[0]->dims[0], out[0]->dims[1], out[0]->dims[2]);
+    return 0;
+}
