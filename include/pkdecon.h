@@ -123,6 +123,56 @@ pk_status pk_decon_batch(pk_ctx *ctx, int32_t method, const pk_batch_item *items
                          const int64_t f_dims[3], const float *psf_b, const int64_t b_dims[3], int32_t numit,
                          const pk_opts *opts);
 
+/* ---- Serving: a request loop on one device (no counterpart in the reference;
+ * its nearest analogue is the per-file loop of XR_decon_data_wrapper.m:332-511).
+ * Continuous batching: requests are admitted FIFO as memory under the HBM
+ * budget given to pk_ctx_create allows, every admitted request advances one
+ * iteration per step in batched launches shared with the others, and a
+ * request's result is identical alone and in any mix.  A request larger than
+ * the budget fails with PK_ERR_BUDGET once nothing else is running.  The
+ * caller's buffers must stay valid until the request is DONE, FAILED or
+ * CANCELLED.  bbox and EdgeErosion are not accepted for served requests. */
+typedef struct {
+    int32_t method;          /* 0 = decon_lucy_function (Biggs RL), 1 = decon_lucy_omw_function */
+    int32_t numit;
+    pk_vol in;
+    pk_vol out;              /* same size as in; class per opts.save16bit */
+    const float *psf_f;      /* copied at submit */
+    int64_t f_dims[3];
+    const float *psf_b;      /* method 1 only; copied at submit */
+    int64_t b_dims[3];
+    pk_opts opts;
+} pk_request;
+
+typedef enum { PK_REQ_QUEUED = 0, PK_REQ_RUNNING = 1, PK_REQ_DONE = 2, PK_REQ_FAILED = 3, PK_REQ_CANCELLED = 4 } pk_req_state;
+
+typedef struct {
+    int32_t state;            /* pk_req_state */
+    int32_t status;           /* pk_status of a failed request */
+    int32_t iterations_done;
+    int32_t reserved;
+    double latency_ms;        /* submit -> output ready (or -> now while pending) */
+    double queue_ms;          /* submit -> admission */
+} pk_req_info;
+
+typedef struct {
+    uint64_t submitted, completed, failed;
+    int32_t active, queued;
+    double voxels;            /* voxels of completed requests */
+} pk_serve_stats;
+
+pk_status pk_serve_start(pk_ctx *ctx, int32_t max_active);
+/* Stops after the requests in flight finish; queued requests are cancelled. */
+pk_status pk_serve_stop(pk_ctx *ctx);
+pk_status pk_serve_submit(pk_ctx *ctx, const pk_request *req, uint64_t *id);
+/* wait_ms: 0 = poll, < 0 = wait until finished, > 0 = wait at most that long. */
+pk_status pk_serve_wait(pk_ctx *ctx, uint64_t id, double wait_ms, pk_req_info *info);
+/* Cancels a queued request (PK_ERR_BUSY once it runs). */
+pk_status pk_serve_cancel(pk_ctx *ctx, uint64_t id);
+/* Forgets a finished request. */
+pk_status pk_serve_release(pk_ctx *ctx, uint64_t id);
+pk_status pk_serve_stats_get(pk_ctx *ctx, pk_serve_stats *out);
+
 /* otf = decon_psf2otf(psf, outSize) (microscopeDataProcessing/decon/decon_psf2otf.m:1-46).
  * otf: host buffer of prod(out_dims) complex single values, interleaved
  * (re, im), the full spectrum as MATLAB returns it. */

@@ -7,6 +7,7 @@ reference's MATLAB call signatures.
 from ._lib import Context, PkDeconError, default_context, load  # noqa: F401
 from .backprojector import omw_backprojector_generation  # noqa: F401
 from .batch import decon_batch  # noqa: F401
+from .serve import Server  # noqa: F401
 from .decon import (  # noqa: F401
     crop_psf_to_data,
     decon_lucy_function,
@@ -28,6 +29,7 @@ __all__ = [
     "findGoodFactorNumber",
     "omw_backprojector_generation",
     "decon_batch",
+    "Server",
     "crop_psf_to_data",
     "fft_r2c",
     "fft_c2r",

@@ -44,6 +44,13 @@ EXPORTS = [
     "pk_decon_lucy",
     "pk_decon_lucy_omw",
     "pk_decon_batch",
+    "pk_serve_start",
+    "pk_serve_stop",
+    "pk_serve_submit",
+    "pk_serve_wait",
+    "pk_serve_cancel",
+    "pk_serve_release",
+    "pk_serve_stats_get",
     "pk_psf2otf",
     "pk_bp_opts_default",
     "pk_omw_backprojector",

@@ -35,6 +35,23 @@ pk_status guarded(pk_ctx* c, Fn&& fn) {
 }
 }  // namespace
 
+// Serving entry points take the request-queue lock only, never the context
+// lock the scheduler holds while it runs a step.
+template <class Fn>
+static pk_status serve_guarded(pk_ctx* c, Fn&& fn) {
+    if (!c) return PK_ERR_INVALID_ARG;
+    try {
+        fn();
+        return PK_OK;
+    } catch (const pk::Error& e) {
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        c->last_error = e.msg;
+        return e.st;
+    } catch (...) {
+        return PK_ERR_INVALID_ARG;
+    }
+}
+
 extern "C" {
 
 void pk_opts_default(pk_opts* o) {
@@ -75,6 +92,7 @@ pk_status pk_ctx_create(int device, uint64_t hbm_budget_bytes, pk_ctx** out) {
 
 void pk_ctx_destroy(pk_ctx* c) {
     if (!c) return;
+    pk::serve_stop(c);  // joins the scheduler (which takes the context lock itself)
     {
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         (void)hipSetDevice(c->device);
@@ -240,6 +258,48 @@ int32_t pk_profile_read(pk_ctx* c, pk_prof_entry* out, int32_t max) {
         ++n;
     }
     return n;
+}
+
+pk_status pk_serve_start(pk_ctx* c, int32_t max_active) {
+    return guarded(c, [&] { pk::serve_start(c, max_active); });
+}
+
+pk_status pk_serve_stop(pk_ctx* c) {
+    return serve_guarded(c, [&] { pk::serve_stop(c); });
+}
+
+pk_status pk_serve_submit(pk_ctx* c, const pk_request* req, uint64_t* id) {
+    return serve_guarded(c, [&] {
+        if (!req || !id) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        *id = pk::serve_submit(c, *req);
+    });
+}
+
+pk_status pk_serve_wait(pk_ctx* c, uint64_t id, double wait_ms, pk_req_info* info) {
+    return serve_guarded(c, [&] {
+        std::string err;
+        pk_req_info tmp{};
+        pk::serve_poll(c, id, info ? info : &tmp, wait_ms, &err);
+        if (!err.empty()) {
+            std::lock_guard<std::recursive_mutex> lk(c->mu);
+            c->last_error = err;
+        }
+    });
+}
+
+pk_status pk_serve_cancel(pk_ctx* c, uint64_t id) {
+    return serve_guarded(c, [&] { pk::serve_cancel(c, id); });
+}
+
+pk_status pk_serve_release(pk_ctx* c, uint64_t id) {
+    return serve_guarded(c, [&] { pk::serve_release(c, id); });
+}
+
+pk_status pk_serve_stats_get(pk_ctx* c, pk_serve_stats* out) {
+    return serve_guarded(c, [&] {
+        if (!out) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        *out = pk::serve_stats(c);
+    });
 }
 
 int64_t pk_find_good_factor_number(int64_t n, int32_t dir, int32_t allow_odd_seven, int32_t max_prime) {

@@ -413,10 +413,12 @@ const float2* pk_ctx_impl::otf_for(const VolPlan& vp, const float* psf, const in
     }
     if (pd[0] > vp.n1 || pd[1] > vp.n2 || pd[2] > vp.n3)
         fail(PK_ERR_OUT_SIZE, "outSizeIsSmallerThanPsfSize: PSF larger than the volume");
-    // evict least recently used
+    // evict least recently used among the unpinned entries
     while (otf.size() >= otf_max_entries) {
-        auto lru = std::min_element(otf.begin(), otf.end(),
-                                    [](const OtfEntry& a, const OtfEntry& b) { return a.tick < b.tick; });
+        auto lru = otf.end();
+        for (auto e = otf.begin(); e != otf.end(); ++e)
+            if (e->pins == 0 && (lru == otf.end() || e->tick < lru->tick)) lru = e;
+        if (lru == otf.end()) break;  // all in use: grow
         otf.erase(lru);
     }
     // PSF ./ sum(PSF(:)): sum accumulated in double, divided in single
@@ -438,6 +440,14 @@ const float2* pk_ctx_impl::otf_for(const VolPlan& vp, const float* psf, const in
     build_otf(vp, q.data(), pd, ent.spec->as<float2>());
     otf.push_back(std::move(ent));
     return otf.back().spec->as<float2>();
+}
+
+void pk_ctx_impl::otf_pin(const float2* spec, int delta) {
+    for (auto& e : otf)
+        if (e.spec->as<float2>() == spec) {
+            e.pins += delta;
+            return;
+        }
 }
 
 void pk_ctx_impl::build_otf(const VolPlan& vp, const float* q, const int64_t pd[3], float2* dst) {

@@ -65,8 +65,40 @@ struct OtfEntry {
     int64_t pdims[3];
     uint64_t hash;
     uint64_t tick;
+    int pins = 0;  // volumes in flight that read this OTF: never evicted while > 0
     std::unique_ptr<DevBuf> spec;
 };
+
+// One volume in flight in the step engine (batched or served).
+struct Chunk {
+    int method = 1, numit = 0, k = 0, par = 0;  // k: iterations done; par: Biggs buffer parity
+    pk_opts o{};
+    const VolPlan* vp = nullptr;
+    int64_t d[3] = {0, 0, 0};  // caller's size (the plan may be padded)
+    const float2* Hf = nullptr;
+    const float2* Hb = nullptr;
+    float *I = nullptr, *J = nullptr, *J3 = nullptr, *Y = nullptr, *J4 = nullptr, *lam = nullptr;
+    float2* A = nullptr;
+    double* part = nullptr;
+    const uint16_t* raw16 = nullptr;
+    const float* rawf = nullptr;
+    const void* raw_user = nullptr;
+    bool u16 = true, in_host = false;
+    void* dout = nullptr;
+    void* out_host = nullptr;
+    size_t out_bytes = 0;
+    bool done() const;
+};
+
+struct Server;
+// request loop (pk_serve.hip)
+void serve_start(struct pk_ctx_impl* ctx, int max_active);
+void serve_stop(struct pk_ctx_impl* ctx);
+uint64_t serve_submit(struct pk_ctx_impl* ctx, const pk_request& q);
+void serve_poll(struct pk_ctx_impl* ctx, uint64_t id, pk_req_info* info, double wait_ms, std::string* err);
+void serve_cancel(struct pk_ctx_impl* ctx, uint64_t id);
+void serve_release(struct pk_ctx_impl* ctx, uint64_t id);
+pk_serve_stats serve_stats(struct pk_ctx_impl* ctx);
 
 struct Workspace {
     DevBuf I, J, A, J3, Y, J4, partials, lambda, raw, out, mask, mask2, count, tmp;
@@ -131,6 +163,17 @@ struct pk_ctx_impl {
     void reset();
     void decon_batch(int method, const pk_batch_item* items, int n, const float* psf_f, const int64_t fd[3],
                      const float* psf_b, const int64_t bd[3], int numit, const pk_opts* o);
+    // step engine
+    void chunk_validate(const pk_vol& in, const pk_vol& out, const pk_opts& o);
+    size_t chunk_prepare(Chunk& c, const pk_vol& in, const pk_vol& out, const float* psf_f, const int64_t fd[3],
+                         const float* psf_b, const int64_t bd[3]);
+    void chunk_place(Chunk& c, char* base);
+    void run_step(std::vector<Chunk*>& act);
+    void chunk_finish(Chunk& c);
+    void otf_pin(const float2* spec, int delta);
+    std::unique_ptr<Server> srv;
+    pk_ctx_impl();  // defined where Server is complete (pk_serve.hip)
+    ~pk_ctx_impl();
 
     // omw_backprojector_generation (host steps + GPU nearest-edge search).
     // thr = {OTFCumThresh, OTFAreaThresh, hann lo, hann hi, minIntThrsh}.

@@ -1,0 +1,138 @@
+"""Request loop serving decon jobs on one device (continuous batching under an
+HBM budget) -- Python front end of pk_serve_* in include/pkdecon.h."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import PkDeconError, PkOpts, PkVol
+from .batch import _vol_of
+from .decon import _opts_struct, _parse, _psf_array
+
+QUEUED, RUNNING, DONE, FAILED, CANCELLED = range(5)
+
+
+class PkRequest(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int32),
+        ("numit", ctypes.c_int32),
+        ("inp", PkVol),
+        ("out", PkVol),
+        ("psf_f", ctypes.c_void_p),
+        ("f_dims", ctypes.c_int64 * 3),
+        ("psf_b", ctypes.c_void_p),
+        ("b_dims", ctypes.c_int64 * 3),
+        ("opts", PkOpts),
+    ]
+
+
+class PkReqInfo(ctypes.Structure):
+    _fields_ = [
+        ("state", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+        ("iterations_done", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("latency_ms", ctypes.c_double),
+        ("queue_ms", ctypes.c_double),
+    ]
+
+
+class PkServeStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64),
+        ("active", ctypes.c_int32),
+        ("queued", ctypes.c_int32),
+        ("voxels", ctypes.c_double),
+    ]
+
+
+def _bind(L):
+    vp = ctypes.c_void_p
+    if getattr(L, "_pk_serve_bound", False):
+        return L
+    L.pk_serve_start.argtypes = [vp, ctypes.c_int32]
+    L.pk_serve_stop.argtypes = [vp]
+    L.pk_serve_submit.argtypes = [vp, ctypes.POINTER(PkRequest), ctypes.POINTER(ctypes.c_uint64)]
+    L.pk_serve_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(PkReqInfo)]
+    L.pk_serve_cancel.argtypes = [vp, ctypes.c_uint64]
+    L.pk_serve_release.argtypes = [vp, ctypes.c_uint64]
+    L.pk_serve_stats_get.argtypes = [vp, ctypes.POINTER(PkServeStats)]
+    for n in ("pk_serve_start", "pk_serve_stop", "pk_serve_submit", "pk_serve_wait", "pk_serve_cancel",
+              "pk_serve_release", "pk_serve_stats_get"):
+        getattr(L, n).restype = ctypes.c_int
+    L._pk_serve_bound = True
+    return L
+
+
+class Request:
+    """A submitted job; keeps its input, output and PSF buffers alive."""
+
+    def __init__(self, server, rid, keep, out):
+        self.server, self.id, self._keep, self.out = server, rid, keep, out
+
+    def info(self, wait_ms=0.0) -> dict:
+        i = PkReqInfo()
+        self.server.ctx.check(self.server.L.pk_serve_wait(self.server.ctx.handle, self.id, float(wait_ms),
+                                                          ctypes.byref(i)))
+        return {"state": i.state, "status": i.status, "iterations_done": i.iterations_done,
+                "latency_ms": i.latency_ms, "queue_ms": i.queue_ms}
+
+    def result(self, timeout_s=None):
+        """Wait for the job; returns the deconvolved volume or raises PkDeconError."""
+        inf = self.info(-1.0 if timeout_s is None else timeout_s * 1e3)
+        if inf["state"] == DONE:
+            return self.out
+        if inf["state"] in (FAILED, CANCELLED):
+            raise PkDeconError(inf["status"], _lib.load().pk_last_error(self.server.ctx.handle).decode())
+        raise TimeoutError(f"request {self.id} not finished")
+
+    def cancel(self):
+        self.server.ctx.check(self.server.L.pk_serve_cancel(self.server.ctx.handle, self.id))
+
+
+class Server:
+    """Continuous-batching decon server on one device.  The context's HBM
+    budget (pk_ctx_create) bounds the device memory of requests in flight."""
+
+    def __init__(self, ctx=None, max_active: int = 64):
+        self.ctx = ctx or _lib.default_context()
+        self.L = _bind(_lib.load())
+        self.ctx.check(self.L.pk_serve_start(self.ctx.handle, int(max_active)))
+
+    def submit(self, I, PSF_f, PSF_b=None, NUMIT=2, method="omw", **kwargs) -> Request:
+        o = _parse(kwargs)
+        m = {"omw": 1, "rl": 0, "simplified": 0}[method]
+        opts, _ = _opts_struct(o, (1, 1, 1), False)
+        obj, vin, kind = _vol_of(I)
+        d = tuple(vin.dims)
+        if kind == "torch":
+            import torch
+            out = torch.empty((d[2], d[1], d[0]), dtype=torch.uint16 if o["save16bit"] else torch.float32,
+                              device=obj.device).permute(2, 1, 0)
+            torch.cuda.synchronize()
+            vout = PkVol(ctypes.c_void_p(out.data_ptr()), 1 if o["save16bit"] else 2,
+                         _lib.PK_MEM_DEVICE, (ctypes.c_int64 * 3)(*d))
+        else:
+            out = np.empty(d, dtype=np.uint16 if o["save16bit"] else np.float32, order="F")
+            vout = PkVol(ctypes.c_void_p(out.ctypes.data), 1 if o["save16bit"] else 2, _lib.PK_MEM_HOST,
+                         (ctypes.c_int64 * 3)(*d))
+        pf, fd = _psf_array(PSF_f)
+        pb, bd = _psf_array(PSF_b if m == 1 else PSF_f)
+        q = PkRequest(m, int(NUMIT), vin, vout, ctypes.c_void_p(pf.ctypes.data), (ctypes.c_int64 * 3)(*fd),
+                      ctypes.c_void_p(pb.ctypes.data), (ctypes.c_int64 * 3)(*bd), opts)
+        rid = ctypes.c_uint64()
+        self.ctx.check(self.L.pk_serve_submit(self.ctx.handle, ctypes.byref(q), ctypes.byref(rid)))
+        return Request(self, rid.value, (obj, pf, pb), out)
+
+    def stats(self) -> dict:
+        s = PkServeStats()
+        self.ctx.check(self.L.pk_serve_stats_get(self.ctx.handle, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in PkServeStats._fields_}
+
+    def stop(self):
+        if self.ctx is not None and self.ctx.handle:
+            self.ctx.check(self.L.pk_serve_stop(self.ctx.handle))

@@ -1,0 +1,80 @@
+"""Serving: continuous batching under an HBM budget.  A request's result is
+the same alone and in the mix; a request larger than the budget is rejected
+while the others finish; queued requests can be cancelled."""
+import numpy as np
+import pytest
+
+from petakit5d_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _vol(shape, seed):
+    return synth.rng(seed, 0).integers(80, 400, size=shape).astype(np.uint16)
+
+
+def test_mixed_stream_equals_single_calls():
+    import petakit5d_amd as pk
+    print("[serve] mixed stream", flush=True)
+    psf1 = synth.gaussian_psf((9, 9, 11), 1.0, 2.5)
+    psf2 = synth.gaussian_psf((9, 9, 13), 1.3, 3.2)
+    ctx = pk.Context(0, 0)
+    srv = pk.Server(ctx, max_active=3)  # forces queueing: later requests join as others finish
+    jobs = []
+    for i, (shape, meth, it, psf) in enumerate([((64, 48, 32), "omw", 2, psf1), ((50, 40, 24), "rl", 5, psf2),
+                                                ((96, 64, 30), "omw", 2, psf2), ((48, 48, 40), "rl", 3, psf1),
+                                                ((40, 36, 20), "omw", 3, psf1), ((72, 60, 27), "rl", 4, psf2)]):
+        I = _vol(shape, 30 + i)
+        pb = np.ascontiguousarray(psf[::-1, ::-1, ::-1])
+        jobs.append((srv.submit(I, psf, pb, it, method=meth, Background=100, save16bit=False), I, psf, pb, meth, it))
+    for req, I, psf, pb, meth, it in jobs:
+        J = req.result(timeout_s=60)
+        inf = req.info()
+        assert inf["latency_ms"] > 0 and inf["iterations_done"] == it
+        if meth == "omw":
+            R, _ = pk.decon_lucy_omw_function(I, psf, pb, it, Background=100, save16bit=False, pad=True, ctx=ctx)
+        else:
+            R, _, _ = pk.decon_lucy_function(I, psf, it, Background=100, save16bit=False, pad=True, ctx=ctx)
+        np.testing.assert_array_equal(J, R)
+    st = srv.stats()
+    assert st["completed"] == 6 and st["failed"] == 0
+    srv.stop()
+    ctx.close()
+
+
+def test_budget_rejects_oversized_request():
+    import petakit5d_amd as pk
+    print("[serve] budget", flush=True)
+    psf = synth.gaussian_psf((7, 7, 9), 1.0, 2.5)
+    ctx = pk.Context(0, 64 << 20)  # 64 MiB budget
+    srv = pk.Server(ctx, max_active=4)
+    small = [srv.submit(_vol((32, 32, 16), 50 + i), psf, psf, 2, method="omw", Background=100) for i in range(3)]
+    big = srv.submit(_vol((256, 256, 128), 60), psf, psf, 2, method="omw", Background=100)
+    for r in small:
+        r.result(timeout_s=60)
+    with pytest.raises(pk.PkDeconError) as e:
+        big.result(timeout_s=60)
+    assert e.value.status == "PK_ERR_BUDGET"
+    srv.stop()
+    ctx.close()
+
+
+def test_cancel_queued_request():
+    import petakit5d_amd as pk
+    psf = synth.gaussian_psf((7, 7, 9), 1.0, 2.5)
+    ctx = pk.Context(0, 0)
+    srv = pk.Server(ctx, max_active=1)
+    reqs = [srv.submit(_vol((64, 64, 32), 70 + i), psf, psf, 3, method="omw", Background=100) for i in range(4)]
+    last = reqs[-1]
+    try:
+        last.cancel()
+        cancelled = True
+    except pk.PkDeconError as err:  # it may already be running on a fast device
+        cancelled = False
+        assert err.status == "PK_ERR_BUSY"
+    for r in reqs[:-1]:
+        r.result(timeout_s=60)
+    if cancelled:
+        assert last.info()["state"] == 4
+    srv.stop()
+    ctx.close()
