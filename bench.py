@@ -1,28 +1,37 @@
 #!/usr/bin/env python3
-"""One-GPU benchmark of the OMW deconvolution hot path.
+"""One-GPU benchmark of the RL / OMW deconvolution hot path.
 
-Workload: decon_lucy_omw_function with 2 iterations on a 512x512x256 uint16
-volume (20k Gaussian blobs, smooth background, blurred by a 64x64x101 PSF
-sheared 31.5 deg, Poisson noise), Background = 100, OTFs cached on the device,
+Default workload (the headline metric): decon_lucy_omw_function with 2
+iterations on a 512x512x256 uint16 volume (20k Gaussian blobs, smooth
+background, blurred by a 64x64x101 PSF sheared 31.5 deg, Poisson noise),
+Background = 100, OMW back projector (alpha 0.005), OTFs cached on the device,
 input and uint16 output resident in HBM.  One step = one decon call.
 
 Prints ONE JSON line.  `value` is deconvolved Mvoxel/s on one GPU; `roofline`
 is for the pass with the largest share of device time (algorithmic bytes per
-launch / average launch time, HIP events on the engine's stream);
-`cpu_baseline` times the oracle (the CPU single-precision restatement) on the
-host cores for the same call.
+launch / mean launch time, HIP events on the engine's stream); `plain_gpu` is
+the same call on the MATLAB-structured unfused path with OTFs rebuilt per
+call; `cpu_baseline` times the oracle (the CPU single-precision restatement)
+on the host cores for the same call.
+
+Other workloads (--workload): rl128 (Biggs RL 15 it. on 128x128x64), batched
+(64 random-size chunks, OMW-2 and RL-10 in batched launches), served
+(continuous-batching server, Poisson arrivals, p50/p99 vs the plain path),
+large (2048x2048x512, OMW-2 and Biggs-10).
 """
 import argparse
 import ctypes
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+PLAIN = 3  # PK_FLAG_UNFUSED | PK_FLAG_NO_OTF_CACHE
 
 
 def parse():
@@ -30,10 +39,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--shape", type=int, nargs=3, default=[512, 512, 256])
-    ap.add_argument("--iters", type=int, default=2)
+    ap.add_argument("--workload", default="omw", choices=["omw", "rl128", "batched", "served", "large"])
+    ap.add_argument("--shape", type=int, nargs=3, default=None)
+    ap.add_argument("--iters", type=int, default=None)
+    ap.add_argument("--duration", type=float, default=20.0, help="served: seconds of Poisson arrivals")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     return ap.parse_args()
 
 
@@ -41,150 +53,414 @@ def col_major_cuda(a, torch):
     return torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).cuda().permute(2, 1, 0)
 
 
+def empty_cm(shape, dtype, torch):
+    return torch.empty((shape[2], shape[1], shape[0]), dtype=dtype, device="cuda").permute(2, 1, 0)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pk_oracle
+
+    pk_oracle.build()
+    return pk_oracle
+
+
+def back_projector(psf):
+    """OMW back projector (omw_backprojector_generation, alpha 0.005, defaults)."""
+    from petakit5d_amd import omw_backprojector_generation
+
+    t = time.perf_counter()
+    b, _, _, _ = omw_backprojector_generation(psf, 0.005, None, OTFCumThresh=0.9, hannWinBounds=[0.8, 1])
+    return b, time.perf_counter() - t
+
+
+class Caller:
+    """Direct C-ABI calls on device-resident volumes (no per-call Python copies)."""
+
+    def __init__(self, ctx, L, method, I, psf_f, psf_b, iters, background=100.0, flags=0, out_dtype=None):
+        import torch
+
+        from petakit5d_amd import _lib
+        self.ctx, self.L, self.method, self.iters = ctx, L, method, iters
+        shape = tuple(I.shape)
+        self.out = empty_cm(shape, out_dtype or torch.uint16, torch)
+        code_in = _lib.PK_UINT16 if I.dtype == torch.uint16 else _lib.PK_SINGLE
+        self.vin = _lib.PkVol(ctypes.c_void_p(I.data_ptr()), code_in, _lib.PK_MEM_DEVICE, (ctypes.c_int64 * 3)(*shape))
+        out16 = self.out.dtype == torch.uint16
+        self.vout = _lib.PkVol(ctypes.c_void_p(self.out.data_ptr()), _lib.PK_UINT16 if out16 else _lib.PK_SINGLE,
+                               _lib.PK_MEM_DEVICE, (ctypes.c_int64 * 3)(*shape))
+        self.opts = _lib.PkOpts()
+        L.pk_opts_default(ctypes.byref(self.opts))
+        self.opts.background = background
+        self.opts.flags = flags
+        self.opts.save16bit = 1 if out16 else 0
+        self.pf = np.asfortranarray(psf_f, dtype=np.float32)
+        self.pb = np.asfortranarray(psf_b if psf_b is not None else psf_f, dtype=np.float32)
+        self.fd, self.bd = _lib.i64x3(self.pf.shape), _lib.i64x3(self.pb.shape)
+
+    def __call__(self):
+        L, c = self.L, self.ctx
+        if self.method == "omw":
+            st = L.pk_decon_lucy_omw(c.handle, ctypes.byref(self.vin), ctypes.c_void_p(self.pf.ctypes.data), self.fd,
+                                     ctypes.c_void_p(self.pb.ctypes.data), self.bd, self.iters, ctypes.byref(self.opts),
+                                     ctypes.byref(self.vout), None)
+        else:
+            st = L.pk_decon_lucy(c.handle, ctypes.byref(self.vin), ctypes.c_void_p(self.pf.ctypes.data), self.fd,
+                                 self.iters, ctypes.byref(self.opts), ctypes.byref(self.vout), None)
+        c.check(st)
+
+
+def timed(fn, steps, warmup, torch, ctx=None):
+    for _ in range(max(warmup, 1)):
+        fn()
+    torch.cuda.synchronize()
+    if ctx is not None:
+        ctx.profile(True)
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        fn()
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = None
+    if ctx is not None:
+        prof = ctx.profile_read()
+        ctx.profile(False)
+    return (t1 - t0) / steps * 1e3, lat, prof
+
+
+def roofline(prof, steps):
+    name, (launches, tot_ms, bpl) = max(prof.items(), key=lambda kv: kv[1][1])
+    avg_ms = tot_ms / launches
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    passes = {k: {"launches_per_step": v[0] / steps, "avg_ms": round(v[1] / v[0], 4),
+                  "GBps": round(v[2] / (v[1] / v[0] * 1e-3) / 1e9, 1) if v[2] else None}
+              for k, v in sorted(prof.items())}
+    return ({"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None},
+            passes, sum(v[1] for v in prof.values()) / steps)
+
+
+def base_result(metric, value, unit, args, ms, config):
+    return {"metric": metric, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": config}
+
+
+# ---------------------------------------------------------------------------------------------- workloads
+def wl_omw(args, torch, pk, L):
+    from petakit5d_amd import synth
+    shape = tuple(args.shape or (512, 512, 256))
+    iters = args.iters or 2
+    t_gen = time.perf_counter()
+    I, psf = synth.config2(shape=shape)
+    psf_b, t_bp = back_projector(psf)
+    t_gen = time.perf_counter() - t_gen
+    ctx = pk.Context(0)
+    It = col_major_cuda(I, torch)
+    call = Caller(ctx, L, "omw", It, psf, psf_b, iters)
+    ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx)
+    nvox = float(np.prod(shape))
+    value = nvox / 1e6 / (ms / 1e3)
+    rl, passes, dev_ms = roofline(prof, args.steps)
+    N = nvox
+    S = (shape[0] // 2 + 1) * shape[1] * shape[2]
+    model = iters * (24 * N + 176 * S) + 12 * N
+    res = base_result("deconvolved Mvoxel/s (OMW, 2 it)", value, "Mvoxel/s", args, ms, {
+        "workload": f"OMW decon, {iters} iterations, {shape[0]}x{shape[1]}x{shape[2]} uint16, 20k blobs + sinusoidal "
+                    "background, 64x64x101 PSF sheared 31.5 deg, Poisson; OMW back projector alpha 0.005; OTFs cached, "
+                    "input/output resident in HBM; one step = one decon call",
+        "volume": list(shape), "psf": [64, 64, 101], "iterations": iters,
+        "back_projector": "omw_backprojector_generation(psf, 0.005, [], OTFCumThresh 0.9, hannWinBounds [0.8 1])",
+        "background": 100, "output": "uint16"})
+    res["roofline"] = rl
+    res["call_model"] = {"bytes_per_call": model, "GBps_vs_wall": round(model / (ms * 1e-3) / 1e9, 1),
+                         "frac_vs_wall": round(model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                         "device_ms_per_step": round(dev_ms, 4)}
+    res["passes"] = passes
+    res["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                         "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+    res["gen_s"] = round(t_gen, 1)
+    res["backprojector_s"] = round(t_bp, 2)
+    if not args.no_plain:
+        plain = Caller(ctx, L, "omw", It, psf, psf_b, iters, flags=PLAIN)
+        pms, _, _ = timed(plain, max(2, args.steps // 3), 1, torch)
+        res["plain_gpu"] = {"value": round(nvox / 1e6 / (pms / 1e3), 2), "unit": "Mvoxel/s", "ms_per_step": round(pms, 3),
+                            "path": "MATLAB-structured: one kernel per gpuArray statement, OTFs rebuilt per call",
+                            "speedup": round(value / (nvox / 1e6 / (pms / 1e3)), 2)}
+    # PCIe-inclusive rate (host uint16 in and out through the same entry point)
+    t = time.perf_counter()
+    for _ in range(3):
+        pk.decon_lucy_omw_function(I, psf, psf_b, iters, Background=100, ctx=ctx)
+    res["host_io_Mvoxel_s"] = round(nvox / 1e6 / ((time.perf_counter() - t) / 3), 1)
+    cpu_and_parity(args, res, ctx, pk, I, psf, psf_b, iters, "omw", nvox, shape)
+    return res
+
+
+def cpu_and_parity(args, res, ctx, pk, I, psf, psf_b, iters, method, nvox, shape):
+    if args.no_check and args.no_cpu_baseline:
+        return
+    po = oracle()
+    if method == "omw":
+        Jg, _ = pk.decon_lucy_omw_function(I, psf, psf_b, iters, Background=100, save16bit=False, ctx=ctx)
+    else:
+        Jg, _, _ = pk.decon_lucy_function(I, psf, iters, Background=100, save16bit=False, ctx=ctx)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    tc = time.perf_counter()
+    if method == "omw":
+        Jc, _ = po.decon_lucy_omw_function(I, psf, psf_b, iters, Background=100, save16bit=False)
+    else:
+        Jc, _, _ = po.decon_lucy_function(I, psf, iters, Background=100, save16bit=False)
+    tc = time.perf_counter() - tc
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = {"value": round(nvox / 1e6 / tc, 3), "unit": "Mvoxel/s", "cores": threads, "kind": "port",
+                               "sample": f"one full {shape[0]}x{shape[1]}x{shape[2]} {method.upper()}-{iters} call "
+                                         "(oracle FP32, full-spectrum FFTs, OTF build included), " f"{tc:.1f} s"}
+    if not args.no_check:
+        par = {"rel_l2_vs_cpu_fp32": rel_l2(Jg, Jc)}
+        if nvox <= 2 ** 22:
+            if method == "omw":
+                J64, _ = po.decon_lucy_omw_function(I, psf, psf_b, iters, Background=100, save16bit=False,
+                                                    precision="f64")
+            else:
+                J64, _, _ = po.decon_lucy_function(I, psf, iters, Background=100, save16bit=False, precision="f64")
+            par["rel_l2_vs_fp64"] = rel_l2(Jg, J64)
+        res["parity"] = par
+
+
+def wl_rl128(args, torch, pk, L):
+    from petakit5d_amd import synth
+    I, psf = synth.config1()
+    iters = args.iters or 15
+    ctx = pk.Context(0)
+    It = col_major_cuda(I, torch)
+    call = Caller(ctx, L, "rl", It, psf, None, iters)
+    ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx)
+    nvox = float(I.size)
+    res = base_result("deconvolved Mvoxel/s (Biggs RL, 15 it)", nvox / 1e6 / (ms / 1e3), "Mvoxel/s", args, ms, {
+        "workload": "Biggs RL 15 iterations, 128x128x64 uint16, 200 emitters, 31x31x41 PSF, background 100, Poisson; "
+                    "one step = one decon call", "volume": list(I.shape), "iterations": iters})
+    res["roofline"], res["passes"], _ = roofline(prof, args.steps)
+    if not args.no_plain:
+        pms, _, _ = timed(Caller(ctx, L, "rl", It, psf, None, iters, flags=PLAIN), max(2, args.steps // 3), 1, torch)
+        res["plain_gpu"] = {"value": round(nvox / 1e6 / (pms / 1e3), 2), "ms_per_step": round(pms, 3)}
+    cpu_and_parity(args, res, ctx, pk, I, psf, None, iters, "rl", nvox, I.shape)
+    return res
+
+
+def _chunk_sizes(n, seed):
+    from petakit5d_amd import synth
+    g = synth.rng(3, seed)
+    return [(int(g.integers(128, 385)), int(g.integers(128, 385)), int(g.integers(64, 257))) for _ in range(n)]
+
+
+def wl_batched(args, torch, pk, L):
+    from petakit5d_amd import synth
+    psf = synth.config2_psf()
+    psf_b, _ = back_projector(psf)
+    sizes = _chunk_sizes(64, 0)
+    # chunks are crops of one blob volume (the config-2 generator at the same blob density)
+    pool = synth._blob_volume_torch((384, 384, 256), psf, 11250, 3, 100.0, as_tensor=True)
+    g = synth.rng(3, 1)
+    chunks = []
+    for s in sizes:
+        o = [int(g.integers(0, 384 - s[0] + 1)), int(g.integers(0, 384 - s[1] + 1)), int(g.integers(0, 256 - s[2] + 1))]
+        chunks.append(pool[o[0]:o[0] + s[0], o[1]:o[1] + s[1], o[2]:o[2] + s[2]].permute(2, 1, 0).contiguous()
+                      .permute(2, 1, 0))
+    del pool
+    ctx = pk.Context(0)
+    nvox = float(sum(np.prod(s) for s in sizes))
+    out = {}
+    for method, iters in (("omw", args.iters or 2), ("rl", 10)):
+        fn = lambda: pk.decon_batch(chunks, psf, psf_b, iters, method=method, Background=100, ctx=ctx)  # noqa: E731
+        ms, _, prof = timed(fn, max(2, args.steps // 3), 1, torch, ctx)
+        rec = {"Mvoxel_s": round(nvox / 1e6 / (ms / 1e3), 1), "ms_per_batch": round(ms, 2),
+               "launches_per_batch": int(sum(v[0] for v in prof.values()) / max(2, args.steps // 3))}
+        # one fused call per chunk (chunks padded by the Python wrapper when not smooth)
+        def seq(plain=False):
+            for c in chunks:
+                pf = pk.crop_psf_to_data(psf, c.shape)
+                pb = pk.crop_psf_to_data(psf_b, c.shape)
+                if method == "omw":
+                    pk.decon_lucy_omw_function(c, pf, pb, iters, Background=100, pad=True, ctx=ctx,
+                                               flags=PLAIN if plain else 0)
+                else:
+                    pk.decon_lucy_function(c, pf, iters, Background=100, pad=True, ctx=ctx, flags=PLAIN if plain else 0)
+        sms, _, _ = timed(seq, 1, 1, torch)
+        rec["one_call_per_chunk_Mvoxel_s"] = round(nvox / 1e6 / (sms / 1e3), 1)
+        if not args.no_plain:
+            pms, _, _ = timed(lambda: seq(True), 1, 0, torch)
+            rec["plain_gpu_Mvoxel_s"] = round(nvox / 1e6 / (pms / 1e3), 1)
+            rec["speedup_vs_plain"] = round(rec["Mvoxel_s"] / rec["plain_gpu_Mvoxel_s"], 2)
+        out[f"{method}{iters}"] = rec
+    v = out["omw2"]["Mvoxel_s"] if "omw2" in out else list(out.values())[0]["Mvoxel_s"]
+    res = base_result("deconvolved Mvoxel/s (batched OMW, 2 it)", v, "Mvoxel/s", args, out["omw2"]["ms_per_batch"], {
+        "workload": "64 chunks, x,y ~ U[128,384], z ~ U[64,256] (padded to 2/3/5-smooth), cropped from a blob volume, "
+                    "one shared 64x64x101 PSF (centre-cropped for chunks < 101 in z), device-resident uint16; "
+                    "one step = one batch", "chunks": 64, "voxels": nvox})
+    res["batched"] = out
+    return res
+
+
+def wl_served(args, torch, pk, L):
+    from petakit5d_amd import synth
+    g = synth.rng(4, 0)
+    psfs = [synth.gaussian_psf((64, 64, 101), 1.0, 2.5), synth.gaussian_psf((64, 64, 101), 1.3, 3.2)]
+    bps = [back_projector(p)[0] for p in psfs]
+    pool = synth._blob_volume_torch((1024, 1024, 256), psfs[0], 20000, 4, 100.0, as_tensor=True)
+    nshape = 24
+    shapes = []
+    for _ in range(nshape):
+        x = int(round(np.exp(g.uniform(np.log(96), np.log(1024)))))
+        y = int(round(np.exp(g.uniform(np.log(96), np.log(1024)))))
+        z = int(round(np.exp(g.uniform(np.log(64), np.log(256)))))
+        shapes.append((x, y, z))
+    vols = [np.asfortranarray(pool[:s[0], :s[1], :s[2]].cpu().numpy()) for s in shapes]
+    del pool
+    torch.cuda.empty_cache()
+
+    def job(i):
+        meth = "omw" if g.random() < 0.7 else "rl"
+        ch = int(g.integers(0, 2))
+        return vols[i % nshape], meth, (2 if meth == "omw" else 15), ch
+
+    ctx = pk.Context(0, 200 << 30)
+    srv = pk.Server(ctx, max_active=64)
+
+    def submit(v, meth, it, ch):
+        return srv.submit(v, psfs[ch], bps[ch], it, method=meth, Background=100)
+
+    # warm plans / OTF cache for every (shape, channel), then measure saturated capacity
+    warm = [submit(vols[i], "omw", 1, c) for i in range(nshape) for c in (0, 1)]
+    for r in warm:
+        r.result()
+    jobs = [job(i) for i in range(48)]
+    t0 = time.perf_counter()
+    reqs = [submit(*j) for j in jobs]
+    for r in reqs:
+        r.result()
+    cap_s = time.perf_counter() - t0
+    cap_vox = sum(float(np.prod(j[0].shape)) for j in jobs)
+    cap_rps = len(jobs) / cap_s
+    qps = 0.7 * cap_rps
+    # open-loop Poisson arrivals at 0.7x capacity
+    arr = []
+    t = 0.0
+    while t < args.duration:
+        t += g.exponential(1.0 / qps)
+        arr.append((t, job(len(arr))))
+    t0 = time.perf_counter()
+    live = []
+    for at, j in arr:
+        d = at - (time.perf_counter() - t0)
+        if d > 0:
+            time.sleep(d)
+        live.append((at, submit(*j)))
+    lat = []
+    for at, r in live:
+        r.result()
+        lat.append(r.info()["latency_ms"])
+    wall = time.perf_counter() - t0
+    vox = sum(float(np.prod(j[0].shape)) for _, j in arr)
+    srv.stop()
+    res = base_result("served decon Mvoxel/s + p99 latency (OMW-2 / Biggs-15 mix)", vox / 1e6 / wall, "Mvoxel/s", args,
+                      float(np.mean(lat)), {
+                          "workload": f"Poisson arrivals at {qps:.2f}/s (0.7x measured capacity) for {args.duration:.0f} s; "
+                                      "sizes log-uniform 96x96x64..1024x1024x256 (24 shapes), 2 channels "
+                                      "(64x64x101 PSFs sigma 1.0/2.5, 1.3/3.2), 70% OMW-2 / 30% Biggs-15, HBM budget "
+                                      "200 GB, host uint16 in/out", "requests": len(arr)})
+    res["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2)}
+    res["capacity"] = {"req_per_s": round(cap_rps, 2), "Mvoxel_s": round(cap_vox / 1e6 / cap_s, 1)}
+    if not args.no_plain:
+        # the same trace on the plain path: one unfused call per request, FIFO, one worker
+        lat_p = []
+        done = threading.Event()
+        q = list(arr)
+
+        def worker():
+            t00 = time.perf_counter()
+            for at, (v, meth, it, ch) in q:
+                d = at - (time.perf_counter() - t00)
+                if d > 0:
+                    time.sleep(d)
+                if meth == "omw":
+                    pk.decon_lucy_omw_function(v, psfs[ch], bps[ch], it, Background=100, pad=True, ctx=ctx, flags=PLAIN)
+                else:
+                    pk.decon_lucy_function(v, psfs[ch], it, Background=100, pad=True, ctx=ctx, flags=PLAIN)
+                lat_p.append((time.perf_counter() - t00 - at) * 1e3)
+            done.set()
+
+        tp = time.perf_counter()
+        th = threading.Thread(target=worker)
+        th.start()
+        th.join()
+        pw = time.perf_counter() - tp
+        busy_vox = vox
+        res["plain_gpu"] = {"Mvoxel_s": round(busy_vox / 1e6 / pw, 1),
+                            "latency_ms": {"p50": round(float(np.percentile(lat_p, 50)), 2),
+                                           "p99": round(float(np.percentile(lat_p, 99)), 2)},
+                            "path": "one MATLAB-structured unfused call per request, OTFs rebuilt per call, FIFO"}
+        res["capacity_speedup_vs_plain"] = round(res["capacity"]["Mvoxel_s"] / res["plain_gpu"]["Mvoxel_s"], 2)
+    return res
+
+
+def wl_large(args, torch, pk, L):
+    from petakit5d_amd import synth
+    shape = tuple(args.shape or (2048, 2048, 512))
+    t_gen = time.perf_counter()
+    It, psf = synth.config5(shape=shape)
+    psf_b, _ = back_projector(psf)
+    t_gen = time.perf_counter() - t_gen
+    ctx = pk.Context(0)
+    nvox = float(np.prod(shape))
+    out = {}
+    for method, iters in (("omw", 2), ("rl", 10)):
+        call = Caller(ctx, L, method, It, psf, psf_b, iters)
+        ms, _, prof = timed(call, max(1, args.steps // 5), 1, torch, ctx)
+        rl, passes, dev_ms = roofline(prof, max(1, args.steps // 5))
+        out[f"{method}{iters}"] = {"Mvoxel_s": round(nvox / 1e6 / (ms / 1e3), 1), "ms_per_call": round(ms, 2),
+                                   "roofline": rl, "device_GB": round(ctx.device_bytes() / 2 ** 30, 1)}
+        del call
+    res = base_result("deconvolved Mvoxel/s (OMW, 2 it, 2048x2048x512)", out["omw2"]["Mvoxel_s"], "Mvoxel/s", args,
+                      out["omw2"]["ms_per_call"], {
+                          "workload": f"{shape[0]}x{shape[1]}x{shape[2]} uint16 (2^31 voxels), 500k blobs, 64x64x101 "
+                                      "PSF, fully resident in HBM; OMW-2 and Biggs-10", "volume": list(shape)})
+    res["large"] = out
+    res["gen_s"] = round(t_gen, 1)
+    if not args.no_check:
+        po = oracle()
+        sub = np.asfortranarray(It[:256, :256, :256].cpu().numpy())
+        Jg, _ = pk.decon_lucy_omw_function(sub, psf, psf_b, 2, Background=100, save16bit=False, ctx=ctx)
+        J64, _ = po.decon_lucy_omw_function(sub, psf, psf_b, 2, Background=100, save16bit=False, precision="f64")
+        J32, _ = po.decon_lucy_omw_function(sub, psf, psf_b, 2, Background=100, save16bit=False)
+        res["parity_256cube"] = {"rel_l2_vs_fp64": rel_l2(Jg, J64), "rel_l2_vs_cpu_fp32": rel_l2(Jg, J32)}
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(volumes are independent units, see the sharding notes)", file=sys.stderr)
+              "(volumes are independent units; see DESIGN.md on sharding)", file=sys.stderr)
         sys.exit(2)
     import torch
 
     import petakit5d_amd as pk
-    from petakit5d_amd import _lib, synth
+    from petakit5d_amd import _lib
 
-    shape = tuple(args.shape)
-    t_gen = time.perf_counter()
-    I, psf = synth.config2(shape=shape)
-    psf_b, bp_kind = back_projector(psf)
-    t_gen = time.perf_counter() - t_gen
-
-    ctx = pk.Context(0)
     L = _lib.load()
-    It = col_major_cuda(I, torch)
-    out = torch.empty((shape[2], shape[1], shape[0]), dtype=torch.uint16, device="cuda").permute(2, 1, 0)
-    vin = _lib.PkVol(ctypes.c_void_p(It.data_ptr()), _lib.PK_UINT16, _lib.PK_MEM_DEVICE, (ctypes.c_int64 * 3)(*shape))
-    vout = _lib.PkVol(ctypes.c_void_p(out.data_ptr()), _lib.PK_UINT16, _lib.PK_MEM_DEVICE,
-                      (ctypes.c_int64 * 3)(*shape))
-    opts = _lib.PkOpts()
-    L.pk_opts_default(ctypes.byref(opts))
-    opts.background = 100.0
-    pf = np.asfortranarray(psf, dtype=np.float32)
-    pb = np.asfortranarray(psf_b, dtype=np.float32)
-    fd, bd = _lib.i64x3(pf.shape), _lib.i64x3(pb.shape)
-    torch.cuda.synchronize()
-
-    def step():
-        ctx.check(L.pk_decon_lucy_omw(ctx.handle, ctypes.byref(vin), ctypes.c_void_p(pf.ctypes.data), fd,
-                                      ctypes.c_void_p(pb.ctypes.data), bd, args.iters, ctypes.byref(opts),
-                                      ctypes.byref(vout), None))
-
-    for _ in range(max(args.warmup, 1)):  # the first call builds and caches the two OTFs
-        step()
-    torch.cuda.synchronize()
-    ctx.profile(True)
-    lat = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        step()
-        lat.append(time.perf_counter() - ts)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = ctx.profile_read()
-    ctx.profile(False)
-    ms = (t1 - t0) / args.steps * 1e3
-    nvox = float(np.prod(shape))
-    value = nvox / 1e6 / (ms / 1e3)
-
-    # dominant pass: largest total device time
-    name, (launches, tot_ms, bpl) = max(prof.items(), key=lambda kv: kv[1][1])
-    avg_ms = tot_ms / launches
-    achieved = bpl / (avg_ms * 1e-3) / 1e9
-    dev_ms = sum(v[1] for v in prof.values()) / args.steps
-    N = nvox
-    S = (shape[0] // 2 + 1) * shape[1] * shape[2]
-    model_bytes = args.iters * (24 * N + 176 * S) + 12 * N  # per call, OTFs cached
-    passes = {k: {"launches_per_step": v[0] / args.steps, "avg_ms": round(v[1] / v[0], 4),
-                  "GBps": round(v[2] / (v[1] / v[0] * 1e-3) / 1e9, 1)} for k, v in sorted(prof.items())}
-
-    result = {
-        "metric": "deconvolved Mvoxel/s (OMW, 2 it)",
-        "value": round(value, 2),
-        "unit": "Mvoxel/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"OMW decon, {args.iters} iterations, {shape[0]}x{shape[1]}x{shape[2]} uint16, "
-                        "20k blobs + sinusoidal background, 64x64x101 PSF sheared 31.5 deg, Poisson; "
-                        "OTFs cached, input/output resident in HBM; one step = one decon call",
-            "volume": list(shape),
-            "psf": [64, 64, 101],
-            "iterations": args.iters,
-            "back_projector": bp_kind,
-            "background": 100,
-            "output": "uint16",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": name,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
-        },
-        "call_model": {
-            "bytes_per_call": model_bytes,
-            "GBps_vs_wall": round(model_bytes / (ms * 1e-3) / 1e9, 1),
-            "frac_vs_wall": round(model_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-            "device_ms_per_step": round(dev_ms, 4),
-        },
-        "passes": passes,
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
-                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
-        "gen_s": round(t_gen, 1),
-    }
-
-    if not args.no_check or not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pk_oracle
-
-        pk_oracle.build()
-    if not args.no_check:
-        Jg, _ = pk.decon_lucy_omw_function(I, psf, psf_b, args.iters, Background=100, save16bit=False, ctx=ctx)
-    if not args.no_cpu_baseline or not args.no_check:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        tc = time.perf_counter()
-        Jc, _ = pk_oracle.decon_lucy_omw_function(I, psf, psf_b, args.iters, Background=100, save16bit=False)
-        tc = time.perf_counter() - tc
-        if not args.no_cpu_baseline:
-            result["cpu_baseline"] = {"value": round(nvox / 1e6 / tc, 3), "unit": "Mvoxel/s", "cores": threads,
-                                      "kind": "port",
-                                      "sample": f"one full {shape[0]}x{shape[1]}x{shape[2]} OMW-{args.iters} call "
-                                                "(oracle FP32, full-spectrum FFTs, OTF build included), "
-                                                f"{tc:.1f} s"}
-        if not args.no_check:
-            a, b = Jg.astype(np.float64), Jc.astype(np.float64)
-            result["parity"] = {"rel_l2_vs_cpu_fp32": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
-    print(json.dumps(result), flush=True)
-
-
-def back_projector(psf):
-    """OMW back projector of the PSF (omw_backprojector_generation, alpha 0.005)."""
-    try:
-        from petakit5d_amd.backprojector import omw_backprojector_generation
-    except ImportError:
-        return np.ascontiguousarray(psf[::-1, ::-1, ::-1]), "flip(PSF) (traditional RL back projector)"
-    b, _, _, _ = omw_backprojector_generation(psf, 0.005, None, OTFCumThresh=0.9, hannWinBounds=[0.8, 1])
-    return b, "OMW, alpha 0.005, OTFCumThresh 0.9, hannWinBounds [0.8 1]"
+    res = {"omw": wl_omw, "rl128": wl_rl128, "batched": wl_batched, "served": wl_served, "large": wl_large}[
+        args.workload](args, torch, pk, L)
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

@@ -64,9 +64,17 @@ typedef struct {
     int32_t edge_erosion;  /* 'EdgeErosion' (0) */
     int32_t save16bit;     /* 'save16bit'   (true): output class uint16, else single */
     int32_t has_bbox;      /* 'bbox' given */
-    int32_t reserved;
+    int32_t flags;         /* PK_FLAG_* (0 = the fused engine) */
     int64_t bbox[6];       /* 'bbox' [s1 s2 s3 e1 e2 e3], 1-based inclusive (decon_lucy_function.m:199-203) */
 } pk_opts;
+
+/* Comparison baselines (no counterpart in the reference's options):
+ * PK_FLAG_UNFUSED runs the MATLAB-structured path -- one kernel per gpuArray
+ * statement of decon_lucy_function.m:121-142 / decon_lucy_omw_function.m:321-328
+ * (3-pass fftn / ifftn, separate OTF product, max, ./, .*) -- on the same
+ * transforms; PK_FLAG_NO_OTF_CACHE rebuilds the OTFs on every call. */
+#define PK_FLAG_UNFUSED 1
+#define PK_FLAG_NO_OTF_CACHE 2
 
 typedef struct pk_ctx pk_ctx;
 

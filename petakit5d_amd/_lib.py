@@ -80,7 +80,7 @@ class PkOpts(ctypes.Structure):
         ("edge_erosion", ctypes.c_int32),
         ("save16bit", ctypes.c_int32),
         ("has_bbox", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
         ("bbox", ctypes.c_int64 * 6),
     ]
 

@@ -523,6 +523,9 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
 
     const VolPlan& vp = plan(n1, n2, n3);
     const int64_t N = vp.N;
+    if (o->flags & PK_FLAG_NO_OTF_CACHE) {  // rebuild the OTFs on every call (comparison baseline)
+        otf.erase(std::remove_if(otf.begin(), otf.end(), [](const OtfEntry& e) { return e.pins == 0; }), otf.end());
+    }
     const float2* Hf = otf_for(vp, psf_f, fd);
     const float2* Hb = method == 1 ? otf_for(vp, psf_b, bd) : nullptr;
 
@@ -535,7 +538,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         ensure(ws.J3, sizeof(float) * N);
         ensure(ws.Y, sizeof(float) * N);
         ensure(ws.J4, sizeof(float) * N);
-        ensure(ws.partials, sizeof(double) * 2 * (size_t)nxb);
+        ensure(ws.partials, sizeof(double) * 2 * (size_t)std::max<int64_t>(nxb, 1024));  // 1024: unfused k_dot2
         ensure(ws.lambda, sizeof(float) * 4);
     }
     const size_t esz = Iv->dtype == PK_UINT16 ? 2 : 4;
@@ -552,85 +555,90 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
     float* J = ws.J.as<float>();
     float2* A = ws.A.as<float2>();
 
-    // prep (decon_lucy_function.m:85-100, GPU branch): uint16 I - uint16(Background)
-    // saturates at 0; single I gives max(I - Background, 0)
-    {
-        XArgs a{};
-                a.spec = A;
-        a.I = I;
-        a.J = J;
-        if (raw16) {
-            a.mode = XM_PREP_U16;
-            a.raw16 = raw16;
-            a.bg = (float)matlab_u16(o->background);
-        } else {
-            a.mode = XM_PREP_F32;
-            a.rawf = rawf;
-            a.bg = (float)o->background;
-        }
-        run_x(vp, a);
-    }
-
-    if (method == 1) {
-        for (int k = 1; k <= numit; ++k) {
-            spectral_conv(vp, A, Hf, false);
-            XArgs r{};
-            r.mode = XM_RATIO;
-            r.spec = A;
-            r.I = I;
-            run_x(vp, r);
-            spectral_conv(vp, A, Hb, false);
-            XArgs u{};
-            u.mode = XM_OMW_UPDATE;
-            u.spec = A;
-            u.J = J;
-            u.r2c_out = k < numit;
-            run_x(vp, u);
-        }
+    if (o->flags & PK_FLAG_UNFUSED) {
+        const float bgv = raw16 ? (float)matlab_u16(o->background) : (float)o->background;
+        decon_unfused(method, vp, raw, raw16 != nullptr, bgv, Hf, Hb, numit, I, J);
     } else {
-        float* J2 = J;
-        float* J3 = ws.J3.as<float>();
-        float* Y = ws.Y.as<float>();
-        float* J4 = ws.J4.as<float>();
-        float* lam = ws.lambda.as<float>();
-        double* part = ws.partials.as<double>();
-        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N)), dim3(256), 0, stream, J3, N, 0.f);
-        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, stream, lam, (int64_t)1, 0.f);
-        PK_HIP(hipGetLastError());
-        for (int k = 1; k <= numit; ++k) {
-            if (k > 2) {
-                hipLaunchKernelGGL(k_lambda, dim3(1), dim3(256), 0, stream, part, (int)nxb, kEps, lam);
-                PK_HIP(hipGetLastError());
+        // prep (decon_lucy_function.m:85-100, GPU branch): uint16 I - uint16(Background)
+        // saturates at 0; single I gives max(I - Background, 0)
+        {
+            XArgs a{};
+                    a.spec = A;
+            a.I = I;
+            a.J = J;
+            if (raw16) {
+                a.mode = XM_PREP_U16;
+                a.raw16 = raw16;
+                a.bg = (float)matlab_u16(o->background);
+            } else {
+                a.mode = XM_PREP_F32;
+                a.rawf = rawf;
+                a.bg = (float)o->background;
             }
-            XArgs y{};
-            y.mode = XM_BIGGS_Y;
-            y.spec = A;
-            y.J = J2;
-            y.J3 = J3;
-            y.Y = Y;
-            y.lambda = lam;
-            run_x(vp, y);
-            spectral_conv(vp, A, Hf, false);
-            XArgs r{};
-            r.mode = XM_RATIO;
-            r.spec = A;
-            r.I = I;
-            run_x(vp, r);
-            spectral_conv(vp, A, Hf, true);
-            XArgs u{};
-            u.mode = XM_BIGGS_UPDATE;
-            u.spec = A;
-            u.Y = Y;
-            u.Jn = J3;  // x_{k-2} is dead once Y is formed
-            u.J4 = J4;
-            u.acc = (k >= 2 && k < numit);
-            u.write_j4 = (k < numit);
-            u.partials = part;
-            u.r2c_out = 0;
-            run_x(vp, u);
-            std::swap(J2, J3);
+            run_x(vp, a);
         }
-        J = J2;
+
+        if (method == 1) {
+            for (int k = 1; k <= numit; ++k) {
+                spectral_conv(vp, A, Hf, false);
+                XArgs r{};
+                r.mode = XM_RATIO;
+                r.spec = A;
+                r.I = I;
+                run_x(vp, r);
+                spectral_conv(vp, A, Hb, false);
+                XArgs u{};
+                u.mode = XM_OMW_UPDATE;
+                u.spec = A;
+                u.J = J;
+                u.r2c_out = k < numit;
+                run_x(vp, u);
+            }
+        } else {
+            float* J2 = J;
+            float* J3 = ws.J3.as<float>();
+            float* Y = ws.Y.as<float>();
+            float* J4 = ws.J4.as<float>();
+            float* lam = ws.lambda.as<float>();
+            double* part = ws.partials.as<double>();
+            hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N)), dim3(256), 0, stream, J3, N, 0.f);
+            hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, stream, lam, (int64_t)1, 0.f);
+            PK_HIP(hipGetLastError());
+            for (int k = 1; k <= numit; ++k) {
+                if (k > 2) {
+                    hipLaunchKernelGGL(k_lambda, dim3(1), dim3(256), 0, stream, part, (int)nxb, kEps, lam);
+                    PK_HIP(hipGetLastError());
+                }
+                XArgs y{};
+                y.mode = XM_BIGGS_Y;
+                y.spec = A;
+                y.J = J2;
+                y.J3 = J3;
+                y.Y = Y;
+                y.lambda = lam;
+                run_x(vp, y);
+                spectral_conv(vp, A, Hf, false);
+                XArgs r{};
+                r.mode = XM_RATIO;
+                r.spec = A;
+                r.I = I;
+                run_x(vp, r);
+                spectral_conv(vp, A, Hf, true);
+                XArgs u{};
+                u.mode = XM_BIGGS_UPDATE;
+                u.spec = A;
+                u.Y = Y;
+                u.Jn = J3;  // x_{k-2} is dead once Y is formed
+                u.J4 = J4;
+                u.acc = (k >= 2 && k < numit);
+                u.write_j4 = (k < numit);
+                u.partials = part;
+                u.r2c_out = 0;
+                run_x(vp, u);
+                std::swap(J2, J3);
+            }
+            J = J2;
+        }
     }
 
     // epilogue (decon_lucy_function.m:177-203)

@@ -171,6 +171,9 @@ struct pk_ctx_impl {
     void run_step(std::vector<Chunk*>& act);
     void chunk_finish(Chunk& c);
     void otf_pin(const float2* spec, int delta);
+    // MATLAB-structured comparison path (pk_plain.hip); leaves the estimate in J
+    void decon_unfused(int method, const VolPlan& vp, const void* raw, bool u16, float bg, const float2* Hf,
+                       const float2* Hb, int numit, float* I, float* J);
     std::unique_ptr<Server> srv;
     pk_ctx_impl();  // defined where Server is complete (pk_serve.hip)
     ~pk_ctx_impl();

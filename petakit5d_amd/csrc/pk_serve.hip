@@ -118,8 +118,34 @@ static void scheduler(pk_ctx_impl* ctx) {
                     admitted.push_back(r);
                 } catch (const Error& e) {
                     unpin(ctx, *r);
-                    if (e.st == PK_ERR_BUDGET) deferred.push_back(r);  // memory frees up as requests finish
-                    else failed.emplace_back(r, e);
+                    r->mem.release();
+                    if (e.st == PK_ERR_BUDGET) {
+                        // cached OTFs nobody reads are the first memory to give back; then wait
+                        const size_t before = ctx->otf.size();
+                        ctx->otf.erase(std::remove_if(ctx->otf.begin(), ctx->otf.end(),
+                                                      [](const OtfEntry& x) { return x.pins == 0; }),
+                                       ctx->otf.end());
+                        bool ok = false;
+                        if (ctx->otf.size() != before) {
+                            try {
+                                const size_t need = ctx->chunk_prepare(r->c, r->in, r->out, r->pf.data(), r->fd,
+                                                                       r->pb.empty() ? nullptr : r->pb.data(), r->bd);
+                                r->pinned = true;
+                                ctx->alloc(r->mem, need);
+                                ctx->chunk_place(r->c, r->mem.as<char>());
+                                r->t_adm = Clock::now();
+                                S.active.push_back(r);
+                                admitted.push_back(r);
+                                ok = true;
+                            } catch (const Error&) {
+                                unpin(ctx, *r);
+                                r->mem.release();
+                            }
+                        }
+                        if (!ok) deferred.push_back(r);  // memory frees up as requests finish
+                    } else {
+                        failed.emplace_back(r, e);
+                    }
                 }
             }
             if (!deferred.empty() && S.active.empty()) {
@@ -201,6 +227,8 @@ void serve_start(pk_ctx_impl* ctx, int max_active) {
     if (ctx->srv) fail(PK_ERR_BUSY, "server already running");
     ctx->srv = std::make_unique<Server>();
     ctx->srv->max_active = max_active > 0 ? max_active : 64;
+    // many request shapes and channels: keep their OTFs (the HBM budget still bounds them)
+    ctx->otf_max_entries = std::max<size_t>(ctx->otf_max_entries, 256);
     ctx->srv->th = std::thread(scheduler, ctx);
 }
 

@@ -15,7 +15,9 @@ CUDA tensors -- a tensor is accepted when its strides are column-major,
 e.g. ``t.permute(2, 1, 0)`` of a contiguous (n3, n2, n1) tensor.
 
 Name-value options are matched case-insensitively like MATLAB's
-inputParser (CaseSensitive=false).  All arithmetic runs in libpkdecon.so on
+inputParser (CaseSensitive=false).  Engine extensions beside the reference's
+options: pad (zero-pad non-2/3/5-smooth sizes), ctx (device context), flags
+(PK_FLAG_* comparison baselines).  All arithmetic runs in libpkdecon.so on
 the GPU; ``useGPU=False`` is rejected because this engine has no CPU branch.
 """
 from __future__ import annotations
@@ -29,10 +31,10 @@ from . import _lib
 from ._lib import PK_MEM_DEVICE, PK_MEM_HOST, PK_SINGLE, PK_UINT16, PkDeconError, PkOpts, PkVol, i64x3
 
 _OPTION_NAMES = ["Background", "useGPU", "save16bit", "dampFactor", "scaleFactor", "deconOffset", "EdgeErosion",
-                 "bbox", "debug", "debug_folder", "saveStep", "pad", "ctx"]
+                 "bbox", "debug", "debug_folder", "saveStep", "pad", "ctx", "flags"]
 _DEFAULTS = {"Background": None, "useGPU": True, "save16bit": True, "dampFactor": 1, "scaleFactor": 1,
              "deconOffset": 0, "EdgeErosion": 0, "bbox": None, "debug": False, "debug_folder": "./debug/",
-             "saveStep": 1, "pad": False, "ctx": None}
+             "saveStep": 1, "pad": False, "ctx": None, "flags": 0}
 
 
 def _parse(kwargs: dict) -> dict:
@@ -158,6 +160,7 @@ def _opts_struct(o: dict, orig_dims, padded: bool) -> tuple:
     s.decon_offset = float(o["deconOffset"])
     s.edge_erosion = int(o["EdgeErosion"])
     s.save16bit = 1 if o["save16bit"] else 0
+    s.flags = int(o.get("flags", 0))  # PK_FLAG_* comparison baselines (engine extension)
     bbox = o["bbox"]
     if bbox is not None and np.size(bbox) > 0:
         b = [int(v) for v in np.asarray(bbox).reshape(-1)]

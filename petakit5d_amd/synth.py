@@ -141,13 +141,15 @@ def _gpu_available() -> bool:
         return False
 
 
-def _blob_volume_torch(shape, psf, n_blobs, config_id, bg_mean):
+def _blob_volume_torch(shape, psf, n_blobs, config_id, bg_mean, dtype=None, as_tensor=False):
     """GPU rendering of blob_volume: blobs, background and PSF blur in one
-    Fourier-domain product (float64), Poisson noise from a seeded torch
-    generator.  Same model as the host version; different random stream for
-    the noise."""
+    Fourier-domain product (float64, or float32 for very large volumes),
+    Poisson noise from a seeded torch generator.  Same model as the host
+    version; different random stream for the noise.  as_tensor=True returns
+    the column-major uint16 CUDA tensor instead of a host array."""
     import torch
     dev = torch.device("cuda")
+    fdt = dtype or torch.float64
     g0, g1, g2 = rng(config_id, 0), rng(config_id, 1), rng(config_id, 2)
     idx, b, w, centers = _blob_params(shape, n_blobs, g0)
     n1, n2, n3 = shape
@@ -159,11 +161,11 @@ def _blob_volume_torch(shape, psf, n_blobs, config_id, bg_mean):
         sel = torch.from_numpy(b == k).to(dev)
         if not bool(sel.any()):
             continue
-        d = torch.zeros(n1 * n2 * n3, dtype=torch.float64, device=dev)
-        d.index_add_(0, flat[sel], torch.from_numpy(w).to(dev)[sel])
+        d = torch.zeros(n1 * n2 * n3, dtype=fdt, device=dev)
+        d.index_add_(0, flat[sel], torch.from_numpy(w).to(dev, fdt)[sel])
         s = float(centers[k])
         # circular Gaussian kernel sampled on the grid, unit sum
-        ax = [torch.arange(m, dtype=torch.float64, device=dev) for m in (n3, n2, n1)]
+        ax = [torch.arange(m, dtype=fdt, device=dev) for m in (n3, n2, n1)]
         ax = [torch.minimum(a, m - a) for a, m in zip(ax, (n3, n2, n1))]
         ker = torch.exp(-(ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2)
                         / (2 * s * s))
@@ -171,19 +173,25 @@ def _blob_volume_torch(shape, psf, n_blobs, config_id, bg_mean):
         term = rfft(d.view(n3, n2, n1)) * rfft(ker)
         acc = term if acc is None else acc + term
         del d, ker
-    bg = torch.full((n3, n2, n1), bg_mean, dtype=torch.float64, device=dev)
+    bg = torch.full((n3, n2, n1), bg_mean, dtype=fdt, device=dev)
     for a, kk, ph in _bg_params(g1, 3):
-        t = [torch.arange(m, dtype=torch.float64, device=dev) / m * kk[dd] for dd, m in enumerate(shape)]
+        t = [torch.arange(m, dtype=fdt, device=dev) / m * kk[dd] for dd, m in enumerate(shape)]
         bg += a * torch.sin(2 * np.pi * (t[2][:, None, None] + t[1][None, :, None] + t[0][None, None, :]) + ph)
     acc = acc + rfft(bg)
     del bg
-    p = torch.from_numpy(np.ascontiguousarray(_wrapped_kernel(psf, shape).transpose(2, 1, 0))).to(dev)
+    p = torch.from_numpy(np.ascontiguousarray(_wrapped_kernel(psf, shape).transpose(2, 1, 0))).to(dev, fdt)
     img = torch.fft.irfftn(acc * rfft(p), s=(n3, n2, n1), dim=(0, 1, 2)).clamp_(min=0)
     del acc, p
     gen = torch.Generator(device=dev)
     gen.manual_seed(int(g2.integers(0, 2 ** 62)))
-    counts = torch.poisson(img, generator=gen).clamp_(max=65535).to(torch.int32).cpu().numpy()
+    counts = torch.poisson(img, generator=gen).clamp_(max=65535)
     del img
+    if as_tensor:
+        out = counts.to(torch.int32).to(torch.uint16).permute(2, 1, 0)  # column-major view, MATLAB dims
+        del counts
+        torch.cuda.empty_cache()
+        return out
+    counts = counts.to(torch.int32).cpu().numpy()
     return np.asfortranarray(counts.astype(np.uint16).transpose(2, 1, 0))
 
 
@@ -218,3 +226,13 @@ def config2(shape=(512, 512, 256), n_blobs=20000, device=None):
     the config-2 PSF, Poisson."""
     psf = config2_psf()
     return blob_volume(shape, psf, n_blobs, 2, device=device), psf
+
+
+def config5(shape=(2048, 2048, 512), n_blobs=500000):
+    """2048x2048x512 uint16 (2^31 voxels), 500k blobs, the config-2 PSF;
+    rendered on the GPU in float32 and returned as a column-major CUDA
+    tensor (it never leaves HBM)."""
+    import torch
+    psf = config2_psf()
+    t = _blob_volume_torch(tuple(shape), psf, n_blobs, 5, 100.0, dtype=torch.float32, as_tensor=True)
+    return t, psf

@@ -130,3 +130,24 @@ def test_errors():
     assert e.value.status == "PK_ERR_UNSUPPORTED_SIZE"
     with pytest.raises(TypeError):
         pk.decon_lucy_function(np.zeros((32, 32, 16)), np.ones((3, 3, 3), np.float32), 2)
+
+
+@pytest.mark.parametrize("method", ["omw", "rl"])
+def test_unfused_plain_path_matches_fused(method):
+    """The MATLAB-structured path (one kernel per gpuArray statement) runs the
+    same transforms and per-element arithmetic: OMW is bitwise identical; Biggs
+    differs only through the order of the lambda reductions."""
+    import petakit5d_amd as pk
+    I, psf = _case((64, 48, 32), (9, 9, 11), seed=21)
+    pb = _flip(psf)
+    if method == "omw":
+        A, _ = pk.decon_lucy_omw_function(I, psf, pb, 3, Background=100, save16bit=False)
+        B, _ = pk.decon_lucy_omw_function(I, psf, pb, 3, Background=100, save16bit=False, flags=PLAIN)
+        np.testing.assert_array_equal(A, B)
+    else:
+        A, _, _ = pk.decon_lucy_function(I, psf, 6, Background=100, save16bit=False)
+        B, _, _ = pk.decon_lucy_function(I, psf, 6, Background=100, save16bit=False, flags=PLAIN)
+        assert rel_l2(A, B) < 1e-5
+
+
+PLAIN = 3  # PK_FLAG_UNFUSED | PK_FLAG_NO_OTF_CACHE
