@@ -198,9 +198,17 @@ def omw_backprojector_generation(psf, alpha, skewed=None, OTFCumThresh=0.9, OTFA
     Y, X, Z = np.meshgrid(np.arange(1, psz3[0] + 1) - c[0], np.arange(1, psz3[1] + 1) - c[1],
                           np.arange(1, psz3[2] + 1) - c[2], indexing="ij")
     # cart2sph / sph2cart (:129-132) with the C library's atan2, hypot, cos and
-    # sin (math module), so exact-distance ties between lattice directions
-    # resolve the same way as in any other glibc-based implementation
+    # sin, so exact-distance ties between lattice directions resolve the same
+    # way as in any other glibc-based implementation.  atan2/cos/sin come from
+    # the math module (thin libm wrappers); math.hypot is CPython's own
+    # algorithm and differs from libm's in the last bit, so hypot is taken
+    # from libm directly.
+    import ctypes
     import math
+    _libm = ctypes.CDLL("libm.so.6")
+    _libm.hypot.restype = ctypes.c_double
+    _libm.hypot.argtypes = [ctypes.c_double, ctypes.c_double]
+    hypot = _libm.hypot
     shp = X.shape
     Xf, Yf, Zf = X.reshape(-1), Y.reshape(-1), Z.reshape(-1)
     rs = np.empty(Xf.size)
@@ -209,8 +217,8 @@ def omw_backprojector_generation(psf, alpha, skewed=None, OTFCumThresh=0.9, OTFA
     Zn = np.empty(Xf.size)
     for i in range(Xf.size):
         x, y, z = float(Xf[i]), float(Yf[i]), float(Zf[i])
-        hxy = math.hypot(x, y)
-        rs[i] = math.hypot(hxy, z)
+        hxy = hypot(x, y)
+        rs[i] = hypot(hxy, z)
         el, az = math.atan2(z, hxy), math.atan2(y, x)
         rc = math.cos(el)
         Xn[i], Yn[i], Zn[i] = rc * math.cos(az), rc * math.sin(az), math.sin(el)

@@ -269,10 +269,15 @@ __global__ void k_knn2(const double* __restrict__ E, int nE, const double* __res
         for (int i = threadIdx.x; i < 3 * nt; i += blockDim.x) se[i] = E[3 * (int64_t)t0 + i];
         __syncthreads();
         for (int j = 0; j < nt; ++j) {
-            const double dy = __dsub_rn(qy, se[3 * j]);
-            const double dx = __dsub_rn(qx, se[3 * j + 1]);
-            const double dz = __dsub_rn(qz, se[3 * j + 2]);
-            const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dy, dy), __dmul_rn(dx, dx)), __dmul_rn(dz, dz));
+            // Separately rounded products and sums, (dy^2 + dx^2) + dz^2, written
+            // as plain operators so the file's contract(off) applies (the
+            // __dmul_rn family are inline header functions that the default
+            // -ffp-contract=fast would fuse): lattice directions produce many
+            // distances that tie up to the last bit, and an FMA reorders them.
+            const double dy = qy - se[3 * j];
+            const double dx = qx - se[3 * j + 1];
+            const double dz = qz - se[3 * j + 2];
+            const double d2 = (dy * dy + dx * dx) + dz * dz;
             if (d2 < b1) {
                 b2 = b1;
                 i2 = i1;

@@ -76,10 +76,17 @@ __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s
             u[m] = v[i + m * Q];
         });
         if (k != 0) {
+            // W^(m k tws) for m = 1..R-1: one table load, the rest by binary
+            // powering (w_2m = w_m^2, w_2m+1 = w_2m w_1; chain depth <= log2 R)
+            float2 w[R];
+            w[1] = pl.tw[k * tws];
+            static_for<2, R>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                w[m] = (m % 2 == 0) ? cmul(w[m / 2], w[m / 2]) : cmul(w[m - 1], w[1]);
+            });
             static_for<1, R>([&](auto mc) {
                 constexpr int m = decltype(mc)::value;
-                const float2 w = pl.tw[m * k * tws];
-                u[m] = INV ? cmulc(u[m], w) : cmul(u[m], w);
+                u[m] = INV ? cmulc(u[m], w[m]) : cmul(u[m], w[m]);
             });
         }
         dft<R, INV>(u);
@@ -256,6 +263,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
     if constexpr (IN) {
         if (packed) {
             const int M = L;
+            const float2 wp = a.rtw[p];  // W^(p + eP) = W^p W^(eP); the second factor is block-uniform
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int m = p + e * P;
@@ -264,7 +272,8 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
                     const float2 xk = a.spec[srow + m];
                     const float2 xmk = cconj(a.spec[srow + (M - m)]);
                     const float2 ev = cadd(xk, xmk);
-                    const float2 od = cmulc(csub(xk, xmk), a.rtw[m]);  // * W^{-m}
+                    const float2 wm = e == 0 ? wp : cmul(wp, a.rtw[e * P]);
+                    const float2 od = cmulc(csub(xk, xmk), wm);  // * W^{-m}
                     z = make_float2(ev.x - od.y, ev.y + od.x);          // ev + i od
                 }
                 v[e] = z;
@@ -343,6 +352,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
             for (int e = 0; e < E; ++e)
                 if (act) lds[lds_slot<true>(lbase, p + e * P, 1)] = v[e];
             __syncthreads();
+            const float2 wp = a.rtw[p];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = p + e * P;
@@ -350,7 +360,8 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
                 const float2 A = lds[lds_slot<true>(lbase, k, 1)];
                 const float2 B = cconj(lds[lds_slot<true>(lbase, k == 0 ? 0 : M - k, 1)]);
                 const float2 s = cadd(A, B);
-                const float2 d = cmul(csub(A, B), a.rtw[k]);
+                const float2 wk = e == 0 ? wp : cmul(wp, a.rtw[e * P]);
+                const float2 d = cmul(csub(A, B), wk);
                 // -i * d = (d.y, -d.x)
                 const float2 X = make_float2(0.5f * (s.x + d.y), 0.5f * (s.y - d.x));
                 if (valid) a.spec[srow + k] = X;
