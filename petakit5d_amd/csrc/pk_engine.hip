@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "pk_launch.hpp"
@@ -123,8 +124,31 @@ void pk_ctx_impl::prof_flush() {
 static const int kEs[] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 27, 30, 32};
 static const int kRadices[] = {32, 30, 27, 25, 24, 20, 18, 16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
 
+// Launch-shape overrides for tuning sweeps, read once per process:
+// PK_FFT_EX / PK_FFT_EY / PK_FFT_EZ force the values-per-thread count of an
+// axis when it is a feasible plan for that length; PK_FFT_TY / PK_FFT_TZ cap
+// the column-tile width of the strided axes.  Unset (the default) means the
+// built-in choice.
+struct Tune {
+    int e[3] = {0, 0, 0};
+    int t[3] = {0, 0, 0};
+    Tune() {
+        const char* names_e[3] = {"PK_FFT_EX", "PK_FFT_EY", "PK_FFT_EZ"};
+        const char* names_t[3] = {nullptr, "PK_FFT_TY", "PK_FFT_TZ"};
+        for (int i = 0; i < 3; ++i) {
+            if (const char* v = std::getenv(names_e[i])) e[i] = std::atoi(v);
+            if (names_t[i])
+                if (const char* v = std::getenv(names_t[i])) t[i] = std::atoi(v);
+        }
+    }
+};
+static const Tune& tune() {
+    static const Tune t;
+    return t;
+}
+
 // Factor L into radices that all divide E; E is a multiple of every prime of L.
-static FftPlan plan_1d(int L, int target_e, int max_threads) {
+static FftPlan plan_1d(int L, int target_e, int max_threads, int force_e = 0) {
     FftPlan best{};
     best.L = L;
     if (L == 1) {
@@ -167,7 +191,7 @@ static FftPlan plan_1d(int L, int target_e, int max_threads) {
             rem /= r;
         }
         if (!ok) continue;
-        const int score = c.nst * 100 + std::abs(E - target_e);
+        const int score = (force_e == E) ? -1 : c.nst * 100 + std::abs(E - target_e);
         if (score < best_score) {
             best_score = score;
             best = c;
@@ -228,7 +252,7 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
     // x (rows)
     {
         AxisLaunch& ax = vp.x;
-        ax.fp = plan_1d(Lx, 16, kMaxThreads);
+        ax.fp = plan_1d(Lx, 16, kMaxThreads, tune().e[0]);
         ax.fp.tw = table(Lx);
         ax.lpad = row_pitch(Lx);
         int rt = std::max(1, kMaxThreads / ax.fp.P);
@@ -238,17 +262,18 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
         ax.lds = std::max<size_t>((size_t)rt * ax.lpad * sizeof(float2), 2 * sizeof(double) * kMaxThreads);
     }
     // y, z (columns)
-    auto colplan = [&](AxisLaunch& ax, int L) {
-        ax.fp = plan_1d(L, 16, kMaxThreads);
+    auto colplan = [&](AxisLaunch& ax, int L, int axis) {
+        ax.fp = plan_1d(L, 16, kMaxThreads, tune().e[axis]);
         ax.fp.tw = table(L);
         int T = std::max(1, kMaxThreads / ax.fp.P);
+        if (tune().t[axis] > 0) T = std::min(T, tune().t[axis]);
         while (T > 1 && (size_t)T * L * sizeof(float2) > 65536) T--;
         ax.tile = T;
         ax.block = dim3(T * ax.fp.P);
         ax.lds = std::max<size_t>((size_t)T * L * sizeof(float2), 16);
     };
-    colplan(vp.y, (int)n2);
-    colplan(vp.z, (int)n3);
+    colplan(vp.y, (int)n2, 1);
+    colplan(vp.z, (int)n3, 2);
 
     if (vp.packed) {
         auto r = rtw.find(n1);

@@ -135,15 +135,17 @@ def test_errors():
 @pytest.mark.parametrize("method", ["omw", "rl"])
 def test_unfused_plain_path_matches_fused(method):
     """The MATLAB-structured path (one kernel per gpuArray statement) runs the
-    same transforms and per-element arithmetic: OMW is bitwise identical; Biggs
-    differs only through the order of the lambda reductions."""
+    same transforms and per-element arithmetic; the two differ only where the
+    compiler contracts a product into an FMA differently in separate kernels
+    (last-bit differences, rel-L2 <= 1e-6) and, for Biggs, through the order
+    of the lambda reductions."""
     import petakit5d_amd as pk
     I, psf = _case((64, 48, 32), (9, 9, 11), seed=21)
     pb = _flip(psf)
     if method == "omw":
         A, _ = pk.decon_lucy_omw_function(I, psf, pb, 3, Background=100, save16bit=False)
         B, _ = pk.decon_lucy_omw_function(I, psf, pb, 3, Background=100, save16bit=False, flags=PLAIN)
-        np.testing.assert_array_equal(A, B)
+        assert rel_l2(A, B) < 1e-6
     else:
         A, _, _ = pk.decon_lucy_function(I, psf, 6, Background=100, save16bit=False)
         B, _, _ = pk.decon_lucy_function(I, psf, 6, Background=100, save16bit=False, flags=PLAIN)
