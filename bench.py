@@ -39,13 +39,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="omw", choices=["omw", "rl128", "batched", "served", "large"])
+    ap.add_argument("--workload", default="omw", choices=["omw", "rl128", "batched", "served", "large", "copy"])
     ap.add_argument("--shape", type=int, nargs=3, default=None)
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--duration", type=float, default=20.0, help="served: seconds of Poisson arrivals")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--events-apart", action="store_true",
+                    help="time the K steps without per-pass events; take per-pass durations from a second K-step run")
     return ap.parse_args()
 
 
@@ -116,29 +118,64 @@ class Caller:
         c.check(st)
 
 
-def timed(fn, steps, warmup, torch, ctx=None):
+def timed(fn, steps, warmup, torch, ctx=None, events_in_timed=True):
+    """W warmup calls, then K timed calls.  With a context, every pass of the
+    K calls is bracketed by HIP events on the engine's stream (per-pass
+    durations); events_in_timed=False times the K calls without events and
+    records the per-pass durations over a second run of K calls."""
     for _ in range(max(warmup, 1)):
         fn()
     torch.cuda.synchronize()
-    if ctx is not None:
-        ctx.profile(True)
-    lat = []
+
+    def loop(profile):
+        if profile:
+            ctx.profile(True)
+        lat = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts = time.perf_counter()
+            fn()
+            lat.append(time.perf_counter() - ts)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        prof = None
+        if profile:
+            prof = ctx.profile_read()
+            ctx.profile(False)
+        return (t1 - t0) / steps * 1e3, lat, prof
+
+    if ctx is None:
+        return loop(False)
+    if events_in_timed:
+        return loop(True)
+    ms, lat, _ = loop(False)
+    ms_ev, _, prof = loop(True)
+    prof = dict(prof)
+    prof["__ms_with_events"] = ms_ev
+    return ms, lat, prof
+
+
+def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
+    """Device-to-device copy of nbytes (read + write) with torch's copy kernel:
+    the achievable HBM rate on this device, for reference."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        ts = time.perf_counter()
-        fn()
-        lat.append(time.perf_counter() - ts)
+    for _ in range(reps):
+        b.copy_(a)
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = None
-    if ctx is not None:
-        prof = ctx.profile_read()
-        ctx.profile(False)
-    return (t1 - t0) / steps * 1e3, lat, prof
+    dt = (time.perf_counter() - t0) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / dt / 1e9
 
 
 def roofline(prof, steps):
+    prof = {k: v for k, v in prof.items() if not k.startswith("__")}
     name, (launches, tot_ms, bpl) = max(prof.items(), key=lambda kv: kv[1][1])
     avg_ms = tot_ms / launches
     achieved = bpl / (avg_ms * 1e-3) / 1e9
@@ -168,7 +205,7 @@ def wl_omw(args, torch, pk, L):
     ctx = pk.Context(0)
     It = col_major_cuda(I, torch)
     call = Caller(ctx, L, "omw", It, psf, psf_b, iters)
-    ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx)
+    ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx, events_in_timed=not args.events_apart)
     nvox = float(np.prod(shape))
     value = nvox / 1e6 / (ms / 1e3)
     rl, passes, dev_ms = roofline(prof, args.steps)
@@ -189,6 +226,9 @@ def wl_omw(args, torch, pk, L):
     res["passes"] = passes
     res["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                          "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)}
+    if "__ms_with_events" in prof:
+        res["call_model"]["ms_per_step_with_events"] = round(prof["__ms_with_events"], 4)
+    res["hbm_copy_GBps"] = round(copy_bandwidth(torch), 1)
     res["gen_s"] = round(t_gen, 1)
     res["backprojector_s"] = round(t_bp, 2)
     if not args.no_plain:
@@ -446,6 +486,17 @@ def wl_large(args, torch, pk, L):
     return res
 
 
+def wl_copy(args, torch, pk, L):
+    """A plain device copy of known size: the calibration kernel for the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) and the achievable copy rate."""
+    nbytes = 1 << 30
+    gbps = copy_bandwidth(torch, nbytes, max(1, args.steps))
+    res = base_result("device copy GB/s", gbps, "GB/s", args, 2.0 * nbytes / gbps / 1e6, {
+        "workload": "torch copy_ of a 1 GiB float32 buffer (1 GiB read + 1 GiB written per step)", "bytes": nbytes})
+    res["dtype"] = "f32"
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -458,7 +509,8 @@ def main():
     from petakit5d_amd import _lib
 
     L = _lib.load()
-    res = {"omw": wl_omw, "rl128": wl_rl128, "batched": wl_batched, "served": wl_served, "large": wl_large}[
+    res = {"omw": wl_omw, "rl128": wl_rl128, "batched": wl_batched, "served": wl_served, "large": wl_large,
+           "copy": wl_copy}[
         args.workload](args, torch, pk, L)
     print(json.dumps(res), flush=True)
 

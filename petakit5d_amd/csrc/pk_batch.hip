@@ -436,8 +436,7 @@ void pk_ctx_impl::chunk_finish(Chunk& c) {
     e.offset = (float)c.o.decon_offset;
     e.apply_scale = (c.o.scale_factor != 1 || c.o.decon_offset != 0);
     e.use_mask = (c.o.decon_offset != 0);
-    const int64_t on = c.d[0] * c.d[1] * c.d[2];
-    hipLaunchKernelGGL(k_epilogue, dim3((unsigned)std::min<int64_t>((on + 255) / 256, 65536)), dim3(256), 0, stream, e);
+    hipLaunchKernelGGL(k_epilogue, dim3(epi_grid(e)), dim3(256), 0, stream, e);
     PK_HIP(hipGetLastError());
     if (c.out_host) PK_HIP(hipMemcpyAsync(c.out_host, c.dout, c.out_bytes, hipMemcpyDeviceToHost, stream));
 }

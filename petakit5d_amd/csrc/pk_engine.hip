@@ -148,7 +148,8 @@ static const Tune& tune() {
 }
 
 // Factor L into radices that all divide E; E is a multiple of every prime of L.
-static FftPlan plan_1d(int L, int target_e, int max_threads, int force_e = 0) {
+// score = stage_w * stages + e_w * |E - target_e|, lowest wins.
+static FftPlan plan_1d(int L, int target_e, int max_threads, int force_e, int stage_w, int e_w) {
     FftPlan best{};
     best.L = L;
     if (L == 1) {
@@ -191,7 +192,7 @@ static FftPlan plan_1d(int L, int target_e, int max_threads, int force_e = 0) {
             rem /= r;
         }
         if (!ok) continue;
-        const int score = (force_e == E) ? -1 : c.nst * 100 + std::abs(E - target_e);
+        const int score = (force_e == E) ? -1 : c.nst * stage_w + e_w * std::abs(E - target_e);
         if (score < best_score) {
             best_score = score;
             best = c;
@@ -252,7 +253,11 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
     // x (rows)
     {
         AxisLaunch& ax = vp.x;
-        ax.fp = plan_1d(Lx, 16, kMaxThreads, tune().e[0]);
+        // rows: occupancy matters more than the number of LDS exchanges (the
+        // pass also holds its real-space operands), so few values per thread
+        // (measured at 512x512x256: E = 4 / 8 / 16 -> 0.24 / 0.26 / 0.29 ms per
+        // ratio pass)
+        ax.fp = plan_1d(Lx, 4, kMaxThreads, tune().e[0], 10, 4);
         ax.fp.tw = table(Lx);
         ax.lpad = row_pitch(Lx);
         int rt = std::max(1, kMaxThreads / ax.fp.P);
@@ -263,7 +268,8 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
     }
     // y, z (columns)
     auto colplan = [&](AxisLaunch& ax, int L, int axis) {
-        ax.fp = plan_1d(L, 16, kMaxThreads, tune().e[axis]);
+        // columns: fewest stages, then values per thread near 16
+        ax.fp = plan_1d(L, 16, kMaxThreads, tune().e[axis], 100, 1);
         ax.fp.tw = table(L);
         int T = std::max(1, kMaxThreads / ax.fp.P);
         if (tune().t[axis] > 0) T = std::min(T, tune().t[axis]);
@@ -721,7 +727,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
     e.use_mask = need_mask;
     if (!(o->damp_factor > 1)) e.damp = 0.f;
     hipEvent_t ee = prof_mark();
-    hipLaunchKernelGGL(k_epilogue, dim3(grid_for(on)), dim3(256), 0, stream, e);
+    hipLaunchKernelGGL(k_epilogue, dim3(epi_grid(e)), dim3(256), 0, stream, e);
     PK_HIP(hipGetLastError());
     prof_done("epilogue", (double)on * (4.0 + (e.damp > 1.f ? 4.0 : 0.0) + (double)osz), ee);
     if (out->mem == PK_MEM_HOST)

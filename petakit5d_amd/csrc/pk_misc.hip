@@ -72,36 +72,61 @@ __device__ __forceinline__ float u16_round(float x) {
 
 // Epilogue (decon_lucy_function.m:177-203): damp, scale / offset / mask,
 // uint16 conversion and the 1-based inclusive bbox crop, one pass over the
-// output voxels.
-__global__ void k_epilogue(EpiArgs a) {
-    const int64_t on = a.o1 * a.o2 * a.o3;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < on; o += stride) {
-        const int64_t x = o % a.o1;
-        const int64_t y = (o / a.o1) % a.o2;
-        const int64_t z = o / (a.o1 * a.o2);
-        const int64_t i = (x + a.b1) + a.n1 * ((y + a.b2) + a.n2 * (z + a.b3));
-        float j = a.J[i];
-        if (a.damp > 1.f) {
-            const float dI = a.damp * a.I[i];
-            // J.*(J<=dI) + min(J, dI.*(J>=dI)); exact equality yields 2J as in the reference
-            const float t1 = (j <= dI) ? j : 0.f;
-            const float t2 = (j >= dI) ? dI : 0.f;
-            j = t1 + (j < t2 ? j : t2);
+// output voxels.  Work items are (output row, 1024-voxel chunk) pairs: the
+// row decomposition is block-uniform, so no per-voxel integer division; when
+// the crop keeps whole dim-1 x dim-2 planes the output is one contiguous run.
+__global__ void __launch_bounds__(256) k_epilogue(EpiArgs a) {
+    const bool flat = (a.o1 == a.n1 && a.o2 == a.n2);
+    const int64_t rlen = flat ? a.o1 * a.o2 * a.o3 : a.o1;
+    const int64_t nrows = flat ? 1 : a.o2 * a.o3;
+    const int64_t chunks = (rlen + kEpiChunk - 1) / kEpiChunk;
+    const int64_t total = nrows * chunks;
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const int64_t r = w / chunks;
+        const int64_t x0 = (w - r * chunks) * kEpiChunk;
+        int64_t src, dst;
+        if (flat) {
+            src = a.b3 * a.n1 * a.n2;
+            dst = 0;
+        } else {
+            const int64_t y = r % a.o2, z = r / a.o2;
+            src = a.b1 + a.n1 * ((y + a.b2) + a.n2 * (z + a.b3));
+            dst = r * a.o1;
         }
-        if (a.apply_scale) {
-            if (a.use_mask) {
-                float m;
-                if (a.mask) m = a.mask[i] ? 1.f : 0.f;
-                else if (a.raw16) m = a.raw16[i] != 0 ? 1.f : 0.f;
-                else m = a.rawf[i] != 0.f ? 1.f : 0.f;
-                j = (j * a.scale + a.offset) * m;
-            } else {
-                j = j * a.scale;
+        float jv[kEpiChunk / 256], dv[kEpiChunk / 256], mv[kEpiChunk / 256];
+#pragma unroll
+        for (int k = 0; k < kEpiChunk / 256; ++k) {
+            const int64_t x = x0 + k * 256 + threadIdx.x;
+            const bool in = x < rlen;
+            const int64_t i = src + (in ? x : 0);
+            jv[k] = a.J[i];
+            dv[k] = a.damp > 1.f ? a.I[i] : 0.f;
+            mv[k] = 1.f;
+            if (a.apply_scale && a.use_mask) {
+                if (a.mask) mv[k] = a.mask[i] ? 1.f : 0.f;
+                else if (a.raw16) mv[k] = a.raw16[i] != 0 ? 1.f : 0.f;
+                else mv[k] = a.rawf[i] != 0.f ? 1.f : 0.f;
             }
         }
-        if (a.out16) a.out16[o] = (uint16_t)u16_round(j);
-        else a.outf[o] = j;
+#pragma unroll
+        for (int k = 0; k < kEpiChunk / 256; ++k) {
+            const int64_t x = x0 + k * 256 + threadIdx.x;
+            if (x >= rlen) continue;
+            float j = jv[k];
+            if (a.damp > 1.f) {
+                const float dI = a.damp * dv[k];
+                // J.*(J<=dI) + min(J, dI.*(J>=dI)); exact equality yields 2J as in the reference
+                const float t1 = (j <= dI) ? j : 0.f;
+                const float t2 = (j >= dI) ? dI : 0.f;
+                j = t1 + (j < t2 ? j : t2);
+            }
+            if (a.apply_scale) {
+                if (a.use_mask) j = (j * a.scale + a.offset) * mv[k];
+                else j = j * a.scale;
+            }
+            if (a.out16) a.out16[dst + x] = (uint16_t)u16_round(j);
+            else a.outf[dst + x] = j;
+        }
     }
 }
 
