@@ -174,7 +174,20 @@ def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
     return 2.0 * nbytes / dt / 1e9
 
 
-def roofline(prof, steps):
+def measured_traffic(pass_name, workload):
+    """HBM bytes per dispatch of a pass, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs committed under profiles/ (calibrated against a copy of
+    known size); None when no measurement exists for this workload."""
+    path = os.path.join(ROOT, "profiles", f"pk_kernels_{workload}_hbm_bytes.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(pass_name)
+        return None if k is None else round(k["traffic_bytes"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def roofline(prof, steps, workload=None):
     prof = {k: v for k, v in prof.items() if not k.startswith("__")}
     name, (launches, tot_ms, bpl) = max(prof.items(), key=lambda kv: kv[1][1])
     avg_ms = tot_ms / launches
@@ -183,7 +196,8 @@ def roofline(prof, steps):
                   "GBps": round(v[2] / (v[1] / v[0] * 1e-3) / 1e9, 1) if v[2] else None}
               for k, v in sorted(prof.items())}
     return ({"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None},
+             "frac": round(achieved / HBM_PEAK_GBPS, 4), "algorithmic_bytes": round(bpl),
+             "traffic": measured_traffic(name, workload) if workload else None},
             passes, sum(v[1] for v in prof.values()) / steps)
 
 
@@ -208,7 +222,8 @@ def wl_omw(args, torch, pk, L):
     ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx, events_in_timed=not args.events_apart)
     nvox = float(np.prod(shape))
     value = nvox / 1e6 / (ms / 1e3)
-    rl, passes, dev_ms = roofline(prof, args.steps)
+    rl, passes, dev_ms = roofline(prof, args.steps, "omw2_512x512x256" if shape == (512, 512, 256) and iters == 2
+                                  else None)
     N = nvox
     S = (shape[0] // 2 + 1) * shape[1] * shape[2]
     model = iters * (24 * N + 176 * S) + 12 * N
