@@ -12,6 +12,13 @@
 #include <type_traits>
 #include <utility>
 
+// Every device kernel includes this header first.  FP contraction is off for
+// the rest of the translation unit and fused multiply-adds are written out
+// (__builtin_fmaf), so the rounding of every operation is fixed by the source:
+// kernels of different shape (compile-time and runtime plans, single and
+// batched launches) that run the same operations produce identical bits.
+#pragma clang fp contract(off)
+
 namespace pk {
 
 // ---------------------------------------------------------------- constexpr trig
@@ -60,10 +67,10 @@ constexpr double sin2pi(long n, long d) {
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
-    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+    return make_float2(__builtin_fmaf(a.x, b.x, a.y * b.y), __builtin_fmaf(a.y, b.x, -(a.x * b.y)));
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
@@ -101,7 +108,7 @@ __device__ __forceinline__ float2 twiddle_const(float2 a) {
     } else {
         constexpr float c = (float)cos2pi(e, R);
         constexpr float s = (float)(INV ? sin2pi(e, R) : -sin2pi(e, R));
-        return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+        return make_float2(__builtin_fmaf(a.x, c, -(a.y * s)), __builtin_fmaf(a.x, s, a.y * c));
     }
 }
 
@@ -118,7 +125,7 @@ __device__ __forceinline__ void dft3(float2& x0, float2& x1, float2& x2) {
     constexpr float s = 0.866025403784438646763723170752936183f;  // sin(2 pi / 3)
     const float2 t1 = cadd(x1, x2);
     const float2 d = csub(x1, x2);
-    const float2 t2 = make_float2(x0.x - 0.5f * t1.x, x0.y - 0.5f * t1.y);
+    const float2 t2 = make_float2(__builtin_fmaf(-0.5f, t1.x, x0.x), __builtin_fmaf(-0.5f, t1.y, x0.y));
     x0 = cadd(x0, t1);
     // forward: X1 = t2 - i s d, X2 = t2 + i s d
     const float2 sd = make_float2(s * d.y, -s * d.x);  // -i s d
@@ -150,10 +157,12 @@ __device__ __forceinline__ void dft5(float2& x0, float2& x1, float2& x2, float2&
     constexpr float s2 = 0.587785252292473129168705954639072769f;   // sin(4 pi/5)
     const float2 a1 = cadd(x1, x4), b1 = csub(x1, x4);
     const float2 a2 = cadd(x2, x3), b2 = csub(x2, x3);
-    const float2 t1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
-    const float2 t2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
-    const float2 u1 = make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
-    const float2 u2 = make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+    const float2 t1 = make_float2(__builtin_fmaf(c2, a2.x, __builtin_fmaf(c1, a1.x, x0.x)),
+                                  __builtin_fmaf(c2, a2.y, __builtin_fmaf(c1, a1.y, x0.y)));
+    const float2 t2 = make_float2(__builtin_fmaf(c1, a2.x, __builtin_fmaf(c2, a1.x, x0.x)),
+                                  __builtin_fmaf(c1, a2.y, __builtin_fmaf(c2, a1.y, x0.y)));
+    const float2 u1 = make_float2(__builtin_fmaf(s2, b2.x, s1 * b1.x), __builtin_fmaf(s2, b2.y, s1 * b1.y));
+    const float2 u2 = make_float2(__builtin_fmaf(-s1, b2.x, s2 * b1.x), __builtin_fmaf(-s1, b2.y, s2 * b1.y));
     x0 = make_float2(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
     // forward: X1 = t1 - i u1, X4 = t1 + i u1, X2 = t2 - i u2, X3 = t2 + i u2
     const float2 iu1 = mul_pi(u1), iu2 = mul_pi(u2);

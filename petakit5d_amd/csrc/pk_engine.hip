@@ -122,7 +122,6 @@ void pk_ctx_impl::prof_flush() {
 
 // ------------------------------------------------------------------ planning
 static const int kEs[] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 27, 30, 32};
-static const int kRadices[] = {32, 30, 27, 25, 24, 20, 18, 16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
 
 // Launch-shape overrides for tuning sweeps, read once per process:
 // PK_FFT_EX / PK_FFT_EY / PK_FFT_EZ force the values-per-thread count of an
@@ -132,7 +131,9 @@ static const int kRadices[] = {32, 30, 27, 25, 24, 20, 18, 16, 15, 12, 10, 9, 8,
 struct Tune {
     int e[3] = {0, 0, 0};
     int t[3] = {0, 0, 0};
+    bool generic_x = false;  // PK_FFT_GENERIC_X=1: always the runtime-plan row kernel
     Tune() {
+        if (const char* v = std::getenv("PK_FFT_GENERIC_X")) generic_x = std::atoi(v) != 0;
         const char* names_e[3] = {"PK_FFT_EX", "PK_FFT_EY", "PK_FFT_EZ"};
         const char* names_t[3] = {nullptr, "PK_FFT_TY", "PK_FFT_TZ"};
         for (int i = 0; i < 3; ++i) {
@@ -173,12 +174,7 @@ static FftPlan plan_1d(int L, int target_e, int max_threads, int force_e, int st
         c.P = L / E;
         int rem = L, ns = 1;
         while (rem > 1) {
-            int r = 0;
-            for (int cand : kRadices)
-                if (E % cand == 0 && rem % cand == 0) {
-                    r = cand;
-                    break;
-                }
+            const int r = radix_for(rem, E);
             if (!r || c.nst >= kMaxStages) {
                 ok = false;
                 break;
@@ -253,11 +249,12 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
     // x (rows)
     {
         AxisLaunch& ax = vp.x;
-        // rows: occupancy matters more than the number of LDS exchanges (the
-        // pass also holds its real-space operands), so few values per thread
-        // (measured at 512x512x256: E = 4 / 8 / 16 -> 0.24 / 0.26 / 0.29 ms per
-        // ratio pass)
-        ax.fp = plan_1d(Lx, 4, kMaxThreads, tune().e[0], 10, 4);
+        // rows: with a compile-time kernel for the length, E near 8 (measured
+        // at 512x512x256, ratio pass: E = 4 / 8 / 16 -> 0.209 / 0.175 / 0.170 ms);
+        // the runtime-plan kernel spends more on index arithmetic and prefers
+        // few values per thread (E = 4 / 8 / 16 -> 0.24 / 0.26 / 0.29 ms)
+        ax.fp = plan_1d(Lx, 8, kMaxThreads, tune().e[0], 10, 4);
+        if (!vp.packed || !xrow_available(ax.fp.L, ax.fp.E)) ax.fp = plan_1d(Lx, 4, kMaxThreads, tune().e[0], 10, 4);
         ax.fp.tw = table(Lx);
         ax.lpad = row_pitch(Lx);
         int rt = std::max(1, kMaxThreads / ax.fp.P);
@@ -316,7 +313,13 @@ void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a) {
     a.eps = kEps;
     const int64_t nblk = (a.nrows + a.RT - 1) / a.RT;
     hipEvent_t e0 = prof_mark();
-    PK_HIP(launch_xpass(vp.x.fp, dim3((unsigned)nblk), vp.x.block, vp.x.lds, stream, a));
+    // compile-time kernel when one exists for this (row length, values per
+    // thread) and the tile is the full 256-thread block it assumes
+    if (vp.packed && vp.x.block.x == (unsigned)kMaxThreads && !tune().generic_x &&
+        launch_xrow(vp.x.fp.L, vp.x.fp.E, dim3((unsigned)nblk), vp.x.lds, stream, a, vp.x.fp.tw))
+        PK_HIP(hipGetLastError());
+    else
+        PK_HIP(launch_xpass(vp.x.fp, dim3((unsigned)nblk), vp.x.block, vp.x.lds, stream, a));
     if (prof_on) {
         // algorithmic bytes: each operand read or written once
         const double N = (double)vp.N, S = (double)vp.Sa;

@@ -50,6 +50,42 @@ __device__ __forceinline__ int lds_slot(int lbase, int pos, int lstride) {
 }
 __host__ __device__ constexpr int row_pitch(int L) { return L + (L >> 4) + 1; }
 
+// Radix of the next Stockham stage: the first entry of this list that divides
+// both the remaining length and E (the host planner and the compile-time row
+// kernels factor with the same rule, so both run the same stage sequence).
+constexpr int kRadixOrder[] = {32, 30, 27, 25, 24, 20, 18, 16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+__host__ __device__ constexpr int radix_for(int rem, int E) {
+    for (int r : kRadixOrder)
+        if (E % r == 0 && rem % r == 0) return r;
+    return 0;
+}
+
+// Inter-stage twiddles of one radix-R butterfly: u[m] *= W_L^(m k tws).
+// R <= 4: each power read from the length-L table (exact table values);
+// larger R: one table read, the rest by binary powering (w_2m = w_m^2,
+// w_2m+1 = w_2m w_1; chain depth <= log2 R).
+template <int R, bool INV>
+__device__ __forceinline__ void stage_twiddle(float2 (&u)[R], const float2* __restrict__ tw, int k, int tws) {
+    if constexpr (R <= 4) {
+        static_for<1, R>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const float2 w = tw[m * k * tws];
+            u[m] = INV ? cmulc(u[m], w) : cmul(u[m], w);
+        });
+    } else {
+        float2 w[R];
+        w[1] = tw[k * tws];
+        static_for<2, R>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            w[m] = (m % 2 == 0) ? cmul(w[m / 2], w[m / 2]) : cmul(w[m - 1], w[1]);
+        });
+        static_for<1, R>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            u[m] = INV ? cmulc(u[m], w[m]) : cmul(u[m], w[m]);
+        });
+    }
+}
+
 // ------------------------------------------------------------------ one stage
 template <int E, int R, bool INV, bool ROWS>
 __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s, int p, float2* lds, int lbase,
@@ -75,20 +111,7 @@ __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s
             constexpr int m = decltype(mc)::value;
             u[m] = v[i + m * Q];
         });
-        if (k != 0) {
-            // W^(m k tws) for m = 1..R-1: one table load, the rest by binary
-            // powering (w_2m = w_m^2, w_2m+1 = w_2m w_1; chain depth <= log2 R)
-            float2 w[R];
-            w[1] = pl.tw[k * tws];
-            static_for<2, R>([&](auto mc) {
-                constexpr int m = decltype(mc)::value;
-                w[m] = (m % 2 == 0) ? cmul(w[m / 2], w[m / 2]) : cmul(w[m - 1], w[1]);
-            });
-            static_for<1, R>([&](auto mc) {
-                constexpr int m = decltype(mc)::value;
-                u[m] = INV ? cmulc(u[m], w[m]) : cmul(u[m], w[m]);
-            });
-        }
+        if (s != 0) stage_twiddle<R, INV>(u, pl.tw, k, tws);
         dft<R, INV>(u);
         if (last) {
             static_for<0, R>([&](auto qc) {
@@ -151,6 +174,93 @@ __device__ __forceinline__ void fft_run(float2 (&v)[E], const FftPlan& pl, int p
     }
 }
 #undef PK_RADIX_CASE
+
+// ------------------------------------------------------------------ compile-time rows
+// The same Stockham stages with L and E fixed at compile time: radices,
+// strides, twiddle strides and LDS offsets are constants.  When a row's P
+// threads lie inside one wavefront (P <= 64) the exchanges only need the
+// wavefront's own ordering: LDS operations of one wavefront execute in
+// order, so a wavefront-scope fence (a compiler ordering point) replaces the
+// workgroup barrier.
+__host__ __device__ constexpr int ct_nst(int L, int E) {
+    int n = 0;
+    for (int rem = L; rem > 1; ++n) rem /= radix_for(rem, E);
+    return n;
+}
+__host__ __device__ constexpr int ct_radix(int L, int E, int s) {
+    int rem = L;
+    for (int i = 0; i < s; ++i) rem /= radix_for(rem, E);
+    return radix_for(rem, E);
+}
+__host__ __device__ constexpr int ct_ns(int L, int E, int s) {
+    int ns = 1, rem = L;
+    for (int i = 0; i < s; ++i) {
+        const int r = radix_for(rem, E);
+        ns *= r;
+        rem /= r;
+    }
+    return ns;
+}
+template <int P>
+__device__ __forceinline__ void row_sync() {
+    if constexpr (P <= 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+template <int L, int E, int S, bool INV>
+__device__ __forceinline__ void ct_stage(float2 (&v)[E], int p, float2* lds, int lbase, const float2* __restrict__ tw) {
+    constexpr int R = ct_radix(L, E, S);
+    constexpr int Ns = ct_ns(L, E, S);
+    constexpr int P = L / E, Q = E / R, tws = L / (Ns * R);
+    constexpr bool last = (S == ct_nst(L, E) - 1);
+    static_assert(R > 0 && E % R == 0, "no stage radix");
+    static_for<0, Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const int j = p + i * P;
+        const int g = j / Ns;
+        const int k = j - g * Ns;
+        float2 u[R];
+        static_for<0, R>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            u[m] = v[i + m * Q];
+        });
+        if constexpr (S != 0) stage_twiddle<R, INV>(u, tw, k, tws);
+        dft<R, INV>(u);
+        if constexpr (last) {
+            static_for<0, R>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                v[i + q * Q] = u[q];
+            });
+        } else {
+            const int ob = g * Ns * R + k;
+            static_for<0, R>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                lds[lds_slot<true>(lbase, ob + q * Ns, 1)] = u[q];
+            });
+        }
+    });
+    if constexpr (!last) {
+        row_sync<P>();
+        static_for<0, E>([&](auto ec) {
+            constexpr int e = decltype(ec)::value;
+            v[e] = lds[lds_slot<true>(lbase, p + e * P, 1)];
+        });
+        row_sync<P>();
+    }
+}
+
+template <int L, int E, bool INV, int S = 0>
+__device__ __forceinline__ void ct_fft(float2 (&v)[E], int p, float2* lds, int lbase, const float2* __restrict__ tw) {
+    if constexpr (S < ct_nst(L, E)) {
+        ct_stage<L, E, S, INV>(v, p, lds, lbase, tw);
+        ct_fft<L, E, INV, S + 1>(v, p, lds, lbase, tw);
+    }
+}
 
 // ------------------------------------------------------------------ x-axis (rows)
 enum XMode : int {
@@ -215,6 +325,47 @@ __device__ __forceinline__ float2 ld_u16(const uint16_t* __restrict__ base, int6
     return make_float2((float)base[xrow + m], 0.f);
 }
 
+// ---- per-element arithmetic shared by the generic and the compile-time row kernels
+// (one definition, so both produce bit-identical results)
+
+// a / b: reciprocal plus one Newton correction of the quotient
+// (q = a r, e = a - b q, q += e r); within 1 ulp of the IEEE quotient for the
+// operand range here (b >= eps, finite a).
+__device__ __forceinline__ float div_nr(float a, float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float q = a * r;
+    const float e = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(e, r, q);
+}
+// C2R pre-step of a packed row: Z_m = (X_m + X*_{M-m}) + i W^{-m} (X_m - X*_{M-m})
+__device__ __forceinline__ float2 c2r_pre(float2 xk, float2 xmk_conj, float2 w) {
+    const float2 ev = cadd(xk, xmk_conj);
+    const float2 od = cmulc(csub(xk, xmk_conj), w);
+    return make_float2(ev.x - od.y, ev.y + od.x);
+}
+// R2C post-step: X_k = 1/2 [(Z_k + Z*_{M-k}) - i W^k (Z_k - Z*_{M-k})]
+__device__ __forceinline__ float2 r2c_post(float2 A, float2 Bc, float2 w) {
+    const float2 s = cadd(A, Bc);
+    const float2 d = cmul(csub(A, Bc), w);
+    return make_float2(0.5f * (s.x + d.y), 0.5f * (s.y - d.x));
+}
+// Real-space step of one value pair (v: inverse-transform output, opA/opB:
+// the mode's real-space operands); returns the forward-transform input.
+template <int MODE>
+__device__ __forceinline__ float2 x_elem(float2 v, float2 opA, float2 opB, float bg, float eps, float lam) {
+    if constexpr (MODE == XM_R2C) return opA;
+    if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32)
+        // uint16: I - uint16(Background) saturates at 0; single: max(I - Background, 0)
+        return fmax0(make_float2(opA.x - bg, opA.y - bg));
+    if constexpr (MODE == XM_RATIO)  // I ./ max(CX, eps)
+        return make_float2(div_nr(opA.x, v.x > eps ? v.x : eps), div_nr(opA.y, v.y > eps ? v.y : eps));
+    if constexpr (MODE == XM_OMW_UPDATE) return fmax0(make_float2(v.x * opA.x, v.y * opA.y));
+    if constexpr (MODE == XM_BIGGS_Y)
+        return fmax0(make_float2(opA.x + lam * (opA.x - opB.x), opA.y + lam * (opA.y - opB.y)));
+    if constexpr (MODE == XM_BIGGS_UPDATE) return fmax0(make_float2(opA.x * v.x, opA.y * v.y));
+    return v;
+}
+
 // One x pass, specialised on its mode: [C2R] -> elementwise -> [R2C].
 template <int E, int MODE>
 __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, float2* lds, int64_t blk) {
@@ -263,19 +414,11 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
     if constexpr (IN) {
         if (packed) {
             const int M = L;
-            const float2 wp = a.rtw[p];  // W^(p + eP) = W^p W^(eP); the second factor is block-uniform
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int m = p + e * P;
                 float2 z = make_float2(0.f, 0.f);
-                if (valid) {
-                    const float2 xk = a.spec[srow + m];
-                    const float2 xmk = cconj(a.spec[srow + (M - m)]);
-                    const float2 ev = cadd(xk, xmk);
-                    const float2 wm = e == 0 ? wp : cmul(wp, a.rtw[e * P]);
-                    const float2 od = cmulc(csub(xk, xmk), wm);  // * W^{-m}
-                    z = make_float2(ev.x - od.y, ev.y + od.x);          // ev + i od
-                }
+                if (valid) z = c2r_pre(a.spec[srow + m], cconj(a.spec[srow + (M - m)]), a.rtw[m]);
                 v[e] = z;
             }
         } else {
@@ -299,29 +442,20 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
         const int m = p + e * P;
         float2 o = make_float2(0.f, 0.f);
         if (valid) {
-            if constexpr (MODE == XM_R2C) {
-                o = opA[e];
-            } else if constexpr (MODE == XM_C2R) {
+            if constexpr (MODE == XM_C2R) {
                 st_real(a.dst, xrow, m, packed, v[e]);
-            } else if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
-                // uint16: I - uint16(Background) saturates at 0; single: max(I - Background, 0)
-                o = fmax0(make_float2(opA[e].x - a.bg, opA[e].y - a.bg));
+            } else {
+                o = x_elem<MODE>(v[e], opA[e], opB[e], a.bg, a.eps, lam);
+            }
+            if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
                 st_real(a.I, xrow, m, packed, o);
                 st_real(a.J, xrow, m, packed, o);
-            } else if constexpr (MODE == XM_RATIO) {
-                const float2 c = v[e];
-                o.x = opA[e].x / (c.x > a.eps ? c.x : a.eps);  // I ./ max(CX, eps)
-                o.y = opA[e].y / (c.y > a.eps ? c.y : a.eps);
             } else if constexpr (MODE == XM_OMW_UPDATE) {
-                o = fmax0(make_float2(v[e].x * opA[e].x, v[e].y * opA[e].y));
                 st_real(a.J, xrow, m, packed, o);
             } else if constexpr (MODE == XM_BIGGS_Y) {
-                const float2 j2 = opA[e], j3 = opB[e];
-                o = fmax0(make_float2(j2.x + lam * (j2.x - j3.x), j2.y + lam * (j2.y - j3.y)));
                 st_real(a.Y, xrow, m, packed, o);
             } else if constexpr (MODE == XM_BIGGS_UPDATE) {
                 const float2 y = opA[e];
-                o = fmax0(make_float2(y.x * v[e].x, y.y * v[e].y));
                 st_real(a.Jn, xrow, m, packed, o);
                 const float2 d = make_float2(o.x - y.x, o.y - y.y);
                 if (a.acc) {
@@ -347,24 +481,18 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
         fft_run<E, false, true>(v, pl, p, lds, lbase, 1, act);
         if (packed) {
             const int M = L;
-            // Z at home positions -> LDS, then X[k] = 1/2[(Z_k + Z*_{M-k}) - i W^k (Z_k - Z*_{M-k})]
+            // Z at home positions -> LDS, then X[k] from Z_k and Z*_{M-k}
 #pragma unroll
             for (int e = 0; e < E; ++e)
                 if (act) lds[lds_slot<true>(lbase, p + e * P, 1)] = v[e];
             __syncthreads();
-            const float2 wp = a.rtw[p];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = p + e * P;
                 if (!act) continue;
                 const float2 A = lds[lds_slot<true>(lbase, k, 1)];
                 const float2 B = cconj(lds[lds_slot<true>(lbase, k == 0 ? 0 : M - k, 1)]);
-                const float2 s = cadd(A, B);
-                const float2 wk = e == 0 ? wp : cmul(wp, a.rtw[e * P]);
-                const float2 d = cmul(csub(A, B), wk);
-                // -i * d = (d.y, -d.x)
-                const float2 X = make_float2(0.5f * (s.x + d.y), 0.5f * (s.y - d.x));
-                if (valid) a.spec[srow + k] = X;
+                if (valid) a.spec[srow + k] = r2c_post(A, B, a.rtw[k]);
             }
             if (valid && p == 0) {
                 const float2 z0 = lds[lbase];  // slot of position 0
@@ -401,6 +529,154 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
             }
         }
     }
+}
+
+// The same pass for an even row of n1 = 2L real values with L and E fixed at
+// compile time and 256-thread blocks of 256/P rows.  Rows past the end of the
+// volume read the last row (in bounds) and store nothing; every wave runs the
+// same instruction stream.
+template <int L, int E, int MODE>
+__device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restrict__ tw, float2* lds, int64_t blk) {
+    constexpr bool IN = (MODE == XM_C2R || MODE == XM_RATIO || MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
+    constexpr bool OUT_ALWAYS = (MODE == XM_R2C || MODE == XM_PREP_U16 || MODE == XM_PREP_F32 ||
+                                 MODE == XM_RATIO || MODE == XM_BIGGS_Y);
+    constexpr bool OUT_MAYBE = (MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
+    constexpr int P = L / E;
+    constexpr int RT = kMaxThreads / P;
+    constexpr int LP = row_pitch(L);
+    constexpr int HP = (L + 1 + 15) / 16 * 16;  // spectrum row pitch for h1 = L + 1
+    static_assert(P * E == L && RT * P == kMaxThreads, "row tile");
+    const int t = threadIdx.x;
+    const int rr = t / P;
+    const int p = t % P;
+    const int64_t row = blk * RT + rr;
+    const bool valid = row < a.nrows;
+    const int64_t rowc = valid ? row : a.nrows - 1;
+    const int lbase = rr * LP;
+    float2* __restrict__ sp = a.spec + rowc * HP;
+    const float2* __restrict__ rt = a.rtw + p;
+    const int64_t r2 = rowc * L;  // real row start in float2 (pairs)
+
+    float2 opA[E], opB[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int m = p + e * P;
+        opA[e] = make_float2(0.f, 0.f);
+        opB[e] = make_float2(0.f, 0.f);
+        if constexpr (MODE == XM_R2C) opA[e] = reinterpret_cast<const float2*>(a.src)[r2 + m];
+        if constexpr (MODE == XM_PREP_U16) {
+            const uint32_t w = reinterpret_cast<const uint32_t*>(a.raw16)[r2 + m];
+            opA[e] = make_float2((float)(w & 0xffffu), (float)(w >> 16));
+        }
+        if constexpr (MODE == XM_PREP_F32) opA[e] = reinterpret_cast<const float2*>(a.rawf)[r2 + m];
+        if constexpr (MODE == XM_RATIO) opA[e] = reinterpret_cast<const float2*>(a.I)[r2 + m];
+        if constexpr (MODE == XM_OMW_UPDATE) opA[e] = reinterpret_cast<const float2*>(a.J)[r2 + m];
+        if constexpr (MODE == XM_BIGGS_Y) {
+            opA[e] = reinterpret_cast<const float2*>(a.J)[r2 + m];
+            opB[e] = reinterpret_cast<const float2*>(a.J3)[r2 + m];
+        }
+        if constexpr (MODE == XM_BIGGS_UPDATE) {
+            opA[e] = reinterpret_cast<const float2*>(a.Y)[r2 + m];
+            if (a.acc) opB[e] = reinterpret_cast<const float2*>(a.J4)[r2 + m];
+        }
+    }
+
+    float2 v[E];
+    if constexpr (IN) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int m = p + e * P;
+            v[e] = c2r_pre(sp[m], cconj(sp[L - m]), rt[e * P]);
+        }
+        ct_fft<L, E, true>(v, p, lds, lbase, tw);
+    }
+
+    double s_num = 0.0, s_den = 0.0;
+    float lam = 0.f;
+    if constexpr (MODE == XM_BIGGS_Y) lam = *a.lambda;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int m = p + e * P;
+        float2 o = make_float2(0.f, 0.f);
+        if constexpr (MODE == XM_C2R) {
+            if (valid) reinterpret_cast<float2*>(a.dst)[r2 + m] = v[e];
+        } else {
+            o = x_elem<MODE>(v[e], opA[e], opB[e], a.bg, a.eps, lam);
+        }
+        if (valid) {
+            if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
+                reinterpret_cast<float2*>(a.I)[r2 + m] = o;
+                reinterpret_cast<float2*>(a.J)[r2 + m] = o;
+            } else if constexpr (MODE == XM_OMW_UPDATE) {
+                reinterpret_cast<float2*>(a.J)[r2 + m] = o;
+            } else if constexpr (MODE == XM_BIGGS_Y) {
+                reinterpret_cast<float2*>(a.Y)[r2 + m] = o;
+            } else if constexpr (MODE == XM_BIGGS_UPDATE) {
+                const float2 y = opA[e];
+                reinterpret_cast<float2*>(a.Jn)[r2 + m] = o;
+                const float2 d = make_float2(o.x - y.x, o.y - y.y);
+                if (a.acc) {
+                    const float2 j4 = opB[e];
+                    s_num += (double)(d.x * j4.x);
+                    s_den += (double)(j4.x * j4.x);
+                    s_num += (double)(d.y * j4.y);
+                    s_den += (double)(j4.y * j4.y);
+                }
+                if (a.write_j4) reinterpret_cast<float2*>(a.J4)[r2 + m] = d;
+            }
+        }
+        v[e] = o;
+    }
+
+    bool out_spec = OUT_ALWAYS;
+    if constexpr (OUT_MAYBE) out_spec = a.r2c_out != 0;
+    if (out_spec) {
+        ct_fft<L, E, false>(v, p, lds, lbase, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) lds[lds_slot<true>(lbase, p + e * P, 1)] = v[e];
+        row_sync<P>();
+        float2 X[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k = p + e * P;
+            const float2 A = lds[lds_slot<true>(lbase, k, 1)];
+            const float2 B = cconj(lds[lds_slot<true>(lbase, (L - k) & (L - 1), 1)]);
+            X[e] = r2c_post(A, B, rt[e * P]);
+        }
+        const float2 z0 = lds[lbase];
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) sp[p + e * P] = X[e];
+            if (p == 0) sp[L] = make_float2(z0.x - z0.y, 0.f);
+        }
+    }
+
+    if constexpr (MODE == XM_BIGGS_UPDATE) {
+        if (a.acc) {
+            __syncthreads();
+            double* red = reinterpret_cast<double*>(lds);
+            red[2 * t] = s_num;
+            red[2 * t + 1] = s_den;
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if (t < w) {
+                    red[2 * t] += red[2 * (t + w)];
+                    red[2 * t + 1] += red[2 * (t + w) + 1];
+                }
+                __syncthreads();
+            }
+            if (t == 0) {
+                a.partials[2 * blk] = red[0];
+                a.partials[2 * blk + 1] = red[1];
+            }
+        }
+    }
+}
+
+template <int L, int E, int MODE>
+__global__ void __launch_bounds__(kMaxThreads) k_xrow(XArgs a, const float2* __restrict__ tw) {
+    extern __shared__ float2 lds[];
+    xrow_body<L, E, MODE>(a, tw, lds, blockIdx.x);
 }
 
 // One kernel per (E, mode) so each mode gets its own register allocation.

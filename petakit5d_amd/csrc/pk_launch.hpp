@@ -42,6 +42,25 @@ hipError_t launch_colpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStr
                                 const int64_t* starts, int n);
 hipError_t launch_colpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
 bool e_supported(int E);
+template <int L, int E>
+inline void launch_modes_xrow(dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw) {
+    const dim3 b(kMaxThreads);
+    switch (a.mode) {
+        case XM_R2C: hipLaunchKernelGGL((k_xrow<L, E, XM_R2C>), grid, b, lds, s, a, tw); break;
+        case XM_C2R: hipLaunchKernelGGL((k_xrow<L, E, XM_C2R>), grid, b, lds, s, a, tw); break;
+        case XM_PREP_U16: hipLaunchKernelGGL((k_xrow<L, E, XM_PREP_U16>), grid, b, lds, s, a, tw); break;
+        case XM_PREP_F32: hipLaunchKernelGGL((k_xrow<L, E, XM_PREP_F32>), grid, b, lds, s, a, tw); break;
+        case XM_RATIO: hipLaunchKernelGGL((k_xrow<L, E, XM_RATIO>), grid, b, lds, s, a, tw); break;
+        case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xrow<L, E, XM_OMW_UPDATE>), grid, b, lds, s, a, tw); break;
+        case XM_BIGGS_Y: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_Y>), grid, b, lds, s, a, tw); break;
+        case XM_BIGGS_UPDATE: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_UPDATE>), grid, b, lds, s, a, tw); break;
+        default: break;
+    }
+}
+// Compile-time row kernels for the (L, E) pairs in pk_kern_xrow_*.hip; false
+// when the pair has none (the caller then uses the generic kernel).
+bool launch_xrow(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
+bool xrow_available(int L, int E);
 template <int E>
 inline void launch_modes_xb(dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch,
                             const int64_t* starts, int n) {
