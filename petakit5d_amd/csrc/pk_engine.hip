@@ -328,8 +328,8 @@ void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a) {
         switch (a.mode) {
             case XM_R2C: name = "x_r2c"; b = 4 * N + 8 * S; break;
             case XM_C2R: name = "x_c2r"; b = 8 * S + 4 * N; break;
-            case XM_PREP_U16: name = "x_prep"; b = 2 * N + 8 * N + 8 * S; break;
-            case XM_PREP_F32: name = "x_prep"; b = 4 * N + 8 * N + 8 * S; break;
+            case XM_PREP_U16: name = "x_prep"; b = 2 * N + (a.J ? 8 : 4) * N + 8 * S; break;
+            case XM_PREP_F32: name = "x_prep"; b = 4 * N + (a.J ? 8 : 4) * N + 8 * S; break;
             case XM_RATIO: name = "x_ratio"; b = 16 * S + 4 * N; break;
             case XM_OMW_UPDATE: name = a.r2c_out ? "x_update_r2c" : "x_update"; b = 8 * S + 8 * N + (a.r2c_out ? 8 * S : 0); break;
             case XM_BIGGS_Y: name = "x_biggs_y"; b = 12 * N + 8 * S; break;
@@ -597,9 +597,9 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         // saturates at 0; single I gives max(I - Background, 0)
         {
             XArgs a{};
-                    a.spec = A;
+            a.spec = A;
             a.I = I;
-            a.J = J;
+            a.J = method == 1 ? nullptr : J;  // OMW: the first update reads I (J_1 = I) instead of a copy
             if (raw16) {
                 a.mode = XM_PREP_U16;
                 a.raw16 = raw16;
@@ -625,6 +625,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
                 u.mode = XM_OMW_UPDATE;
                 u.spec = A;
                 u.J = J;
+                u.Jin = k == 1 ? I : nullptr;
                 u.r2c_out = k < numit;
                 run_x(vp, u);
             }
@@ -708,7 +709,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         dout = ws.out.p;
     }
     EpiArgs e{};
-    e.J = J;
+    e.J = (method == 1 && numit == 0 && !(o->flags & PK_FLAG_UNFUSED)) ? I : J;  // OMW with no iteration: J = I
     e.I = I;
     e.mask = mask;
     e.raw16 = raw16;

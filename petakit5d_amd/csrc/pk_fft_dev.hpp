@@ -295,7 +295,8 @@ struct XArgs {
     const uint16_t* raw16;
     const float* rawf;
     float* I;          // background-subtracted input
-    float* J;          // estimate (OMW J, Biggs J2 for mode 6)
+    float* J;          // estimate (OMW J, Biggs J2 for mode 6); modes 2/3 store it only when non-null
+    const float* Jin;  // mode 5: read the estimate from here instead of J when non-null (first iteration: I)
     const float* J3;   // Biggs x_{k-2}
     float* Y;          // Biggs prediction
     float* Jn;         // Biggs new estimate
@@ -398,7 +399,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
         if constexpr (MODE == XM_PREP_U16) opA[e] = ld_u16(a.raw16, xrow, m, packed);
         if constexpr (MODE == XM_PREP_F32) opA[e] = ld_real(a.rawf, xrow, m, packed);
         if constexpr (MODE == XM_RATIO) opA[e] = ld_real(a.I, xrow, m, packed);
-        if constexpr (MODE == XM_OMW_UPDATE) opA[e] = ld_real(a.J, xrow, m, packed);
+        if constexpr (MODE == XM_OMW_UPDATE) opA[e] = ld_real(a.Jin ? a.Jin : a.J, xrow, m, packed);
         if constexpr (MODE == XM_BIGGS_Y) {
             opA[e] = ld_real(a.J, xrow, m, packed);
             opB[e] = ld_real(a.J3, xrow, m, packed);
@@ -449,7 +450,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
             }
             if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
                 st_real(a.I, xrow, m, packed, o);
-                st_real(a.J, xrow, m, packed, o);
+                if (a.J) st_real(a.J, xrow, m, packed, o);
             } else if constexpr (MODE == XM_OMW_UPDATE) {
                 st_real(a.J, xrow, m, packed, o);
             } else if constexpr (MODE == XM_BIGGS_Y) {
@@ -570,7 +571,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
         }
         if constexpr (MODE == XM_PREP_F32) opA[e] = reinterpret_cast<const float2*>(a.rawf)[r2 + m];
         if constexpr (MODE == XM_RATIO) opA[e] = reinterpret_cast<const float2*>(a.I)[r2 + m];
-        if constexpr (MODE == XM_OMW_UPDATE) opA[e] = reinterpret_cast<const float2*>(a.J)[r2 + m];
+        if constexpr (MODE == XM_OMW_UPDATE) opA[e] = reinterpret_cast<const float2*>(a.Jin ? a.Jin : a.J)[r2 + m];
         if constexpr (MODE == XM_BIGGS_Y) {
             opA[e] = reinterpret_cast<const float2*>(a.J)[r2 + m];
             opB[e] = reinterpret_cast<const float2*>(a.J3)[r2 + m];
@@ -606,7 +607,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
         if (valid) {
             if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
                 reinterpret_cast<float2*>(a.I)[r2 + m] = o;
-                reinterpret_cast<float2*>(a.J)[r2 + m] = o;
+                if (a.J) reinterpret_cast<float2*>(a.J)[r2 + m] = o;
             } else if constexpr (MODE == XM_OMW_UPDATE) {
                 reinterpret_cast<float2*>(a.J)[r2 + m] = o;
             } else if constexpr (MODE == XM_BIGGS_Y) {

@@ -30,6 +30,8 @@ def _flip(p):
     ((64, 64, 32), (15, 15, 21), 15),
     ((45, 30, 27), (9, 9, 11), 7),
     ((50, 36, 1), (7, 7, 1), 5),
+    ((256, 20, 12), (9, 9, 7), 6),   # 128-point rows
+    ((2048, 6, 8), (7, 5, 5), 4),    # 1024-point rows (a row spans two wavefronts)
 ])
 def test_rl_biggs_vs_oracle(oracle, shape, psf_shape, numit):
     import petakit5d_amd as pk
@@ -48,6 +50,11 @@ def test_rl_biggs_vs_oracle(oracle, shape, psf_shape, numit):
 @pytest.mark.parametrize("shape,psf_shape,numit", [
     ((64, 64, 32), (15, 15, 21), 2),
     ((48, 40, 30), (11, 13, 9), 3),
+    ((64, 40, 30), (11, 13, 9), 0),   # no iteration: the epilogue of I
+    ((64, 40, 30), (11, 13, 9), 1),
+    ((1024, 12, 10), (9, 9, 7), 2),   # 512-point rows
+    ((512, 18, 10), (9, 9, 7), 2),    # 256-point rows
+    ((96, 30, 20), (9, 9, 7), 2),     # 48-point rows (runtime-plan kernel)
 ])
 def test_omw_vs_oracle(oracle, shape, psf_shape, numit):
     import petakit5d_amd as pk
