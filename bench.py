@@ -286,13 +286,14 @@ def cpu_and_parity(args, res, ctx, pk, I, psf, psf_b, iters, method, nvox, shape
                                          "(oracle FP32, full-spectrum FFTs, OTF build included), " f"{tc:.1f} s"}
     if not args.no_check:
         par = {"rel_l2_vs_cpu_fp32": rel_l2(Jg, Jc)}
-        if nvox <= 2 ** 22:
+        if nvox <= 2 ** 27:
             if method == "omw":
                 J64, _ = po.decon_lucy_omw_function(I, psf, psf_b, iters, Background=100, save16bit=False,
                                                     precision="f64")
             else:
                 J64, _, _ = po.decon_lucy_function(I, psf, iters, Background=100, save16bit=False, precision="f64")
             par["rel_l2_vs_fp64"] = rel_l2(Jg, J64)
+            par["cpu_fp32_vs_fp64"] = rel_l2(Jc, J64)
         res["parity"] = par
 
 

@@ -10,7 +10,9 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpkdecon.so")
+# PKDECON_LIB selects another build of the library (e.g. one compiled with
+# different launch attributes); default: the in-tree build.
+LIB_PATH = os.environ.get("PKDECON_LIB") or os.path.join(_HERE, "libpkdecon.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "pkdecon.h")
 
 PK_OK = 0

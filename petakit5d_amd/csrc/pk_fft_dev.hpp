@@ -532,6 +532,10 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
     }
 }
 
+// Compile-time row kernels load their real-space operands after the inverse
+// transform when true (fewer live registers, exposed load latency).
+constexpr bool kXrowLateOperands = false;
+
 // The same pass for an even row of n1 = 2L real values with L and E fixed at
 // compile time and 256-thread blocks of 256/P rows.  Rows past the end of the
 // volume read the last row (in bounds) and store nothing; every wave runs the
@@ -559,29 +563,35 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
     const int64_t r2 = rowc * L;  // real row start in float2 (pairs)
 
     float2 opA[E], opB[E];
+    auto load_operands = [&]() {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int m = p + e * P;
-        opA[e] = make_float2(0.f, 0.f);
-        opB[e] = make_float2(0.f, 0.f);
-        if constexpr (MODE == XM_R2C) opA[e] = reinterpret_cast<const float2*>(a.src)[r2 + m];
-        if constexpr (MODE == XM_PREP_U16) {
-            const uint32_t w = reinterpret_cast<const uint32_t*>(a.raw16)[r2 + m];
-            opA[e] = make_float2((float)(w & 0xffffu), (float)(w >> 16));
+        for (int e = 0; e < E; ++e) {
+            const int m = p + e * P;
+            opA[e] = make_float2(0.f, 0.f);
+            opB[e] = make_float2(0.f, 0.f);
+            if constexpr (MODE == XM_R2C) opA[e] = reinterpret_cast<const float2*>(a.src)[r2 + m];
+            if constexpr (MODE == XM_PREP_U16) {
+                const uint32_t w = reinterpret_cast<const uint32_t*>(a.raw16)[r2 + m];
+                opA[e] = make_float2((float)(w & 0xffffu), (float)(w >> 16));
+            }
+            if constexpr (MODE == XM_PREP_F32) opA[e] = reinterpret_cast<const float2*>(a.rawf)[r2 + m];
+            if constexpr (MODE == XM_RATIO) opA[e] = reinterpret_cast<const float2*>(a.I)[r2 + m];
+            if constexpr (MODE == XM_OMW_UPDATE)
+                opA[e] = reinterpret_cast<const float2*>(a.Jin ? a.Jin : a.J)[r2 + m];
+            if constexpr (MODE == XM_BIGGS_Y) {
+                opA[e] = reinterpret_cast<const float2*>(a.J)[r2 + m];
+                opB[e] = reinterpret_cast<const float2*>(a.J3)[r2 + m];
+            }
+            if constexpr (MODE == XM_BIGGS_UPDATE) {
+                opA[e] = reinterpret_cast<const float2*>(a.Y)[r2 + m];
+                if (a.acc) opB[e] = reinterpret_cast<const float2*>(a.J4)[r2 + m];
+            }
         }
-        if constexpr (MODE == XM_PREP_F32) opA[e] = reinterpret_cast<const float2*>(a.rawf)[r2 + m];
-        if constexpr (MODE == XM_RATIO) opA[e] = reinterpret_cast<const float2*>(a.I)[r2 + m];
-        if constexpr (MODE == XM_OMW_UPDATE) opA[e] = reinterpret_cast<const float2*>(a.Jin ? a.Jin : a.J)[r2 + m];
-        if constexpr (MODE == XM_BIGGS_Y) {
-            opA[e] = reinterpret_cast<const float2*>(a.J)[r2 + m];
-            opB[e] = reinterpret_cast<const float2*>(a.J3)[r2 + m];
-        }
-        if constexpr (MODE == XM_BIGGS_UPDATE) {
-            opA[e] = reinterpret_cast<const float2*>(a.Y)[r2 + m];
-            if (a.acc) opB[e] = reinterpret_cast<const float2*>(a.J4)[r2 + m];
-        }
-    }
-
+    };
+    // real-space operands: loaded before the inverse transform (their latency
+    // overlaps it) unless kXrowLateOperands, which trades that overlap for
+    // fewer live registers across the transform
+    if constexpr (!(IN && kXrowLateOperands)) load_operands();
     float2 v[E];
     if constexpr (IN) {
 #pragma unroll
@@ -590,6 +600,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
             v[e] = c2r_pre(sp[m], cconj(sp[L - m]), rt[e * P]);
         }
         ct_fft<L, E, true>(v, p, lds, lbase, tw);
+        if constexpr (kXrowLateOperands) load_operands();
     }
 
     double s_num = 0.0, s_den = 0.0;
