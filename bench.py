@@ -343,8 +343,10 @@ def wl_batched(args, torch, pk, L):
     for method, iters in (("omw", args.iters or 2), ("rl", 10)):
         fn = lambda: pk.decon_batch(chunks, psf, psf_b, iters, method=method, Background=100, ctx=ctx)  # noqa: E731
         ms, _, prof = timed(fn, max(2, args.steps // 3), 1, torch, ctx)
+        nb = max(2, args.steps // 3)
         rec = {"Mvoxel_s": round(nvox / 1e6 / (ms / 1e3), 1), "ms_per_batch": round(ms, 2),
-               "launches_per_batch": int(sum(v[0] for v in prof.values()) / max(2, args.steps // 3))}
+               "launches_per_batch": int(sum(v[0] for v in prof.values()) / nb),
+               "passes_ms_per_batch": {k: round(v[1] / nb, 3) for k, v in sorted(prof.items())}}
         # one fused call per chunk (chunks padded by the Python wrapper when not smooth)
         def seq(plain=False):
             for c in chunks:
@@ -355,7 +357,8 @@ def wl_batched(args, torch, pk, L):
                                                flags=PLAIN if plain else 0)
                 else:
                     pk.decon_lucy_function(c, pf, iters, Background=100, pad=True, ctx=ctx, flags=PLAIN if plain else 0)
-        sms, _, _ = timed(seq, 1, 1, torch)
+        sms, _, sprof = timed(seq, 1, 1, torch, ctx)
+        rec["one_call_per_chunk_passes_ms"] = {k: round(v[1], 3) for k, v in sorted(sprof.items())}
         rec["one_call_per_chunk_Mvoxel_s"] = round(nvox / 1e6 / (sms / 1e3), 1)
         if not args.no_plain:
             pms, _, _ = timed(lambda: seq(True), 1, 0, torch)
@@ -452,10 +455,13 @@ def wl_served(args, torch, pk, L):
                 d = at - (time.perf_counter() - t00)
                 if d > 0:
                     time.sleep(d)
+                # PSFs centre-cropped to small volumes as the server does (RLdecon.m:264-270)
+                pf = pk.crop_psf_to_data(psfs[ch], v.shape)
                 if meth == "omw":
-                    pk.decon_lucy_omw_function(v, psfs[ch], bps[ch], it, Background=100, pad=True, ctx=ctx, flags=PLAIN)
+                    pb = pk.crop_psf_to_data(bps[ch], v.shape)
+                    pk.decon_lucy_omw_function(v, pf, pb, it, Background=100, pad=True, ctx=ctx, flags=PLAIN)
                 else:
-                    pk.decon_lucy_function(v, psfs[ch], it, Background=100, pad=True, ctx=ctx, flags=PLAIN)
+                    pk.decon_lucy_function(v, pf, it, Background=100, pad=True, ctx=ctx, flags=PLAIN)
                 lat_p.append((time.perf_counter() - t00 - at) * 1e3)
             done.set()
 

@@ -15,8 +15,11 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <array>
+#include <tuple>
 #include <vector>
 
 #include "pk_engine.hpp"
@@ -34,6 +37,8 @@ template <class T>
 struct Group {
     int E = 0;
     int mode = 0;
+    int nt = 256;  // block size the kernels are compiled for
+    int L = 0;     // x passes: row length of a compile-time row kernel shared by the group (0: runtime plan)
     std::vector<T> chunks;
     std::vector<int64_t> starts{0};  // block offsets, size chunks+1
     unsigned block = 1;
@@ -41,8 +46,11 @@ struct Group {
     size_t dev_off = 0;
     size_t start_off = 0;
 };
+// Chunks share a launch only when their blocks have the same shape: values per
+// thread, mode, compiled block size, row length of a compile-time row kernel,
+// threads per block and LDS bytes (no idle threads, no over-allocated LDS).
 template <class T>
-using Pass = std::map<std::pair<int, int>, Group<T>>;  // keyed by (E, mode)
+using Pass = std::map<std::array<int64_t, 6>, Group<T>>;
 
 int64_t smooth_up(int64_t n) {
     while (!smooth235(n)) ++n;
@@ -241,9 +249,12 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
     };
     auto add_x = [&](Pass<XChunk>& ps, const Chunk& c, const XArgs& a) {
         const VolPlan& vp = *c.vp;
-        Group<XChunk>& g = ps[{vp.x.fp.E, a.mode}];
+        // chunks whose rows have a compile-time kernel are grouped by row length
+        const int ctL = (vp.packed && vp.x.block.x == 256 && xrow_available(vp.x.fp.L, vp.x.fp.E)) ? vp.x.fp.L : 0;
+        Group<XChunk>& g = ps[{vp.x.fp.E, a.mode, 256, ctL, (int64_t)vp.x.block.x, (int64_t)vp.x.lds}];
         g.E = vp.x.fp.E;
         g.mode = a.mode;
+        g.L = ctL;
         g.chunks.push_back(XChunk{a, vp.x.fp});
         g.starts.push_back(g.starts.back() + (vp.n2 * vp.n3 + vp.x.tile - 1) / vp.x.tile);
         g.block = std::max(g.block, vp.x.block.x);
@@ -274,9 +285,10 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
             gx = (a.acount + a.T - 1) / a.T;
             gy = 1;
         }
-        Group<ColChunk>& g = ps[{ax.fp.E, mode}];
+        Group<ColChunk>& g = ps[{ax.fp.E, mode, ax.nt, 0, (int64_t)ax.block.x, (int64_t)ax.lds}];
         g.E = ax.fp.E;
         g.mode = mode;
+        g.nt = ax.nt;
         g.chunks.push_back(ColChunk{a, ax.fp, gx});
         g.starts.push_back(g.starts.back() + gx * gy);
         g.block = std::max(g.block, ax.block.x);
@@ -364,17 +376,38 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
         std::memcpy(table.data() + lam_off, lam.data(), sizeof(LamChunk) * lam.size());
     }
     if (table.empty()) return;
-    ensure(ws.batch_tab, table.size());
-    PK_HIP(hipMemcpyAsync(ws.batch_tab.p, table.data(), table.size(), hipMemcpyHostToDevice, stream));
-    char* tab = ws.batch_tab.as<char>();
+    const int slot = ws.tab_slot;
+    ws.tab_slot ^= 1;
+    if (ws.tab_ev[slot]) PK_HIP(hipEventSynchronize(ws.tab_ev[slot]));  // its previous upload has been read
+    else PK_HIP(hipEventCreateWithFlags(&ws.tab_ev[slot], hipEventDisableTiming));
+    HostBuf& hb = ws.tab_host[slot];
+    if (hb.bytes < table.size()) {
+        if (hb.p) PK_HIP(hipHostFree(hb.p));
+        hb.p = nullptr;
+        hb.bytes = 0;
+        const size_t want = std::max<size_t>(table.size() * 2, 64 << 10);
+        PK_HIP(hipHostMalloc(&hb.p, want, hipHostMallocDefault));
+        hb.bytes = want;
+    }
+    std::memcpy(hb.p, table.data(), table.size());
+    // the device table is reused two steps later, after this step's kernels in stream order
+    ensure(ws.tab_dev[slot], table.size());
+    PK_HIP(hipMemcpyAsync(ws.tab_dev[slot].p, hb.p, table.size(), hipMemcpyHostToDevice, stream));
+    PK_HIP(hipEventRecord(ws.tab_ev[slot], stream));
+    char* tab = ws.tab_dev[slot].as<char>();
 
     auto run_x = [&](const Pass<XChunk>& ps, const char* name) {
         for (auto& kv : ps) {
             const auto& g = kv.second;
             hipEvent_t e0 = prof_mark();
-            PK_HIP(launch_xpass_batch(g.E, dim3((unsigned)g.starts.back()), dim3(g.block), g.lds, stream, g.mode,
-                                      reinterpret_cast<const XChunk*>(tab + g.dev_off),
-                                      reinterpret_cast<const int64_t*>(tab + g.start_off), (int)g.chunks.size()));
+            const XChunk* chd = reinterpret_cast<const XChunk*>(tab + g.dev_off);
+            const int64_t* std_ = reinterpret_cast<const int64_t*>(tab + g.start_off);
+            if (g.L && launch_xrow_batch(g.L, g.E, dim3((unsigned)g.starts.back()), g.lds, stream, g.mode, chd, std_,
+                                         (int)g.chunks.size()))
+                PK_HIP(hipGetLastError());
+            else
+                PK_HIP(launch_xpass_batch(g.E, dim3((unsigned)g.starts.back()), dim3(g.block), g.lds, stream, g.mode,
+                                          chd, std_, (int)g.chunks.size()));
             prof_done(name, 0, e0);
         }
     };
@@ -382,7 +415,7 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
         for (auto& kv : ps) {
             const auto& g = kv.second;
             hipEvent_t e0 = prof_mark();
-            PK_HIP(launch_colpass_batch(g.E, dim3((unsigned)g.starts.back()), dim3(g.block), g.lds, stream, g.mode,
+            PK_HIP(launch_colpass_batch(g.E, g.nt, dim3((unsigned)g.starts.back()), dim3(g.block), g.lds, stream, g.mode,
                                         reinterpret_cast<const ColChunk*>(tab + g.dev_off),
                                         reinterpret_cast<const int64_t*>(tab + g.start_off), (int)g.chunks.size()));
             prof_done(name, 0, e0);

@@ -132,7 +132,9 @@ struct Tune {
     int e[3] = {0, 0, 0};
     int t[3] = {0, 0, 0};
     bool generic_x = false;  // PK_FFT_GENERIC_X=1: always the runtime-plan row kernel
+    int slab = -1;           // PK_SLAB_PLANES: planes per slab (0 = off), -1 = built-in
     Tune() {
+        if (const char* v = std::getenv("PK_SLAB_PLANES")) slab = std::atoi(v);
         if (const char* v = std::getenv("PK_FFT_GENERIC_X")) generic_x = std::atoi(v) != 0;
         const char* names_e[3] = {"PK_FFT_EX", "PK_FFT_EY", "PK_FFT_EZ"};
         const char* names_t[3] = {nullptr, "PK_FFT_TY", "PK_FFT_TZ"};
@@ -264,16 +266,40 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
         ax.lds = std::max<size_t>((size_t)rt * ax.lpad * sizeof(float2), 2 * sizeof(double) * kMaxThreads);
     }
     // y, z (columns)
+    // y, z (columns): tiles of T adjacent columns (T float2 per row segment);
+    // prefer T = 16 (128-byte segments), then fewest stages, then E near 16,
+    // then the smallest block.  Long columns use 512- / 1024-thread blocks and
+    // up to 160 KB of LDS (T * L * 8 bytes) to keep T >= 8.
     auto colplan = [&](AxisLaunch& ax, int L, int axis) {
-        // columns: fewest stages, then values per thread near 16
-        ax.fp = plan_1d(L, 16, kMaxThreads, tune().e[axis], 100, 1);
+        int best = 1 << 30;
+        for (int E : kEs) {
+            if (L % E) continue;
+            if (tune().e[axis] > 0 && E != tune().e[axis] && L != 1) continue;
+            bool ok = e_supported(E);
+            for (int pr : {2, 3, 5})
+                if (L % pr == 0 && E % pr != 0) ok = false;
+            if (!ok) continue;
+            const FftPlan fp = plan_1d(L, E, kMaxThreads * 4, E, 100, 1);
+            if (fp.E != E) continue;
+            for (int nt : {kMaxThreads, 512, 1024}) {
+                if (nt != kMaxThreads && !col_wide_available(E, nt)) continue;
+                if (fp.P > nt) continue;
+                int T = 1;
+                while (T * 2 <= 16 && T * 2 * fp.P <= nt && (size_t)T * 2 * L * sizeof(float2) <= kMaxLds) T *= 2;
+                if (tune().t[axis] > 0) T = std::min(T, tune().t[axis]);
+                const int score = (16 - T) * 1000 + fp.nst * 100 + std::abs(E - 16) * 2 + (nt > kMaxThreads ? 1 : 0);
+                if (score < best) {
+                    best = score;
+                    ax.fp = fp;
+                    ax.tile = T;
+                    ax.nt = nt;
+                }
+            }
+        }
+        if (best == 1 << 30) fail(PK_ERR_UNSUPPORTED_SIZE, "no transform plan for length " + std::to_string(L));
         ax.fp.tw = table(L);
-        int T = std::max(1, kMaxThreads / ax.fp.P);
-        if (tune().t[axis] > 0) T = std::min(T, tune().t[axis]);
-        while (T > 1 && (size_t)T * L * sizeof(float2) > 65536) T--;
-        ax.tile = T;
-        ax.block = dim3(T * ax.fp.P);
-        ax.lds = std::max<size_t>((size_t)T * L * sizeof(float2), 16);
+        ax.block = dim3(ax.tile * ax.fp.P);
+        ax.lds = std::max<size_t>((size_t)ax.tile * L * sizeof(float2), 16);
     };
     colplan(vp.y, (int)n2, 1);
     colplan(vp.z, (int)n3, 2);
@@ -301,16 +327,39 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
 }
 
 // ------------------------------------------------------------------ passes
-void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a) {
+void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a, int64_t z0, int64_t nz) {
     a.n1 = (int)vp.n1;
     a.h1 = (int)vp.h1;
     a.hp = (int)vp.hp;
     a.packed = vp.packed;
     a.RT = vp.x.tile;
     a.lpad = vp.x.lpad;
-    a.nrows = vp.n2 * vp.n3;
     a.rtw = vp.rtw;
     a.eps = kEps;
+    if (nz <= 0) nz = vp.n3 - z0;
+    a.nrows = vp.n2 * nz;
+    if (z0 > 0) {
+        // slab: every array advanced to plane z0; block partials continue the
+        // whole-volume numbering (slab boundaries fall on block boundaries)
+        const int64_t ro = z0 * vp.n2 * vp.n1;
+        auto adv = [&](auto*& ptr) {
+            if (ptr) ptr += ro;
+        };
+        adv(a.src);
+        adv(a.dst);
+        adv(a.raw16);
+        adv(a.rawf);
+        adv(a.I);
+        adv(a.J);
+        adv(a.Jin);
+        adv(a.J3);
+        adv(a.Y);
+        adv(a.Jn);
+        adv(a.J4);
+        a.spec += z0 * vp.n2 * vp.hp;
+        if (a.partials) a.partials += 2 * (z0 * vp.n2 / a.RT);
+    }
+    const double frac = (double)nz / (double)vp.n3;
     const int64_t nblk = (a.nrows + a.RT - 1) / a.RT;
     hipEvent_t e0 = prof_mark();
     // compile-time kernel when one exists for this (row length, values per
@@ -335,14 +384,17 @@ void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a) {
             case XM_BIGGS_Y: name = "x_biggs_y"; b = 12 * N + 8 * S; break;
             case XM_BIGGS_UPDATE: name = "x_biggs_update"; b = 8 * S + 12 * N + (a.acc ? 4 * N : 0) + (a.r2c_out ? 8 * S : 0); break;
         }
-        prof_done(name, b, e0);
+        prof_done(name, b * frac, e0);
     }
 }
 
 static int64_t xpass_blocks(const VolPlan& vp) { return (vp.n2 * vp.n3 + vp.x.tile - 1) / vp.x.tile; }
 
-void pk_ctx_impl::run_col(const VolPlan& vp, int axis, float2* data, int mode, const float2* otf, float scale) {
+void pk_ctx_impl::run_col(const VolPlan& vp, int axis, float2* data, int mode, const float2* otf, float scale,
+                          int64_t z0, int64_t nz) {
     const AxisLaunch& ax = axis == 2 ? vp.y : vp.z;
+    if (nz <= 0) nz = vp.n3 - z0;
+    if (axis == 2 && z0 > 0) data += z0 * vp.hp * vp.n2;  // dim-2 columns of planes [z0, z0 + nz)
     if (ax.fp.L == 1 && (mode == CM_FWD || mode == CM_INV)) return;  // length-1 transform: identity
     ColArgs a{};
     a.data = data;
@@ -355,7 +407,7 @@ void pk_ctx_impl::run_col(const VolPlan& vp, int axis, float2* data, int mode, c
         a.acount = vp.h1;
         a.bstride = vp.hp * vp.n2;
         a.stride = vp.hp;
-        grid = dim3((unsigned)((vp.h1 + a.T - 1) / a.T), (unsigned)vp.n3);
+        grid = dim3((unsigned)((vp.h1 + a.T - 1) / a.T), (unsigned)nz);
     } else {
         a.acount = vp.hp * vp.n2;
         a.amod = vp.hp;
@@ -365,12 +417,13 @@ void pk_ctx_impl::run_col(const VolPlan& vp, int axis, float2* data, int mode, c
         grid = dim3((unsigned)((a.acount + a.T - 1) / a.T), 1);
     }
     hipEvent_t e0 = prof_mark();
-    PK_HIP(launch_colpass(ax.fp, grid, ax.block, ax.lds, stream, a));
+    PK_HIP(launch_colpass(ax.fp, ax.nt, grid, ax.block, ax.lds, stream, a));
     if (prof_on) {
         const double S = (double)vp.Sa;
         const char* name = axis == 2 ? (mode == CM_INV ? "y_inv" : mode == CM_FWD ? "y_fwd" : "y_conv")
                                      : (mode == CM_INV ? "z_inv" : mode == CM_FWD ? "z_fwd" : "z_conv");
-        prof_done(name, mode >= CM_CONV ? 24 * S : 16 * S, e0);
+        const double frac = axis == 2 ? (double)nz / (double)vp.n3 : 1.0;
+        prof_done(name, (mode >= CM_CONV ? 24 * S : 16 * S) * frac, e0);
     }
 }
 
@@ -394,14 +447,18 @@ void pk_ctx_impl::c2r3d(const VolPlan& vp, float2* spec, float* real) {
     run_x(vp, a);
 }
 
-void pk_ctx_impl::spectral_conv(const VolPlan& vp, float2* A, const float2* H, bool conj) {
-    const float scale = (float)(1.0 / (double)vp.N);  // ifftn's 1/N
-    run_col(vp, 2, A, CM_FWD, nullptr, 1.f);
-    run_col(vp, 3, A, conj ? CM_CONVJ : CM_CONV, H, scale);
-    run_col(vp, 2, A, CM_INV, nullptr, 1.f);
-}
-
 static int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 65536); }
+
+// Planes per slab for the dim-1 / dim-2 passes: the chain (dim-2 inverse,
+// dim-1 step, dim-2 forward) runs slab by slab so its working set stays in the
+// device's memory-side cache; 0 = whole volume.  A slab holds whole x-pass
+// blocks (nz * n2 a multiple of the rows per block).  PK_SLAB_PLANES overrides.
+int64_t pk_ctx_impl::slab_planes(const VolPlan& vp) const {
+    int64_t k = tune().slab >= 0 ? tune().slab : 0;
+    if (k <= 0 || k >= vp.n3) return 0;
+    while (k < vp.n3 && (k * vp.n2) % vp.x.tile != 0) ++k;
+    return k >= vp.n3 ? 0 : k;
+}
 
 // ------------------------------------------------------------------ OTF cache
 // 64-bit content hash of a PSF (cache key): four independent multiply-xorshift
@@ -516,7 +573,15 @@ void pk_ctx_impl::reset() {
     rtw.clear();
     ws.I.release(); ws.J.release(); ws.A.release(); ws.J3.release(); ws.Y.release(); ws.J4.release();
     ws.partials.release(); ws.lambda.release(); ws.raw.release(); ws.out.release(); ws.mask.release();
-    ws.mask2.release(); ws.count.release(); ws.tmp.release(); ws.batch.release(); ws.batch_tab.release();
+    ws.mask2.release(); ws.count.release(); ws.tmp.release(); ws.batch.release();
+    for (int i = 0; i < 2; ++i) {
+        ws.tab_dev[i].release();
+        if (ws.tab_host[i].p) (void)hipHostFree(ws.tab_host[i].p);
+        ws.tab_host[i].p = nullptr;
+        ws.tab_host[i].bytes = 0;
+        if (ws.tab_ev[i]) (void)hipEventDestroy(ws.tab_ev[i]);
+        ws.tab_ev[i] = nullptr;
+    }
 }
 
 // ------------------------------------------------------------------ decon driver
@@ -593,43 +658,68 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         const float bgv = raw16 ? (float)matlab_u16(o->background) : (float)o->background;
         decon_unfused(method, vp, raw, raw16 != nullptr, bgv, Hf, Hb, numit, I, J);
     } else {
+        // The dim-1 and dim-2 passes run slab by slab (planes [z0, z0 + kz)):
+        // each slab's chain (dim-2 inverse, dim-1 step, dim-2 forward) reuses
+        // its data from the memory-side cache; the dim-3 convolution covers the
+        // whole volume between chains.  kz = 0: one slab (the whole volume).
+        const int64_t kz = slab_planes(vp);
+        const int64_t step = kz > 0 ? kz : vp.n3;
+        const float scale = (float)(1.0 / (double)vp.N);  // ifftn's 1/N
+        auto slabs = [&](auto&& body) {
+            for (int64_t z0 = 0; z0 < vp.n3; z0 += step) body(z0, std::min(step, vp.n3 - z0));
+        };
+        auto y_fwd = [&](int64_t z0, int64_t nz) { run_col(vp, 2, A, CM_FWD, nullptr, 1.f, z0, nz); };
+        auto y_inv = [&](int64_t z0, int64_t nz) { run_col(vp, 2, A, CM_INV, nullptr, 1.f, z0, nz); };
+        auto z_conv = [&](const float2* H, bool conj) { run_col(vp, 3, A, conj ? CM_CONVJ : CM_CONV, H, scale); };
+
         // prep (decon_lucy_function.m:85-100, GPU branch): uint16 I - uint16(Background)
         // saturates at 0; single I gives max(I - Background, 0)
-        {
-            XArgs a{};
-            a.spec = A;
-            a.I = I;
-            a.J = method == 1 ? nullptr : J;  // OMW: the first update reads I (J_1 = I) instead of a copy
-            if (raw16) {
-                a.mode = XM_PREP_U16;
-                a.raw16 = raw16;
-                a.bg = (float)matlab_u16(o->background);
-            } else {
-                a.mode = XM_PREP_F32;
-                a.rawf = rawf;
-                a.bg = (float)o->background;
-            }
-            run_x(vp, a);
+        XArgs pa{};
+        pa.spec = A;
+        pa.I = I;
+        pa.J = method == 1 ? nullptr : J;  // OMW: the first update reads I (J_1 = I) instead of a copy
+        if (raw16) {
+            pa.mode = XM_PREP_U16;
+            pa.raw16 = raw16;
+            pa.bg = (float)matlab_u16(o->background);
+        } else {
+            pa.mode = XM_PREP_F32;
+            pa.rawf = rawf;
+            pa.bg = (float)o->background;
         }
 
         if (method == 1) {
+            // J_{k+1} = max(J_k .* real(ifftn(fftn(I ./ max(real(ifftn(fftn(J_k) .* OTF_f)), eps)) .* OTF_b)), 0)
+            slabs([&](int64_t z0, int64_t nz) {
+                run_x(vp, pa, z0, nz);
+                if (numit > 0) y_fwd(z0, nz);
+            });
             for (int k = 1; k <= numit; ++k) {
-                spectral_conv(vp, A, Hf, false);
-                XArgs r{};
-                r.mode = XM_RATIO;
-                r.spec = A;
-                r.I = I;
-                run_x(vp, r);
-                spectral_conv(vp, A, Hb, false);
-                XArgs u{};
-                u.mode = XM_OMW_UPDATE;
-                u.spec = A;
-                u.J = J;
-                u.Jin = k == 1 ? I : nullptr;
-                u.r2c_out = k < numit;
-                run_x(vp, u);
+                z_conv(Hf, false);
+                slabs([&](int64_t z0, int64_t nz) {
+                    y_inv(z0, nz);
+                    XArgs r{};
+                    r.mode = XM_RATIO;
+                    r.spec = A;
+                    r.I = I;
+                    run_x(vp, r, z0, nz);
+                    y_fwd(z0, nz);
+                });
+                z_conv(Hb, false);
+                slabs([&](int64_t z0, int64_t nz) {
+                    y_inv(z0, nz);
+                    XArgs u{};
+                    u.mode = XM_OMW_UPDATE;
+                    u.spec = A;
+                    u.J = J;
+                    u.Jin = k == 1 ? I : nullptr;
+                    u.r2c_out = k < numit;
+                    run_x(vp, u, z0, nz);
+                    if (k < numit) y_fwd(z0, nz);
+                });
             }
         } else {
+            run_x(vp, pa);
             float* J2 = J;
             float* J3 = ws.J3.as<float>();
             float* Y = ws.Y.as<float>();
@@ -640,36 +730,46 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
             hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, stream, lam, (int64_t)1, 0.f);
             PK_HIP(hipGetLastError());
             for (int k = 1; k <= numit; ++k) {
-                if (k > 2) {
+                if (k > 2) {  // lambda from the dot products of every slab of the last update
                     hipLaunchKernelGGL(k_lambda, dim3(1), dim3(256), 0, stream, part, (int)nxb, kEps, lam);
                     PK_HIP(hipGetLastError());
                 }
-                XArgs y{};
-                y.mode = XM_BIGGS_Y;
-                y.spec = A;
-                y.J = J2;
-                y.J3 = J3;
-                y.Y = Y;
-                y.lambda = lam;
-                run_x(vp, y);
-                spectral_conv(vp, A, Hf, false);
-                XArgs r{};
-                r.mode = XM_RATIO;
-                r.spec = A;
-                r.I = I;
-                run_x(vp, r);
-                spectral_conv(vp, A, Hf, true);
-                XArgs u{};
-                u.mode = XM_BIGGS_UPDATE;
-                u.spec = A;
-                u.Y = Y;
-                u.Jn = J3;  // x_{k-2} is dead once Y is formed
-                u.J4 = J4;
-                u.acc = (k >= 2 && k < numit);
-                u.write_j4 = (k < numit);
-                u.partials = part;
-                u.r2c_out = 0;
-                run_x(vp, u);
+                slabs([&](int64_t z0, int64_t nz) {
+                    XArgs y{};
+                    y.mode = XM_BIGGS_Y;
+                    y.spec = A;
+                    y.J = J2;
+                    y.J3 = J3;
+                    y.Y = Y;
+                    y.lambda = lam;
+                    run_x(vp, y, z0, nz);
+                    y_fwd(z0, nz);
+                });
+                z_conv(Hf, false);
+                slabs([&](int64_t z0, int64_t nz) {
+                    y_inv(z0, nz);
+                    XArgs r{};
+                    r.mode = XM_RATIO;
+                    r.spec = A;
+                    r.I = I;
+                    run_x(vp, r, z0, nz);
+                    y_fwd(z0, nz);
+                });
+                z_conv(Hf, true);
+                slabs([&](int64_t z0, int64_t nz) {
+                    y_inv(z0, nz);
+                    XArgs u{};
+                    u.mode = XM_BIGGS_UPDATE;
+                    u.spec = A;
+                    u.Y = Y;
+                    u.Jn = J3;  // x_{k-2} is dead once Y is formed
+                    u.J4 = J4;
+                    u.acc = (k >= 2 && k < numit);
+                    u.write_j4 = (k < numit);
+                    u.partials = part;
+                    u.r2c_out = 0;
+                    run_x(vp, u, z0, nz);
+                });
                 std::swap(J2, J3);
             }
             J = J2;

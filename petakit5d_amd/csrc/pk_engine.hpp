@@ -49,6 +49,7 @@ struct AxisLaunch {
     int lpad = 0;      // x: LDS row pitch
     dim3 block{1, 1, 1};
     size_t lds = 0;
+    int nt = 256;      // y, z: block size the kernel is compiled for (256, 512 or 1024)
 };
 
 struct VolPlan {
@@ -100,9 +101,28 @@ void serve_cancel(struct pk_ctx_impl* ctx, uint64_t id);
 void serve_release(struct pk_ctx_impl* ctx, uint64_t id);
 pk_serve_stats serve_stats(struct pk_ctx_impl* ctx);
 
+// Pinned host staging buffer (hipHostMalloc), so uploads are truly asynchronous.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 struct Workspace {
     DevBuf I, J, A, J3, Y, J4, partials, lambda, raw, out, mask, mask2, count, tmp;
-    DevBuf batch, batch_tab;  // batched decon: per-chunk arena, descriptor tables
+    DevBuf batch;  // batched decon: per-chunk arena
+    // per-step descriptor tables, double-buffered: step s stages its table in
+    // tab_host[s&1] and copies it to tab_dev[s&1]; tab_ev[s&1] marks the copy's
+    // completion, so the host can build step s+1 while step s runs
+    DevBuf tab_dev[2];
+    HostBuf tab_host[2];
+    hipEvent_t tab_ev[2] = {nullptr, nullptr};
+    int tab_slot = 0;
 };
 
 // Per-pass timing with HIP events recorded on the engine's stream.
@@ -153,10 +173,12 @@ struct pk_ctx_impl {
     // transforms (device pointers)
     void r2c3d(const VolPlan& vp, const float* real, float2* spec);
     void c2r3d(const VolPlan& vp, float2* spec, float* real);
-    void run_x(const VolPlan& vp, XArgs a);
-    void run_col(const VolPlan& vp, int axis, float2* data, int mode, const float2* otf, float scale);
+    // z0, nz: the planes [z0, z0 + nz) of dim 3 (a slab); nz = 0 means all
+    void run_x(const VolPlan& vp, XArgs a, int64_t z0 = 0, int64_t nz = 0);
+    void run_col(const VolPlan& vp, int axis, float2* data, int mode, const float2* otf, float scale, int64_t z0 = 0,
+                 int64_t nz = 0);
+    int64_t slab_planes(const VolPlan& vp) const;
     // convolution of the spectrum in ws.A: forward y, z-conv, inverse y
-    void spectral_conv(const VolPlan& vp, float2* A, const float2* H, bool conj);
 
     void decon(int method, const pk_vol* I, const float* psf_f, const int64_t fd[3], const float* psf_b,
                const int64_t bd[3], int numit, const pk_opts* o, pk_vol* out, double* err_mat);

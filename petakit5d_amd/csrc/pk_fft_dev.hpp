@@ -723,8 +723,19 @@ __global__ void __launch_bounds__(kMaxThreads) k_xpass_batch(const XChunk* __res
     extern __shared__ float2 lds[];
     const int c = chunk_of(starts, n, blockIdx.x);
     const XArgs a = ch[c].a;
-    const FftPlan pl = ch[c].pl;
+    const FftPlan& pl = ch[c].pl;  // read in place: a local copy indexed by stage would live in scratch
     xpass_body<E, MODE>(a, pl, lds, (int64_t)blockIdx.x - starts[c]);
+}
+
+// Batched launch of the compile-time row kernel: every chunk of the group has
+// rows of length 2L (same instantiation, same arithmetic as a single call).
+template <int L, int E, int MODE>
+__global__ void __launch_bounds__(kMaxThreads) k_xrow_batch(const XChunk* __restrict__ ch,
+                                                            const int64_t* __restrict__ starts, int n) {
+    extern __shared__ float2 lds[];
+    const int c = chunk_of(starts, n, blockIdx.x);
+    const XArgs a = ch[c].a;
+    xrow_body<L, E, MODE>(a, ch[c].pl.tw, lds, (int64_t)blockIdx.x - starts[c]);
 }
 
 // ------------------------------------------------------------------ strided axes
@@ -792,19 +803,21 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
     }
 }
 
-template <int E, int MODE>
-__global__ void __launch_bounds__(kMaxThreads) k_colpass(ColArgs a, FftPlan pl) {
+// NT: the block size the kernel is compiled for (256; 512 / 1024 for long
+// columns, whose 8-16-column tiles need more threads and up to 160 KB of LDS).
+template <int E, int MODE, int NT = kMaxThreads>
+__global__ void __launch_bounds__(NT) k_colpass(ColArgs a, FftPlan pl) {
     extern __shared__ float2 lds[];
     colpass_body<E, MODE>(a, pl, lds, blockIdx.x, blockIdx.y);
 }
 
-template <int E, int MODE>
-__global__ void __launch_bounds__(kMaxThreads) k_colpass_batch(const ColChunk* __restrict__ ch,
-                                                               const int64_t* __restrict__ starts, int n) {
+template <int E, int MODE, int NT = kMaxThreads>
+__global__ void __launch_bounds__(NT) k_colpass_batch(const ColChunk* __restrict__ ch,
+                                                      const int64_t* __restrict__ starts, int n) {
     extern __shared__ float2 lds[];
     const int c = chunk_of(starts, n, blockIdx.x);
     const ColArgs a = ch[c].a;
-    const FftPlan pl = ch[c].pl;
+    const FftPlan& pl = ch[c].pl;  // read in place: a local copy indexed by stage would live in scratch
     const int64_t lb = (int64_t)blockIdx.x - starts[c];
     colpass_body<E, MODE>(a, pl, lds, lb % ch[c].gx, lb / ch[c].gx);
 }

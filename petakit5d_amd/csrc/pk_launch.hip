@@ -25,6 +25,18 @@ bool launch_colb_part_c(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s,
 bool launch_xrow_a(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
 bool launch_xrow_b(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
 bool launch_xrow_c(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
+bool launch_xrowb_a(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                    const int64_t* starts, int n);
+bool launch_xrowb_b(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                    const int64_t* starts, int n);
+bool launch_xrowb_c(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                    const int64_t* starts, int n);
+bool launch_xrow_batch(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                       const int64_t* starts, int n) {
+    return launch_xrowb_a(L, E, grid, lds, s, mode, ch, starts, n) ||
+           launch_xrowb_b(L, E, grid, lds, s, mode, ch, starts, n) ||
+           launch_xrowb_c(L, E, grid, lds, s, mode, ch, starts, n);
+}
 bool xrow_available_a(int L, int E);
 bool xrow_available_b(int L, int E);
 bool xrow_available_c(int L, int E);
@@ -46,7 +58,12 @@ hipError_t launch_xpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hi
     return hipGetLastError();
 }
 
-hipError_t launch_colpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a) {
+hipError_t launch_colpass(const FftPlan& pl, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                          const ColArgs& a) {
+    if (nt != kMaxThreads) {
+        if (!launch_col_wide(pl, nt, grid, block, lds, s, a)) return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     const bool ok = launch_col_part_a(pl, grid, block, lds, s, a) ||
            launch_col_part_b(pl, grid, block, lds, s, a) ||
            launch_col_part_c(pl, grid, block, lds, s, a);
@@ -66,8 +83,12 @@ hipError_t launch_xpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_colpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch,
-                                const int64_t* starts, int n) {
+hipError_t launch_colpass_batch(int E, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode,
+                                const ColChunk* ch, const int64_t* starts, int n) {
+    if (nt != kMaxThreads) {
+        if (!launch_colb_wide(E, nt, grid, block, lds, s, mode, ch, starts, n)) return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     const bool ok = launch_colb_part_a(E, grid, block, lds, s, mode, ch, starts, n) ||
            launch_colb_part_b(E, grid, block, lds, s, mode, ch, starts, n) ||
            launch_colb_part_c(E, grid, block, lds, s, mode, ch, starts, n);

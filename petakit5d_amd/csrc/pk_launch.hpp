@@ -24,13 +24,25 @@ inline void launch_modes_x(dim3 grid, dim3 block, size_t lds, hipStream_t s, con
         default: break;
     }
 }
-template <int E>
+// Dynamic LDS above 64 KB needs the kernel's limit raised once (gfx950: 160 KB
+// per workgroup).
+constexpr size_t kMaxLds = 160 * 1024;
+template <class K>
+inline void allow_large_lds(K kernel) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kMaxLds);
+}
+template <int E, int NT = kMaxThreads>
 inline void launch_modes_col(dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a, const FftPlan& pl) {
+    auto go = [&](auto kernel) {
+        if (lds > 65536) allow_large_lds(kernel);
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, a, pl);
+    };
     switch (a.mode) {
-        case CM_FWD: hipLaunchKernelGGL((k_colpass<E, CM_FWD>), grid, block, lds, s, a, pl); break;
-        case CM_INV: hipLaunchKernelGGL((k_colpass<E, CM_INV>), grid, block, lds, s, a, pl); break;
-        case CM_CONV: hipLaunchKernelGGL((k_colpass<E, CM_CONV>), grid, block, lds, s, a, pl); break;
-        case CM_CONVJ: hipLaunchKernelGGL((k_colpass<E, CM_CONVJ>), grid, block, lds, s, a, pl); break;
+        case CM_FWD: go(k_colpass<E, CM_FWD, NT>); break;
+        case CM_INV: go(k_colpass<E, CM_INV, NT>); break;
+        case CM_CONV: go(k_colpass<E, CM_CONV, NT>); break;
+        case CM_CONVJ: go(k_colpass<E, CM_CONVJ, NT>); break;
         default: break;
     }
 }
@@ -38,9 +50,10 @@ inline void launch_modes_col(dim3 grid, dim3 block, size_t lds, hipStream_t s, c
 hipError_t launch_xpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
 hipError_t launch_xpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch,
                               const int64_t* starts, int n);
-hipError_t launch_colpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch,
-                                const int64_t* starts, int n);
-hipError_t launch_colpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
+hipError_t launch_colpass_batch(int E, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode,
+                                const ColChunk* ch, const int64_t* starts, int n);
+hipError_t launch_colpass(const FftPlan& pl, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                          const ColArgs& a);
 bool e_supported(int E);
 template <int L, int E>
 inline void launch_modes_xrow(dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw) {
@@ -57,6 +70,27 @@ inline void launch_modes_xrow(dim3 grid, size_t lds, hipStream_t s, const XArgs&
         default: break;
     }
 }
+template <int L, int E>
+inline void launch_modes_xrowb(dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch, const int64_t* starts,
+                               int n) {
+    const dim3 b(kMaxThreads);
+    switch (mode) {
+        case XM_R2C: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_R2C>), grid, b, lds, s, ch, starts, n); break;
+        case XM_C2R: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_C2R>), grid, b, lds, s, ch, starts, n); break;
+        case XM_PREP_U16: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_PREP_U16>), grid, b, lds, s, ch, starts, n); break;
+        case XM_PREP_F32: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_PREP_F32>), grid, b, lds, s, ch, starts, n); break;
+        case XM_RATIO: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_RATIO>), grid, b, lds, s, ch, starts, n); break;
+        case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_OMW_UPDATE>), grid, b, lds, s, ch, starts, n); break;
+        case XM_BIGGS_Y: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_Y>), grid, b, lds, s, ch, starts, n); break;
+        case XM_BIGGS_UPDATE:
+            hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_UPDATE>), grid, b, lds, s, ch, starts, n);
+            break;
+        default: break;
+    }
+}
+// Batched compile-time row kernels (same (L, E) pairs); false when absent.
+bool launch_xrow_batch(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                       const int64_t* starts, int n);
 // Compile-time row kernels for the (L, E) pairs in pk_kern_xrow_*.hip; false
 // when the pair has none (the caller then uses the generic kernel).
 bool launch_xrow(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
@@ -75,15 +109,24 @@ inline void launch_modes_xb(dim3 grid, dim3 block, size_t lds, hipStream_t s, in
         default: break;
     }
 }
-template <int E>
+template <int E, int NT = kMaxThreads>
 inline void launch_modes_colb(dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch,
                               const int64_t* starts, int n) {
+    auto go = [&](auto kernel) {
+        if (lds > 65536) allow_large_lds(kernel);
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, ch, starts, n);
+    };
     switch (mode) {
-        case CM_FWD: hipLaunchKernelGGL((k_colpass_batch<E, CM_FWD>), grid, block, lds, s, ch, starts, n); break;
-        case CM_INV: hipLaunchKernelGGL((k_colpass_batch<E, CM_INV>), grid, block, lds, s, ch, starts, n); break;
-        case CM_CONV: hipLaunchKernelGGL((k_colpass_batch<E, CM_CONV>), grid, block, lds, s, ch, starts, n); break;
-        case CM_CONVJ: hipLaunchKernelGGL((k_colpass_batch<E, CM_CONVJ>), grid, block, lds, s, ch, starts, n); break;
+        case CM_FWD: go(k_colpass_batch<E, CM_FWD, NT>); break;
+        case CM_INV: go(k_colpass_batch<E, CM_INV, NT>); break;
+        case CM_CONV: go(k_colpass_batch<E, CM_CONV, NT>); break;
+        case CM_CONVJ: go(k_colpass_batch<E, CM_CONVJ, NT>); break;
         default: break;
     }
 }
+// Column kernels compiled for 512- / 1024-thread blocks (pk_kern_col_wide.hip).
+bool col_wide_available(int E, int nt);
+bool launch_col_wide(const FftPlan& pl, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
+bool launch_colb_wide(int E, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch,
+                      const int64_t* starts, int n);
 }  // namespace pk

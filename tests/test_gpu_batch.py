@@ -18,7 +18,7 @@ def _chunks(sizes, seed=4):
     return out
 
 
-SIZES = [(64, 48, 40), (50, 45, 32), (72, 64, 27), (33, 40, 20), (96, 80, 36), (64, 64, 30)]
+SIZES = [(64, 48, 40), (50, 45, 32), (72, 64, 27), (33, 40, 20), (96, 80, 36), (64, 64, 30), (16, 1024, 30)]
 
 
 @pytest.mark.parametrize("method,numit", [("omw", 2), ("rl", 5)])

@@ -55,6 +55,9 @@ def test_rl_biggs_vs_oracle(oracle, shape, psf_shape, numit):
     ((1024, 12, 10), (9, 9, 7), 2),   # 512-point rows
     ((512, 18, 10), (9, 9, 7), 2),    # 256-point rows
     ((96, 30, 20), (9, 9, 7), 2),     # 48-point rows (runtime-plan kernel)
+    ((16, 2048, 6), (5, 9, 5), 2),    # 2048-point dim-2 columns (1024-thread blocks, 128 KB LDS)
+    ((20, 1024, 8), (5, 9, 5), 2),    # 1024-point dim-2 columns (512-thread blocks)
+    ((24, 12, 1024), (5, 5, 9), 2),   # 1024-point dim-3 convolution columns
 ])
 def test_omw_vs_oracle(oracle, shape, psf_shape, numit):
     import petakit5d_amd as pk
