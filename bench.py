@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--bp", default="wb", choices=["wb", "omw"],
+                    help="OMW back projector: Wiener-Butterworth (BackProjector) or omw_backprojector_generation")
     ap.add_argument("--events-apart", action="store_true",
                     help="time the K steps without per-pass events; take per-pass durations from a second K-step run")
     return ap.parse_args()
@@ -73,12 +75,21 @@ def oracle():
     return pk_oracle
 
 
-def back_projector(psf):
-    """OMW back projector (omw_backprojector_generation, alpha 0.005, defaults)."""
-    from petakit5d_amd import omw_backprojector_generation
+BP_DESC = {"wb": "BackProjector(psf, 'wiener-butterworth', 0.005, 1, 10): Wiener-Butterworth, alpha 0.005, "
+                 "order 10, cutoff gain and resolution by the reference defaults",
+           "omw": "omw_backprojector_generation(psf, 0.005, [], OTFCumThresh 0.9, hannWinBounds [0.8 1])"}
+
+
+def back_projector(psf, kind="wb"):
+    """The back projector PSF_b (single) for OMW decon: Wiener-Butterworth
+    (BackProjector, alpha 0.005, order 10) or the OMW generator."""
+    from petakit5d_amd import BackProjector, omw_backprojector_generation
 
     t = time.perf_counter()
-    b, _, _, _ = omw_backprojector_generation(psf, 0.005, None, OTFCumThresh=0.9, hannWinBounds=[0.8, 1])
+    if kind == "wb":
+        b = BackProjector(psf, "wiener-butterworth", 0.005, 1, 10)[0].astype(np.float32)
+    else:
+        b, _, _, _ = omw_backprojector_generation(psf, 0.005, None, OTFCumThresh=0.9, hannWinBounds=[0.8, 1])
     return b, time.perf_counter() - t
 
 
@@ -214,7 +225,7 @@ def wl_omw(args, torch, pk, L):
     iters = args.iters or 2
     t_gen = time.perf_counter()
     I, psf = synth.config2(shape=shape)
-    psf_b, t_bp = back_projector(psf)
+    psf_b, t_bp = back_projector(psf, args.bp)
     t_gen = time.perf_counter() - t_gen
     ctx = pk.Context(0)
     It = col_major_cuda(I, torch)
@@ -229,10 +240,10 @@ def wl_omw(args, torch, pk, L):
     model = iters * (24 * N + 176 * S) + 12 * N
     res = base_result("deconvolved Mvoxel/s (OMW, 2 it)", value, "Mvoxel/s", args, ms, {
         "workload": f"OMW decon, {iters} iterations, {shape[0]}x{shape[1]}x{shape[2]} uint16, 20k blobs + sinusoidal "
-                    "background, 64x64x101 PSF sheared 31.5 deg, Poisson; OMW back projector alpha 0.005; OTFs cached, "
-                    "input/output resident in HBM; one step = one decon call",
+                    f"background, 64x64x101 PSF sheared 31.5 deg, Poisson; {args.bp.upper()} back projector alpha 0.005; "
+                    "OTFs cached, input/output resident in HBM; one step = one decon call",
         "volume": list(shape), "psf": [64, 64, 101], "iterations": iters,
-        "back_projector": "omw_backprojector_generation(psf, 0.005, [], OTFCumThresh 0.9, hannWinBounds [0.8 1])",
+        "back_projector": BP_DESC[args.bp],
         "background": 100, "output": "uint16"})
     res["roofline"] = rl
     # algorithmic bytes of the engine's passes per call (each operand of each
@@ -326,7 +337,7 @@ def _chunk_sizes(n, seed):
 def wl_batched(args, torch, pk, L):
     from petakit5d_amd import synth
     psf = synth.config2_psf()
-    psf_b, _ = back_projector(psf)
+    psf_b, _ = back_projector(psf, args.bp)
     sizes = _chunk_sizes(64, 0)
     # chunks are crops of one blob volume (the config-2 generator at the same blob density)
     pool = synth._blob_volume_torch((384, 384, 256), psf, 11250, 3, 100.0, as_tensor=True)
@@ -378,7 +389,7 @@ def wl_served(args, torch, pk, L):
     from petakit5d_amd import synth
     g = synth.rng(4, 0)
     psfs = [synth.gaussian_psf((64, 64, 101), 1.0, 2.5), synth.gaussian_psf((64, 64, 101), 1.3, 3.2)]
-    bps = [back_projector(p)[0] for p in psfs]
+    bps = [back_projector(p, args.bp)[0] for p in psfs]
     pool = synth._blob_volume_torch((1024, 1024, 256), psfs[0], 20000, 4, 100.0, as_tensor=True)
     nshape = 24
     shapes = []
@@ -484,7 +495,7 @@ def wl_large(args, torch, pk, L):
     shape = tuple(args.shape or (2048, 2048, 512))
     t_gen = time.perf_counter()
     It, psf = synth.config5(shape=shape)
-    psf_b, _ = back_projector(psf)
+    psf_b, _ = back_projector(psf, args.bp)
     t_gen = time.perf_counter() - t_gen
     ctx = pk.Context(0)
     nvox = float(np.prod(shape))

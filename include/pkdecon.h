@@ -210,6 +210,20 @@ pk_status pk_omw_backprojector(pk_ctx *ctx, const float *psf, const int64_t psf_
                                int32_t skewed, const pk_bp_opts *opts, float *b_omw, float *otf_bp_omw,
                                float *abs_otf_c, float *otf_mask, int32_t *skewed_used);
 
+/* [PSF_bp, OTF_bp] = BackProjector(PSF_fp, bp_type, alpha, beta, n, resFlag, iRes)
+ * (third_parties/WBDeconvolution/BackProjector.m:1-233): traditional (flipped
+ * PSF), gaussian, butterworth, wiener and wiener-butterworth back projectors.
+ * alpha == 1 / beta == 1 take the traditional back projector's cutoff gain, as
+ * the reference does; res_flag 0/1/2 = FWHM/sqrt(2) / FWHM / ires (pixels).
+ * Host computation (no device use; ctx may be NULL, then no error text).
+ * psf_fp: host single, dims; psf_bp: host double, prod(dims); otf_bp: host
+ * complex double interleaved, 2*prod(dims) (may be NULL); info (may be NULL):
+ * 10 doubles {FWHMx, FWHMy, FWHMz, tx, ty, tz, beta_fp, alpha, beta, beta_wienerx}. */
+enum { PK_BP_TRADITIONAL = 0, PK_BP_GAUSSIAN = 1, PK_BP_BUTTERWORTH = 2, PK_BP_WIENER = 3, PK_BP_WIENER_BUTTERWORTH = 4 };
+pk_status pk_wb_backprojector(pk_ctx *ctx, const float *psf_fp, const int64_t dims[3], int32_t bp_type, double alpha,
+                              double beta, int32_t n, int32_t res_flag, const double ires[3], double *psf_bp,
+                              double *otf_bp, double *info);
+
 /* Transform diagnostics (MATLAB fftn / ifftn of real data, decon_lucy_function.m:136,142).
  * Device pointers.  r2c: real N floats -> half spectrum (floor(n1/2)+1)*n2*n3
  * complex (interleaved); c2r: half spectrum -> real, unnormalised (N * ifftn). */

@@ -4,6 +4,7 @@
  *   J = pk_decon_mex('rl',  I, PSF, NUMIT, opts)          % decon_lucy_function
  *   J = pk_decon_mex('omw', I, PSF_f, PSF_b, NUMIT, opts) % decon_lucy_omw_function
  *   [b, otf_bp, abs_otf_c, mask] = pk_decon_mex('bp', psf, alpha, skewed, bpopts)
+ *   [psf_bp, otf_bp] = pk_decon_mex('wbbp', psf, bp_type, alpha, beta, n, resFlag, iRes)  % BackProjector
  *   otf = pk_decon_mex('psf2otf', psf, outSize)
  *   n = pk_decon_mex('fgfn', n, dir, allowOddSeven, maxPrime)
  *   pk_decon_mex('reset')
@@ -193,6 +194,29 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         if (nlhs > 1) plhs[1] = otf; else mxDestroyArray(otf);
         if (nlhs > 2) plhs[2] = absc; else mxDestroyArray(absc);
         if (nlhs > 3) plhs[3] = mask; else mxDestroyArray(mask);
+    } else if (!strcmp(op, "wbbp")) {
+        /* host computation: no device context is needed */
+        if (nrhs != 8 || !mxIsChar(prhs[2]))
+            mexErrMsgIdAndTxt("pkdecon:invalidArg", "wbbp takes psf, bp_type, alpha, beta, n, resFlag, iRes");
+        static const char *types[] = {"traditional", "gaussian", "butterworth", "wiener", "wiener-butterworth"};
+        char *ty = mxArrayToString(prhs[2]);
+        int32_t t = -1;
+        for (int i = 0; i < 5; ++i)
+            if (!strcmp(ty, types[i])) t = i;
+        mxFree(ty);
+        if (t < 0) mexErrMsgIdAndTxt("pkdecon:invalidArg", "bp_type does not match any back-projector type");
+        int64_t pd[3];
+        dims3(prhs[1], pd);
+        double ires[3] = {0, 0, 0};
+        const size_t ni = mxGetNumberOfElements(prhs[7]);
+        for (int i = 0; i < 3 && i < (int)ni; ++i) ires[i] = mxGetPr(prhs[7])[i];
+        mwSize dm[3] = {(mwSize)pd[0], (mwSize)pd[1], (mwSize)pd[2]};
+        plhs[0] = mxCreateNumericArray(3, dm, mxDOUBLE_CLASS, mxREAL);
+        mxArray *otf = mxCreateNumericArray(3, dm, mxDOUBLE_CLASS, mxCOMPLEX);
+        check(pk_wb_backprojector(NULL, single_arg(prhs[1], "psf"), pd, t, mxGetScalar(prhs[3]), mxGetScalar(prhs[4]),
+                                  (int32_t)mxGetScalar(prhs[5]), (int32_t)mxGetScalar(prhs[6]), ires,
+                                  (double *)mxGetData(plhs[0]), (double *)mxGetData(otf), NULL));
+        if (nlhs > 1) plhs[1] = otf; else mxDestroyArray(otf);
     } else {
         mexErrMsgIdAndTxt("pkdecon:invalidArg", "unknown operation '%s'", op);
     }

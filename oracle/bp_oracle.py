@@ -247,3 +247,144 @@ def omw_backprojector_generation(psf, alpha, skewed=None, OTFCumThresh=0.9, OTFA
     omw = (om.astype(np.float32) * w).astype(np.complex64)  # :161
     b = np.fft.fftshift(np.real(np.fft.ifftn(omw.astype(np.complex128)))).astype(np.float32)  # :163
     return b, omw, abs_OTF_c, mask, bool(skewed)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# BackProjector (third_parties/WBDeconvolution/BackProjector.m:1-233): restated with numpy.  The reference's
+# PSF is single, so its FWHMs, cutoffs and Butterworth masks are single-precision values (float32 here); its
+# spectra are double (complex128 here).
+def wb_fwhm(y):
+    """fwhm(x, y), x = 1:N (BackProjector.m:474-509): NaN for a step-like profile; raises IndexError where the
+    reference's unbounded crossing search would index past the end."""
+    y = np.asarray(y, np.float32)
+    y = (y / y.max()).astype(np.float32)
+    N = y.size
+    lev = np.float32(0.5)
+    center = int(np.argmax(y)) if y[0] < lev else int(np.argmin(y))  # 0-based, first occurrence
+    s = np.sign(y - lev)
+    i = 1
+    while s[i] == s[i - 1]:
+        i += 1
+        if i >= N:
+            raise IndexError("index exceeds the profile (leading crossing)")
+    tlead = np.float32(i) + (lev - y[i - 1]) / (y[i] - y[i - 1])
+    i = center + 1
+    if i > N - 1:
+        raise IndexError("index exceeds the profile (trailing crossing)")
+    while i <= N - 2 and s[i] == s[i - 1]:
+        i += 1
+    if i != N - 1:
+        ttrail = np.float32(i) + (lev - y[i - 1]) / (y[i] - y[i - 1])
+        return np.float32(ttrail - tlead)
+    return np.float32(np.nan)
+
+
+def wb_fwhm_psf(p):
+    """fwhm_PSF(PSF) with cFlag 0, fitFlag 0 (:257-379): profiles through the first maximum (column-major)."""
+    p = np.asarray(p, np.float32)
+    if p.ndim == 2:
+        p = p[:, :, None]
+    sx, sy, sz = p.shape
+    if sx == 1 or sy == 1:
+        return np.float32(wb_fwhm(p.reshape(-1, order="F"))), np.float32(0), np.float32(0)
+    ind = int(np.argmax(p.reshape(-1, order="F")))
+    ix, iy, iz = np.unravel_index(ind, p.shape, order="F")
+    fx = wb_fwhm(p[:, iy, iz])
+    fy = wb_fwhm(p[ix, :, iz])
+    fz = wb_fwhm(p[ix, iy, :]) if sz > 1 else np.float32(0)
+    return fx, fy, fz
+
+
+def _mround(x):
+    return float(np.sign(x) * np.floor(np.abs(x) + 0.5))
+
+
+def wb_backprojector(psf, bp_type="traditional", alpha=0.001, beta=1.0, n=10, resFlag=1, iRes=(0, 0, 0)):
+    """Returns (PSF_bp, OTF_bp, info) like BackProjector(PSF_fp, bp_type, alpha, beta, n, resFlag, iRes)."""
+    p = np.asarray(psf, np.float32)
+    if p.ndim == 2:
+        p = p[:, :, None]
+    if p.shape[0] == 1 or p.shape[1] == 1:
+        raise ValueError("BackProjector needs a 2-D or 3-D PSF")
+    S = np.array(p.shape)
+    Sc = (S + 1) / 2.0  # :68-70, double
+    Soz = int(_mround((S[2] + 1) / 2.0))
+    fw = np.array(wb_fwhm_psf(p), np.float32)
+    flipped = p[::-1, ::-1, ::-1].astype(np.float64)  # flipPSF (:535-544), double
+    otf_flip = np.fft.fftn(np.fft.ifftshift(flipped))
+    oabs = np.fft.fftshift(np.abs(otf_flip))
+    M = oabs.max()
+    on = oabs / M
+    if resFlag == 0:
+        res = (fw / np.float32(np.sqrt(2.0))).astype(np.float32)
+    elif resFlag == 1:
+        res = fw
+    elif resFlag == 2:
+        res = np.asarray(iRes, np.float64)
+    else:
+        raise ValueError("Processing terminated, please set resFlag as 0, 1, or 2")
+    if resFlag == 2:
+        t = 1.0 / res / (1.0 / S)
+        lo = [int(max(_mround(Sc[a] - t[a]), 1)) for a in range(3)]
+        hi = [int(min(_mround(Sc[a] + t[a]), S[a])) for a in range(3)]
+    else:
+        px = (1.0 / S).astype(np.float32)
+        t = ((np.float32(1) / res) / px).astype(np.float32)
+        lo = [int(max(_mround(float(np.float32(Sc[a]) - t[a])), 1)) for a in range(3)]
+        hi = [int(min(_mround(float(np.float32(Sc[a]) + t[a])), S[a])) for a in range(3)]
+    lx = on.max(axis=(1, 2))  # max over z then over y (:117-121)
+    ly = on.max(axis=(0, 2))
+    lz = on.max(axis=(0, 1))
+    # 2-D PSF: squeeze(max(OTF_abs_norm,[],1)) stays a 1 x Sy row; both z indices are 1
+    bz = (lz[lo[2] - 1] + lz[hi[2] - 1]) / 2 if S[2] > 1 else on[:, 0, 0].max()
+    beta_fp = ((lx[lo[0] - 1] + lx[hi[0] - 1]) / 2 + (ly[lo[1] - 1] + ly[hi[1] - 1]) / 2 + bz) / 3
+    if alpha == 1:
+        alpha = beta_fp
+    if beta == 1:
+        beta = beta_fp
+    # w = sum(((index - Sc) / t)^2), single unless iRes was given (:167-177)
+    I, J, K = np.meshgrid(np.arange(1, S[0] + 1), np.arange(1, S[1] + 1), np.arange(1, S[2] + 1), indexing="ij")
+    if resFlag == 2:
+        w = (((I - Sc[0]) / t[0]) ** 2 + ((J - Sc[1]) / t[1]) ** 2 + ((K - Sc[2]) / t[2]) ** 2).astype(np.float32)
+    else:
+        a = (I - Sc[0]).astype(np.float32) / t[0]
+        b = (J - Sc[1]).astype(np.float32) / t[1]
+        c = (K - Sc[2]).astype(np.float32) / t[2]
+        w = (a * a + b * b + c * c).astype(np.float32)
+
+    def butter(ee):
+        m = np.float32(1) / np.sqrt(np.float32(1) + np.float32(ee) * np.power(w, np.float32(n)))
+        return np.fft.ifftshift(m.astype(np.float64))
+
+    beta_w = 0.0
+    if bp_type == "traditional":
+        out, otf = flipped, otf_flip
+    elif bp_type == "gaussian":  # gen_gaussianPSF_3D (:511-533)
+        sig = (fw / np.float32(2.3548)).astype(np.float32)
+        q = (sig * sig * np.float32(2)).astype(np.float32)
+        coef = np.float32(1) / (np.float32((2 * np.pi) ** 1.5) * sig[0] * sig[1] * sig[2])
+        a = (I - Sc[0]).astype(np.float32)
+        b = (J - Sc[1]).astype(np.float32)
+        c = (K - Sc[2]).astype(np.float32)
+        dd = a * a / q[0] + b * b / q[1] + c * c / q[2]
+        out = (coef * np.exp(-dd)).astype(np.float32).astype(np.float64)
+        otf = np.fft.fftn(np.fft.ifftshift(out))
+    elif bp_type == "butterworth":
+        otf = butter(1.0 / beta ** 2 - 1.0).astype(np.complex128)
+        out = np.fft.fftshift(np.real(np.fft.ifftn(otf)))
+    elif bp_type in ("wiener", "wiener-butterworth"):
+        onrm = otf_flip / M
+        wn = onrm / (np.abs(onrm) ** 2 + alpha)
+        if bp_type == "wiener":
+            otf = wn
+        else:
+            wa = np.fft.fftshift(np.abs(wn))
+            tl = np.abs(wa[:, :, Soz - 1]).max(axis=1)
+            beta_w = (tl[lo[0] - 1] + tl[hi[0] - 1]) / 2
+            otf = butter(beta_w / beta ** 2 - 1.0) * wn
+        out = np.fft.fftshift(np.real(np.fft.ifftn(otf)))
+    else:
+        raise ValueError("bp_type does not match any back-projector type")
+    info = {"FWHM": [float(x) for x in fw], "cutoff": [float(x) for x in t], "beta_fp": float(beta_fp),
+            "alpha": float(alpha), "beta": float(beta), "beta_wienerx": float(beta_w)}
+    return out, otf, info

@@ -5,7 +5,7 @@ declared in include/pkdecon.h); this package is its Python front end with the
 reference's MATLAB call signatures.
 """
 from ._lib import Context, PkDeconError, default_context, load  # noqa: F401
-from .backprojector import omw_backprojector_generation  # noqa: F401
+from .backprojector import BackProjector, omw_backprojector_generation  # noqa: F401
 from .batch import decon_batch  # noqa: F401
 from .serve import Server  # noqa: F401
 from .decon import (  # noqa: F401
@@ -28,6 +28,7 @@ __all__ = [
     "decon_psf2otf",
     "findGoodFactorNumber",
     "omw_backprojector_generation",
+    "BackProjector",
     "decon_batch",
     "Server",
     "crop_psf_to_data",

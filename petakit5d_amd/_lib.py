@@ -56,6 +56,7 @@ EXPORTS = [
     "pk_psf2otf",
     "pk_bp_opts_default",
     "pk_omw_backprojector",
+    "pk_wb_backprojector",
     "pk_fft_r2c",
     "pk_fft_c2r",
     "pk_profile_enable",
@@ -162,6 +163,9 @@ def load():
         L.pk_bp_opts_default.restype = None
         L.pk_omw_backprojector.argtypes = [vp, vp, i64p, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(PkBpOpts),
                                            vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int32)]
+        L.pk_wb_backprojector.argtypes = [vp, vp, i64p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), vp, vp,
+                                          ctypes.POINTER(ctypes.c_double)]
         L.pk_fft_r2c.argtypes = [vp, vp, vp, i64p]
         L.pk_fft_c2r.argtypes = [vp, vp, vp, i64p]
         L.pk_profile_enable.argtypes = [vp, ctypes.c_int32]

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <vector>
 
@@ -529,6 +530,258 @@ void pk_ctx_impl::omw_backprojector(const float* psf_in, const int64_t pd[3], do
     if (abs_otf_c_out) std::memcpy(abs_otf_c_out, absOc.data(), sizeof(float) * (size_t)P);
     if (mask_out) std::memcpy(mask_out, mcount.data(), sizeof(float) * (size_t)P);
     if (skewed_used) *skewed_used = skewed ? 1 : 0;
+}
+
+}  // namespace pk
+
+// ==================================================================== Wiener-Butterworth back projector
+// BackProjector(PSF_fp, bp_type, alpha, beta, n, resFlag, iRes)
+// (third_parties/WBDeconvolution/BackProjector.m:1-233).  Host code: it runs
+// once per PSF on PSF-sized arrays (odd and prime sizes included).  The
+// reference computes the FWHMs, the cutoff frequencies and the Butterworth
+// mask in single precision (its PSF is single) and the spectra in double;
+// this follows the same split.
+namespace pk {
+
+// fwhm(x, y) with x = 1:N (BackProjector.m:474-509).  false: the leading
+// crossing search runs off the end (the reference raises an index error);
+// width NaN: no trailing crossing ("step-like pulse").
+static bool wb_fwhm(const std::vector<float>& yin, float& width) {
+    const int N = (int)yin.size();
+    if (N < 2) return false;
+    float mx = yin[0];
+    for (float v : yin) mx = v > mx ? v : mx;
+    std::vector<float> y((size_t)N);
+    for (int i = 0; i < N; ++i) y[(size_t)i] = yin[(size_t)i] / mx;
+    const float lev = 0.5f;
+    auto sgn = [](float v) { return (v > 0.f) - (v < 0.f); };
+    int center = 0;  // 0-based index of the first max (or min when y(1) >= 0.5)
+    for (int i = 1; i < N; ++i)
+        if (y[0] < lev ? y[(size_t)i] > y[(size_t)center] : y[(size_t)i] < y[(size_t)center]) center = i;
+    int i = 1;
+    while (sgn(y[(size_t)i] - lev) == sgn(y[(size_t)i - 1] - lev)) {
+        if (++i >= N) return false;
+    }
+    const float ip1 = (lev - y[(size_t)i - 1]) / (y[(size_t)i] - y[(size_t)i - 1]);
+    const float tlead = (float)i + ip1;  // x(i-1) + interp * (x(i) - x(i-1)), x = 1:N
+    i = center + 1;
+    if (i > N - 1) return false;
+    while (i <= N - 2 && sgn(y[(size_t)i] - lev) == sgn(y[(size_t)i - 1] - lev)) ++i;
+    if (i != N - 1) {
+        const float ip2 = (lev - y[(size_t)i - 1]) / (y[(size_t)i] - y[(size_t)i - 1]);
+        width = ((float)i + ip2) - tlead;
+    } else {
+        width = std::numeric_limits<float>::quiet_NaN();
+    }
+    return true;
+}
+
+// fwhm_PSF(PSF) with cFlag = 0, fitFlag = 0, pixelSize = 1 (:257-379): lines
+// through the first maximum (column-major order).
+static void wb_fwhm_psf(const std::vector<float>& p, const int64_t d[3], float f[3]) {
+    auto need = [](bool ok) {
+        if (!ok) fail(PK_ERR_INVALID_ARG, "fwhm: the PSF profile has no half-maximum crossing (index exceeds)");
+    };
+    f[0] = f[1] = f[2] = 0.f;
+    const Vol3 v{{d[0], d[1], d[2]}};
+    if (d[0] == 1 || d[1] == 1) {  // 1-D input: the whole array is the profile
+        need(wb_fwhm(p, f[0]));
+        return;
+    }
+    size_t im = 0;
+    for (size_t i = 1; i < p.size(); ++i)
+        if (p[i] > p[im]) im = i;
+    const int64_t ix = (int64_t)im % d[0], iy = ((int64_t)im / d[0]) % d[1], iz = (int64_t)im / (d[0] * d[1]);
+    std::vector<float> line;
+    line.resize((size_t)d[0]);
+    for (int64_t i = 0; i < d[0]; ++i) line[(size_t)i] = p[(size_t)v.at(i, iy, iz)];
+    need(wb_fwhm(line, f[0]));
+    line.resize((size_t)d[1]);
+    for (int64_t j = 0; j < d[1]; ++j) line[(size_t)j] = p[(size_t)v.at(ix, j, iz)];
+    need(wb_fwhm(line, f[1]));
+    if (d[2] > 1) {
+        line.resize((size_t)d[2]);
+        for (int64_t k = 0; k < d[2]; ++k) line[(size_t)k] = p[(size_t)v.at(ix, iy, k)];
+        need(wb_fwhm(line, f[2]));
+    }
+}
+
+static double mround(double x) { return x < 0 ? -std::floor(-x + 0.5) : std::floor(x + 0.5); }  // MATLAB round
+
+void wb_backprojector(const float* psf_in, const int64_t d[3], int bp_type, double alpha, double beta, int n,
+                      int res_flag, const double ires[3], double* psf_bp, double* otf_bp, double* info) {
+    const Vol3 v{{d[0], d[1], d[2]}};
+    const int64_t P = v.n();
+    if (P < 1) fail(PK_ERR_INVALID_ARG, "empty PSF");
+    if (bp_type < PK_BP_TRADITIONAL || bp_type > PK_BP_WIENER_BUTTERWORTH)
+        fail(PK_ERR_INVALID_ARG, "bp_type does not match any back-projector type");
+    if (res_flag < 0 || res_flag > 2) fail(PK_ERR_INVALID_ARG, "Processing terminated, please set resFlag as 0, 1, or 2");
+    if (d[0] == 1 || d[1] == 1) fail(PK_ERR_INVALID_ARG, "BackProjector needs a 2-D or 3-D PSF");
+    std::vector<float> psf(psf_in, psf_in + P);  // single(PSF_fp)
+    const double Sc[3] = {(d[0] + 1) / 2.0, (d[1] + 1) / 2.0, (d[2] + 1) / 2.0};
+    const int64_t Soz = (int64_t)mround((d[2] + 1) / 2.0);
+    float fw[3];
+    wb_fwhm_psf(psf, d, fw);
+
+    // traditional back projector: flipped PSF (double) and its OTF
+    std::vector<cd> otf_flip((size_t)P);
+    std::vector<double> flipped((size_t)P);
+    for (int64_t k = 0; k < d[2]; ++k)
+        for (int64_t j = 0; j < d[1]; ++j)
+            for (int64_t i = 0; i < d[0]; ++i)
+                flipped[(size_t)v.at(i, j, k)] = (double)psf[(size_t)v.at(d[0] - 1 - i, d[1] - 1 - j, d[2] - 1 - k)];
+    const int64_t fsh[3] = {d[0] / 2, d[1] / 2, d[2] / 2}, ish[3] = {-(d[0] / 2), -(d[1] / 2), -(d[2] / 2)};
+    {
+        std::vector<double> s = shift3(flipped, v, ish);
+        for (int64_t i = 0; i < P; ++i) otf_flip[(size_t)i] = cd(s[(size_t)i], 0.0);
+        fft3_host(otf_flip, d, false);
+    }
+    std::vector<double> oabs((size_t)P);
+    for (int64_t i = 0; i < P; ++i) oabs[(size_t)i] = std::abs(otf_flip[(size_t)i]);
+    oabs = shift3(oabs, v, fsh);
+    double M = oabs[0];
+    for (double a : oabs) M = a > M ? a : M;
+    for (auto& a : oabs) a /= M;  // OTF_abs_norm
+
+    // resolution cutoff (single unless given), cutoff in pixels of the OTF
+    float t[3];
+    double td[3];
+    for (int a = 0; a < 3; ++a) {
+        const float px = (float)(1.0 / (double)d[a]);
+        if (res_flag == 2) {
+            td[a] = 1.0 / ires[a] / (1.0 / (double)d[a]);
+        } else {
+            const float res = res_flag == 0 ? fw[a] / (float)std::sqrt(2.0) : fw[a];
+            t[a] = (1.0f / res) / px;
+            td[a] = (double)t[a];
+        }
+    }
+    // 1-based index of an OTF profile sample: max(round(Sc - t), 1) / min(round(Sc + t), S)
+    auto lo_idx = [&](int a) {
+        const double x = res_flag == 2 ? mround(Sc[a] - td[a]) : mround((double)((float)Sc[a] - t[a]));
+        return (int64_t)std::max(x, 1.0);
+    };
+    auto hi_idx = [&](int a) {
+        const double x = res_flag == 2 ? mround(Sc[a] + td[a]) : mround((double)((float)Sc[a] + t[a]));
+        return (int64_t)std::min(x, (double)d[a]);
+    };
+    // cutoff gains of the traditional back projector (:117-137)
+    std::vector<double> lx((size_t)d[0], -1e300), ly((size_t)d[1], -1e300), lz((size_t)d[2], -1e300);
+    for (int64_t k = 0; k < d[2]; ++k)
+        for (int64_t j = 0; j < d[1]; ++j)
+            for (int64_t i = 0; i < d[0]; ++i) {
+                const double a = oabs[(size_t)v.at(i, j, k)];
+                lx[(size_t)i] = std::max(lx[(size_t)i], a);
+                ly[(size_t)j] = std::max(ly[(size_t)j], a);
+                lz[(size_t)k] = std::max(lz[(size_t)k], a);
+            }
+    const double bx = (lx[(size_t)lo_idx(0) - 1] + lx[(size_t)hi_idx(0) - 1]) / 2;
+    const double by = (ly[(size_t)lo_idx(1) - 1] + ly[(size_t)hi_idx(1) - 1]) / 2;
+    double bz;
+    if (d[2] > 1) {
+        bz = (lz[(size_t)lo_idx(2) - 1] + lz[(size_t)hi_idx(2) - 1]) / 2;
+    } else {
+        // 2-D PSF: squeeze(max(OTF_abs_norm, [], 1)) stays a 1 x Sy row, so the
+        // z "profile" is max over x of the OTF magnitude along y; both z
+        // indices are 1 (:131-135)
+        double m = -1e300;
+        for (int64_t i = 0; i < d[0]; ++i) m = std::max(m, oabs[(size_t)v.at(i, 0, 0)]);
+        bz = m;
+    }
+    const double beta_fp = (bx + by + bz) / 3;
+    if (alpha == 1) alpha = beta_fp;
+    if (beta == 1) beta = beta_fp;
+
+    // Butterworth mask 1 / sqrt(1 + ee w^n), w = sum ((i - Sc) / t)^2, in single (:167-177, :213-222)
+    auto butterworth = [&](double ee) {
+        std::vector<double> m((size_t)P);
+        const float eef = (float)ee, nf = (float)n;
+        for (int64_t k = 0; k < d[2]; ++k)
+            for (int64_t j = 0; j < d[1]; ++j)
+                for (int64_t i = 0; i < d[0]; ++i) {
+                    float w;
+                    if (res_flag == 2) {
+                        const double a = (i + 1 - Sc[0]) / td[0], b = (j + 1 - Sc[1]) / td[1], c = (k + 1 - Sc[2]) / td[2];
+                        w = (float)(a * a + b * b + c * c);
+                    } else {
+                        const float a = (float)(i + 1 - Sc[0]) / t[0], b = (float)(j + 1 - Sc[1]) / t[1],
+                                    c = (float)(k + 1 - Sc[2]) / t[2];
+                        w = a * a + b * b + c * c;
+                    }
+                    m[(size_t)v.at(i, j, k)] = (double)(1.0f / std::sqrt(1.0f + eef * std::pow(w, nf)));
+                }
+        return shift3(m, v, ish);  // ifftshift
+    };
+    std::vector<cd> otf((size_t)P);
+    std::vector<double> out((size_t)P);
+    auto psf_from_otf = [&]() {  // PSF_bp = fftshift(real(ifftn(OTF_bp)))
+        std::vector<cd> tmp = otf;
+        fft3_host(tmp, d, true);
+        std::vector<double> r((size_t)P);
+        for (int64_t i = 0; i < P; ++i) r[(size_t)i] = tmp[(size_t)i].real();
+        out = shift3(r, v, fsh);
+    };
+    double beta_w = 0;
+    switch (bp_type) {
+        case PK_BP_TRADITIONAL:
+            out = flipped;
+            otf = otf_flip;
+            break;
+        case PK_BP_GAUSSIAN: {  // gen_gaussianPSF_3D(Sx, Sy, Sz, FWHMx, FWHMy, FWHMz), single (:511-533)
+            const float sx = fw[0] / 2.3548f, sy = fw[1] / 2.3548f, sz = fw[2] / 2.3548f;
+            const float qx = sx * sx * 2, qy = sy * sy * 2, qz = sz * sz * 2;
+            const float coef = 1.0f / ((float)std::pow(2 * kPi, 1.5) * sx * sy * sz);
+            for (int64_t k = 0; k < d[2]; ++k)
+                for (int64_t j = 0; j < d[1]; ++j)
+                    for (int64_t i = 0; i < d[0]; ++i) {
+                        const float a = (float)(i + 1 - Sc[0]), b = (float)(j + 1 - Sc[1]), c = (float)(k + 1 - Sc[2]);
+                        const float dd = a * a / qx + b * b / qy + c * c / qz;
+                        out[(size_t)v.at(i, j, k)] = (double)(coef * std::exp(-dd));
+                    }
+            std::vector<double> s = shift3(out, v, ish);
+            for (int64_t i = 0; i < P; ++i) otf[(size_t)i] = cd(s[(size_t)i], 0.0);
+            fft3_host(otf, d, false);
+            break;
+        }
+        case PK_BP_BUTTERWORTH: {
+            std::vector<double> m = butterworth(1.0 / (beta * beta) - 1.0);
+            for (int64_t i = 0; i < P; ++i) otf[(size_t)i] = cd(m[(size_t)i], 0.0);
+            psf_from_otf();
+            break;
+        }
+        case PK_BP_WIENER:
+        case PK_BP_WIENER_BUTTERWORTH: {
+            std::vector<cd> wn((size_t)P);
+            for (int64_t i = 0; i < P; ++i) {
+                const cd o = otf_flip[(size_t)i] / M;  // OTF_flip_norm
+                const double a = std::abs(o);
+                wn[(size_t)i] = o / (a * a + alpha);
+            }
+            if (bp_type == PK_BP_WIENER) {
+                otf = wn;
+            } else {
+                // cutoff gain of the Wiener filter on the central z slice (:207-212)
+                std::vector<double> wa((size_t)P);
+                for (int64_t i = 0; i < P; ++i) wa[(size_t)i] = std::abs(wn[(size_t)i]);
+                wa = shift3(wa, v, fsh);
+                std::vector<double> tl((size_t)d[0], -1e300);
+                for (int64_t j = 0; j < d[1]; ++j)
+                    for (int64_t i = 0; i < d[0]; ++i)
+                        tl[(size_t)i] = std::max(tl[(size_t)i], wa[(size_t)v.at(i, j, Soz - 1)]);
+                beta_w = (tl[(size_t)lo_idx(0) - 1] + tl[(size_t)hi_idx(0) - 1]) / 2;
+                std::vector<double> m = butterworth(beta_w / (beta * beta) - 1.0);
+                for (int64_t i = 0; i < P; ++i) otf[(size_t)i] = m[(size_t)i] * wn[(size_t)i];
+            }
+            psf_from_otf();
+            break;
+        }
+    }
+    std::memcpy(psf_bp, out.data(), sizeof(double) * (size_t)P);
+    if (otf_bp) std::memcpy(otf_bp, otf.data(), sizeof(double) * 2 * (size_t)P);
+    if (info) {
+        const double vals[10] = {fw[0], fw[1], fw[2], td[0], td[1], td[2], beta_fp, alpha, beta, beta_w};
+        std::memcpy(info, vals, sizeof(vals));
+    }
 }
 
 }  // namespace pk

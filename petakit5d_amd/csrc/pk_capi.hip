@@ -52,6 +52,30 @@ static pk_status serve_guarded(pk_ctx* c, Fn&& fn) {
     }
 }
 
+// Host-only entry points: no device, no context lock; the context (optional)
+// only receives the error text.
+template <class Fn>
+static pk_status host_guarded(pk_ctx* c, Fn&& fn) {
+    try {
+        fn();
+        if (c) {
+            std::lock_guard<std::recursive_mutex> lk(c->mu);
+            c->last_error.clear();
+        }
+        return PK_OK;
+    } catch (const pk::Error& e) {
+        if (c) {
+            std::lock_guard<std::recursive_mutex> lk(c->mu);
+            c->last_error = e.msg;
+        }
+        return e.st;
+    } catch (const std::bad_alloc&) {
+        return PK_ERR_NO_MEMORY;
+    } catch (...) {
+        return PK_ERR_INVALID_ARG;
+    }
+}
+
 extern "C" {
 
 void pk_opts_default(pk_opts* o) {
@@ -323,6 +347,18 @@ int64_t pk_find_good_factor_number(int64_t n, int32_t dir, int32_t allow_odd_sev
         n += dir;
     }
     return -1;
+}
+
+pk_status pk_wb_backprojector(pk_ctx* c, const float* psf_fp, const int64_t dims[3], int32_t bp_type, double alpha,
+                              double beta, int32_t n, int32_t res_flag, const double ires[3], double* psf_bp,
+                              double* otf_bp, double* info) {
+    return host_guarded(c, [&] {
+        if (!psf_fp || !dims || !psf_bp) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1) pk::fail(PK_ERR_INVALID_ARG, "empty PSF");
+        if (res_flag == 2 && !ires) pk::fail(PK_ERR_INVALID_ARG, "resFlag 2 needs iRes");
+        const double zero[3] = {0, 0, 0};
+        pk::wb_backprojector(psf_fp, dims, bp_type, alpha, beta, n, res_flag, ires ? ires : zero, psf_bp, otf_bp, info);
+    });
 }
 
 }  // extern "C"

@@ -91,6 +91,10 @@ struct Chunk {
     bool done() const;
 };
 
+// BackProjector.m (host): see pk_wb_backprojector in pkdecon.h
+void wb_backprojector(const float* psf, const int64_t d[3], int bp_type, double alpha, double beta, int n,
+                      int res_flag, const double ires[3], double* psf_bp, double* otf_bp, double* info);
+
 struct Server;
 // request loop (pk_serve.hip)
 void serve_start(struct pk_ctx_impl* ctx, int max_active);

@@ -87,5 +87,17 @@ int main(void) {
     mxArray *I16 = mxCreateNumericArray(3, d, mxUINT16_CLASS, mxREAL);
     const mxArray *a5[] = {str("rl"), I16, P, dbl(2), s};
     if (!call(2, out, 5, a5)) printf("rl ok %zu %zu %zu\n", out[0]->dims[0], out[0]->dims[1], out[0]->dims[2]);
+    /* BackProjector (host): delta PSF, wiener-butterworth; then an unknown type */
+    mwSize wd[3] = {9, 9, 9};
+    mxArray *D = mxCreateNumericArray(3, wd, mxSINGLE_CLASS, mxREAL);
+    ((float *)D->data)[4 + 9 * (4 + 9 * 4)] = 1.f;
+    mwSize rd[2] = {1, 3};
+    mxArray *ir = mxCreateNumericArray(2, rd, mxDOUBLE_CLASS, mxREAL);
+    const mxArray *a6[] = {str("wbbp"), D, str("wiener-butterworth"), dbl(0.005), dbl(1), dbl(10), dbl(1), ir};
+    if (!call(2, out, 8, a6))
+        printf("wbbp ok %zu %zu %zu %d\n", out[0]->dims[0], out[0]->dims[1], out[0]->dims[2],
+               ((double *)out[0]->data)[4 + 9 * (4 + 9 * 4)] > 0);
+    const mxArray *a7[] = {str("wbbp"), D, str("nosuchtype"), dbl(0.005), dbl(1), dbl(10), dbl(1), ir};
+    call(2, out, 8, a7);
     return 0;
 }

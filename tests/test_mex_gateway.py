@@ -23,3 +23,6 @@ def test_mex_gateway(tmp_path):
     assert out[3] == "error pkdecon:unsupportedType"
     # fourth call reaches the engine: no device here, a shape-checked result on a GPU host
     assert out[4] in ("error pkdecon:noDevice", "rl ok 8 8 4")
+    # BackProjector runs on the host: the same answer with or without a device
+    assert out[5] == "wbbp ok 9 9 9 1"
+    assert out[6] == "error pkdecon:invalidArg"

@@ -71,3 +71,84 @@ def test_hull_and_knn_primitives(bo):
 def test_errors(bo):
     with pytest.raises(ValueError, match="nonzero PSF"):
         bo.omw_backprojector_generation(np.zeros((5, 5, 5)), 0.005)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# BackProjector (WB) oracle: pinned by closed forms, not by retyping its formulas.
+def test_wb_fwhm_tent_is_exact(bo):
+    # a tent of half-width h peaking on a sample: linear interpolation between
+    # samples is exact on its flanks, so FWHM = h (also for non-integer h)
+    for h, c in [(6.0, 20.0), (9.0, 15.0), (5.5, 30.0), (7.25, 12.0)]:
+        x = np.arange(1, 41, dtype=np.float64)
+        y = np.maximum(0.0, 1.0 - np.abs(x - c) / h).astype(np.float32)
+        assert abs(float(bo.wb_fwhm(y)) - h) < 1e-5
+
+
+def test_wb_fwhm_step_and_no_crossing(bo):
+    assert np.isnan(bo.wb_fwhm(np.array([0.0, 0.1, 0.6, 1.0, 1.0], np.float32)))  # no trailing edge
+    with pytest.raises(IndexError):
+        bo.wb_fwhm(np.array([0.0, 0.0, 0.0, 0.0], np.float32) + np.array([0.4, 0.45, 0.49, 0.45], np.float32))
+
+
+def test_wb_gaussian_fwhm_close_to_analytic(bo):
+    from petakit5d_amd import synth
+    psf = synth.gaussian_psf((63, 63, 81), 2.0, 4.0, 0.0)
+    fx, fy, fz = bo.wb_fwhm_psf(psf)
+    assert abs(fx / (2.3548 * 2.0) - 1) < 0.03 and abs(fy / (2.3548 * 2.0) - 1) < 0.03
+    assert abs(fz / (2.3548 * 4.0) - 1) < 0.03
+
+
+def _psf(bo):
+    from petakit5d_amd import synth
+    return synth.gaussian_psf((33, 33, 41), 1.2, 3.0, 15.0)
+
+
+def test_wb_traditional_is_flipped_psf(bo):
+    p = _psf(bo)
+    out, otf, _ = bo.wb_backprojector(p, "traditional")
+    np.testing.assert_array_equal(out, p[::-1, ::-1, ::-1].astype(np.float64))
+    # its OTF is the DFT of the centred flipped PSF: the DC term is the PSF sum
+    assert abs(otf[0, 0, 0] - p.astype(np.float64).sum()) < 1e-9
+
+
+def test_wb_wiener_identity(bo):
+    p = _psf(bo)
+    out, otf, info = bo.wb_backprojector(p, "wiener", 0.005)
+    o = np.fft.fftn(np.fft.ifftshift(p[::-1, ::-1, ::-1].astype(np.float64)))
+    on = o / np.abs(o).max()
+    np.testing.assert_allclose(otf * (np.abs(on) ** 2 + 0.005), on, rtol=0, atol=1e-12)
+    # PSF_bp is the centred inverse DFT: its sum is the DC gain
+    assert abs(out.sum() - otf[0, 0, 0].real) < 1e-9
+
+
+def test_wb_butterworth_gain_at_cutoff(bo):
+    # odd size 33 (centre 17), iRes = 33/4 -> cutoff t = 4 pixels: voxel 21 on axis 1 has w = 1 exactly,
+    # where 1/sqrt(1 + (1/beta^2 - 1) w^n) = beta; at the centre (DC) the mask is 1
+    p = _psf(bo)
+    beta = 0.3
+    _, otf, info = bo.wb_backprojector(p, "butterworth", 0.005, beta, 10, 2, (33 / 4, 33 / 4, 41 / 5))
+    m = np.fft.fftshift(otf.real)
+    assert abs(info["cutoff"][0] - 4.0) < 1e-12
+    assert abs(m[16, 16, 20] - 1.0) < 1e-7
+    assert abs(m[20, 16, 20] - beta) < 1e-6
+    assert np.all(np.diff(m[16:, 16, 20]) <= 0)  # monotone along the axis
+
+
+def test_wb_wiener_butterworth_dc_and_alpha_one(bo):
+    p = _psf(bo)
+    _, otf_w, _ = bo.wb_backprojector(p, "wiener", 0.005)
+    _, otf_wb, info = bo.wb_backprojector(p, "wiener-butterworth", 0.005, 1, 10)
+    assert abs(otf_wb[0, 0, 0] - otf_w[0, 0, 0]) < 1e-12  # mask(DC) = 1
+    assert info["beta"] == info["beta_fp"] and 0 < info["beta_fp"] < 1  # beta == 1 -> traditional cutoff gain
+    _, _, info2 = bo.wb_backprojector(p, "wiener", 1)
+    assert info2["alpha"] == info2["beta_fp"]
+
+
+def test_wb_gaussian_back_projector_normalised(bo):
+    p = _psf(bo)
+    out, otf, info = bo.wb_backprojector(p, "gaussian")
+    # the generated Gaussian uses the measured FWHMs: centred, unit integral (sampled), FWHMs reproduced
+    assert np.unravel_index(np.argmax(out), out.shape) == (16, 16, 20)
+    assert abs(out.sum() - 1) < 0.02
+    f = bo.wb_fwhm_psf(out.astype(np.float32))
+    assert all(abs(a / b - 1) < 0.05 for a, b in zip(f, info["FWHM"]))
