@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch shapes instead of measuring them for the workload's sizes")
     ap.add_argument("--bp", default="wb", choices=["wb", "omw"],
                     help="OMW back projector: Wiener-Butterworth (BackProjector) or omw_backprojector_generation")
     ap.add_argument("--events-apart", action="store_true",
@@ -229,6 +231,10 @@ def wl_omw(args, torch, pk, L):
     t_gen = time.perf_counter() - t_gen
     ctx = pk.Context(0)
     It = col_major_cuda(I, torch)
+    tune = None
+    if not args.no_autotune:  # per-size launch-shape / pass-order measurement, before the timed region
+        t = time.perf_counter()
+        tune = {"report": ctx.autotune(shape), "s": round(time.perf_counter() - t, 3)}
     call = Caller(ctx, L, "omw", It, psf, psf_b, iters)
     ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx, events_in_timed=not args.events_apart)
     nvox = float(np.prod(shape))
@@ -259,6 +265,8 @@ def wl_omw(args, torch, pk, L):
     if "__ms_with_events" in prof:
         res["call_model"]["ms_per_step_with_events"] = round(prof["__ms_with_events"], 4)
     res["hbm_copy_GBps"] = round(copy_bandwidth(torch), 1)
+    if tune:
+        res["autotune"] = tune
     res["gen_s"] = round(t_gen, 1)
     res["backprojector_s"] = round(t_bp, 2)
     if not args.no_plain:
@@ -352,7 +360,9 @@ def wl_batched(args, torch, pk, L):
     nvox = float(sum(np.prod(s) for s in sizes))
     out = {}
     for method, iters in (("omw", args.iters or 2), ("rl", 10)):
-        fn = lambda: pk.decon_batch(chunks, psf, psf_b, iters, method=method, Background=100, ctx=ctx)  # noqa: E731
+        fl = 0 if args.no_autotune else 4  # PK_FLAG_AUTOTUNE: shapes measured per chunk size in the warm-up batch
+        fn = lambda: pk.decon_batch(chunks, psf, psf_b, iters, method=method, Background=100, ctx=ctx,  # noqa: E731
+                                    flags=fl)
         ms, _, prof = timed(fn, max(2, args.steps // 3), 1, torch, ctx)
         nb = max(2, args.steps // 3)
         rec = {"Mvoxel_s": round(nvox / 1e6 / (ms / 1e3), 1), "ms_per_batch": round(ms, 2),
@@ -410,8 +420,10 @@ def wl_served(args, torch, pk, L):
     ctx = pk.Context(0, 200 << 30)
     srv = pk.Server(ctx, max_active=64)
 
+    fl = 0 if args.no_autotune else 4  # PK_FLAG_AUTOTUNE: each size is measured the first time it arrives (warm-up)
+
     def submit(v, meth, it, ch):
-        return srv.submit(v, psfs[ch], bps[ch], it, method=meth, Background=100)
+        return srv.submit(v, psfs[ch], bps[ch], it, method=meth, Background=100, flags=fl)
 
     # warm plans / OTF cache for every (shape, channel), then measure saturated capacity
     warm = [submit(vols[i], "omw", 1, c) for i in range(nshape) for c in (0, 1)]

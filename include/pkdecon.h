@@ -75,6 +75,9 @@ typedef struct {
  * transforms; PK_FLAG_NO_OTF_CACHE rebuilds the OTFs on every call. */
 #define PK_FLAG_UNFUSED 1
 #define PK_FLAG_NO_OTF_CACHE 2
+/* PK_FLAG_AUTOTUNE: measure the launch shapes and pass order of a padded size
+ * the first time the context meets it (pk_autotune), then use them. */
+#define PK_FLAG_AUTOTUNE 4
 
 typedef struct pk_ctx pk_ctx;
 
@@ -223,6 +226,18 @@ enum { PK_BP_TRADITIONAL = 0, PK_BP_GAUSSIAN = 1, PK_BP_BUTTERWORTH = 2, PK_BP_W
 pk_status pk_wb_backprojector(pk_ctx *ctx, const float *psf_fp, const int64_t dims[3], int32_t bp_type, double alpha,
                               double beta, int32_t n, int32_t res_flag, const double ires[3], double *psf_bp,
                               double *otf_bp, double *info);
+
+/* Launch-shape autotuner (no counterpart in the reference, whose FFTs are
+ * MATLAB's): times the candidate row-pass values-per-thread counts, the
+ * strided passes' values per thread, block sizes and tile widths, and the pass
+ * order (which strided axis carries the fused OTF convolution) for the padded
+ * size dims on the context's GPU, and keeps the fastest for later calls of that
+ * size.  Results are shared by the process's contexts and appended to the file
+ * named by the environment variable PK_TUNE_FILE when set (read back at the
+ * next plan of the size).  report (may be NULL) receives a one-line summary.
+ * Call it before serving: requests in flight keep the shapes they started with
+ * only when nothing retunes their size. */
+pk_status pk_autotune(pk_ctx *ctx, const int64_t dims[3], char *report, size_t report_len);
 
 /* Transform diagnostics (MATLAB fftn / ifftn of real data, decon_lucy_function.m:136,142).
  * Device pointers.  r2c: real N floats -> half spectrum (floor(n1/2)+1)*n2*n3

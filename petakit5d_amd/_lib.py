@@ -57,6 +57,7 @@ EXPORTS = [
     "pk_bp_opts_default",
     "pk_omw_backprojector",
     "pk_wb_backprojector",
+    "pk_autotune",
     "pk_fft_r2c",
     "pk_fft_c2r",
     "pk_profile_enable",
@@ -166,6 +167,7 @@ def load():
         L.pk_wb_backprojector.argtypes = [vp, vp, i64p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), vp, vp,
                                           ctypes.POINTER(ctypes.c_double)]
+        L.pk_autotune.argtypes = [vp, i64p, ctypes.c_char_p, ctypes.c_size_t]
         L.pk_fft_r2c.argtypes = [vp, vp, vp, i64p]
         L.pk_fft_c2r.argtypes = [vp, vp, vp, i64p]
         L.pk_profile_enable.argtypes = [vp, ctypes.c_int32]
@@ -207,6 +209,13 @@ class Context:
 
     def set_stream(self, stream_ptr: int | None):
         self.check(self._L.pk_set_stream(self.handle, ctypes.c_void_p(stream_ptr or 0)))
+
+    def autotune(self, dims) -> str:
+        """Measure and keep the fastest launch shapes / pass order for a
+        padded size (pk_autotune); returns the report line."""
+        buf = ctypes.create_string_buffer(2048)
+        self.check(self._L.pk_autotune(self.handle, i64x3(tuple(int(x) for x in dims)), buf, len(buf)))
+        return buf.value.decode(errors="replace")
 
     def profile(self, enable: bool):
         """Start (clearing) or stop per-pass HIP-event timing."""

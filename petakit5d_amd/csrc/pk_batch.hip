@@ -140,7 +140,11 @@ void pk_ctx_impl::chunk_validate(const pk_vol& in, const pk_vol& out, const pk_o
 size_t pk_ctx_impl::chunk_prepare(Chunk& c, const pk_vol& in, const pk_vol& out, const float* psf_f,
                                   const int64_t fd[3], const float* psf_b, const int64_t bd[3]) {
     for (int d = 0; d < 3; ++d) c.d[d] = in.dims[d];
-    c.vp = &plan(smooth_up(c.d[0]), smooth_up(c.d[1]), smooth_up(c.d[2]));
+    {
+        const int64_t a = smooth_up(c.d[0]), b = smooth_up(c.d[1]), z = smooth_up(c.d[2]);
+        if ((c.o.flags & PK_FLAG_AUTOTUNE) && !plan(a, b, z).tuned) autotune(a, b, z);
+        c.vp = &plan(a, b, z);
+    }
     std::vector<float> cf;
     int64_t cfd[3];
     crop_psf(psf_f, fd, c.d, cf, cfd);
@@ -337,15 +341,16 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
             a.lambda = c.lam;
             add_x(biggs_y, c, a);
         }
-        add_col(yf, c, 2, CM_FWD, nullptr);
-        add_col(zc1, c, 3, CM_CONV, c.Hf);
-        add_col(yi, c, 2, CM_INV, nullptr);
+        const int cax = c.vp->conv_axis, oax = cax == 3 ? 2 : 3;  // pass order of the chunk's plan
+        add_col(yf, c, oax, CM_FWD, nullptr);
+        add_col(zc1, c, cax, CM_CONV, c.Hf);
+        add_col(yi, c, oax, CM_INV, nullptr);
         {
             XArgs a = base_x(c, XM_RATIO);
             a.I = c.I;
             add_x(ratio, c, a);
         }
-        add_col(zc2, c, 3, c.method == 1 ? CM_CONV : CM_CONVJ, c.method == 1 ? c.Hb : c.Hf);
+        add_col(zc2, c, cax, c.method == 1 ? CM_CONV : CM_CONVJ, c.method == 1 ? c.Hb : c.Hf);
         if (c.method == 1) {
             XArgs a = base_x(c, XM_OMW_UPDATE);
             a.J = c.J;

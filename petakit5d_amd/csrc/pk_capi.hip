@@ -349,6 +349,17 @@ int64_t pk_find_good_factor_number(int64_t n, int32_t dir, int32_t allow_odd_sev
     return -1;
 }
 
+pk_status pk_autotune(pk_ctx* c, const int64_t dims[3], char* report, size_t report_len) {
+    return guarded(c, [&] {
+        if (!dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) pk::fail(PK_ERR_INVALID_ARG, "bad dims");
+        const std::string r = c->autotune(dims[0], dims[1], dims[2]);
+        if (report && report_len) {
+            std::strncpy(report, r.c_str(), report_len - 1);
+            report[report_len - 1] = 0;
+        }
+    });
+}
+
 pk_status pk_wb_backprojector(pk_ctx* c, const float* psf_fp, const int64_t dims[3], int32_t bp_type, double alpha,
                               double beta, int32_t n, int32_t res_flag, const double ires[3], double* psf_bp,
                               double* otf_bp, double* info) {

@@ -25,9 +25,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "pk_launch.hpp"
 #include "pk_misc.hpp"
@@ -214,6 +217,127 @@ static void fill_twiddles(std::vector<float2>& t, int L) {
     }
 }
 
+// Row-pass launch shapes for length Lx (complex points), best first by the
+// built-in rule: with a compile-time kernel for the length, E near 8
+// (measured at 512x512x256, ratio pass: E = 4 / 8 / 16 -> 0.209 / 0.175 /
+// 0.170 ms); the runtime-plan kernel spends more on index arithmetic and
+// prefers few values per thread (E = 4 / 8 / 16 -> 0.24 / 0.26 / 0.29 ms).
+static AxisLaunch x_axis(const FftPlan& fp) {
+    AxisLaunch ax;
+    ax.fp = fp;
+    ax.lpad = row_pitch(fp.L);
+    int rt = std::max(1, kMaxThreads / fp.P);
+    while (rt > 1 && (size_t)rt * ax.lpad * sizeof(float2) > 65536) rt--;
+    ax.tile = rt;
+    ax.block = dim3(rt * fp.P);
+    ax.lds = std::max<size_t>((size_t)rt * ax.lpad * sizeof(float2), 2 * sizeof(double) * kMaxThreads);
+    return ax;
+}
+static std::vector<AxisLaunch> x_candidates(int Lx, bool packed) {
+    std::vector<AxisLaunch> out;
+    FftPlan first = plan_1d(Lx, 8, kMaxThreads, tune().e[0], 10, 4);
+    if (!packed || !xrow_available(first.L, first.E)) first = plan_1d(Lx, 4, kMaxThreads, tune().e[0], 10, 4);
+    out.push_back(x_axis(first));
+    if (tune().e[0] > 0) return out;
+    for (int E : {4, 8, 16}) {
+        if (E == first.E || Lx % E || Lx / E > kMaxThreads) continue;
+        const FftPlan fp = plan_1d(Lx, E, kMaxThreads, E, 10, 4);
+        if (fp.E == E) out.push_back(x_axis(fp));
+    }
+    return out;
+}
+
+// Strided-pass (y, z) launch shapes for length L: tiles of T adjacent
+// columns (T float2 per row segment), sorted by the built-in rule: T = 16
+// (128-byte segments) first, then fewest stages, then E near 16, then the
+// smallest block.  Long columns use 512- / 1024-thread blocks and up to
+// 160 KB of LDS (T * L * 8 bytes) to keep T >= 8.
+static std::vector<AxisLaunch> col_candidates(int L, int axis) {
+    std::vector<std::pair<int, AxisLaunch>> c;
+    for (int E : kEs) {
+        if (L % E) continue;
+        if (tune().e[axis] > 0 && E != tune().e[axis] && L != 1) continue;
+        bool ok = e_supported(E);
+        for (int pr : {2, 3, 5})
+            if (L % pr == 0 && E % pr != 0) ok = false;
+        if (!ok) continue;
+        const FftPlan fp = plan_1d(L, E, kMaxThreads * 4, E, 100, 1);
+        if (fp.E != E) continue;
+        for (int nt : {kMaxThreads, 512, 1024}) {
+            if (nt != kMaxThreads && !col_wide_available(E, nt)) continue;
+            if (fp.P > nt) continue;
+            int T = 1;
+            while (T * 2 <= 16 && T * 2 * fp.P <= nt && (size_t)T * 2 * L * sizeof(float2) <= kMaxLds) T *= 2;
+            if (tune().t[axis] > 0) T = std::min(T, tune().t[axis]);
+            AxisLaunch ax;
+            ax.fp = fp;
+            ax.tile = T;
+            ax.nt = nt;
+            ax.block = dim3(T * fp.P);
+            ax.lds = std::max<size_t>((size_t)T * L * sizeof(float2), 16);
+            c.emplace_back((16 - T) * 1000 + fp.nst * 100 + std::abs(E - 16) * 2 + (nt > kMaxThreads ? 1 : 0), ax);
+        }
+    }
+    if (c.empty()) fail(PK_ERR_UNSUPPORTED_SIZE, "no transform plan for length " + std::to_string(L));
+    std::stable_sort(c.begin(), c.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<AxisLaunch> out;
+    for (auto& e : c) out.push_back(e.second);
+    return out;
+}
+
+// A column launch with given values per thread, block size and tile width;
+// false when the combination is not available for length L.
+static bool col_axis(int L, int E, int nt, int T, AxisLaunch& ax) {
+    if (L % E || !e_supported(E) || T < 1) return false;
+    if (nt != kMaxThreads && !col_wide_available(E, nt)) return false;
+    const FftPlan fp = plan_1d(L, E, kMaxThreads * 4, E, 100, 1);
+    if (fp.E != E || fp.P * T > nt || (size_t)T * L * sizeof(float2) > kMaxLds) return false;
+    ax = AxisLaunch{};
+    ax.fp = fp;
+    ax.tile = T;
+    ax.nt = nt;
+    ax.block = dim3(T * fp.P);
+    ax.lds = std::max<size_t>((size_t)T * L * sizeof(float2), 16);
+    return true;
+}
+
+// Tuned shapes per padded size, shared by the process's contexts and kept in
+// the text file PK_TUNE_FILE when set (one line per size:
+// n1 n2 n3  xE  yE yNT yT  zE zNT zT  conv_axis).
+struct Wisdom {
+    std::mutex mu;
+    std::map<std::vector<int64_t>, std::array<int, 8>> table;
+    bool loaded = false;
+    static Wisdom& get() {
+        static Wisdom w;
+        return w;
+    }
+    void load() {
+        if (loaded) return;
+        loaded = true;
+        const char* f = std::getenv("PK_TUNE_FILE");
+        if (!f) return;
+        FILE* fp = std::fopen(f, "r");
+        if (!fp) return;
+        long long a, b, c;
+        int v[8];
+        while (std::fscanf(fp, "%lld %lld %lld %d %d %d %d %d %d %d %d", &a, &b, &c, &v[0], &v[1], &v[2], &v[3], &v[4],
+                           &v[5], &v[6], &v[7]) == 11)
+            table[{a, b, c}] = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+        std::fclose(fp);
+    }
+    void store(const std::vector<int64_t>& k, const std::array<int, 8>& v) {
+        table[k] = v;
+        const char* f = std::getenv("PK_TUNE_FILE");
+        if (!f) return;
+        FILE* fp = std::fopen(f, "a");
+        if (!fp) return;
+        std::fprintf(fp, "%lld %lld %lld %d %d %d %d %d %d %d %d\n", (long long)k[0], (long long)k[1], (long long)k[2],
+                     v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+        std::fclose(fp);
+    }
+};
+
 const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
     std::vector<int64_t> key{n1, n2, n3};
     auto it = plans.find(key);
@@ -248,61 +372,14 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
         return r;
     };
 
-    // x (rows)
-    {
-        AxisLaunch& ax = vp.x;
-        // rows: with a compile-time kernel for the length, E near 8 (measured
-        // at 512x512x256, ratio pass: E = 4 / 8 / 16 -> 0.209 / 0.175 / 0.170 ms);
-        // the runtime-plan kernel spends more on index arithmetic and prefers
-        // few values per thread (E = 4 / 8 / 16 -> 0.24 / 0.26 / 0.29 ms)
-        ax.fp = plan_1d(Lx, 8, kMaxThreads, tune().e[0], 10, 4);
-        if (!vp.packed || !xrow_available(ax.fp.L, ax.fp.E)) ax.fp = plan_1d(Lx, 4, kMaxThreads, tune().e[0], 10, 4);
-        ax.fp.tw = table(Lx);
-        ax.lpad = row_pitch(Lx);
-        int rt = std::max(1, kMaxThreads / ax.fp.P);
-        while (rt > 1 && (size_t)rt * ax.lpad * sizeof(float2) > 65536) rt--;
-        ax.tile = rt;
-        ax.block = dim3(rt * ax.fp.P);
-        ax.lds = std::max<size_t>((size_t)rt * ax.lpad * sizeof(float2), 2 * sizeof(double) * kMaxThreads);
-    }
+    // x (rows): the first candidate is the built-in choice
+    vp.x = x_candidates(Lx, vp.packed)[0];
+    vp.x.fp.tw = table(Lx);
     // y, z (columns)
-    // y, z (columns): tiles of T adjacent columns (T float2 per row segment);
-    // prefer T = 16 (128-byte segments), then fewest stages, then E near 16,
-    // then the smallest block.  Long columns use 512- / 1024-thread blocks and
-    // up to 160 KB of LDS (T * L * 8 bytes) to keep T >= 8.
-    auto colplan = [&](AxisLaunch& ax, int L, int axis) {
-        int best = 1 << 30;
-        for (int E : kEs) {
-            if (L % E) continue;
-            if (tune().e[axis] > 0 && E != tune().e[axis] && L != 1) continue;
-            bool ok = e_supported(E);
-            for (int pr : {2, 3, 5})
-                if (L % pr == 0 && E % pr != 0) ok = false;
-            if (!ok) continue;
-            const FftPlan fp = plan_1d(L, E, kMaxThreads * 4, E, 100, 1);
-            if (fp.E != E) continue;
-            for (int nt : {kMaxThreads, 512, 1024}) {
-                if (nt != kMaxThreads && !col_wide_available(E, nt)) continue;
-                if (fp.P > nt) continue;
-                int T = 1;
-                while (T * 2 <= 16 && T * 2 * fp.P <= nt && (size_t)T * 2 * L * sizeof(float2) <= kMaxLds) T *= 2;
-                if (tune().t[axis] > 0) T = std::min(T, tune().t[axis]);
-                const int score = (16 - T) * 1000 + fp.nst * 100 + std::abs(E - 16) * 2 + (nt > kMaxThreads ? 1 : 0);
-                if (score < best) {
-                    best = score;
-                    ax.fp = fp;
-                    ax.tile = T;
-                    ax.nt = nt;
-                }
-            }
-        }
-        if (best == 1 << 30) fail(PK_ERR_UNSUPPORTED_SIZE, "no transform plan for length " + std::to_string(L));
-        ax.fp.tw = table(L);
-        ax.block = dim3(ax.tile * ax.fp.P);
-        ax.lds = std::max<size_t>((size_t)ax.tile * L * sizeof(float2), 16);
-    };
-    colplan(vp.y, (int)n2, 1);
-    colplan(vp.z, (int)n3, 2);
+    vp.y = col_candidates((int)n2, 1)[0];
+    vp.y.fp.tw = table((int)n2);
+    vp.z = col_candidates((int)n3, 2)[0];
+    vp.z.fp.tw = table((int)n3);
 
     if (vp.packed) {
         auto r = rtw.find(n1);
@@ -323,7 +400,149 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
         }
         vp.rtw = r->second->as<float2>();
     }
+    {  // shapes measured earlier for this size (this process or PK_TUNE_FILE)
+        Wisdom& w = Wisdom::get();
+        std::lock_guard<std::mutex> lk(w.mu);
+        w.load();
+        auto t = w.table.find(key);
+        if (t != w.table.end()) {
+            const auto& v = t->second;
+            const FftPlan fx = plan_1d(Lx, v[0], kMaxThreads, v[0], 10, 4);
+            AxisLaunch ay, az;
+            if (fx.E == v[0] && col_axis((int)n2, v[1], v[2], v[3], ay) && col_axis((int)n3, v[4], v[5], v[6], az) &&
+                (v[7] == 2 || v[7] == 3)) {
+                vp.x = x_axis(fx);
+                vp.x.fp.tw = table(Lx);
+                vp.y = ay;
+                vp.y.fp.tw = table((int)n2);
+                vp.z = az;
+                vp.z.fp.tw = table((int)n3);
+                vp.conv_axis = v[7];
+                vp.tuned = true;
+            }
+        }
+    }
     return plans.emplace(key, vp).first->second;
+}
+
+// ------------------------------------------------------------------ autotune
+// Measures the candidate launch shapes of one padded size on the context's
+// stream and keeps the fastest: row-pass values per thread (ratio pass), for
+// each strided axis the values per thread, block size and tile width of its
+// forward pass and of its convolution pass, and the pass order (which strided
+// axis carries the convolution: 2 * forward(other axis) + convolution(this
+// axis) per transform pair).  Buffer contents do not affect the timings: the
+// workspace spectrum, I and a scratch spectrum standing in for an OTF are
+// zeroed first.  Returns a one-line report.
+std::string pk_ctx_impl::autotune(int64_t n1, int64_t n2, int64_t n3) {
+    const std::vector<int64_t> key{n1, n2, n3};
+    const VolPlan& cur = plan(n1, n2, n3);
+    VolPlan vp = cur;
+    const int Lx = (int)(vp.packed ? n1 / 2 : n1);
+    ensure(ws.A, sizeof(float2) * (size_t)vp.S);
+    ensure(ws.I, sizeof(float) * (size_t)vp.N);
+    ensure(ws.tmp, sizeof(float2) * (size_t)vp.S);
+    PK_HIP(hipMemsetAsync(ws.A.p, 0, sizeof(float2) * (size_t)vp.S, stream));
+    PK_HIP(hipMemsetAsync(ws.I.p, 0, sizeof(float) * (size_t)vp.N, stream));
+    PK_HIP(hipMemsetAsync(ws.tmp.p, 0, sizeof(float2) * (size_t)vp.S, stream));
+    const bool prof_was = prof_on;
+    prof_on = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PK_HIP(hipEventCreate(&e0));
+    PK_HIP(hipEventCreate(&e1));
+    auto timed = [&](auto&& launch) {
+        launch();  // warm-up (code load, caches)
+        PK_HIP(hipEventRecord(e0, stream));
+        for (int r = 0; r < 3; ++r) launch();
+        PK_HIP(hipEventRecord(e1, stream));
+        PK_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return ms / 3.f;
+    };
+    float2* A = ws.A.as<float2>();
+    const float2* H = ws.tmp.as<float2>();
+    const float scale = (float)(1.0 / (double)vp.N);
+    std::string rep;
+    char buf[160];
+    // rows
+    {
+        float best = 1e30f;
+        AxisLaunch pick = vp.x;
+        for (AxisLaunch c : x_candidates(Lx, vp.packed)) {
+            c.fp.tw = vp.x.fp.tw;
+            VolPlan t = vp;
+            t.x = c;
+            XArgs a{};
+            a.mode = XM_RATIO;
+            a.spec = A;
+            a.I = ws.I.as<float>();
+            const float ms = timed([&] { run_x(t, a); });
+            std::snprintf(buf, sizeof buf, " x:E%d=%.3f", c.fp.E, ms);
+            rep += buf;
+            if (ms < best) {
+                best = ms;
+                pick = c;
+            }
+        }
+        vp.x = pick;
+    }
+    // strided axes: best forward and best convolution shape per axis
+    AxisLaunch fwd[2], conv[2];
+    float tf[2], tc[2];
+    for (int i = 0; i < 2; ++i) {
+        const int axis = i + 2, L = (int)(axis == 2 ? n2 : n3);
+        const float2* tw = (axis == 2 ? vp.y : vp.z).fp.tw;
+        tf[i] = tc[i] = 1e30f;
+        fwd[i] = conv[i] = axis == 2 ? vp.y : vp.z;
+        std::vector<AxisLaunch> cands = col_candidates(L, axis - 1);
+        if (cands.size() > 4) cands.resize(4);
+        for (AxisLaunch c : cands) {
+            c.fp.tw = tw;
+            VolPlan t = vp;
+            (axis == 2 ? t.y : t.z) = c;
+            const float mf = timed([&] { run_col(t, axis, A, CM_FWD, nullptr, 1.f); });
+            const float mc = timed([&] { run_col(t, axis, A, CM_CONV, H, scale); });
+            std::snprintf(buf, sizeof buf, " %c:E%d/%d/T%d=%.3f/%.3f", axis == 2 ? 'y' : 'z', c.fp.E, c.nt, c.tile, mf,
+                          mc);
+            rep += buf;
+            if (mf < tf[i]) {
+                tf[i] = mf;
+                fwd[i] = c;
+            }
+            if (mc < tc[i]) {
+                tc[i] = mc;
+                conv[i] = c;
+            }
+        }
+    }
+    const float order_z = 2 * tf[0] + tc[1], order_y = 2 * tf[1] + tc[0];
+    if (order_y < order_z) {
+        vp.conv_axis = 2;
+        vp.y = conv[0];
+        vp.z = fwd[1];
+    } else {
+        vp.conv_axis = 3;
+        vp.y = fwd[0];
+        vp.z = conv[1];
+    }
+    std::snprintf(buf, sizeof buf, " | conv along dim %d (%.3f vs %.3f ms per transform pair)", vp.conv_axis,
+                  std::min(order_y, order_z), std::max(order_y, order_z));
+    rep += buf;
+    vp.tuned = true;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    prof_on = prof_was;
+    plans[key] = vp;
+    {
+        Wisdom& w = Wisdom::get();
+        std::lock_guard<std::mutex> lk(w.mu);
+        w.store(key, {vp.x.fp.E, vp.y.fp.E, vp.y.nt, vp.y.tile, vp.z.fp.E, vp.z.nt, vp.z.tile, vp.conv_axis});
+    }
+    std::snprintf(buf, sizeof buf, "%lldx%lldx%lld: x E%d; y E%d/%d/T%d; z E%d/%d/T%d; conv dim %d |",
+                  (long long)n1, (long long)n2, (long long)n3, vp.x.fp.E, vp.y.fp.E, vp.y.nt, vp.y.tile, vp.z.fp.E,
+                  vp.z.nt, vp.z.tile, vp.conv_axis);
+    return std::string(buf) + rep;
 }
 
 // ------------------------------------------------------------------ passes
@@ -455,7 +674,7 @@ static int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 
 // blocks (nz * n2 a multiple of the rows per block).  PK_SLAB_PLANES overrides.
 int64_t pk_ctx_impl::slab_planes(const VolPlan& vp) const {
     int64_t k = tune().slab >= 0 ? tune().slab : 0;
-    if (k <= 0 || k >= vp.n3) return 0;
+    if (k <= 0 || k >= vp.n3 || vp.conv_axis != 3) return 0;
     while (k < vp.n3 && (k * vp.n2) % vp.x.tile != 0) ++k;
     return k >= vp.n3 ? 0 : k;
 }
@@ -620,6 +839,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         fail(PK_ERR_INVALID_ARG, "output class must be uint16 when save16bit, single otherwise");
     if (!out->data || !Iv->data) fail(PK_ERR_INVALID_ARG, "null data pointer");
 
+    if ((o->flags & PK_FLAG_AUTOTUNE) && !plan(n1, n2, n3).tuned) autotune(n1, n2, n3);
     const VolPlan& vp = plan(n1, n2, n3);
     const int64_t N = vp.N;
     if (o->flags & PK_FLAG_NO_OTF_CACHE) {  // rebuild the OTFs on every call (comparison baseline)
@@ -668,9 +888,14 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         auto slabs = [&](auto&& body) {
             for (int64_t z0 = 0; z0 < vp.n3; z0 += step) body(z0, std::min(step, vp.n3 - z0));
         };
-        auto y_fwd = [&](int64_t z0, int64_t nz) { run_col(vp, 2, A, CM_FWD, nullptr, 1.f, z0, nz); };
-        auto y_inv = [&](int64_t z0, int64_t nz) { run_col(vp, 2, A, CM_INV, nullptr, 1.f, z0, nz); };
-        auto z_conv = [&](const float2* H, bool conj) { run_col(vp, 3, A, conj ? CM_CONVJ : CM_CONV, H, scale); };
+        // the strided pass pair around the convolution (dim 2 unless the plan
+        // convolves along dim 2); slabs only split dim-2 passes
+        const int outer = vp.conv_axis == 3 ? 2 : 3;
+        auto y_fwd = [&](int64_t z0, int64_t nz) { run_col(vp, outer, A, CM_FWD, nullptr, 1.f, z0, nz); };
+        auto y_inv = [&](int64_t z0, int64_t nz) { run_col(vp, outer, A, CM_INV, nullptr, 1.f, z0, nz); };
+        auto z_conv = [&](const float2* H, bool conj) {
+            run_col(vp, vp.conv_axis, A, conj ? CM_CONVJ : CM_CONV, H, scale);
+        };
 
         // prep (decon_lucy_function.m:85-100, GPU branch): uint16 I - uint16(Background)
         // saturates at 0; single I gives max(I - Background, 0)

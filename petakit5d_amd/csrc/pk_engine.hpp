@@ -59,6 +59,11 @@ struct VolPlan {
     int packed = 0;
     AxisLaunch x, y, z;
     const float2* rtw = nullptr;  // W_{n1}^k, k in [0, n1/2]
+    // pass order: the strided axis (2 or 3) that carries the fused forward
+    // transform x OTF x inverse transform; the other strided axis takes a
+    // forward pass before it and an inverse pass after it
+    int conv_axis = 3;
+    bool tuned = false;  // launch shapes and pass order measured by autotune()
 };
 
 struct OtfEntry {
@@ -182,6 +187,7 @@ struct pk_ctx_impl {
     void run_col(const VolPlan& vp, int axis, float2* data, int mode, const float2* otf, float scale, int64_t z0 = 0,
                  int64_t nz = 0);
     int64_t slab_planes(const VolPlan& vp) const;
+    std::string autotune(int64_t n1, int64_t n2, int64_t n3);
     // convolution of the spectrum in ws.A: forward y, z-conv, inverse y
 
     void decon(int method, const pk_vol* I, const float* psf_f, const int64_t fd[3], const float* psf_b,

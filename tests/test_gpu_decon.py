@@ -73,6 +73,21 @@ def test_omw_vs_oracle(oracle, shape, psf_shape, numit):
     assert e32 <= TOL
 
 
+def test_omw_wiener_butterworth_backprojector_vs_oracle(oracle):
+    """OMW-2 with the Wiener-Butterworth back projector (BackProjector, alpha
+    0.005, order 10): it lifts high frequencies, so FP32 rounding matters more
+    than with the flipped PSF; the bar stays 1e-4 against CPU FP32."""
+    import petakit5d_amd as pk
+    I, psf = _case((64, 64, 48), (21, 21, 31), shear=31.5)
+    pb = pk.BackProjector(psf, "wiener-butterworth", 0.005, 1, 10)[0].astype(np.float32)
+    J, _ = pk.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False)
+    R32, _ = oracle.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False)
+    R64, _ = oracle.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False, precision="f64")
+    e32, e64 = rel_l2(J, R32), rel_l2(J, R64)
+    print(f"[omw wb] rel-L2 vs CPU FP32 {e32:.3e}  vs FP64 {e64:.3e}  (CPU FP32 vs FP64 {rel_l2(R32, R64):.3e})")
+    assert e32 <= TOL
+
+
 def test_config1_biggs15(oracle):
     """128x128x64, 200 emitters, PSF 31x31x41, Biggs RL 15 iterations."""
     import petakit5d_amd as pk
@@ -163,3 +178,30 @@ def test_unfused_plain_path_matches_fused(method):
 
 
 PLAIN = 3  # PK_FLAG_UNFUSED | PK_FLAG_NO_OTF_CACHE
+
+
+def test_autotune_keeps_parity_and_batch_identity(oracle, tmp_path, monkeypatch):
+    """pk_autotune measures launch shapes / pass order for a size; results stay
+    within the bar, batched and single calls stay bitwise identical, and the
+    choice is appended to PK_TUNE_FILE."""
+    import petakit5d_amd as pk
+    tf = tmp_path / "tune.txt"
+    monkeypatch.setenv("PK_TUNE_FILE", str(tf))
+    ctx = pk.Context(0)
+    shape = (96, 80, 60)
+    rep = ctx.autotune(shape)
+    print(f"[autotune] {rep}")
+    assert rep.startswith("96x80x60:") and "conv dim" in rep
+    line = tf.read_text().split()
+    assert line[:3] == ["96", "80", "60"] and line[-1] in ("2", "3")
+    I, psf = _case(shape, (11, 11, 13), shear=20.0)
+    pb = _flip(psf)
+    J, _ = pk.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False, ctx=ctx)
+    R32, _ = oracle.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False)
+    assert rel_l2(J, R32) <= TOL
+    B = pk.decon_batch([I, I[:, :, :48].copy()], psf, pb, 2, method="omw", Background=100, save16bit=False, ctx=ctx)
+    np.testing.assert_array_equal(B[0], J)
+    Jr, _, _ = pk.decon_lucy_function(I, psf, 4, Background=100, save16bit=False, ctx=ctx)
+    Rr, _, _ = oracle.decon_lucy_function(I, psf, 4, Background=100, save16bit=False)
+    assert rel_l2(Jr, Rr) <= TOL
+    ctx.close()
