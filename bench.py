@@ -50,8 +50,10 @@ def parse():
                     help="keep the built-in launch shapes instead of measuring them for the workload's sizes")
     ap.add_argument("--bp", default="wb", choices=["wb", "omw"],
                     help="OMW back projector: Wiener-Butterworth (BackProjector) or omw_backprojector_generation")
-    ap.add_argument("--events-apart", action="store_true",
-                    help="time the K steps without per-pass events; take per-pass durations from a second K-step run")
+    ap.add_argument("--events-in-timed", action="store_true",
+                    help="record the per-pass HIP events inside the timed K steps (the iteration loops then run "
+                         "kernel by kernel instead of as HIP graphs); default: time the K steps as they run in "
+                         "production and take the per-pass durations from a second, instrumented K-step run")
     return ap.parse_args()
 
 
@@ -236,7 +238,7 @@ def wl_omw(args, torch, pk, L):
         t = time.perf_counter()
         tune = {"report": ctx.autotune(shape), "s": round(time.perf_counter() - t, 3)}
     call = Caller(ctx, L, "omw", It, psf, psf_b, iters)
-    ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx, events_in_timed=not args.events_apart)
+    ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx, events_in_timed=args.events_in_timed)
     nvox = float(np.prod(shape))
     value = nvox / 1e6 / (ms / 1e3)
     rl, passes, dev_ms = roofline(prof, args.steps, "omw2_512x512x256" if shape == (512, 512, 256) and iters == 2
@@ -323,7 +325,7 @@ def wl_rl128(args, torch, pk, L):
     ctx = pk.Context(0)
     It = col_major_cuda(I, torch)
     call = Caller(ctx, L, "rl", It, psf, None, iters)
-    ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx)
+    ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx, events_in_timed=args.events_in_timed)
     nvox = float(I.size)
     res = base_result("deconvolved Mvoxel/s (Biggs RL, 15 it)", nvox / 1e6 / (ms / 1e3), "Mvoxel/s", args, ms, {
         "workload": "Biggs RL 15 iterations, 128x128x64 uint16, 200 emitters, 31x31x41 PSF, background 100, Poisson; "
@@ -514,7 +516,7 @@ def wl_large(args, torch, pk, L):
     out = {}
     for method, iters in (("omw", 2), ("rl", 10)):
         call = Caller(ctx, L, method, It, psf, psf_b, iters)
-        ms, _, prof = timed(call, max(1, args.steps // 5), 1, torch, ctx)
+        ms, _, prof = timed(call, max(1, args.steps // 5), 1, torch, ctx, events_in_timed=args.events_in_timed)
         rl, passes, dev_ms = roofline(prof, max(1, args.steps // 5))
         out[f"{method}{iters}"] = {"Mvoxel_s": round(nvox / 1e6 / (ms / 1e3), 1), "ms_per_call": round(ms, 2),
                                    "roofline": rl, "device_GB": round(ctx.device_bytes() / 2 ** 30, 1)}

@@ -135,9 +135,11 @@ struct Tune {
     int e[3] = {0, 0, 0};
     int t[3] = {0, 0, 0};
     bool generic_x = false;  // PK_FFT_GENERIC_X=1: always the runtime-plan row kernel
+    bool graphs = true;      // PK_GRAPHS=0: launch the iteration loops kernel by kernel
     int slab = -1;           // PK_SLAB_PLANES: planes per slab (0 = off), -1 = built-in
     Tune() {
         if (const char* v = std::getenv("PK_SLAB_PLANES")) slab = std::atoi(v);
+        if (const char* v = std::getenv("PK_GRAPHS")) graphs = std::atoi(v) != 0;
         if (const char* v = std::getenv("PK_FFT_GENERIC_X")) generic_x = std::atoi(v) != 0;
         const char* names_e[3] = {"PK_FFT_EX", "PK_FFT_EY", "PK_FFT_EZ"};
         const char* names_t[3] = {nullptr, "PK_FFT_TY", "PK_FFT_TZ"};
@@ -534,6 +536,7 @@ std::string pk_ctx_impl::autotune(int64_t n1, int64_t n2, int64_t n3) {
     (void)hipEventDestroy(e1);
     prof_on = prof_was;
     plans[key] = vp;
+    clear_graphs();  // captured loops hold the old launch shapes
     {
         Wisdom& w = Wisdom::get();
         std::lock_guard<std::mutex> lk(w.mu);
@@ -543,6 +546,54 @@ std::string pk_ctx_impl::autotune(int64_t n1, int64_t n2, int64_t n3) {
                   (long long)n1, (long long)n2, (long long)n3, vp.x.fp.E, vp.y.fp.E, vp.y.nt, vp.y.tile, vp.z.fp.E,
                   vp.z.nt, vp.z.tile, vp.conv_axis);
     return std::string(buf) + rep;
+}
+
+// ------------------------------------------------------------------ graphs
+void pk_ctx_impl::clear_graphs() {
+    for (auto& g : graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
+template <class Fn>
+void pk_ctx_impl::run_graphed(const std::vector<uint64_t>& key, Fn&& fn) {
+    // not while profiling (per-pass events), not on the legacy null stream, and PK_GRAPHS=0 disables
+    if (!tune().graphs || prof_on || !stream) {
+        fn();
+        return;
+    }
+    GraphEntry* e = nullptr;
+    for (auto& g : graphs)
+        if (g.key == key) e = &g;
+    if (!e) {  // first sighting: eager (code objects loaded, attributes set, plan verified)
+        if (graphs.size() >= 16) {
+            auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                        [](const GraphEntry& a, const GraphEntry& b) { return a.tick < b.tick; });
+            if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+            graphs.erase(lru);
+        }
+        graphs.push_back(GraphEntry{key, nullptr, ++graph_tick});
+        fn();
+        return;
+    }
+    e->tick = ++graph_tick;
+    if (!e->exec) {
+        PK_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            fn();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        PK_HIP(hipStreamEndCapture(stream, &g));
+        const hipError_t ie = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PK_HIP(ie);
+    }
+    PK_HIP(hipGraphLaunch(e->exec, stream));
 }
 
 // ------------------------------------------------------------------ passes
@@ -784,6 +835,7 @@ void pk_ctx_impl::build_otf(const VolPlan& vp, const float* q, const int64_t pd[
 void pk_ctx_impl::reset() {
     PK_HIP(hipStreamSynchronize(stream));
     prof_flush();
+    clear_graphs();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     ev_pool.clear();
     otf.clear();
@@ -897,6 +949,14 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
             run_col(vp, vp.conv_axis, A, conj ? CM_CONVJ : CM_CONV, H, scale);
         };
 
+        // the iteration loop reads only the workspace, the cached OTFs and the
+        // plan's tables, so it is replayed as a graph for a repeated key
+        auto u64 = [](const void* p) { return (uint64_t)reinterpret_cast<uintptr_t>(p); };
+        const std::vector<uint64_t> loop_key{
+            (uint64_t)method, (uint64_t)numit, u64(&vp), (uint64_t)vp.N, u64(Hf), u64(Hb), u64(ws.I.p), u64(ws.J.p),
+            u64(ws.A.p), u64(ws.J3.p), u64(ws.Y.p), u64(ws.J4.p), u64(ws.lambda.p), u64(ws.partials.p),
+            (uint64_t)kz, u64(stream)};
+
         // prep (decon_lucy_function.m:85-100, GPU branch): uint16 I - uint16(Background)
         // saturates at 0; single I gives max(I - Background, 0)
         XArgs pa{};
@@ -919,6 +979,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
                 run_x(vp, pa, z0, nz);
                 if (numit > 0) y_fwd(z0, nz);
             });
+            run_graphed(loop_key, [&] {
             for (int k = 1; k <= numit; ++k) {
                 z_conv(Hf, false);
                 slabs([&](int64_t z0, int64_t nz) {
@@ -943,6 +1004,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
                     if (k < numit) y_fwd(z0, nz);
                 });
             }
+            });
         } else {
             run_x(vp, pa);
             float* J2 = J;
@@ -951,6 +1013,9 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
             float* J4 = ws.J4.as<float>();
             float* lam = ws.lambda.as<float>();
             double* part = ws.partials.as<double>();
+            run_graphed(loop_key, [&] {
+            J2 = J;
+            J3 = ws.J3.as<float>();
             hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N)), dim3(256), 0, stream, J3, N, 0.f);
             hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, stream, lam, (int64_t)1, 0.f);
             PK_HIP(hipGetLastError());
@@ -997,7 +1062,8 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
                 });
                 std::swap(J2, J3);
             }
-            J = J2;
+            });
+            J = (numit % 2 == 0) ? ws.J.as<float>() : ws.J3.as<float>();  // x_numit after the swaps
         }
     }
 

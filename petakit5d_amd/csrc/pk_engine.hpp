@@ -188,6 +188,18 @@ struct pk_ctx_impl {
                  int64_t nz = 0);
     int64_t slab_planes(const VolPlan& vp) const;
     std::string autotune(int64_t n1, int64_t n2, int64_t n3);
+    // Iteration loops replayed as HIP graphs: a key's first call runs eagerly,
+    // the second captures and instantiates, later calls launch the graph.
+    struct GraphEntry {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t tick = 0;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_tick = 0;
+    template <class Fn>
+    void run_graphed(const std::vector<uint64_t>& key, Fn&& fn);
+    void clear_graphs();
     // convolution of the spectrum in ws.A: forward y, z-conv, inverse y
 
     void decon(int method, const pk_vol* I, const float* psf_f, const int64_t fd[3], const float* psf_b,

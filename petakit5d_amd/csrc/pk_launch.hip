@@ -2,7 +2,18 @@
 // compiled for its values-per-thread count.
 #include "pk_launch.hpp"
 
+#include <mutex>
+#include <set>
+
 namespace pk {
+void raise_lds_once(const void* kernel) {
+    static std::mutex mu;
+    static std::set<const void*> seen;
+    std::lock_guard<std::mutex> lk(mu);
+    if (seen.insert(kernel).second)
+        (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+}
+
 bool launch_x_part_a(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
 bool launch_xb_part_a(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch, const int64_t* starts, int n);
 bool launch_x_part_b(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);

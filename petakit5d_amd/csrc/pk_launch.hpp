@@ -27,10 +27,11 @@ inline void launch_modes_x(dim3 grid, dim3 block, size_t lds, hipStream_t s, con
 // Dynamic LDS above 64 KB needs the kernel's limit raised once (gfx950: 160 KB
 // per workgroup).
 constexpr size_t kMaxLds = 160 * 1024;
+// Raised once per kernel (its first launch happens outside any graph capture).
+void raise_lds_once(const void* kernel);
 template <class K>
 inline void allow_large_lds(K kernel) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kMaxLds);
+    raise_lds_once(reinterpret_cast<const void*>(kernel));
 }
 template <int E, int NT = kMaxThreads>
 inline void launch_modes_col(dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a, const FftPlan& pl) {

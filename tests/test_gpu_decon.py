@@ -205,3 +205,28 @@ def test_autotune_keeps_parity_and_batch_identity(oracle, tmp_path, monkeypatch)
     Rr, _, _ = oracle.decon_lucy_function(I, psf, 4, Background=100, save16bit=False)
     assert rel_l2(Jr, Rr) <= TOL
     ctx.close()
+
+
+@pytest.mark.parametrize("method", ["omw", "rl"])
+def test_graph_replays_match_eager(method):
+    """The iteration loop of a repeated call is captured (2nd call) and replayed
+    (later calls) as a HIP graph; every replay, with other eager work and
+    synchronisations in between, equals the first (eager) call bitwise."""
+    import petakit5d_amd as pk
+    ctx = pk.Context(0)
+    I, psf = _case((64, 48, 40), (9, 9, 11), seed=31)
+    I2, psf2 = _case((48, 40, 30), (7, 7, 9), seed=32)
+    pb = _flip(psf)
+
+    def run():
+        if method == "omw":
+            return pk.decon_lucy_omw_function(I, psf, pb, 3, Background=100, save16bit=False, ctx=ctx)[0]
+        return pk.decon_lucy_function(I, psf, 5, Background=100, save16bit=False, ctx=ctx)[0]
+
+    ref = run()  # eager
+    for r in range(4):  # capture, then replays
+        other = pk.decon_lucy_function(I2, psf2, 2, Background=100, save16bit=False, ctx=ctx)[0]  # eager work between
+        assert np.isfinite(other).all()
+        got = run()
+        np.testing.assert_array_equal(got, ref, err_msg=f"replay {r}")
+    ctx.close()
