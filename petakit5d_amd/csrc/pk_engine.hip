@@ -237,15 +237,36 @@ static AxisLaunch x_axis(const FftPlan& fp) {
 }
 static std::vector<AxisLaunch> x_candidates(int Lx, bool packed) {
     std::vector<AxisLaunch> out;
-    FftPlan first = plan_1d(Lx, 8, kMaxThreads, tune().e[0], 10, 4);
-    if (!packed || !xrow_available(first.L, first.E)) first = plan_1d(Lx, 4, kMaxThreads, tune().e[0], 10, 4);
-    out.push_back(x_axis(first));
-    if (tune().e[0] > 0) return out;
-    for (int E : {4, 8, 16}) {
-        if (E == first.E || Lx % E || Lx / E > kMaxThreads) continue;
-        const FftPlan fp = plan_1d(Lx, E, kMaxThreads, E, 10, 4);
-        if (fp.E == E) out.push_back(x_axis(fp));
+    if (tune().e[0] > 0) {
+        out.push_back(x_axis(plan_1d(Lx, tune().e[0], kMaxThreads, tune().e[0], 10, 4)));
+        return out;
     }
+    // every feasible values-per-thread count, scored toward 8 (compile-time
+    // kernels) and toward 4 (runtime-plan kernel)
+    std::vector<std::pair<int, FftPlan>> ct, gen;
+    for (int E : kEs) {
+        if (Lx % E || Lx / E > kMaxThreads) continue;
+        bool ok = e_supported(E);
+        for (int pr : {2, 3, 5})
+            if (Lx % pr == 0 && E % pr != 0) ok = false;
+        if (!ok) continue;
+        const FftPlan fp = plan_1d(Lx, E, kMaxThreads, E, 10, 4);
+        if (fp.E != E) continue;
+        if (packed && xrow_available(Lx, E)) ct.emplace_back(fp.nst * 10 + 4 * std::abs(E - 8), fp);
+        gen.emplace_back(fp.nst * 10 + 4 * std::abs(E - 4), fp);
+    }
+    if (ct.empty() && gen.empty()) fail(PK_ERR_UNSUPPORTED_SIZE, "no transform plan for length " + std::to_string(Lx));
+    auto by = [](const auto& a, const auto& b) { return a.first < b.first; };
+    std::stable_sort(ct.begin(), ct.end(), by);
+    std::stable_sort(gen.begin(), gen.end(), by);
+    for (auto& c : ct) out.push_back(x_axis(c.second));
+    for (auto& g : gen) {
+        if (out.size() >= 4) break;
+        bool dup = false;
+        for (auto& o : out) dup |= o.fp.E == g.second.E;
+        if (!dup) out.push_back(x_axis(g.second));
+    }
+    if (out.size() > 4) out.resize(4);
     return out;
 }
 

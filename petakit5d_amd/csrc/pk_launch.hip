@@ -33,31 +33,6 @@ bool launch_colb_part_b(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s,
 bool launch_col_part_c(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
 bool launch_colb_part_c(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch, const int64_t* starts, int n);
 
-bool launch_xrow_a(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
-bool launch_xrow_b(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
-bool launch_xrow_c(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
-bool launch_xrowb_a(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
-                    const int64_t* starts, int n);
-bool launch_xrowb_b(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
-                    const int64_t* starts, int n);
-bool launch_xrowb_c(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
-                    const int64_t* starts, int n);
-bool launch_xrow_batch(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
-                       const int64_t* starts, int n) {
-    return launch_xrowb_a(L, E, grid, lds, s, mode, ch, starts, n) ||
-           launch_xrowb_b(L, E, grid, lds, s, mode, ch, starts, n) ||
-           launch_xrowb_c(L, E, grid, lds, s, mode, ch, starts, n);
-}
-bool xrow_available_a(int L, int E);
-bool xrow_available_b(int L, int E);
-bool xrow_available_c(int L, int E);
-
-bool launch_xrow(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw) {
-    return launch_xrow_a(L, E, grid, lds, s, a, tw) || launch_xrow_b(L, E, grid, lds, s, a, tw) ||
-           launch_xrow_c(L, E, grid, lds, s, a, tw);
-}
-bool xrow_available(int L, int E) { return xrow_available_a(L, E) || xrow_available_b(L, E) || xrow_available_c(L, E); }
-
 hipError_t launch_xpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a) {
     const bool ok = launch_x_part_a(pl, grid, block, lds, s, a) ||
            launch_x_part_b(pl, grid, block, lds, s, a) ||

@@ -230,3 +230,23 @@ def test_graph_replays_match_eager(method):
         got = run()
         np.testing.assert_array_equal(got, ref, err_msg=f"replay {r}")
     ctx.close()
+
+
+@pytest.mark.parametrize("n1", [72, 96, 120, 144, 160, 192, 240, 288, 480, 640, 1920])
+def test_row_lengths_vs_oracle(oracle, n1):
+    """Every family of compile-time row kernels (row length n1 = 2L with L =
+    P * E, P a power of two, E = 8/10/12/18/30) against the oracle, OMW-2 and
+    Biggs-4, through the single and the batched entry points."""
+    import petakit5d_amd as pk
+    shape = (n1, 12, 10)
+    print(f"[rows] n1={n1}", flush=True)
+    I, psf = _case(shape, (7, 7, 5), seed=40 + n1 % 17)
+    pb = _flip(psf)
+    J, _ = pk.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False)
+    R, _ = oracle.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False)
+    assert rel_l2(J, R) <= TOL
+    B = pk.decon_batch([I], psf, pb, 2, method="omw", Background=100, save16bit=False)
+    np.testing.assert_array_equal(B[0], J)
+    Jr, _, _ = pk.decon_lucy_function(I, psf, 4, Background=100, save16bit=False)
+    Rr, _, _ = oracle.decon_lucy_function(I, psf, 4, Background=100, save16bit=False)
+    assert rel_l2(Jr, Rr) <= TOL
