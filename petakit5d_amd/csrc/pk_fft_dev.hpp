@@ -652,7 +652,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
         for (int e = 0; e < E; ++e) {
             const int k = p + e * P;
             const float2 A = lds[lds_slot<true>(lbase, k, 1)];
-            const float2 B = cconj(lds[lds_slot<true>(lbase, (L - k) & (L - 1), 1)]);
+            const float2 B = cconj(lds[lds_slot<true>(lbase, k == 0 ? 0 : L - k, 1)]);
             X[e] = r2c_post(A, B, rt[e * P]);
         }
         const float2 z0 = lds[lbase];
