@@ -514,18 +514,22 @@ def wl_large(args, torch, pk, L):
     ctx = pk.Context(0)
     nvox = float(np.prod(shape))
     out = {}
+    tune = None if args.no_autotune else ctx.autotune(shape)
     for method, iters in (("omw", 2), ("rl", 10)):
         call = Caller(ctx, L, method, It, psf, psf_b, iters)
         ms, _, prof = timed(call, max(1, args.steps // 5), 1, torch, ctx, events_in_timed=args.events_in_timed)
         rl, passes, dev_ms = roofline(prof, max(1, args.steps // 5))
         out[f"{method}{iters}"] = {"Mvoxel_s": round(nvox / 1e6 / (ms / 1e3), 1), "ms_per_call": round(ms, 2),
-                                   "roofline": rl, "device_GB": round(ctx.device_bytes() / 2 ** 30, 1)}
+                                   "roofline": rl, "passes": passes,
+                                   "device_GB": round(ctx.device_bytes() / 2 ** 30, 1)}
         del call
     res = base_result("deconvolved Mvoxel/s (OMW, 2 it, 2048x2048x512)", out["omw2"]["Mvoxel_s"], "Mvoxel/s", args,
                       out["omw2"]["ms_per_call"], {
                           "workload": f"{shape[0]}x{shape[1]}x{shape[2]} uint16 (2^31 voxels), 500k blobs, 64x64x101 "
                                       "PSF, fully resident in HBM; OMW-2 and Biggs-10", "volume": list(shape)})
     res["large"] = out
+    if tune:
+        res["autotune"] = tune
     res["gen_s"] = round(t_gen, 1)
     if not args.no_check:
         po = oracle()

@@ -136,10 +136,15 @@ struct Tune {
     int t[3] = {0, 0, 0};
     bool generic_x = false;  // PK_FFT_GENERIC_X=1: always the runtime-plan row kernel
     bool graphs = true;      // PK_GRAPHS=0: launch the iteration loops kernel by kernel
+    // XCD-aware tile order in the strided passes (PK_XCD_REMAP=0 turns it off);
+    // measured: 512x512x256 z_conv 0.190 -> 0.174 ms, 2048-point dim-2 passes
+    // 5.19 -> 4.64 ms
+    int xcd = 1;
     int slab = -1;           // PK_SLAB_PLANES: planes per slab (0 = off), -1 = built-in
     Tune() {
         if (const char* v = std::getenv("PK_SLAB_PLANES")) slab = std::atoi(v);
         if (const char* v = std::getenv("PK_GRAPHS")) graphs = std::atoi(v) != 0;
+        if (const char* v = std::getenv("PK_XCD_REMAP")) xcd = std::atoi(v);
         if (const char* v = std::getenv("PK_FFT_GENERIC_X")) generic_x = std::atoi(v) != 0;
         const char* names_e[3] = {"PK_FFT_EX", "PK_FFT_EY", "PK_FFT_EZ"};
         const char* names_t[3] = {nullptr, "PK_FFT_TY", "PK_FFT_TZ"};
@@ -715,6 +720,8 @@ void pk_ctx_impl::run_col(const VolPlan& vp, int axis, float2* data, int mode, c
         a.stride = vp.hp * vp.n2;
         grid = dim3((unsigned)((a.acount + a.T - 1) / a.T), 1);
     }
+    // the remap needs a whole number of tiles per XCD
+    a.xcd = (tune().xcd && ((int64_t)grid.x * grid.y) % 8 == 0) ? 1 : 0;
     hipEvent_t e0 = prof_mark();
     PK_HIP(launch_colpass(ax.fp, ax.nt, grid, ax.block, ax.lds, stream, a));
     if (prof_on) {

@@ -752,6 +752,8 @@ struct ColArgs {
     int T;               // columns per block
     int mode;
     float scale;         // applied with the OTF product
+    int xcd;             // >0: XCD-aware tile order (blocks go round-robin to the 8 XCDs; with this each
+                         // XCD walks a contiguous range of tiles, sharing pages and L2 lines)
 };
 
 struct ColChunk {
@@ -808,7 +810,15 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
 template <int E, int MODE, int NT = kMaxThreads>
 __global__ void __launch_bounds__(NT) k_colpass(ColArgs a, FftPlan pl) {
     extern __shared__ float2 lds[];
-    colpass_body<E, MODE>(a, pl, lds, blockIdx.x, blockIdx.y);
+    int64_t bx = blockIdx.x, by = blockIdx.y;
+    if (a.xcd) {  // block b runs on XCD b % 8: give XCD k the k-th eighth of the tiles, in order
+        const int64_t gx = gridDim.x, total = gx * gridDim.y;
+        const int64_t b = bx + by * gx;
+        const int64_t t = (b & 7) * (total >> 3) + (b >> 3);
+        bx = t % gx;
+        by = t / gx;
+    }
+    colpass_body<E, MODE>(a, pl, lds, bx, by);
 }
 
 template <int E, int MODE, int NT = kMaxThreads>
