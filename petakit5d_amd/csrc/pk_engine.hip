@@ -297,19 +297,32 @@ static std::vector<AxisLaunch> col_candidates(int L, int axis) {
             int T = 1;
             while (T * 2 <= 16 && T * 2 * fp.P <= nt && (size_t)T * 2 * L * sizeof(float2) <= kMaxLds) T *= 2;
             if (tune().t[axis] > 0) T = std::min(T, tune().t[axis]);
-            AxisLaunch ax;
-            ax.fp = fp;
-            ax.tile = T;
-            ax.nt = nt;
-            ax.block = dim3(T * fp.P);
-            ax.lds = std::max<size_t>((size_t)T * L * sizeof(float2), 16);
-            c.emplace_back((16 - T) * 1000 + fp.nst * 100 + std::abs(E - 16) * 2 + (nt > kMaxThreads ? 1 : 0), ax);
+            // also 32-column tiles (256-byte row segments) where they fit: ranked
+            // after the 16-column shape of the same E, for the autotuner to measure
+            for (int Tc : {T, 2 * T}) {
+                if (Tc > T && (T != 16 || Tc * fp.P > nt || (size_t)Tc * L * sizeof(float2) > kMaxLds ||
+                               tune().t[axis] > 0))
+                    continue;
+                AxisLaunch ax;
+                ax.fp = fp;
+                ax.tile = Tc;
+                ax.nt = nt;
+                ax.block = dim3(Tc * fp.P);
+                ax.lds = std::max<size_t>((size_t)Tc * L * sizeof(float2), 16);
+                c.emplace_back((16 - T) * 1000 + fp.nst * 100 + std::abs(E - 16) * 2 + (nt > kMaxThreads ? 1 : 0) +
+                                   (Tc > T ? 50 : 0),
+                               ax);
+            }
         }
     }
     if (c.empty()) fail(PK_ERR_UNSUPPORTED_SIZE, "no transform plan for length " + std::to_string(L));
     std::stable_sort(c.begin(), c.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
     std::vector<AxisLaunch> out;
-    for (auto& e : c) out.push_back(e.second);
+    for (auto& e : c) {  // one block size per (E, tile): the smallest that fits ranks first
+        bool dup = false;
+        for (auto& o : out) dup |= o.fp.E == e.second.fp.E && o.tile == e.second.tile;
+        if (!dup) out.push_back(e.second);
+    }
     return out;
 }
 
