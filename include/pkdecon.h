@@ -78,6 +78,10 @@ typedef struct {
 /* PK_FLAG_AUTOTUNE: measure the launch shapes and pass order of a padded size
  * the first time the context meets it (pk_autotune), then use them. */
 #define PK_FLAG_AUTOTUNE 4
+/* PK_FLAG_EPILOGUE_PASS: run the epilogue (damp, scale / offset / mask,
+ * uint16, bbox) as a pass of its own after the last update instead of inside
+ * it; the results are bitwise identical (comparison baseline). */
+#define PK_FLAG_EPILOGUE_PASS 8
 
 typedef struct pk_ctx pk_ctx;
 

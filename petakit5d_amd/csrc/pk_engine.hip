@@ -701,6 +701,15 @@ void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a, int64_t z0, int64_t nz) {
             case XM_BIGGS_Y: name = "x_biggs_y"; b = 12 * N + 8 * S; break;
             case XM_BIGGS_UPDATE: name = "x_biggs_update"; b = 8 * S + 12 * N + (a.acc ? 4 * N : 0) + (a.r2c_out ? 8 * S : 0); break;
         }
+        if (a.mode == XM_OMW_FINAL || a.mode == XM_BIGGS_FINAL) {
+            // fused last update: spectrum and estimate operand in, the
+            // epilogue's output (and damp / mask operands) out; no J store
+            const EpiArgs& ep = a.ep;
+            const double on = (double)ep.o1 * ep.o2 * ep.o3;
+            const double mb = (ep.apply_scale && ep.use_mask) ? (ep.mask ? 1.0 : ep.raw16 ? 2.0 : 4.0) : 0.0;
+            name = a.mode == XM_OMW_FINAL ? "x_update_epi" : "x_biggs_update_epi";
+            b = 8 * S + 4 * N + on * ((ep.out16 ? 2.0 : 4.0) + (ep.damp > 1.f ? 4.0 : 0.0) + mb);
+        }
         prof_done(name, b * frac, e0);
     }
 }
@@ -975,6 +984,66 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
     float* J = ws.J.as<float>();
     float2* A = ws.A.as<float2>();
 
+    // epilogue (decon_lucy_function.m:177-203)
+    const bool need_mask = o->decon_offset != 0 || o->edge_erosion > 0;
+    const uint8_t* mask = nullptr;
+    if (o->edge_erosion > 0) {
+        ensure(ws.mask, (size_t)N);
+        ensure(ws.mask2, (size_t)N);
+        ensure(ws.count, 16);
+        PK_HIP(hipMemsetAsync(ws.count.p, 0, 16, stream));
+        auto* cnt = ws.count.as<unsigned long long>();
+        uint8_t* m1 = ws.mask.as<uint8_t>();
+        uint8_t* m2 = ws.mask2.as<uint8_t>();
+        const int is3d = n3 > 1;
+        hipLaunchKernelGGL(k_mask_init, dim3(grid_for(N)), dim3(256), 0, stream, raw16, rawf, N, m1, cnt);
+        hipLaunchKernelGGL(k_mask_faces, dim3(grid_for(N)), dim3(256), 0, stream, m1, n1, n2, n3, is3d, cnt,
+                           (int)o->edge_erosion);
+        hipLaunchKernelGGL(k_mask_erode_disk, dim3(grid_for(N)), dim3(256), 0, stream, m1, m2, n1, n2, n3,
+                           (int)o->edge_erosion - 1, cnt);
+        mask = m2;
+        if (is3d) {
+            hipLaunchKernelGGL(k_mask_erode_z, dim3(grid_for(N)), dim3(256), 0, stream, m2, m1, n1, n2, n3,
+                               (int)o->edge_erosion - 1, cnt);
+            mask = m1;
+        }
+        PK_HIP(hipGetLastError());
+    }
+    const int64_t on = od[0] * od[1] * od[2];
+    const size_t osz = o->save16bit ? 2 : 4;
+    void* dout = out->data;
+    if (out->mem == PK_MEM_HOST) {
+        ensure(ws.out, osz * on);
+        dout = ws.out.p;
+    }
+    EpiArgs e{};
+    e.I = I;
+    e.mask = mask;
+    e.raw16 = raw16;
+    e.rawf = rawf;
+    e.out16 = o->save16bit ? static_cast<uint16_t*>(dout) : nullptr;
+    e.outf = o->save16bit ? nullptr : static_cast<float*>(dout);
+    e.n1 = n1;
+    e.n2 = n2;
+    e.o1 = od[0];
+    e.o2 = od[1];
+    e.o3 = od[2];
+    e.b1 = b0[0];
+    e.b2 = b0[1];
+    e.b3 = b0[2];
+    e.damp = (float)o->damp_factor;
+    e.scale = (float)o->scale_factor;
+    e.offset = (float)o->decon_offset;
+    e.apply_scale = (o->scale_factor != 1 || o->decon_offset != 0 || o->edge_erosion > 0);
+    e.use_mask = need_mask;
+    if (!(o->damp_factor > 1)) e.damp = 0.f;
+    // pair stores in the fused form: even dim-1 crop start and size, aligned output
+    e.pair = vp.packed && b0[0] % 2 == 0 && od[0] % 2 == 0 &&
+             reinterpret_cast<uintptr_t>(dout) % (2 * osz) == 0;
+    // the last update writes the output itself (fused epilogue) when the row
+    // pass covers the whole volume in one launch
+    const int64_t kz = (o->flags & PK_FLAG_UNFUSED) ? 0 : slab_planes(vp);
+    const bool fuse_epi = !(o->flags & (PK_FLAG_UNFUSED | PK_FLAG_EPILOGUE_PASS)) && numit > 0 && kz == 0;
     if (o->flags & PK_FLAG_UNFUSED) {
         const float bgv = raw16 ? (float)matlab_u16(o->background) : (float)o->background;
         decon_unfused(method, vp, raw, raw16 != nullptr, bgv, Hf, Hb, numit, I, J);
@@ -983,7 +1052,6 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         // each slab's chain (dim-2 inverse, dim-1 step, dim-2 forward) reuses
         // its data from the memory-side cache; the dim-3 convolution covers the
         // whole volume between chains.  kz = 0: one slab (the whole volume).
-        const int64_t kz = slab_planes(vp);
         const int64_t step = kz > 0 ? kz : vp.n3;
         const float scale = (float)(1.0 / (double)vp.N);  // ifftn's 1/N
         auto slabs = [&](auto&& body) {
@@ -1004,7 +1072,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         const std::vector<uint64_t> loop_key{
             (uint64_t)method, (uint64_t)numit, u64(&vp), (uint64_t)vp.N, u64(Hf), u64(Hb), u64(ws.I.p), u64(ws.J.p),
             u64(ws.A.p), u64(ws.J3.p), u64(ws.Y.p), u64(ws.J4.p), u64(ws.lambda.p), u64(ws.partials.p),
-            (uint64_t)kz, u64(stream)};
+            (uint64_t)kz, u64(stream), (uint64_t)fuse_epi};
 
         // prep (decon_lucy_function.m:85-100, GPU branch): uint16 I - uint16(Background)
         // saturates at 0; single I gives max(I - Background, 0)
@@ -1049,11 +1117,20 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
                     u.J = J;
                     u.Jin = k == 1 ? I : nullptr;
                     u.r2c_out = k < numit;
-                    run_x(vp, u, z0, nz);
+                    if (!(fuse_epi && k == numit)) run_x(vp, u, z0, nz);
                     if (k < numit) y_fwd(z0, nz);
                 });
             }
             });
+            if (fuse_epi) {  // last update + epilogue
+                XArgs u{};
+                u.mode = XM_OMW_FINAL;
+                u.spec = A;
+                u.J = J;
+                u.Jin = numit == 1 ? I : nullptr;
+                u.ep = e;
+                run_x(vp, u);
+            }
         } else {
             run_x(vp, pa);
             float* J2 = J;
@@ -1107,73 +1184,30 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
                     u.write_j4 = (k < numit);
                     u.partials = part;
                     u.r2c_out = 0;
-                    run_x(vp, u, z0, nz);
+                    if (!(fuse_epi && k == numit)) run_x(vp, u, z0, nz);
                 });
                 std::swap(J2, J3);
             }
             });
+            if (fuse_epi) {  // last update + epilogue
+                XArgs u{};
+                u.mode = XM_BIGGS_FINAL;
+                u.spec = A;
+                u.Y = Y;
+                u.ep = e;
+                run_x(vp, u);
+            }
             J = (numit % 2 == 0) ? ws.J.as<float>() : ws.J3.as<float>();  // x_numit after the swaps
         }
     }
 
-    // epilogue (decon_lucy_function.m:177-203)
-    const bool need_mask = o->decon_offset != 0 || o->edge_erosion > 0;
-    const uint8_t* mask = nullptr;
-    if (o->edge_erosion > 0) {
-        ensure(ws.mask, (size_t)N);
-        ensure(ws.mask2, (size_t)N);
-        ensure(ws.count, 16);
-        PK_HIP(hipMemsetAsync(ws.count.p, 0, 16, stream));
-        auto* cnt = ws.count.as<unsigned long long>();
-        uint8_t* m1 = ws.mask.as<uint8_t>();
-        uint8_t* m2 = ws.mask2.as<uint8_t>();
-        const int is3d = n3 > 1;
-        hipLaunchKernelGGL(k_mask_init, dim3(grid_for(N)), dim3(256), 0, stream, raw16, rawf, N, m1, cnt);
-        hipLaunchKernelGGL(k_mask_faces, dim3(grid_for(N)), dim3(256), 0, stream, m1, n1, n2, n3, is3d, cnt,
-                           (int)o->edge_erosion);
-        hipLaunchKernelGGL(k_mask_erode_disk, dim3(grid_for(N)), dim3(256), 0, stream, m1, m2, n1, n2, n3,
-                           (int)o->edge_erosion - 1, cnt);
-        mask = m2;
-        if (is3d) {
-            hipLaunchKernelGGL(k_mask_erode_z, dim3(grid_for(N)), dim3(256), 0, stream, m2, m1, n1, n2, n3,
-                               (int)o->edge_erosion - 1, cnt);
-            mask = m1;
-        }
+    if (!fuse_epi) {
+        e.J = (method == 1 && numit == 0 && !(o->flags & PK_FLAG_UNFUSED)) ? I : J;  // OMW with no iteration: J = I
+        hipEvent_t ee = prof_mark();
+        hipLaunchKernelGGL(k_epilogue, dim3(epi_grid(e)), dim3(256), 0, stream, e);
         PK_HIP(hipGetLastError());
+        prof_done("epilogue", (double)on * (4.0 + (e.damp > 1.f ? 4.0 : 0.0) + (double)osz), ee);
     }
-    const int64_t on = od[0] * od[1] * od[2];
-    const size_t osz = o->save16bit ? 2 : 4;
-    void* dout = out->data;
-    if (out->mem == PK_MEM_HOST) {
-        ensure(ws.out, osz * on);
-        dout = ws.out.p;
-    }
-    EpiArgs e{};
-    e.J = (method == 1 && numit == 0 && !(o->flags & PK_FLAG_UNFUSED)) ? I : J;  // OMW with no iteration: J = I
-    e.I = I;
-    e.mask = mask;
-    e.raw16 = raw16;
-    e.rawf = rawf;
-    e.out16 = o->save16bit ? static_cast<uint16_t*>(dout) : nullptr;
-    e.outf = o->save16bit ? nullptr : static_cast<float*>(dout);
-    e.n1 = n1;
-    e.n2 = n2;
-    e.o1 = od[0];
-    e.o2 = od[1];
-    e.o3 = od[2];
-    e.b1 = b0[0];
-    e.b2 = b0[1];
-    e.b3 = b0[2];
-    e.damp = (float)o->damp_factor;
-    e.scale = (float)o->scale_factor;
-    e.offset = (float)o->decon_offset;
-    e.apply_scale = (o->scale_factor != 1 || o->decon_offset != 0 || o->edge_erosion > 0);
-    e.use_mask = need_mask;
-    if (!(o->damp_factor > 1)) e.damp = 0.f;
-    hipEvent_t ee = prof_mark();
-    hipLaunchKernelGGL(k_epilogue, dim3(epi_grid(e)), dim3(256), 0, stream, e);
-    PK_HIP(hipGetLastError());
-    prof_done("epilogue", (double)on * (4.0 + (e.damp > 1.f ? 4.0 : 0.0) + (double)osz), ee);
     if (out->mem == PK_MEM_HOST)
         PK_HIP(hipMemcpyAsync(out->data, dout, osz * on, hipMemcpyDeviceToHost, stream));
     PK_HIP(hipStreamSynchronize(stream));

@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include "pk_dft.hpp"
+#include "pk_misc.hpp"
 
 namespace pk {
 
@@ -272,7 +273,13 @@ enum XMode : int {
     XM_OMW_UPDATE = 5,   // spectrum -> t; J = max(t * J, 0); [spectrum of J]
     XM_BIGGS_Y = 6,      // Y = max(J2 + lambda (J2 - J3), 0) -> Y; spectrum of Y
     XM_BIGGS_UPDATE = 7, // spectrum -> t; Jn = max(Y * t, 0) -> Jn; J4 = Jn - Y; dot partials
+    // last update with the epilogue fused (whole volume, single volume only):
+    // the new estimate goes through the epilogue into ep.out16 / ep.outf
+    XM_OMW_FINAL = 8,    // mode 5 without the estimate store or the forward transform
+    XM_BIGGS_FINAL = 9,  // mode 7 without Jn, J4, the dot partials or the forward transform
 };
+// the iteration mode a fused final mode extends
+constexpr int xm_base(int m) { return m == XM_OMW_FINAL ? XM_OMW_UPDATE : m == XM_BIGGS_FINAL ? XM_BIGGS_UPDATE : m; }
 
 struct XArgs {
     int n1;            // real row length
@@ -303,6 +310,7 @@ struct XArgs {
     float* J4;         // Biggs x_{k-1} - y_{k-1}
     const float* lambda;  // device scalar
     double* partials;     // 2 doubles per block
+    EpiArgs ep;           // modes 8/9: epilogue operands and the (cropped) output
 };
 
 __device__ __forceinline__ float fmax0(float a) { return a > 0.f ? a : 0.f; }  // max(a,0), NaN -> 0
@@ -368,8 +376,10 @@ __device__ __forceinline__ float2 x_elem(float2 v, float2 opA, float2 opB, float
 }
 
 // One x pass, specialised on its mode: [C2R] -> elementwise -> [R2C].
-template <int E, int MODE>
+template <int E, int MODE_>
 __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, float2* lds, int64_t blk) {
+    constexpr int MODE = xm_base(MODE_);
+    constexpr bool EPI = MODE_ != MODE;  // fused epilogue
     constexpr bool IN = (MODE == XM_C2R || MODE == XM_RATIO || MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
     constexpr bool OUT_ALWAYS = (MODE == XM_R2C || MODE == XM_PREP_U16 || MODE == XM_PREP_F32 ||
                                  MODE == XM_RATIO || MODE == XM_BIGGS_Y);
@@ -406,7 +416,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
         }
         if constexpr (MODE == XM_BIGGS_UPDATE) {
             opA[e] = ld_real(a.Y, xrow, m, packed);
-            if (a.acc) opB[e] = ld_real(a.J4, xrow, m, packed);
+            if (!EPI && a.acc) opB[e] = ld_real(a.J4, xrow, m, packed);
         }
     }
 
@@ -438,6 +448,10 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
     double s_num = 0.0, s_den = 0.0;
     float lam = 0.f;
     if constexpr (MODE == XM_BIGGS_Y) lam = *a.lambda;
+    EpiRow er{0, 0, false};
+    if constexpr (EPI) {
+        if (valid) er = epi_row(a.ep, row);
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = p + e * P;
@@ -452,14 +466,22 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
                 st_real(a.I, xrow, m, packed, o);
                 if (a.J) st_real(a.J, xrow, m, packed, o);
             } else if constexpr (MODE == XM_OMW_UPDATE) {
-                st_real(a.J, xrow, m, packed, o);
+                if constexpr (EPI) {
+                    if (er.keep) epi_put(a.ep, er, packed ? 2 * (int64_t)m : m, packed, o);
+                } else {
+                    st_real(a.J, xrow, m, packed, o);
+                }
             } else if constexpr (MODE == XM_BIGGS_Y) {
                 st_real(a.Y, xrow, m, packed, o);
             } else if constexpr (MODE == XM_BIGGS_UPDATE) {
                 const float2 y = opA[e];
-                st_real(a.Jn, xrow, m, packed, o);
+                if constexpr (EPI) {
+                    if (er.keep) epi_put(a.ep, er, packed ? 2 * (int64_t)m : m, packed, o);
+                } else {
+                    st_real(a.Jn, xrow, m, packed, o);
+                }
                 const float2 d = make_float2(o.x - y.x, o.y - y.y);
-                if (a.acc) {
+                if (!EPI && a.acc) {
                     const float2 j4 = opB[e];
                     s_num += (double)(d.x * j4.x);
                     s_den += (double)(j4.x * j4.x);
@@ -468,7 +490,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
                         s_den += (double)(j4.y * j4.y);
                     }
                 }
-                if (a.write_j4) st_real(a.J4, xrow, m, packed, d);
+                if (!EPI && a.write_j4) st_real(a.J4, xrow, m, packed, d);
             }
             if (!packed) o.y = 0.f;  // odd rows: the forward transform input is real
         }
@@ -477,7 +499,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
 
     // ---- forward transform along the row (real -> half spectrum)
     bool out_spec = OUT_ALWAYS;
-    if constexpr (OUT_MAYBE) out_spec = a.r2c_out != 0;
+    if constexpr (OUT_MAYBE) out_spec = !EPI && a.r2c_out != 0;
     if (out_spec) {
         fft_run<E, false, true>(v, pl, p, lds, lbase, 1, act);
         if (packed) {
@@ -510,7 +532,7 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
     }
 
     // ---- block partial sums of the Biggs step length (fixed order)
-    if constexpr (MODE == XM_BIGGS_UPDATE) {
+    if constexpr (MODE == XM_BIGGS_UPDATE && !EPI) {
         if (a.acc) {
             __syncthreads();
             double* red = reinterpret_cast<double*>(lds);
@@ -540,8 +562,10 @@ constexpr bool kXrowLateOperands = false;
 // compile time and 256-thread blocks of 256/P rows.  Rows past the end of the
 // volume read the last row (in bounds) and store nothing; every wave runs the
 // same instruction stream.
-template <int L, int E, int MODE>
+template <int L, int E, int MODE_>
 __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restrict__ tw, float2* lds, int64_t blk) {
+    constexpr int MODE = xm_base(MODE_);
+    constexpr bool EPI = MODE_ != MODE;  // fused epilogue
     constexpr bool IN = (MODE == XM_C2R || MODE == XM_RATIO || MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
     constexpr bool OUT_ALWAYS = (MODE == XM_R2C || MODE == XM_PREP_U16 || MODE == XM_PREP_F32 ||
                                  MODE == XM_RATIO || MODE == XM_BIGGS_Y);
@@ -584,7 +608,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
             }
             if constexpr (MODE == XM_BIGGS_UPDATE) {
                 opA[e] = reinterpret_cast<const float2*>(a.Y)[r2 + m];
-                if (a.acc) opB[e] = reinterpret_cast<const float2*>(a.J4)[r2 + m];
+                if (!EPI && a.acc) opB[e] = reinterpret_cast<const float2*>(a.J4)[r2 + m];
             }
         }
     };
@@ -606,6 +630,10 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
     double s_num = 0.0, s_den = 0.0;
     float lam = 0.f;
     if constexpr (MODE == XM_BIGGS_Y) lam = *a.lambda;
+    EpiRow er{0, 0, false};
+    if constexpr (EPI) {
+        if (valid) er = epi_row(a.ep, row);
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = p + e * P;
@@ -620,28 +648,36 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
                 reinterpret_cast<float2*>(a.I)[r2 + m] = o;
                 if (a.J) reinterpret_cast<float2*>(a.J)[r2 + m] = o;
             } else if constexpr (MODE == XM_OMW_UPDATE) {
-                reinterpret_cast<float2*>(a.J)[r2 + m] = o;
+                if constexpr (EPI) {
+                    if (er.keep) epi_put(a.ep, er, 2 * (int64_t)m, true, o);
+                } else {
+                    reinterpret_cast<float2*>(a.J)[r2 + m] = o;
+                }
             } else if constexpr (MODE == XM_BIGGS_Y) {
                 reinterpret_cast<float2*>(a.Y)[r2 + m] = o;
             } else if constexpr (MODE == XM_BIGGS_UPDATE) {
                 const float2 y = opA[e];
-                reinterpret_cast<float2*>(a.Jn)[r2 + m] = o;
+                if constexpr (EPI) {
+                    if (er.keep) epi_put(a.ep, er, 2 * (int64_t)m, true, o);
+                } else {
+                    reinterpret_cast<float2*>(a.Jn)[r2 + m] = o;
+                }
                 const float2 d = make_float2(o.x - y.x, o.y - y.y);
-                if (a.acc) {
+                if (!EPI && a.acc) {
                     const float2 j4 = opB[e];
                     s_num += (double)(d.x * j4.x);
                     s_den += (double)(j4.x * j4.x);
                     s_num += (double)(d.y * j4.y);
                     s_den += (double)(j4.y * j4.y);
                 }
-                if (a.write_j4) reinterpret_cast<float2*>(a.J4)[r2 + m] = d;
+                if (!EPI && a.write_j4) reinterpret_cast<float2*>(a.J4)[r2 + m] = d;
             }
         }
         v[e] = o;
     }
 
     bool out_spec = OUT_ALWAYS;
-    if constexpr (OUT_MAYBE) out_spec = a.r2c_out != 0;
+    if constexpr (OUT_MAYBE) out_spec = !EPI && a.r2c_out != 0;
     if (out_spec) {
         ct_fft<L, E, false>(v, p, lds, lbase, tw);
 #pragma unroll
@@ -663,7 +699,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
         }
     }
 
-    if constexpr (MODE == XM_BIGGS_UPDATE) {
+    if constexpr (MODE == XM_BIGGS_UPDATE && !EPI) {
         if (a.acc) {
             __syncthreads();
             double* red = reinterpret_cast<double*>(lds);

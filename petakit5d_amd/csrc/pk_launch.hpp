@@ -21,6 +21,8 @@ inline void launch_modes_x(dim3 grid, dim3 block, size_t lds, hipStream_t s, con
         case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xpass<E, XM_OMW_UPDATE>), grid, block, lds, s, a, pl); break;
         case XM_BIGGS_Y: hipLaunchKernelGGL((k_xpass<E, XM_BIGGS_Y>), grid, block, lds, s, a, pl); break;
         case XM_BIGGS_UPDATE: hipLaunchKernelGGL((k_xpass<E, XM_BIGGS_UPDATE>), grid, block, lds, s, a, pl); break;
+        case XM_OMW_FINAL: hipLaunchKernelGGL((k_xpass<E, XM_OMW_FINAL>), grid, block, lds, s, a, pl); break;
+        case XM_BIGGS_FINAL: hipLaunchKernelGGL((k_xpass<E, XM_BIGGS_FINAL>), grid, block, lds, s, a, pl); break;
         default: break;
     }
 }
@@ -68,6 +70,8 @@ inline void launch_modes_xrow(dim3 grid, size_t lds, hipStream_t s, const XArgs&
         case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xrow<L, E, XM_OMW_UPDATE>), grid, b, lds, s, a, tw); break;
         case XM_BIGGS_Y: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_Y>), grid, b, lds, s, a, tw); break;
         case XM_BIGGS_UPDATE: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_UPDATE>), grid, b, lds, s, a, tw); break;
+        case XM_OMW_FINAL: hipLaunchKernelGGL((k_xrow<L, E, XM_OMW_FINAL>), grid, b, lds, s, a, tw); break;
+        case XM_BIGGS_FINAL: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_FINAL>), grid, b, lds, s, a, tw); break;
         default: break;
     }
 }

@@ -63,39 +63,11 @@ __global__ void k_lambda(const double* __restrict__ partials, int nblk, float ep
     }
 }
 
-__device__ __forceinline__ float u16_round(float x) {
-    // MATLAB uint16(): round half away from zero, saturate, NaN -> 0
-    if (!(x > 0.f)) return 0.f;
-    if (x >= 65535.f) return 65535.f;
-    return roundf(x);
-}
-
 // Epilogue (decon_lucy_function.m:177-203): damp, scale / offset / mask,
 // uint16 conversion and the 1-based inclusive bbox crop, one pass over the
 // output voxels.  Work items are (output row, 1024-voxel chunk) pairs: the
 // row decomposition is block-uniform, so no per-voxel integer division; when
 // the crop keeps whole dim-1 x dim-2 planes the output is one contiguous run.
-__device__ __forceinline__ float epi_voxel(const EpiArgs& a, float j, float dI_raw, float m) {
-    if (a.damp > 1.f) {
-        const float dI = a.damp * dI_raw;
-        // J.*(J<=dI) + min(J, dI.*(J>=dI)); exact equality yields 2J as in the reference
-        const float t1 = (j <= dI) ? j : 0.f;
-        const float t2 = (j >= dI) ? dI : 0.f;
-        j = t1 + (j < t2 ? j : t2);
-    }
-    if (a.apply_scale) {
-        if (a.use_mask) j = (j * a.scale + a.offset) * m;
-        else j = j * a.scale;
-    }
-    return j;
-}
-__device__ __forceinline__ float epi_mask(const EpiArgs& a, int64_t i) {
-    if (!(a.apply_scale && a.use_mask)) return 1.f;
-    if (a.mask) return a.mask[i] ? 1.f : 0.f;
-    if (a.raw16) return a.raw16[i] != 0 ? 1.f : 0.f;
-    return a.rawf[i] != 0.f ? 1.f : 0.f;
-}
-
 __global__ void __launch_bounds__(256) k_epilogue(EpiArgs a) {
     const bool flat = (a.o1 == a.n1 && a.o2 == a.n2);
     const int64_t rlen = flat ? a.o1 * a.o2 * a.o3 : a.o1;

@@ -19,7 +19,94 @@ struct EpiArgs {
     int64_t b1, b2, b3;       // 0-based bbox start
     float damp, scale, offset;
     int apply_scale, use_mask;
+    int pair;                 // fused form: dim-1 crop start and size even, output aligned for pair stores
 };
+
+// ---- per-voxel epilogue step (decon_lucy_function.m:177-203), shared by
+// k_epilogue and the fused last update of the row pass, so both give the same
+// bits.  Products and sums round separately, as MATLAB evaluates them.
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ float u16_round(float x) {
+    // MATLAB uint16(): round half away from zero, saturate, NaN -> 0
+    if (!(x > 0.f)) return 0.f;
+    if (x >= 65535.f) return 65535.f;
+    return roundf(x);
+}
+__device__ __forceinline__ float epi_voxel(const EpiArgs& a, float j, float dI_raw, float m) {
+    if (a.damp > 1.f) {
+        const float dI = a.damp * dI_raw;
+        // J.*(J<=dI) + min(J, dI.*(J>=dI)); exact equality yields 2J as in the reference
+        const float t1 = (j <= dI) ? j : 0.f;
+        const float t2 = (j >= dI) ? dI : 0.f;
+        j = t1 + (j < t2 ? j : t2);
+    }
+    if (a.apply_scale) {
+        if (a.use_mask) j = (j * a.scale + a.offset) * m;
+        else j = j * a.scale;
+    }
+    return j;
+}
+__device__ __forceinline__ float epi_mask(const EpiArgs& a, int64_t i) {
+    if (!(a.apply_scale && a.use_mask)) return 1.f;
+    if (a.mask) return a.mask[i] ? 1.f : 0.f;
+    if (a.raw16) return a.raw16[i] != 0 ? 1.f : 0.f;
+    return a.rawf[i] != 0.f ? 1.f : 0.f;
+}
+
+// Fused form: one dim-1 row of the estimate, row index `row` over dims 2 x 3.
+struct EpiRow {
+    int64_t src;  // volume index of x = 0
+    int64_t dst;  // output index of x = 0 (x outside the crop never stores)
+    bool keep;    // the row lies inside the crop in dims 2 and 3
+};
+__device__ __forceinline__ EpiRow epi_row(const EpiArgs& a, int64_t row) {
+    const int64_t y = row % a.n2, z = row / a.n2;
+    EpiRow r;
+    r.src = row * a.n1;
+    r.keep = y >= a.b2 && y < a.b2 + a.o2 && z >= a.b3 && z < a.b3 + a.o3;
+    r.dst = ((y - a.b2) + a.o2 * (z - a.b3)) * a.o1 - a.b1;
+    return r;
+}
+// Values o.x (and o.y when pair) of the estimate at x0 (and x0 + 1) of row r.
+__device__ __forceinline__ void epi_put(const EpiArgs& a, const EpiRow& r, int64_t x0, bool pair, float2 o) {
+    float dv0 = 0.f, dv1 = 0.f;
+    if (a.damp > 1.f) {
+        if (pair) {
+            const float2 d = *reinterpret_cast<const float2*>(a.I + r.src + x0);
+            dv0 = d.x;
+            dv1 = d.y;
+        } else {
+            dv0 = a.I[r.src + x0];
+        }
+    }
+    const float v0 = epi_voxel(a, o.x, dv0, epi_mask(a, r.src + x0));
+    if (!pair) {
+        if (x0 < a.b1 || x0 >= a.b1 + a.o1) return;
+        if (a.out16) a.out16[r.dst + x0] = (uint16_t)u16_round(v0);
+        else a.outf[r.dst + x0] = v0;
+        return;
+    }
+    const float v1 = epi_voxel(a, o.y, dv1, epi_mask(a, r.src + x0 + 1));
+    if (a.pair) {  // x0 even, crop start and size even: both values in or both out
+        if (x0 < a.b1 || x0 >= a.b1 + a.o1) return;
+        if (a.out16) {
+            const uint32_t w = (uint32_t)u16_round(v0) | ((uint32_t)u16_round(v1) << 16);
+            *reinterpret_cast<uint32_t*>(a.out16 + r.dst + x0) = w;
+        } else {
+            *reinterpret_cast<float2*>(a.outf + r.dst + x0) = make_float2(v0, v1);
+        }
+        return;
+    }
+    if (x0 >= a.b1 && x0 < a.b1 + a.o1) {
+        if (a.out16) a.out16[r.dst + x0] = (uint16_t)u16_round(v0);
+        else a.outf[r.dst + x0] = v0;
+    }
+    if (x0 + 1 >= a.b1 && x0 + 1 < a.b1 + a.o1) {
+        if (a.out16) a.out16[r.dst + x0 + 1] = (uint16_t)u16_round(v1);
+        else a.outf[r.dst + x0 + 1] = v1;
+    }
+}
 
 __global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v);
 __global__ void k_scatter_psf(const float* __restrict__ q, int p1, int p2, int p3, float* __restrict__ dst,
