@@ -365,7 +365,8 @@ def wl_batched(args, torch, pk, L):
         fl = 0 if args.no_autotune else 4  # PK_FLAG_AUTOTUNE: shapes measured per chunk size in the warm-up batch
         fn = lambda: pk.decon_batch(chunks, psf, psf_b, iters, method=method, Background=100, ctx=ctx,  # noqa: E731
                                     flags=fl)
-        ms, _, prof = timed(fn, max(2, args.steps // 3), 1, torch, ctx)
+        ms, _, prof = timed(fn, max(2, args.steps // 3), 1, torch, ctx, events_in_timed=args.events_in_timed)
+        prof.pop("__ms_with_events", None)
         nb = max(2, args.steps // 3)
         rec = {"Mvoxel_s": round(nvox / 1e6 / (ms / 1e3), 1), "ms_per_batch": round(ms, 2),
                "launches_per_batch": int(sum(v[0] for v in prof.values()) / nb),
@@ -380,7 +381,8 @@ def wl_batched(args, torch, pk, L):
                                                flags=PLAIN if plain else 0)
                 else:
                     pk.decon_lucy_function(c, pf, iters, Background=100, pad=True, ctx=ctx, flags=PLAIN if plain else 0)
-        sms, _, sprof = timed(seq, 1, 1, torch, ctx)
+        sms, _, sprof = timed(seq, 1, 1, torch, ctx, events_in_timed=args.events_in_timed)
+        sprof.pop("__ms_with_events", None)
         rec["one_call_per_chunk_passes_ms"] = {k: round(v[1], 3) for k, v in sorted(sprof.items())}
         rec["one_call_per_chunk_Mvoxel_s"] = round(nvox / 1e6 / (sms / 1e3), 1)
         if not args.no_plain:

@@ -58,21 +58,6 @@ int64_t smooth_up(int64_t n) {
 }
 }  // namespace
 
-__global__ void k_pad_copy(const void* __restrict__ src, void* __restrict__ dst, int esz, int64_t s1, int64_t s2,
-                           int64_t s3, int64_t d1, int64_t d2, int64_t d3) {
-    const int64_t n = d1 * d2 * d3;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int64_t x = i % d1, y = (i / d1) % d2, z = i / (d1 * d2);
-        const bool in = x < s1 && y < s2 && z < s3;
-        const int64_t si = x + s1 * (y + s2 * z);
-        if (esz == 2)
-            static_cast<uint16_t*>(dst)[i] = in ? static_cast<const uint16_t*>(src)[si] : (uint16_t)0;
-        else
-            static_cast<float*>(dst)[i] = in ? static_cast<const float*>(src)[si] : 0.f;
-    }
-}
-
 // Per-chunk Biggs step length: block c reduces chunk c's partials in the same
 // fixed order as the single-volume k_lambda.
 struct LamChunk {
@@ -138,24 +123,38 @@ void pk_ctx_impl::chunk_validate(const pk_vol& in, const pk_vol& out, const pk_o
 
 // Plans and OTFs of a chunk; returns the device bytes its state needs.
 size_t pk_ctx_impl::chunk_prepare(Chunk& c, const pk_vol& in, const pk_vol& out, const float* psf_f,
-                                  const int64_t fd[3], const float* psf_b, const int64_t bd[3]) {
+                                  const int64_t fd[3], uint64_t hf, const float* psf_b, const int64_t bd[3],
+                                  uint64_t hb) {
     for (int d = 0; d < 3; ++d) c.d[d] = in.dims[d];
     {
         const int64_t a = smooth_up(c.d[0]), b = smooth_up(c.d[1]), z = smooth_up(c.d[2]);
         if ((c.o.flags & PK_FLAG_AUTOTUNE) && !plan(a, b, z).tuned) autotune(a, b, z);
         c.vp = &plan(a, b, z);
     }
-    std::vector<float> cf;
-    int64_t cfd[3];
-    crop_psf(psf_f, fd, c.d, cf, cfd);
-    c.Hf = otf_for(*c.vp, cf.data(), cfd);
+    // The cropped PSF is a function of the whole PSF and the crop window, so
+    // its cache key is the whole PSF's key with the window folded in (the
+    // whole PSF's own key when nothing is cropped); the crop is only
+    // materialised to build a missing OTF.
+    auto otf_cropped = [&](const float* p, const int64_t pd[3], uint64_t h) {
+        int64_t win[6];
+        bool whole = true;
+        for (int d = 0; d < 3; ++d) {
+            win[d] = pd[d] > c.d[d] ? (pd[d] - c.d[d]) / 2 : 0;
+            win[d + 3] = std::min(pd[d], c.d[d]);
+            whole &= win[d + 3] == pd[d];
+        }
+        const uint64_t key = whole ? h : key_mix(h, win, 6);
+        if (const float2* hit = otf_find(*c.vp, key, win + 3)) return hit;
+        std::vector<float> q;
+        int64_t qd[3];
+        crop_psf(p, pd, c.d, q, qd);
+        return otf_build(*c.vp, key, q.data(), qd);
+    };
+    c.Hf = otf_cropped(psf_f, fd, hf);
     otf_pin(c.Hf, +1);
     if (c.method == 1) {
-        std::vector<float> cb;
-        int64_t cbd[3];
-        crop_psf(psf_b, bd, c.d, cb, cbd);
         try {
-            c.Hb = otf_for(*c.vp, cb.data(), cbd);
+            c.Hb = otf_cropped(psf_b, bd, hb);
         } catch (...) {
             otf_pin(c.Hf, -1);
             c.Hf = nullptr;
@@ -173,7 +172,7 @@ size_t pk_ctx_impl::chunk_prepare(Chunk& c, const pk_vol& in, const pk_vol& out,
     const int64_t nxb = (vp.n2 * vp.n3 + vp.x.tile - 1) / vp.x.tile;
     size_t need = 2 * align_up(sizeof(float) * vp.N) + align_up(sizeof(float2) * vp.S);
     if (c.method == 0) need += 3 * align_up(sizeof(float) * vp.N) + align_up(sizeof(double) * 2 * nxb) + kAlign;
-    if (padded || c.in_host) need += align_up(esz * vp.N);
+    if (c.in_host) need += align_up(esz * (size_t)(c.d[0] * c.d[1] * c.d[2]));  // staging, caller's layout
     if (c.out_host) need += align_up(c.out_bytes);
     c.raw_user = in.data;
     c.dout = c.out_host ? nullptr : out.data;
@@ -201,23 +200,14 @@ void pk_ctx_impl::chunk_place(Chunk& c, char* base) {
         c.lam = reinterpret_cast<float*>(take(kAlign));
     }
     const size_t esz = c.u16 ? 2 : 4;
-    const bool padded = vp.n1 != c.d[0] || vp.n2 != c.d[1] || vp.n3 != c.d[2];
+    // the input stays in the caller's (unpadded) layout: the prep pass reads
+    // it there and supplies the zero padding, the epilogue's mask reads it at
+    // output positions
     const void* raw = c.raw_user;
-    if (padded || c.in_host) {
-        void* dst = take(esz * vp.N);
-        const void* src = c.raw_user;
+    if (c.in_host) {
         const size_t ub = esz * (size_t)(c.d[0] * c.d[1] * c.d[2]);
-        if (c.in_host) {
-            // unpadded host chunk lands in J (free until the prep pass) when it needs padding
-            void* st = padded ? static_cast<void*>(c.J) : dst;
-            PK_HIP(hipMemcpyAsync(st, c.raw_user, ub, hipMemcpyHostToDevice, stream));
-            src = st;
-        }
-        if (padded) {
-            hipLaunchKernelGGL(k_pad_copy, dim3((unsigned)std::min<int64_t>((vp.N + 255) / 256, 65536)), dim3(256), 0,
-                               stream, src, dst, (int)esz, c.d[0], c.d[1], c.d[2], vp.n1, vp.n2, vp.n3);
-            PK_HIP(hipGetLastError());
-        }
+        void* dst = take(ub);
+        PK_HIP(hipMemcpyAsync(dst, c.raw_user, ub, hipMemcpyHostToDevice, stream));
         raw = dst;
     }
     c.raw16 = c.u16 ? static_cast<const uint16_t*>(raw) : nullptr;
@@ -225,6 +215,33 @@ void pk_ctx_impl::chunk_place(Chunk& c, char* base) {
     if (c.out_host) c.dout = take(c.out_bytes);
     c.k = 0;
     c.par = 0;
+}
+
+// Epilogue operands of a chunk (decon_lucy_function.m:177-203; no bbox or
+// erosion in the step engine): output = the chunk's own (unpadded) size.
+static EpiArgs chunk_epi(const Chunk& c) {
+    const VolPlan& vp = *c.vp;
+    EpiArgs e{};
+    e.J = (c.method == 0 && c.par == 1) ? c.J3 : c.J;
+    e.I = c.I;
+    e.raw16 = c.raw16;
+    e.rawf = c.rawf;
+    e.out16 = c.o.save16bit ? static_cast<uint16_t*>(c.dout) : nullptr;
+    e.outf = c.o.save16bit ? nullptr : static_cast<float*>(c.dout);
+    e.n1 = vp.n1;
+    e.n2 = vp.n2;
+    e.o1 = c.d[0];
+    e.o2 = c.d[1];
+    e.o3 = c.d[2];
+    e.damp = c.o.damp_factor > 1 ? (float)c.o.damp_factor : 0.f;
+    e.scale = (float)c.o.scale_factor;
+    e.offset = (float)c.o.decon_offset;
+    e.apply_scale = (c.o.scale_factor != 1 || c.o.decon_offset != 0);
+    e.use_mask = (c.o.decon_offset != 0);
+    e.mask_at_out = 1;
+    const size_t osz = c.o.save16bit ? 2 : 4;
+    e.pair = vp.packed && c.d[0] % 2 == 0 && reinterpret_cast<uintptr_t>(c.dout) % (2 * osz) == 0;
+    return e;
 }
 
 // ------------------------------------------------------------------ one step
@@ -318,6 +335,13 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
             XArgs a = base_x(c, c.u16 ? XM_PREP_U16 : XM_PREP_F32);
             a.raw16 = c.raw16;
             a.rawf = c.rawf;
+            const VolPlan& vp = *c.vp;
+            if (vp.n1 != c.d[0] || vp.n2 != c.d[1] || vp.n3 != c.d[2]) {  // padded plan: read the caller's layout
+                a.s0 = c.d[0];
+                a.s1 = c.d[1];
+                a.s2 = c.d[2];
+                a.pn2 = vp.n2;
+            }
             a.bg = c.u16 ? (float)std::min(65535.0, std::max(0.0, std::round(c.o.background))) : (float)c.o.background;
             a.I = c.I;
             a.J = c.J;
@@ -351,19 +375,22 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
             add_x(ratio, c, a);
         }
         add_col(zc2, c, cax, c.method == 1 ? CM_CONV : CM_CONVJ, c.method == 1 ? c.Hb : c.Hf);
+        const bool last = kn == c.numit;  // the last update writes the output through the epilogue
         if (c.method == 1) {
-            XArgs a = base_x(c, XM_OMW_UPDATE);
+            XArgs a = base_x(c, last ? XM_OMW_FINAL : XM_OMW_UPDATE);
             a.J = c.J;
-            a.r2c_out = kn < c.numit;
+            a.r2c_out = !last;
+            if (last) a.ep = chunk_epi(c);
             add_x(update, c, a);
         } else {
-            XArgs a = base_x(c, XM_BIGGS_UPDATE);
+            XArgs a = base_x(c, last ? XM_BIGGS_FINAL : XM_BIGGS_UPDATE);
             a.Y = c.Y;
             a.Jn = c.par ? c.J : c.J3;  // x_{k-2}'s buffer receives x_k
             a.J4 = c.J4;
-            a.acc = (kn >= 2 && kn < c.numit);
-            a.write_j4 = kn < c.numit;
+            a.acc = (kn >= 2 && !last);
+            a.write_j4 = !last;
             a.partials = c.part;
+            if (last) a.ep = chunk_epi(c);
             add_x(update, c, a);
         }
     }
@@ -456,26 +483,12 @@ bool Chunk::done() const { return k >= std::max(numit, 1); }
 
 // Epilogue (decon_lucy_function.m:177-197) and output copy of a finished chunk.
 void pk_ctx_impl::chunk_finish(Chunk& c) {
-    const VolPlan& vp = *c.vp;
-    EpiArgs e{};
-    e.J = (c.method == 0 && c.par == 1) ? c.J3 : c.J;
-    e.I = c.I;
-    e.raw16 = c.raw16;
-    e.rawf = c.rawf;
-    e.out16 = c.o.save16bit ? static_cast<uint16_t*>(c.dout) : nullptr;
-    e.outf = c.o.save16bit ? nullptr : static_cast<float*>(c.dout);
-    e.n1 = vp.n1;
-    e.n2 = vp.n2;
-    e.o1 = c.d[0];
-    e.o2 = c.d[1];
-    e.o3 = c.d[2];
-    e.damp = c.o.damp_factor > 1 ? (float)c.o.damp_factor : 0.f;
-    e.scale = (float)c.o.scale_factor;
-    e.offset = (float)c.o.decon_offset;
-    e.apply_scale = (c.o.scale_factor != 1 || c.o.decon_offset != 0);
-    e.use_mask = (c.o.decon_offset != 0);
-    hipLaunchKernelGGL(k_epilogue, dim3(epi_grid(e)), dim3(256), 0, stream, e);
-    PK_HIP(hipGetLastError());
+    // NUMIT > 0: the last update wrote the output (fused epilogue)
+    if (c.numit == 0) {
+        const EpiArgs e = chunk_epi(c);
+        hipLaunchKernelGGL(k_epilogue, dim3(epi_grid(e)), dim3(256), 0, stream, e);
+        PK_HIP(hipGetLastError());
+    }
     if (c.out_host) PK_HIP(hipMemcpyAsync(c.out_host, c.dout, c.out_bytes, hipMemcpyDeviceToHost, stream));
 }
 
@@ -492,6 +505,9 @@ void pk_ctx_impl::decon_batch(int method, const pk_batch_item* items, int n, con
     for (int i = 0; i < n; ++i)
         distinct[{smooth_up(items[i].in.dims[0]), smooth_up(items[i].in.dims[1]), smooth_up(items[i].in.dims[2])}]++;
     otf_max_entries = std::max<size_t>(otf_max_entries, 2 * distinct.size() + 4);
+    // one hash per PSF for the whole batch
+    const uint64_t hf = psf_key(psf_f, fd);
+    const uint64_t hb = method == 1 ? psf_key(psf_b, bd) : 0;
     std::vector<Chunk> ch((size_t)n);
     std::vector<size_t> offs((size_t)n);
     size_t arena = 0;
@@ -510,7 +526,7 @@ void pk_ctx_impl::decon_batch(int method, const pk_batch_item* items, int n, con
         ch[(size_t)i].numit = numit;
         ch[(size_t)i].o = *o;
         offs[(size_t)i] = arena;
-        arena += chunk_prepare(ch[(size_t)i], items[i].in, items[i].out, psf_f, fd, psf_b, bd);
+        arena += chunk_prepare(ch[(size_t)i], items[i].in, items[i].out, psf_f, fd, hf, psf_b, bd, hb);
     }
     ensure(ws.batch, arena);
     std::vector<Chunk*> act;

@@ -819,10 +819,20 @@ static uint64_t content_hash(const void* p, size_t n, uint64_t seed) {
 
 // OTF of PSF ./ sum(PSF(:)) at the volume size (decon_lucy_function.m:76,
 // decon_lucy_omw_function.m:286-287), half spectrum, cached on the device.
-const float2* pk_ctx_impl::otf_for(const VolPlan& vp, const float* psf, const int64_t pd[3]) {
+uint64_t psf_key(const float* psf, const int64_t pd[3]) {
     const int64_t P = pd[0] * pd[1] * pd[2];
-    const uint64_t h = content_hash(psf, sizeof(float) * (size_t)P,
-                                    content_hash(pd, sizeof(int64_t) * 3, 0x5eed));
+    return content_hash(psf, sizeof(float) * (size_t)P, content_hash(pd, sizeof(int64_t) * 3, 0x5eed));
+}
+
+uint64_t key_mix(uint64_t h, const int64_t* v, int n) { return content_hash(v, sizeof(int64_t) * (size_t)n, h); }
+
+const float2* pk_ctx_impl::otf_for(const VolPlan& vp, const float* psf, const int64_t pd[3]) {
+    const uint64_t h = psf_key(psf, pd);
+    if (const float2* hit = otf_find(vp, h, pd)) return hit;
+    return otf_build(vp, h, psf, pd);
+}
+
+const float2* pk_ctx_impl::otf_find(const VolPlan& vp, uint64_t h, const int64_t pd[3]) {
     for (auto& e : otf) {
         if (e.dims[0] == vp.n1 && e.dims[1] == vp.n2 && e.dims[2] == vp.n3 && e.hash == h &&
             e.pdims[0] == pd[0] && e.pdims[1] == pd[1] && e.pdims[2] == pd[2]) {
@@ -830,6 +840,11 @@ const float2* pk_ctx_impl::otf_for(const VolPlan& vp, const float* psf, const in
             return e.spec->as<float2>();
         }
     }
+    return nullptr;
+}
+
+const float2* pk_ctx_impl::otf_build(const VolPlan& vp, uint64_t h, const float* psf, const int64_t pd[3]) {
+    const int64_t P = pd[0] * pd[1] * pd[2];
     if (pd[0] > vp.n1 || pd[1] > vp.n2 || pd[2] > vp.n3)
         fail(PK_ERR_OUT_SIZE, "outSizeIsSmallerThanPsfSize: PSF larger than the volume");
     // evict least recently used among the unpinned entries

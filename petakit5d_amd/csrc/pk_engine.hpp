@@ -26,6 +26,11 @@ void hip_check(hipError_t e, const char* what);
 
 bool smooth235(int64_t n);
 
+// OTF cache keys: content hash of a PSF and its size; key_mix folds further
+// integers (a crop window) into a key
+uint64_t psf_key(const float* psf, const int64_t pd[3]);
+uint64_t key_mix(uint64_t h, const int64_t* v, int n);
+
 // Device allocation owned by a context (counted against its budget).
 struct pk_ctx_impl;
 struct DevBuf {
@@ -176,6 +181,9 @@ struct pk_ctx_impl {
 
     const VolPlan& plan(int64_t n1, int64_t n2, int64_t n3);
     const float2* otf_for(const VolPlan& vp, const float* psf, const int64_t pdims[3]);
+    // cache lookup by key (nullptr on a miss) and build of a missing entry
+    const float2* otf_find(const VolPlan& vp, uint64_t key, const int64_t pdims[3]);
+    const float2* otf_build(const VolPlan& vp, uint64_t key, const float* psf, const int64_t pdims[3]);
     // R2C of the zero-padded, circularly shifted PSF q (decon_psf2otf.m:19-27) into dst
     void build_otf(const VolPlan& vp, const float* q, const int64_t pdims[3], float2* dst);
 
@@ -209,8 +217,9 @@ struct pk_ctx_impl {
                      const float* psf_b, const int64_t bd[3], int numit, const pk_opts* o);
     // step engine
     void chunk_validate(const pk_vol& in, const pk_vol& out, const pk_opts& o);
+    // hf, hb: psf_key of psf_f / psf_b (hashed once per batch or request)
     size_t chunk_prepare(Chunk& c, const pk_vol& in, const pk_vol& out, const float* psf_f, const int64_t fd[3],
-                         const float* psf_b, const int64_t bd[3]);
+                         uint64_t hf, const float* psf_b, const int64_t bd[3], uint64_t hb);
     void chunk_place(Chunk& c, char* base);
     void run_step(std::vector<Chunk*>& act);
     void chunk_finish(Chunk& c);

@@ -311,6 +311,9 @@ struct XArgs {
     const float* lambda;  // device scalar
     double* partials;     // 2 doubles per block
     EpiArgs ep;           // modes 8/9: epilogue operands and the (cropped) output
+    // modes 2/3: raw16 / rawf in the caller's unpadded s0 x s1 x s2 layout
+    // inside the n1 x pn2 x n3 plan (zeros past its edge); s0 = 0: same layout
+    int64_t s0, s1, s2, pn2;
 };
 
 __device__ __forceinline__ float fmax0(float a) { return a > 0.f ? a : 0.f; }  // max(a,0), NaN -> 0
@@ -332,6 +335,26 @@ __device__ __forceinline__ float2 ld_u16(const uint16_t* __restrict__ base, int6
         return make_float2((float)(w & 0xffffu), (float)(w >> 16));
     }
     return make_float2((float)base[xrow + m], 0.f);
+}
+
+// Raw input of a padded volume read from the caller's unpadded layout: value
+// pair at x0, x0 + 1 (one value when !pair) of plan row `row`; positions
+// outside the s0 x s1 x s2 source read as 0 (the zero padding).
+struct RawRow {
+    int64_t src;  // source index of x = 0
+    bool in;      // the row lies inside the source in dims 2 and 3
+};
+__device__ __forceinline__ RawRow raw_row(const XArgs& a, int64_t row, int64_t n2) {
+    const int64_t y = row % n2, z = row / n2;
+    return RawRow{(y + a.s1 * z) * a.s0, y < a.s1 && z < a.s2};
+}
+template <class T>
+__device__ __forceinline__ float2 ld_raw_padded(const T* __restrict__ base, const XArgs& a, const RawRow& r,
+                                                int64_t x0, bool pair) {
+    float2 v = make_float2(0.f, 0.f);
+    if (r.in && x0 < a.s0) v.x = (float)base[r.src + x0];
+    if (pair && r.in && x0 + 1 < a.s0) v.y = (float)base[r.src + x0 + 1];
+    return v;
 }
 
 // ---- per-element arithmetic shared by the generic and the compile-time row kernels
@@ -398,6 +421,10 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
     const int64_t xrow = row * a.n1;  // real row start
 
     // ---- prefetch the real-space operands so their latency overlaps the C2R
+    RawRow rr_src{0, false};
+    if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
+        if (a.s0 && valid) rr_src = raw_row(a, row, a.pn2);
+    }
     float2 opA[E], opB[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -406,8 +433,12 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
         opB[e] = make_float2(0.f, 0.f);
         if (!valid) continue;
         if constexpr (MODE == XM_R2C) opA[e] = ld_real(a.src, xrow, m, packed);
-        if constexpr (MODE == XM_PREP_U16) opA[e] = ld_u16(a.raw16, xrow, m, packed);
-        if constexpr (MODE == XM_PREP_F32) opA[e] = ld_real(a.rawf, xrow, m, packed);
+        if constexpr (MODE == XM_PREP_U16)
+            opA[e] = a.s0 ? ld_raw_padded(a.raw16, a, rr_src, packed ? 2 * (int64_t)m : m, packed)
+                          : ld_u16(a.raw16, xrow, m, packed);
+        if constexpr (MODE == XM_PREP_F32)
+            opA[e] = a.s0 ? ld_raw_padded(a.rawf, a, rr_src, packed ? 2 * (int64_t)m : m, packed)
+                          : ld_real(a.rawf, xrow, m, packed);
         if constexpr (MODE == XM_RATIO) opA[e] = ld_real(a.I, xrow, m, packed);
         if constexpr (MODE == XM_OMW_UPDATE) opA[e] = ld_real(a.Jin ? a.Jin : a.J, xrow, m, packed);
         if constexpr (MODE == XM_BIGGS_Y) {
@@ -586,6 +617,10 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
     const float2* __restrict__ rt = a.rtw + p;
     const int64_t r2 = rowc * L;  // real row start in float2 (pairs)
 
+    RawRow rr_src{0, false};
+    if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
+        if (a.s0) rr_src = raw_row(a, rowc, a.pn2);
+    }
     float2 opA[E], opB[E];
     auto load_operands = [&]() {
 #pragma unroll
@@ -595,10 +630,16 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
             opB[e] = make_float2(0.f, 0.f);
             if constexpr (MODE == XM_R2C) opA[e] = reinterpret_cast<const float2*>(a.src)[r2 + m];
             if constexpr (MODE == XM_PREP_U16) {
-                const uint32_t w = reinterpret_cast<const uint32_t*>(a.raw16)[r2 + m];
-                opA[e] = make_float2((float)(w & 0xffffu), (float)(w >> 16));
+                if (a.s0) {
+                    opA[e] = ld_raw_padded(a.raw16, a, rr_src, 2 * (int64_t)m, true);
+                } else {
+                    const uint32_t w = reinterpret_cast<const uint32_t*>(a.raw16)[r2 + m];
+                    opA[e] = make_float2((float)(w & 0xffffu), (float)(w >> 16));
+                }
             }
-            if constexpr (MODE == XM_PREP_F32) opA[e] = reinterpret_cast<const float2*>(a.rawf)[r2 + m];
+            if constexpr (MODE == XM_PREP_F32)
+                opA[e] = a.s0 ? ld_raw_padded(a.rawf, a, rr_src, 2 * (int64_t)m, true)
+                              : reinterpret_cast<const float2*>(a.rawf)[r2 + m];
             if constexpr (MODE == XM_RATIO) opA[e] = reinterpret_cast<const float2*>(a.I)[r2 + m];
             if constexpr (MODE == XM_OMW_UPDATE)
                 opA[e] = reinterpret_cast<const float2*>(a.Jin ? a.Jin : a.J)[r2 + m];

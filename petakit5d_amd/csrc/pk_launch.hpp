@@ -90,6 +90,10 @@ inline void launch_modes_xrowb(dim3 grid, size_t lds, hipStream_t s, int mode, c
         case XM_BIGGS_UPDATE:
             hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_UPDATE>), grid, b, lds, s, ch, starts, n);
             break;
+        case XM_OMW_FINAL: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_OMW_FINAL>), grid, b, lds, s, ch, starts, n); break;
+        case XM_BIGGS_FINAL:
+            hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_FINAL>), grid, b, lds, s, ch, starts, n);
+            break;
         default: break;
     }
 }
@@ -111,6 +115,8 @@ inline void launch_modes_xb(dim3 grid, dim3 block, size_t lds, hipStream_t s, in
         case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xpass_batch<E, XM_OMW_UPDATE>), grid, block, lds, s, ch, starts, n); break;
         case XM_BIGGS_Y: hipLaunchKernelGGL((k_xpass_batch<E, XM_BIGGS_Y>), grid, block, lds, s, ch, starts, n); break;
         case XM_BIGGS_UPDATE: hipLaunchKernelGGL((k_xpass_batch<E, XM_BIGGS_UPDATE>), grid, block, lds, s, ch, starts, n); break;
+        case XM_OMW_FINAL: hipLaunchKernelGGL((k_xpass_batch<E, XM_OMW_FINAL>), grid, block, lds, s, ch, starts, n); break;
+        case XM_BIGGS_FINAL: hipLaunchKernelGGL((k_xpass_batch<E, XM_BIGGS_FINAL>), grid, block, lds, s, ch, starts, n); break;
         default: break;
     }
 }

@@ -97,10 +97,11 @@ __global__ void __launch_bounds__(256) k_epilogue(EpiArgs a) {
             const float4 jv = *reinterpret_cast<const float4*>(a.J + i);
             float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
             if (a.damp > 1.f) dv = *reinterpret_cast<const float4*>(a.I + i);
-            const float o0 = epi_voxel(a, jv.x, dv.x, epi_mask(a, i));
-            const float o1 = epi_voxel(a, jv.y, dv.y, epi_mask(a, i + 1));
-            const float o2 = epi_voxel(a, jv.z, dv.z, epi_mask(a, i + 2));
-            const float o3 = epi_voxel(a, jv.w, dv.w, epi_mask(a, i + 3));
+            const int64_t mi = a.mask_at_out ? dst + x : i;  // mask index
+            const float o0 = epi_voxel(a, jv.x, dv.x, epi_mask(a, mi));
+            const float o1 = epi_voxel(a, jv.y, dv.y, epi_mask(a, mi + 1));
+            const float o2 = epi_voxel(a, jv.z, dv.z, epi_mask(a, mi + 2));
+            const float o3 = epi_voxel(a, jv.w, dv.w, epi_mask(a, mi + 3));
             if (a.out16) {
                 ushort4 u;
                 u.x = (uint16_t)u16_round(o0);
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(256) k_epilogue(EpiArgs a) {
             const int64_t i = src + (in ? x : 0);
             jv[k] = a.J[i];
             dv[k] = a.damp > 1.f ? a.I[i] : 0.f;
-            mv[k] = epi_mask(a, i);
+            mv[k] = epi_mask(a, a.mask_at_out ? dst + (in ? x : 0) : i);
         }
 #pragma unroll
         for (int k = 0; k < kEpiChunk / 256; ++k) {

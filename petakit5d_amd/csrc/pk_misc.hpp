@@ -20,6 +20,7 @@ struct EpiArgs {
     float damp, scale, offset;
     int apply_scale, use_mask;
     int pair;                 // fused form: dim-1 crop start and size even, output aligned for pair stores
+    int mask_at_out;          // raw16 / rawf in the output's (unpadded) layout: the I ~= 0 mask reads there
 };
 
 // ---- per-voxel epilogue step (decon_lucy_function.m:177-203), shared by
@@ -80,14 +81,15 @@ __device__ __forceinline__ void epi_put(const EpiArgs& a, const EpiRow& r, int64
             dv0 = a.I[r.src + x0];
         }
     }
-    const float v0 = epi_voxel(a, o.x, dv0, epi_mask(a, r.src + x0));
+    const int64_t mi = (a.mask_at_out ? r.dst : r.src) + x0;  // mask index of x0
+    const float v0 = epi_voxel(a, o.x, dv0, epi_mask(a, mi));
     if (!pair) {
         if (x0 < a.b1 || x0 >= a.b1 + a.o1) return;
         if (a.out16) a.out16[r.dst + x0] = (uint16_t)u16_round(v0);
         else a.outf[r.dst + x0] = v0;
         return;
     }
-    const float v1 = epi_voxel(a, o.y, dv1, epi_mask(a, r.src + x0 + 1));
+    const float v1 = epi_voxel(a, o.y, dv1, epi_mask(a, mi + 1));
     if (a.pair) {  // x0 even, crop start and size even: both values in or both out
         if (x0 < a.b1 || x0 >= a.b1 + a.o1) return;
         if (a.out16) {

@@ -39,6 +39,7 @@ struct ServeReq {
     pk_vol in{}, out{};
     std::vector<float> pf, pb;
     int64_t fd[3] = {0, 0, 0}, bd[3] = {0, 0, 0};
+    uint64_t hf = 0, hb = 0;  // OTF cache keys of pf / pb, hashed at submission
     DevBuf mem;
     bool pinned = false;
     Clock::time_point t_sub, t_adm, t_done;
@@ -108,8 +109,8 @@ static void scheduler(pk_ctx_impl* ctx) {
                 }
                 try {
                     ctx->chunk_validate(r->in, r->out, r->c.o);
-                    const size_t need = ctx->chunk_prepare(r->c, r->in, r->out, r->pf.data(), r->fd,
-                                                           r->pb.empty() ? nullptr : r->pb.data(), r->bd);
+                    const size_t need = ctx->chunk_prepare(r->c, r->in, r->out, r->pf.data(), r->fd, r->hf,
+                                                           r->pb.empty() ? nullptr : r->pb.data(), r->bd, r->hb);
                     r->pinned = true;
                     ctx->alloc(r->mem, need);
                     ctx->chunk_place(r->c, r->mem.as<char>());
@@ -128,8 +129,9 @@ static void scheduler(pk_ctx_impl* ctx) {
                         bool ok = false;
                         if (ctx->otf.size() != before) {
                             try {
-                                const size_t need = ctx->chunk_prepare(r->c, r->in, r->out, r->pf.data(), r->fd,
-                                                                       r->pb.empty() ? nullptr : r->pb.data(), r->bd);
+                                const size_t need = ctx->chunk_prepare(r->c, r->in, r->out, r->pf.data(), r->fd, r->hf,
+                                                                       r->pb.empty() ? nullptr : r->pb.data(), r->bd,
+                                                                       r->hb);
                                 r->pinned = true;
                                 ctx->alloc(r->mem, need);
                                 ctx->chunk_place(r->c, r->mem.as<char>());
@@ -262,7 +264,9 @@ uint64_t serve_submit(pk_ctx_impl* ctx, const pk_request& q) {
         if (nb < 1) fail(PK_ERR_INVALID_ARG, "empty PSF_b");
         r->pb.assign(q.psf_b, q.psf_b + nb);
         std::memcpy(r->bd, q.b_dims, sizeof(r->bd));
+        r->hb = psf_key(r->pb.data(), r->bd);
     }
+    r->hf = psf_key(r->pf.data(), r->fd);
     r->t_sub = Clock::now();
     Server& S = *ctx->srv;
     {

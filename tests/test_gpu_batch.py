@@ -51,3 +51,34 @@ def test_batch_device_chunks_uint16():
     for d, h in zip(outs, host):
         assert d.is_cuda and d.dtype == torch.uint16
         np.testing.assert_array_equal(d.cpu().numpy(), h)
+
+
+@pytest.mark.parametrize("method,numit", [("omw", 2), ("rl", 3), ("omw", 0)])
+@pytest.mark.parametrize("kw", [
+    dict(save16bit=True),
+    dict(save16bit=False, scaleFactor=2.0, deconOffset=5.0),
+    dict(save16bit=True, dampFactor=1.5, scaleFactor=0.5),
+], ids=lambda k: "-".join(f"{a}={b}" for a, b in k.items()))
+@pytest.mark.parametrize("dtype", ["uint16", "single"])
+def test_batch_epilogue_options_match_single(method, numit, kw, dtype):
+    """The batched last update writes each chunk's output through the epilogue
+    and the prep pass reads each chunk in its own (unpadded) layout; every
+    epilogue option must give the bits of one single-volume call per chunk."""
+    import petakit5d_amd as pk
+    print(f"[batch epilogue] {method} numit={numit} {kw} {dtype}", flush=True)
+    psf = synth.gaussian_psf((9, 9, 11), 1.2, 3.0)
+    psf_b = np.ascontiguousarray(psf[::-1, ::-1, ::-1])
+    # padded to 64x48x40 (no padding), 50x45x36, 48x40x27 and 45x40x24 (odd rows, padded)
+    chunks = _chunks([(64, 48, 40), (50, 45, 33), (47, 40, 27), (43, 40, 21)], seed=12)
+    for c in chunks:
+        c[:2, :, :] = 0  # an I ~= 0 mask inside each chunk
+    if dtype == "single":
+        chunks = [c.astype(np.float32) for c in chunks]
+    outs = pk.decon_batch(chunks, psf, psf_b, numit, method=method, Background=100, **kw)
+    for c, J in zip(chunks, outs):
+        if method == "omw":
+            R, _ = pk.decon_lucy_omw_function(c, psf, psf_b, numit, Background=100, pad=True, **kw)
+        else:
+            R, _, _ = pk.decon_lucy_function(c, psf, numit, Background=100, pad=True, **kw)
+        assert J.shape == c.shape and J.dtype == R.dtype
+        np.testing.assert_array_equal(J, R)
