@@ -1,11 +1,18 @@
 #!/usr/bin/env python3
 """Writes pk_kern_xrow_*.hip: the (row length L, values per thread E) pairs
-that get a compile-time row kernel.  A pair qualifies when L = P * E with P a
-power of two dividing 256 (whole 256-thread blocks of 256 / P rows) and E a
-compiled values-per-thread count holding every odd prime factor of L; per L
-the E with the lowest planner score (10 * stages + 4 * |E - 8|) is chosen, the
-same rule the host planner applies.  Run from this directory after changing
-the rule; the build compiles whatever the files list."""
+that get a compile-time row kernel, one per 2/3/5-smooth L in [32, 1024] that a rule below covers.
+
+The first rule keeps every stage on home registers: E holds every prime of L
+and P = L / E is a power of two dividing 256 (whole 256-thread blocks of
+256 / P rows); the E with the lowest score 10 * stages + 4 * |E - 8| wins
+(the host planner's rule).  For L <= 512 where that rule finds nothing, or only E > 16
+(whose kernels need so many registers that one wave per SIMD fits), the
+second rule takes any E with P <= 64 (floor(64 / P) rows per wavefront, the
+64 mod P lanes left idle) or P = 128 / 256, and lets a stage take a radix
+that does not divide E (such a stage runs from LDS); it scores
+10 * stages + 10 * LDS-fed stages + 4 * |E - 8| + 100 x the idle-lane
+fraction.  Run from this directory after changing a rule; the build compiles
+whatever the files list."""
 import os
 
 E_LIST = [1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 27, 30, 32]
@@ -31,7 +38,20 @@ def stages(L, E):
     return n
 
 
-def pick(L):
+def ct_stages(L, E):
+    """Stages and LDS-fed stages of the compile-time rule (ct_radix_for)."""
+    n, mm, rem = 0, 0, L
+    while rem > 1:
+        r = next((r for r in RADICES if E % r == 0 and rem % r == 0), 0)
+        if not r:
+            r = next(r for r in RADICES if rem % r == 0)
+            mm += 1
+        rem //= r
+        n += 1
+    return n, mm
+
+
+def pick_registers(L):
     best = None
     for E in E_LIST:
         if L % E or any(L % p == 0 and E % p for p in (2, 3, 5)):
@@ -46,6 +66,28 @@ def pick(L):
         if best is None or sc < best[0]:
             best = (sc, E)
     return None if best is None else best[1]
+
+
+def pick_mixed(L):
+    best = None
+    for E in E_LIST:
+        if L % E:
+            continue
+        P = L // E
+        if P > 64 and P not in (128, 256):
+            continue
+        nst, mm = ct_stages(L, E)
+        sc = nst * 10 + mm * 10 + 4 * abs(E - 8) + (int(100 * (64 % P) / 64) if P <= 64 else 0)
+        if best is None or sc < best[0]:
+            best = (sc, E)
+    return None if best is None else best[1]
+
+
+def pick(L):
+    E = pick_registers(L)
+    if (E and E <= 16) or L > 512:
+        return E
+    return pick_mixed(L) or E
 
 
 def main():

@@ -167,7 +167,6 @@ size_t pk_ctx_impl::chunk_prepare(Chunk& c, const pk_vol& in, const pk_vol& out,
     c.out_host = out.mem == PK_MEM_HOST ? out.data : nullptr;
     c.out_bytes = (c.o.save16bit ? 2 : 4) * (size_t)(c.d[0] * c.d[1] * c.d[2]);
     const VolPlan& vp = *c.vp;
-    const bool padded = vp.n1 != c.d[0] || vp.n2 != c.d[1] || vp.n3 != c.d[2];
     const size_t esz = c.u16 ? 2 : 4;
     const int64_t nxb = (vp.n2 * vp.n3 + vp.x.tile - 1) / vp.x.tile;
     size_t need = 2 * align_up(sizeof(float) * vp.N) + align_up(sizeof(float2) * vp.S);

@@ -229,16 +229,34 @@ static void fill_twiddles(std::vector<float2>& t, int L) {
 // (measured at 512x512x256, ratio pass: E = 4 / 8 / 16 -> 0.209 / 0.175 /
 // 0.170 ms); the runtime-plan kernel spends more on index arithmetic and
 // prefers few values per thread (E = 4 / 8 / 16 -> 0.24 / 0.26 / 0.29 ms).
-static AxisLaunch x_axis(const FftPlan& fp) {
+static AxisLaunch x_axis(const FftPlan& fp, bool compiled = false) {
     AxisLaunch ax;
     ax.fp = fp;
     ax.lpad = row_pitch(fp.L);
+    if (compiled) {
+        // compile-time kernel: whole 256-thread blocks, rows packed per wavefront
+        ax.tile = xrow_tile(fp.P);
+        ax.block = dim3(kMaxThreads);
+        ax.lds = std::max<size_t>((size_t)xrow_lds_rows(fp.P) * ax.lpad * sizeof(float2),
+                                  2 * sizeof(double) * kMaxThreads);
+        return ax;
+    }
     int rt = std::max(1, kMaxThreads / fp.P);
     while (rt > 1 && (size_t)rt * ax.lpad * sizeof(float2) > 65536) rt--;
     ax.tile = rt;
     ax.block = dim3(rt * fp.P);
     ax.lds = std::max<size_t>((size_t)rt * ax.lpad * sizeof(float2), 2 * sizeof(double) * kMaxThreads);
     return ax;
+}
+// Plan of a compile-time row kernel: L and E are all the kernel takes from it
+// (its stages are compile-time constants; the table pointer is set by the caller).
+static FftPlan ct_row_plan(int L, int E) {
+    FftPlan fp{};
+    fp.L = L;
+    fp.E = E;
+    fp.P = L / E;
+    fp.nst = ct_nst(L, E);
+    return fp;
 }
 static std::vector<AxisLaunch> x_candidates(int Lx, bool packed) {
     std::vector<AxisLaunch> out;
@@ -251,20 +269,23 @@ static std::vector<AxisLaunch> x_candidates(int Lx, bool packed) {
     std::vector<std::pair<int, FftPlan>> ct, gen;
     for (int E : kEs) {
         if (Lx % E || Lx / E > kMaxThreads) continue;
+        if (packed && !tune().generic_x && xrow_available(Lx, E)) {
+            const FftPlan fp = ct_row_plan(Lx, E);
+            ct.emplace_back(fp.nst * 10 + 4 * std::abs(E - 8), fp);
+        }
         bool ok = e_supported(E);
         for (int pr : {2, 3, 5})
             if (Lx % pr == 0 && E % pr != 0) ok = false;
         if (!ok) continue;
         const FftPlan fp = plan_1d(Lx, E, kMaxThreads, E, 10, 4);
         if (fp.E != E) continue;
-        if (packed && xrow_available(Lx, E)) ct.emplace_back(fp.nst * 10 + 4 * std::abs(E - 8), fp);
         gen.emplace_back(fp.nst * 10 + 4 * std::abs(E - 4), fp);
     }
     if (ct.empty() && gen.empty()) fail(PK_ERR_UNSUPPORTED_SIZE, "no transform plan for length " + std::to_string(Lx));
     auto by = [](const auto& a, const auto& b) { return a.first < b.first; };
     std::stable_sort(ct.begin(), ct.end(), by);
     std::stable_sort(gen.begin(), gen.end(), by);
-    for (auto& c : ct) out.push_back(x_axis(c.second));
+    for (auto& c : ct) out.push_back(x_axis(c.second, true));
     for (auto& g : gen) {
         if (out.size() >= 4) break;
         bool dup = false;
@@ -448,11 +469,12 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
         auto t = w.table.find(key);
         if (t != w.table.end()) {
             const auto& v = t->second;
-            const FftPlan fx = plan_1d(Lx, v[0], kMaxThreads, v[0], 10, 4);
+            const bool ctx = vp.packed && !tune().generic_x && v[0] > 0 && Lx % v[0] == 0 && xrow_available(Lx, v[0]);
+            const FftPlan fx = ctx ? ct_row_plan(Lx, v[0]) : plan_1d(Lx, v[0], kMaxThreads, v[0], 10, 4);
             AxisLaunch ay, az;
             if (fx.E == v[0] && col_axis((int)n2, v[1], v[2], v[3], ay) && col_axis((int)n3, v[4], v[5], v[6], az) &&
                 (v[7] == 2 || v[7] == 3)) {
-                vp.x = x_axis(fx);
+                vp.x = x_axis(fx, ctx);
                 vp.x.fp.tw = table(Lx);
                 vp.y = ay;
                 vp.y.fp.tw = table((int)n2);

@@ -183,20 +183,36 @@ __device__ __forceinline__ void fft_run(float2 (&v)[E], const FftPlan& pl, int p
 // wavefront's own ordering: LDS operations of one wavefront execute in
 // order, so a wavefront-scope fence (a compiler ordering point) replaces the
 // workgroup barrier.
+//
+// A compile-time stage may also take a radix that does not divide E (E then
+// need not hold every prime of L, so long rows with the primes 2, 3 and 5
+// keep few values per thread).  Such a stage reads its butterflies' inputs
+// from LDS instead of home registers: the L / R butterflies are dealt
+// round-robin over the P threads (the last round predicated when P does not
+// divide L / R), and the outputs go to the usual Stockham positions.  Each
+// butterfly does the same operations either way, so the bits of a transform
+// depend only on its radix sequence.
+__host__ __device__ constexpr int ct_radix_for(int rem, int E) {
+    const int r = radix_for(rem, E);
+    if (r) return r;
+    for (int q : kRadixOrder)
+        if (rem % q == 0) return q;
+    return 0;
+}
 __host__ __device__ constexpr int ct_nst(int L, int E) {
     int n = 0;
-    for (int rem = L; rem > 1; ++n) rem /= radix_for(rem, E);
+    for (int rem = L; rem > 1; ++n) rem /= ct_radix_for(rem, E);
     return n;
 }
 __host__ __device__ constexpr int ct_radix(int L, int E, int s) {
     int rem = L;
-    for (int i = 0; i < s; ++i) rem /= radix_for(rem, E);
-    return radix_for(rem, E);
+    for (int i = 0; i < s; ++i) rem /= ct_radix_for(rem, E);
+    return ct_radix_for(rem, E);
 }
 __host__ __device__ constexpr int ct_ns(int L, int E, int s) {
     int ns = 1, rem = L;
     for (int i = 0; i < s; ++i) {
-        const int r = radix_for(rem, E);
+        const int r = ct_radix_for(rem, E);
         ns *= r;
         rem /= r;
     }
@@ -213,45 +229,91 @@ __device__ __forceinline__ void row_sync() {
     }
 }
 
+// One stage.  Data enter in home registers, except after a stage whose
+// successor reads LDS itself (the outputs are then left in LDS); they leave
+// in home registers after the last stage.
 template <int L, int E, int S, bool INV>
 __device__ __forceinline__ void ct_stage(float2 (&v)[E], int p, float2* lds, int lbase, const float2* __restrict__ tw) {
     constexpr int R = ct_radix(L, E, S);
     constexpr int Ns = ct_ns(L, E, S);
-    constexpr int P = L / E, Q = E / R, tws = L / (Ns * R);
+    constexpr int P = L / E, tws = L / (Ns * R);
     constexpr bool last = (S == ct_nst(L, E) - 1);
-    static_assert(R > 0 && E % R == 0, "no stage radix");
-    static_for<0, Q>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        const int j = p + i * P;
-        const int g = j / Ns;
-        const int k = j - g * Ns;
-        float2 u[R];
-        static_for<0, R>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            u[m] = v[i + m * Q];
+    constexpr bool matched = (R > 0 && E % R == 0);
+    constexpr bool next_from_lds = !last && (E % ct_radix(L, E, S + 1) != 0);
+    static_assert(R > 0, "no stage radix");
+    if constexpr (matched) {
+        constexpr int Q = E / R;
+        static_for<0, Q>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            const int j = p + i * P;
+            const int g = j / Ns;
+            const int k = j - g * Ns;
+            float2 u[R];
+            static_for<0, R>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                u[m] = v[i + m * Q];
+            });
+            if constexpr (S != 0) stage_twiddle<R, INV>(u, tw, k, tws);
+            dft<R, INV>(u);
+            if constexpr (last) {
+                static_for<0, R>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    v[i + q * Q] = u[q];
+                });
+            } else {
+                const int ob = g * Ns * R + k;
+                static_for<0, R>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    lds[lds_slot<true>(lbase, ob + q * Ns, 1)] = u[q];
+                });
+            }
         });
-        if constexpr (S != 0) stage_twiddle<R, INV>(u, tw, k, tws);
-        dft<R, INV>(u);
-        if constexpr (last) {
-            static_for<0, R>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                v[i + q * Q] = u[q];
+    } else {
+        constexpr int NBF = L / R;             // butterflies of the stage
+        constexpr int NB = (NBF + P - 1) / P;  // rounds per thread
+        if constexpr (S == 0) {  // inputs still in home registers
+            static_for<0, E>([&](auto ec) {
+                constexpr int e = decltype(ec)::value;
+                lds[lds_slot<true>(lbase, p + e * P, 1)] = v[e];
             });
-        } else {
-            const int ob = g * Ns * R + k;
-            static_for<0, R>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                lds[lds_slot<true>(lbase, ob + q * Ns, 1)] = u[q];
-            });
+            row_sync<P>();
         }
-    });
-    if constexpr (!last) {
-        row_sync<P>();
-        static_for<0, E>([&](auto ec) {
-            constexpr int e = decltype(ec)::value;
-            v[e] = lds[lds_slot<true>(lbase, p + e * P, 1)];
+        float2 u[NB][R];
+        static_for<0, NB>([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            const int j = p + b * P;
+            const int jj = (NBF % P == 0 || b + 1 < NB || j < NBF) ? j : 0;  // idle slot: an in-bounds read
+            static_for<0, R>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                u[b][m] = lds[lds_slot<true>(lbase, jj + m * NBF, 1)];
+            });
         });
+        row_sync<P>();  // every input read before any output lands
+        static_for<0, NB>([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            const int j = p + b * P;
+            if (NBF % P == 0 || b + 1 < NB || j < NBF) {
+                const int g = j / Ns;
+                const int k = j - g * Ns;
+                if constexpr (S != 0) stage_twiddle<R, INV>(u[b], tw, k, tws);
+                dft<R, INV>(u[b]);
+                const int ob = g * Ns * R + k;
+                static_for<0, R>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    lds[lds_slot<true>(lbase, ob + q * Ns, 1)] = u[b][q];
+                });
+            }
+        });
+    }
+    if constexpr (!(matched && last)) {  // outputs in LDS
         row_sync<P>();
+        if constexpr (!next_from_lds) {
+            static_for<0, E>([&](auto ec) {
+                constexpr int e = decltype(ec)::value;
+                v[e] = lds[lds_slot<true>(lbase, p + e * P, 1)];
+            });
+            row_sync<P>();
+        }
     }
 }
 
@@ -588,11 +650,26 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
 // Compile-time row kernels load their real-space operands after the inverse
 // transform when true (fewer live registers, exposed load latency).
 constexpr bool kXrowLateOperands = false;
+#ifndef XROW_LATE_E
+#define XROW_LATE_E 100
+#endif
+
+// Rows per 256-thread block of the compile-time row kernels.  A row of
+// P <= 64 threads never straddles a wavefront: each wavefront holds
+// floor(64 / P) rows, and the 64 mod P lanes left over (none when P divides
+// 64) run a dummy row in an LDS slot of their wavefront's own and store
+// nothing.  Longer rows (P = 128, 256) span whole wavefronts.
+__host__ __device__ constexpr int xrow_tile(int P) { return P <= 64 ? 4 * (64 / P) : kMaxThreads / P; }
+// LDS rows the kernel addresses: the row tile plus one dummy row per wavefront
+// when lanes are left over.
+__host__ __device__ constexpr int xrow_lds_rows(int P) {
+    return xrow_tile(P) + ((P <= 64 && 64 % P != 0) ? kMaxThreads / 64 : 0);
+}
 
 // The same pass for an even row of n1 = 2L real values with L and E fixed at
-// compile time and 256-thread blocks of 256/P rows.  Rows past the end of the
-// volume read the last row (in bounds) and store nothing; every wave runs the
-// same instruction stream.
+// compile time and 256-thread blocks of xrow_tile(P) rows.  Rows past the end
+// of the volume (and the left-over lanes) read the last row (in bounds) and
+// store nothing; every wave runs the same instruction stream.
 template <int L, int E, int MODE_>
 __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restrict__ tw, float2* lds, int64_t blk) {
     constexpr int MODE = xm_base(MODE_);
@@ -602,15 +679,26 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
                                  MODE == XM_RATIO || MODE == XM_BIGGS_Y);
     constexpr bool OUT_MAYBE = (MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
     constexpr int P = L / E;
-    constexpr int RT = kMaxThreads / P;
+    constexpr int RT = xrow_tile(P);
     constexpr int LP = row_pitch(L);
     constexpr int HP = (L + 1 + 15) / 16 * 16;  // spectrum row pitch for h1 = L + 1
-    static_assert(P * E == L && RT * P == kMaxThreads, "row tile");
+    static_assert(P * E == L && (P <= 64 || RT * P == kMaxThreads), "row tile");
     const int t = threadIdx.x;
-    const int rr = t / P;
-    const int p = t % P;
+    int rr, p;
+    bool lane_on = true;
+    if constexpr (P <= 64) {
+        constexpr int W = 64 / P;
+        const int w = t >> 6, l = t & 63;
+        const int rw = l / P;
+        p = l - rw * P;
+        lane_on = rw < W;
+        rr = lane_on ? w * W + rw : RT + w;  // RT + w: the wavefront's dummy row
+    } else {
+        rr = t / P;
+        p = t % P;
+    }
     const int64_t row = blk * RT + rr;
-    const bool valid = row < a.nrows;
+    const bool valid = lane_on && row < a.nrows;
     const int64_t rowc = valid ? row : a.nrows - 1;
     const int lbase = rr * LP;
     float2* __restrict__ sp = a.spec + rowc * HP;
@@ -656,7 +744,8 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
     // real-space operands: loaded before the inverse transform (their latency
     // overlaps it) unless kXrowLateOperands, which trades that overlap for
     // fewer live registers across the transform
-    if constexpr (!(IN && kXrowLateOperands)) load_operands();
+    constexpr bool LATE = kXrowLateOperands || E >= XROW_LATE_E;
+    if constexpr (!(IN && LATE)) load_operands();
     float2 v[E];
     if constexpr (IN) {
 #pragma unroll
@@ -665,7 +754,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
             v[e] = c2r_pre(sp[m], cconj(sp[L - m]), rt[e * P]);
         }
         ct_fft<L, E, true>(v, p, lds, lbase, tw);
-        if constexpr (kXrowLateOperands) load_operands();
+        if constexpr (LATE) load_operands();
     }
 
     double s_num = 0.0, s_den = 0.0;

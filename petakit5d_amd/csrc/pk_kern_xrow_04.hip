@@ -1,62 +1,62 @@
 // pk_kern_xrow_04.hip -- compile-time row-transform kernels, single-volume and batched
-// launches, for (row length / values per thread) 960/30, 1024/8, 256/4, 256/16, 128/4, 512/16.  Written by gen_xrow.py.
+// launches, for (row length / values per thread) 162/9, 180/9, 192/12, 200/10, 216/8, 225/15.  Written by gen_xrow.py.
 #include "pk_launch.hpp"
 
 namespace pk {
 bool launch_xrow_04(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw) {
-    if (L == 960 && E == 30) {
-        launch_modes_xrow<960, 30>(grid, lds, s, a, tw);
+    if (L == 162 && E == 9) {
+        launch_modes_xrow<162, 9>(grid, lds, s, a, tw);
         return true;
     }
-    if (L == 1024 && E == 8) {
-        launch_modes_xrow<1024, 8>(grid, lds, s, a, tw);
+    if (L == 180 && E == 9) {
+        launch_modes_xrow<180, 9>(grid, lds, s, a, tw);
         return true;
     }
-    if (L == 256 && E == 4) {
-        launch_modes_xrow<256, 4>(grid, lds, s, a, tw);
+    if (L == 192 && E == 12) {
+        launch_modes_xrow<192, 12>(grid, lds, s, a, tw);
         return true;
     }
-    if (L == 256 && E == 16) {
-        launch_modes_xrow<256, 16>(grid, lds, s, a, tw);
+    if (L == 200 && E == 10) {
+        launch_modes_xrow<200, 10>(grid, lds, s, a, tw);
         return true;
     }
-    if (L == 128 && E == 4) {
-        launch_modes_xrow<128, 4>(grid, lds, s, a, tw);
+    if (L == 216 && E == 8) {
+        launch_modes_xrow<216, 8>(grid, lds, s, a, tw);
         return true;
     }
-    if (L == 512 && E == 16) {
-        launch_modes_xrow<512, 16>(grid, lds, s, a, tw);
+    if (L == 225 && E == 15) {
+        launch_modes_xrow<225, 15>(grid, lds, s, a, tw);
         return true;
     }
     return false;
 }
 bool launch_xrowb_04(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
                       const int64_t* starts, int n) {
-    if (L == 960 && E == 30) {
-        launch_modes_xrowb<960, 30>(grid, lds, s, mode, ch, starts, n);
+    if (L == 162 && E == 9) {
+        launch_modes_xrowb<162, 9>(grid, lds, s, mode, ch, starts, n);
         return true;
     }
-    if (L == 1024 && E == 8) {
-        launch_modes_xrowb<1024, 8>(grid, lds, s, mode, ch, starts, n);
+    if (L == 180 && E == 9) {
+        launch_modes_xrowb<180, 9>(grid, lds, s, mode, ch, starts, n);
         return true;
     }
-    if (L == 256 && E == 4) {
-        launch_modes_xrowb<256, 4>(grid, lds, s, mode, ch, starts, n);
+    if (L == 192 && E == 12) {
+        launch_modes_xrowb<192, 12>(grid, lds, s, mode, ch, starts, n);
         return true;
     }
-    if (L == 256 && E == 16) {
-        launch_modes_xrowb<256, 16>(grid, lds, s, mode, ch, starts, n);
+    if (L == 200 && E == 10) {
+        launch_modes_xrowb<200, 10>(grid, lds, s, mode, ch, starts, n);
         return true;
     }
-    if (L == 128 && E == 4) {
-        launch_modes_xrowb<128, 4>(grid, lds, s, mode, ch, starts, n);
+    if (L == 216 && E == 8) {
+        launch_modes_xrowb<216, 8>(grid, lds, s, mode, ch, starts, n);
         return true;
     }
-    if (L == 512 && E == 16) {
-        launch_modes_xrowb<512, 16>(grid, lds, s, mode, ch, starts, n);
+    if (L == 225 && E == 15) {
+        launch_modes_xrowb<225, 15>(grid, lds, s, mode, ch, starts, n);
         return true;
     }
     return false;
 }
-bool xrow_available_04(int L, int E) { return (L == 960 && E == 30) || (L == 1024 && E == 8) || (L == 256 && E == 4) || (L == 256 && E == 16) || (L == 128 && E == 4) || (L == 512 && E == 16); }
+bool xrow_available_04(int L, int E) { return (L == 162 && E == 9) || (L == 180 && E == 9) || (L == 192 && E == 12) || (L == 200 && E == 10) || (L == 216 && E == 8) || (L == 225 && E == 15); }
 }  // namespace pk
