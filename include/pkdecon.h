@@ -243,6 +243,15 @@ pk_status pk_wb_backprojector(pk_ctx *ctx, const float *psf_fp, const int64_t di
  * only when nothing retunes their size. */
 pk_status pk_autotune(pk_ctx *ctx, const int64_t dims[3], char *report, size_t report_len);
 
+/* Sets the launch shapes of a padded size without measuring (no counterpart
+ * in the reference): shapes = {row values per thread, dim-2 values per
+ * thread, block size, tile width, dim-3 values per thread, block size, tile
+ * width, pass order (2 or 3: the strided axis carrying the OTF convolution)},
+ * the fields of one PK_TUNE_FILE line.  Applies to this context's later calls
+ * and to plans the process's other contexts make afterwards; not written to
+ * the file.  PK_ERR_INVALID_ARG when a shape is not feasible for the size. */
+pk_status pk_tune_set(pk_ctx *ctx, const int64_t dims[3], const int32_t shapes[8]);
+
 /* Transform diagnostics (MATLAB fftn / ifftn of real data, decon_lucy_function.m:136,142).
  * Device pointers.  r2c: real N floats -> half spectrum (floor(n1/2)+1)*n2*n3
  * complex (interleaved); c2r: half spectrum -> real, unnormalised (N * ifftn). */

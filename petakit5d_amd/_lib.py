@@ -58,6 +58,7 @@ EXPORTS = [
     "pk_omw_backprojector",
     "pk_wb_backprojector",
     "pk_autotune",
+    "pk_tune_set",
     "pk_fft_r2c",
     "pk_fft_c2r",
     "pk_profile_enable",
@@ -168,6 +169,7 @@ def load():
                                           ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), vp, vp,
                                           ctypes.POINTER(ctypes.c_double)]
         L.pk_autotune.argtypes = [vp, i64p, ctypes.c_char_p, ctypes.c_size_t]
+        L.pk_tune_set.argtypes = [vp, i64p, ctypes.POINTER(ctypes.c_int32)]
         L.pk_fft_r2c.argtypes = [vp, vp, vp, i64p]
         L.pk_fft_c2r.argtypes = [vp, vp, vp, i64p]
         L.pk_profile_enable.argtypes = [vp, ctypes.c_int32]
@@ -216,6 +218,12 @@ class Context:
         buf = ctypes.create_string_buffer(2048)
         self.check(self._L.pk_autotune(self.handle, i64x3(tuple(int(x) for x in dims)), buf, len(buf)))
         return buf.value.decode(errors="replace")
+
+    def tune_set(self, dims, shapes):
+        """Set the launch shapes of a padded size without measuring
+        (pk_tune_set): (xE, yE, yNT, yT, zE, zNT, zT, conv_axis)."""
+        arr = (ctypes.c_int32 * 8)(*[int(x) for x in shapes])
+        self.check(self._L.pk_tune_set(self.handle, i64x3(tuple(int(x) for x in dims)), arr))
 
     def profile(self, enable: bool):
         """Start (clearing) or stop per-pass HIP-event timing."""

@@ -38,7 +38,8 @@ struct Group {
     int E = 0;
     int mode = 0;
     int nt = 256;  // block size the kernels are compiled for
-    int L = 0;     // x passes: row length of a compile-time row kernel shared by the group (0: runtime plan)
+    int L = 0;     // x passes: row length of a compile-time row kernel shared by the group (0: runtime plan);
+                   // strided passes: 1 for plans with LDS-fed stages
     std::vector<T> chunks;
     std::vector<int64_t> starts{0};  // block offsets, size chunks+1
     unsigned block = 1;
@@ -305,8 +306,10 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
             gx = (a.acount + a.T - 1) / a.T;
             gy = 1;
         }
-        Group<ColChunk>& g = ps[{ax.fp.E, mode, ax.nt, 0, (int64_t)ax.block.x, (int64_t)ax.lds}];
+        const bool mixed = plan_mixed_stages(ax.fp);  // kernels of their own
+        Group<ColChunk>& g = ps[{ax.fp.E, mode, ax.nt, mixed ? 1 : 0, (int64_t)ax.block.x, (int64_t)ax.lds}];
         g.E = ax.fp.E;
+        g.L = mixed ? 1 : 0;
         g.mode = mode;
         g.nt = ax.nt;
         g.chunks.push_back(ColChunk{a, ax.fp, gx});
@@ -446,7 +449,8 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
         for (auto& kv : ps) {
             const auto& g = kv.second;
             hipEvent_t e0 = prof_mark();
-            PK_HIP(launch_colpass_batch(g.E, g.nt, dim3((unsigned)g.starts.back()), dim3(g.block), g.lds, stream, g.mode,
+            PK_HIP(launch_colpass_batch(g.E, g.nt, g.L != 0, dim3((unsigned)g.starts.back()), dim3(g.block), g.lds,
+                                        stream, g.mode,
                                         reinterpret_cast<const ColChunk*>(tab + g.dev_off),
                                         reinterpret_cast<const int64_t*>(tab + g.start_off), (int)g.chunks.size()));
             prof_done(name, 0, e0);

@@ -360,6 +360,15 @@ pk_status pk_autotune(pk_ctx* c, const int64_t dims[3], char* report, size_t rep
     });
 }
 
+pk_status pk_tune_set(pk_ctx* c, const int64_t dims[3], const int32_t shapes[8]) {
+    return guarded(c, [&] {
+        if (!dims || !shapes || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) pk::fail(PK_ERR_INVALID_ARG, "bad dims");
+        std::array<int, 8> v;
+        for (int i = 0; i < 8; ++i) v[i] = shapes[i];
+        c->set_shapes(dims[0], dims[1], dims[2], v);
+    });
+}
+
 pk_status pk_wb_backprojector(pk_ctx* c, const float* psf_fp, const int64_t dims[3], int32_t bp_type, double alpha,
                               double beta, int32_t n, int32_t res_flag, const double ires[3], double* psf_bp,
                               double* otf_bp, double* info) {

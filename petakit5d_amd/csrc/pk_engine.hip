@@ -296,6 +296,57 @@ static std::vector<AxisLaunch> x_candidates(int Lx, bool packed) {
     return out;
 }
 
+// Plan of values-per-thread E for a length E does not cover (a prime of L
+// missing from E): a radix dividing E where one divides the remaining length,
+// else the largest of kMixedRadices dividing it (that stage runs from LDS).
+// nst = 0 when no such plan exists; *mm receives the number of LDS-fed stages.
+static FftPlan plan_mixed(int L, int E, int* mm) {
+    FftPlan c{};
+    c.L = L;
+    c.E = E;
+    c.P = L / E;
+    *mm = 0;
+    int rem = L, ns = 1;
+    while (rem > 1) {
+        int r = radix_for(rem, E);
+        if (!r) {
+            for (int q : kMixedRadices)
+                if (rem % q == 0) {
+                    r = q;
+                    break;
+                }
+            ++*mm;
+        }
+        if (!r || c.nst >= kMaxStages) {
+            c.nst = 0;
+            return c;
+        }
+        c.R[c.nst] = r;
+        c.Ns[c.nst] = ns;
+        c.mag[c.nst] = ns > 1 ? (uint32_t)((1ull << 32) / (uint64_t)ns + 1ull) : 0u;
+        c.tws[c.nst] = L / (ns * r);
+        c.nst++;
+        ns *= r;
+        rem /= r;
+    }
+    return c;
+}
+// Column plan for (L, E): the all-register plan when E holds every prime of
+// L, else a mixed plan with at most two LDS-fed stages; E = 0 when neither.
+static FftPlan col_plan(int L, int E, int* mm) {
+    *mm = 0;
+    bool full = true;
+    for (int pr : {2, 3, 5})
+        if (L % pr == 0 && E % pr != 0) full = false;
+    if (full) {
+        const FftPlan fp = plan_1d(L, E, kMaxThreads * 4, E, 100, 1);
+        if (fp.E == E) return fp;
+    }
+    FftPlan fp = plan_mixed(L, E, mm);
+    if (L == 1 || fp.nst == 0 || *mm > 2 || !col_mixed_available(E, kMaxThreads)) fp.E = 0;
+    return fp;
+}
+
 // Strided-pass (y, z) launch shapes for length L: tiles of T adjacent
 // columns (T float2 per row segment), sorted by the built-in rule: T = 16
 // (128-byte segments) first, then fewest stages, then E near 16, then the
@@ -306,14 +357,12 @@ static std::vector<AxisLaunch> col_candidates(int L, int axis) {
     for (int E : kEs) {
         if (L % E) continue;
         if (tune().e[axis] > 0 && E != tune().e[axis] && L != 1) continue;
-        bool ok = e_supported(E);
-        for (int pr : {2, 3, 5})
-            if (L % pr == 0 && E % pr != 0) ok = false;
-        if (!ok) continue;
-        const FftPlan fp = plan_1d(L, E, kMaxThreads * 4, E, 100, 1);
+        if (!e_supported(E)) continue;
+        int mm = 0;
+        const FftPlan fp = L == 1 ? plan_1d(L, E, kMaxThreads * 4, E, 100, 1) : col_plan(L, E, &mm);
         if (fp.E != E) continue;
         for (int nt : {kMaxThreads, 512, 1024}) {
-            if (nt != kMaxThreads && !col_wide_available(E, nt)) continue;
+            if (nt != kMaxThreads && (mm > 0 || !col_wide_available(E, nt))) continue;
             if (fp.P > nt) continue;
             int T = 1;
             while (T * 2 <= 16 && T * 2 * fp.P <= nt && (size_t)T * 2 * L * sizeof(float2) <= kMaxLds) T *= 2;
@@ -330,8 +379,8 @@ static std::vector<AxisLaunch> col_candidates(int L, int axis) {
                 ax.nt = nt;
                 ax.block = dim3(Tc * fp.P);
                 ax.lds = std::max<size_t>((size_t)Tc * L * sizeof(float2), 16);
-                c.emplace_back((16 - T) * 1000 + fp.nst * 100 + std::abs(E - 16) * 2 + (nt > kMaxThreads ? 1 : 0) +
-                                   (Tc > T ? 50 : 0),
+                c.emplace_back((16 - T) * 1000 + (fp.nst + mm) * 100 + std::abs(E - 16) * 2 +
+                                   (nt > kMaxThreads ? 1 : 0) + (Tc > T ? 50 : 0),
                                ax);
             }
         }
@@ -352,7 +401,9 @@ static std::vector<AxisLaunch> col_candidates(int L, int axis) {
 static bool col_axis(int L, int E, int nt, int T, AxisLaunch& ax) {
     if (L % E || !e_supported(E) || T < 1) return false;
     if (nt != kMaxThreads && !col_wide_available(E, nt)) return false;
-    const FftPlan fp = plan_1d(L, E, kMaxThreads * 4, E, 100, 1);
+    int mm = 0;
+    const FftPlan fp = L == 1 ? plan_1d(L, E, kMaxThreads * 4, E, 100, 1) : col_plan(L, E, &mm);
+    if (mm > 0 && nt != kMaxThreads) return false;  // LDS-fed stages: 256-thread kernels only
     if (fp.E != E || fp.P * T > nt || (size_t)T * L * sizeof(float2) > kMaxLds) return false;
     ax = AxisLaunch{};
     ax.fp = fp;
@@ -399,6 +450,41 @@ struct Wisdom {
         std::fclose(fp);
     }
 };
+
+// Launch shapes of one wisdom line {xE, yE, yNT, yT, zE, zNT, zT, conv_axis}
+// applied to a plan; false (plan unchanged) when a shape is infeasible.
+bool pk_ctx_impl::apply_shapes(VolPlan& vp, const std::array<int, 8>& v) {
+    const int Lx = (int)(vp.packed ? vp.n1 / 2 : vp.n1);
+    const bool ctx = vp.packed && !tune().generic_x && v[0] > 0 && Lx % v[0] == 0 && xrow_available(Lx, v[0]);
+    if (!ctx && (v[0] < 1 || Lx % v[0] != 0)) return false;
+    const FftPlan fx = ctx ? ct_row_plan(Lx, v[0]) : plan_1d(Lx, v[0], kMaxThreads, v[0], 10, 4);
+    AxisLaunch ay, az;
+    if (fx.E != v[0] || !col_axis((int)vp.n2, v[1], v[2], v[3], ay) || !col_axis((int)vp.n3, v[4], v[5], v[6], az) ||
+        !(v[7] == 2 || v[7] == 3))
+        return false;
+    const float2 *tx = vp.x.fp.tw, *ty = vp.y.fp.tw, *tz = vp.z.fp.tw;  // the plan's tables, per length
+    vp.x = x_axis(fx, ctx);
+    vp.x.fp.tw = tx;
+    vp.y = ay;
+    vp.y.fp.tw = ty;
+    vp.z = az;
+    vp.z.fp.tw = tz;
+    vp.conv_axis = v[7];
+    vp.tuned = true;
+    return true;
+}
+
+void pk_ctx_impl::set_shapes(int64_t n1, int64_t n2, int64_t n3, const std::array<int, 8>& v) {
+    const std::vector<int64_t> key{n1, n2, n3};
+    VolPlan vp = plan(n1, n2, n3);
+    if (!apply_shapes(vp, v)) fail(PK_ERR_INVALID_ARG, "launch shapes not feasible for this size");
+    plans[key] = vp;
+    clear_graphs();  // captured loops hold the old launch shapes
+    Wisdom& w = Wisdom::get();
+    std::lock_guard<std::mutex> lk(w.mu);
+    w.load();
+    w.table[key] = v;
+}
 
 const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
     std::vector<int64_t> key{n1, n2, n3};
@@ -467,23 +553,7 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
         std::lock_guard<std::mutex> lk(w.mu);
         w.load();
         auto t = w.table.find(key);
-        if (t != w.table.end()) {
-            const auto& v = t->second;
-            const bool ctx = vp.packed && !tune().generic_x && v[0] > 0 && Lx % v[0] == 0 && xrow_available(Lx, v[0]);
-            const FftPlan fx = ctx ? ct_row_plan(Lx, v[0]) : plan_1d(Lx, v[0], kMaxThreads, v[0], 10, 4);
-            AxisLaunch ay, az;
-            if (fx.E == v[0] && col_axis((int)n2, v[1], v[2], v[3], ay) && col_axis((int)n3, v[4], v[5], v[6], az) &&
-                (v[7] == 2 || v[7] == 3)) {
-                vp.x = x_axis(fx, ctx);
-                vp.x.fp.tw = table(Lx);
-                vp.y = ay;
-                vp.y.fp.tw = table((int)n2);
-                vp.z = az;
-                vp.z.fp.tw = table((int)n3);
-                vp.conv_axis = v[7];
-                vp.tuned = true;
-            }
-        }
+        if (t != w.table.end()) apply_shapes(vp, t->second);
     }
     return plans.emplace(key, vp).first->second;
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -196,6 +197,8 @@ struct pk_ctx_impl {
                  int64_t nz = 0);
     int64_t slab_planes(const VolPlan& vp) const;
     std::string autotune(int64_t n1, int64_t n2, int64_t n3);
+    void set_shapes(int64_t n1, int64_t n2, int64_t n3, const std::array<int, 8>& v);
+    bool apply_shapes(VolPlan& vp, const std::array<int, 8>& v);
     // Iteration loops replayed as HIP graphs: a key's first call runs eagerly,
     // the second captures and instantiates, later calls launch the graph.
     struct GraphEntry {

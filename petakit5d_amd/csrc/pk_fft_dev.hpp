@@ -129,20 +129,89 @@ __device__ __forceinline__ void stage_r(float2 (&v)[E], const FftPlan& pl, int s
     }
 }
 
-#define PK_RADIX_CASE(RR)                                                    \
-    case RR:                                                                 \
-        if constexpr (E % RR == 0) stage_r<E, RR, INV, ROWS>(v, pl, s, p, lds, lbase, lstride, last, act); \
+// A stage whose radix R does not divide E (plans with such stages keep few
+// values per thread for lengths with the primes 2, 3 and 5): the L / R
+// butterflies are dealt round-robin over the P threads, at most ceil(E / R)
+// rounds each (the last predicated), and read their inputs from LDS, where
+// the previous stage left its outputs (stage 0 first stores the home
+// registers).  Outputs go to the usual Stockham positions in LDS.  Each
+// butterfly does the same operations as in stage_r.
+template <int E, int R, bool INV, bool ROWS>
+__device__ __forceinline__ void stage_mm(float2 (&v)[E], const FftPlan& pl, int s, int p, float2* lds, int lbase,
+                                         int lstride, bool act) {
+    constexpr int NB = (E + R - 1) / R;
+    const int P = pl.P;
+    const int NBF = pl.L / R;
+    const int Ns = pl.Ns[s];
+    const int tws = pl.tws[s];
+    const uint32_t mag = pl.mag[s];
+    if (s == 0) {
+        if (act) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) lds[lds_slot<ROWS>(lbase, p + e * P, lstride)] = v[e];
+        }
+        __syncthreads();
+    }
+    float2 u[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = p + b * P;
+        const int jj = j < NBF ? j : 0;  // idle round: an in-bounds read
+        static_for<0, R>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            u[b][m] = act ? lds[lds_slot<ROWS>(lbase, jj + m * NBF, lstride)] : make_float2(0.f, 0.f);
+        });
+    }
+    __syncthreads();  // every input read before any output lands
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = p + b * P;
+        if (act && j < NBF) {
+            int g, k;
+            if (s == 0) {
+                g = j;
+                k = 0;
+            } else {
+                g = (int)__umulhi((uint32_t)j, mag);
+                k = j - g * Ns;
+            }
+            if (s != 0) stage_twiddle<R, INV>(u[b], pl.tw, k, tws);
+            dft<R, INV>(u[b]);
+            const int ob = g * Ns * R + k;
+            static_for<0, R>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                lds[lds_slot<ROWS>(lbase, ob + q * Ns, lstride)] = u[b][q];
+            });
+        }
+    }
+}
+
+// Radices a plan may run from LDS (stage_mm), largest first.
+constexpr int kMixedRadices[] = {5, 4, 3, 2};
+// A plan with at least one stage whose radix does not divide E.
+__host__ __device__ inline bool plan_mixed_stages(const FftPlan& pl) {
+    for (int s = 0; s < pl.nst; ++s)
+        if (pl.E % pl.R[s] != 0) return true;
+    return false;
+}
+
+#define PK_RADIX_CASE(RR)                                                                                    \
+    case RR:                                                                                                 \
+        if constexpr (E % RR == 0) stage_r<E, RR, INV, ROWS>(v, pl, s, p, lds, lbase, lstride, last, act);   \
+        else if constexpr (MIXED && RR <= 5) stage_mm<E, RR, INV, ROWS>(v, pl, s, p, lds, lbase, lstride, act); \
         break;
 
 // Full transform of the E home values of every thread (all threads of the
 // block must call it: it contains barriers).  act = false marks a thread that
 // owns no tile slot (batched launches with a block larger than the chunk's
-// tile): it joins the barriers but never touches LDS.
-template <int E, bool INV, bool ROWS>
+// tile): it joins the barriers but never touches LDS.  MIXED: the plan may
+// hold stages whose radix (2..5) does not divide E.
+template <int E, bool INV, bool ROWS, bool MIXED = false>
 __device__ __forceinline__ void fft_run(float2 (&v)[E], const FftPlan& pl, int p, float2* lds, int lbase,
                                         int lstride, bool act = true) {
     for (int s = 0; s < pl.nst; ++s) {
         const bool last = (s == pl.nst - 1);
+        const bool mm = MIXED && (E % pl.R[s] != 0);
         switch (pl.R[s]) {
             PK_RADIX_CASE(2)
             PK_RADIX_CASE(3)
@@ -165,12 +234,15 @@ __device__ __forceinline__ void fft_run(float2 (&v)[E], const FftPlan& pl, int p
             default:
                 break;
         }
-        if (!last) {
+        if (!last || mm) {  // outputs in LDS: back to home registers unless the next stage reads LDS itself
             __syncthreads();
+            const bool next_mm = MIXED && !last && (E % pl.R[s + 1] != 0);
+            if (!next_mm) {
 #pragma unroll
-            for (int e = 0; e < E; ++e)
-                if (act) v[e] = lds[lds_slot<ROWS>(lbase, p + e * pl.P, lstride)];
-            __syncthreads();
+                for (int e = 0; e < E; ++e)
+                    if (act) v[e] = lds[lds_slot<ROWS>(lbase, p + e * pl.P, lstride)];
+                __syncthreads();
+            }
         }
     }
 }
@@ -928,7 +1000,9 @@ struct ColChunk {
     int64_t gx;  // blocks along x of the chunk's own grid
 };
 
-template <int E, int MODE>
+// MIXED: the plan may hold stages whose radix does not divide E (compiled as
+// kernels of their own, so the all-register plans keep their register budget).
+template <int E, int MODE, bool MIXED = false>
 __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl, float2* lds, int64_t bx,
                                              int64_t by) {
     const int t = threadIdx.x;
@@ -948,22 +1022,22 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? ptr0[e * step] : make_float2(0.f, 0.f);
     if constexpr (MODE == CM_INV) {
-        fft_run<E, true, false>(v, pl, p, lds, c, T, act);
+        fft_run<E, true, false, MIXED>(v, pl, p, lds, c, T, act);
     } else if constexpr (MODE == CM_FWD) {
-        fft_run<E, false, false>(v, pl, p, lds, c, T, act);
+        fft_run<E, false, false, MIXED>(v, pl, p, lds, c, T, act);
     } else {
         // OTF values prefetched before the forward transform
         const float2* __restrict__ h0 = a.otf + (valid ? off0 : 0);
         float2 h[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) h[e] = valid ? h0[e * step] : make_float2(0.f, 0.f);
-        fft_run<E, false, false>(v, pl, p, lds, c, T, act);
+        fft_run<E, false, false, MIXED>(v, pl, p, lds, c, T, act);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const float2 hs = make_float2(h[e].x * a.scale, h[e].y * a.scale);
             v[e] = (MODE == CM_CONVJ) ? cmulc(v[e], hs) : cmul(v[e], hs);
         }
-        fft_run<E, true, false>(v, pl, p, lds, c, T, act);
+        fft_run<E, true, false, MIXED>(v, pl, p, lds, c, T, act);
     }
     if (valid) {
 #pragma unroll
@@ -973,7 +1047,7 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
 
 // NT: the block size the kernel is compiled for (256; 512 / 1024 for long
 // columns, whose 8-16-column tiles need more threads and up to 160 KB of LDS).
-template <int E, int MODE, int NT = kMaxThreads>
+template <int E, int MODE, int NT = kMaxThreads, bool MIXED = false>
 __global__ void __launch_bounds__(NT) k_colpass(ColArgs a, FftPlan pl) {
     extern __shared__ float2 lds[];
     int64_t bx = blockIdx.x, by = blockIdx.y;
@@ -984,10 +1058,10 @@ __global__ void __launch_bounds__(NT) k_colpass(ColArgs a, FftPlan pl) {
         bx = t % gx;
         by = t / gx;
     }
-    colpass_body<E, MODE>(a, pl, lds, bx, by);
+    colpass_body<E, MODE, MIXED>(a, pl, lds, bx, by);
 }
 
-template <int E, int MODE, int NT = kMaxThreads>
+template <int E, int MODE, int NT = kMaxThreads, bool MIXED = false>
 __global__ void __launch_bounds__(NT) k_colpass_batch(const ColChunk* __restrict__ ch,
                                                       const int64_t* __restrict__ starts, int n) {
     extern __shared__ float2 lds[];
@@ -995,7 +1069,7 @@ __global__ void __launch_bounds__(NT) k_colpass_batch(const ColChunk* __restrict
     const ColArgs a = ch[c].a;
     const FftPlan& pl = ch[c].pl;  // read in place: a local copy indexed by stage would live in scratch
     const int64_t lb = (int64_t)blockIdx.x - starts[c];
-    colpass_body<E, MODE>(a, pl, lds, lb % ch[c].gx, lb / ch[c].gx);
+    colpass_body<E, MODE, MIXED>(a, pl, lds, lb % ch[c].gx, lb / ch[c].gx);
 }
 
 }  // namespace pk

@@ -46,6 +46,10 @@ hipError_t launch_xpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hi
 
 hipError_t launch_colpass(const FftPlan& pl, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s,
                           const ColArgs& a) {
+    if (plan_mixed_stages(pl)) {
+        if (nt != kMaxThreads || !launch_col_mixed(pl, grid, block, lds, s, a)) return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     if (nt != kMaxThreads) {
         if (!launch_col_wide(pl, nt, grid, block, lds, s, a)) return hipErrorInvalidValue;
         return hipGetLastError();
@@ -69,8 +73,13 @@ hipError_t launch_xpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_colpass_batch(int E, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode,
+hipError_t launch_colpass_batch(int E, int nt, bool mixed, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode,
                                 const ColChunk* ch, const int64_t* starts, int n) {
+    if (mixed) {
+        if (nt != kMaxThreads || !launch_colb_mixed(E, grid, block, lds, s, mode, ch, starts, n))
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     if (nt != kMaxThreads) {
         if (!launch_colb_wide(E, nt, grid, block, lds, s, mode, ch, starts, n)) return hipErrorInvalidValue;
         return hipGetLastError();

@@ -35,17 +35,17 @@ template <class K>
 inline void allow_large_lds(K kernel) {
     raise_lds_once(reinterpret_cast<const void*>(kernel));
 }
-template <int E, int NT = kMaxThreads>
+template <int E, int NT = kMaxThreads, bool MIXED = false>
 inline void launch_modes_col(dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a, const FftPlan& pl) {
     auto go = [&](auto kernel) {
         if (lds > 65536) allow_large_lds(kernel);
         hipLaunchKernelGGL(kernel, grid, block, lds, s, a, pl);
     };
     switch (a.mode) {
-        case CM_FWD: go(k_colpass<E, CM_FWD, NT>); break;
-        case CM_INV: go(k_colpass<E, CM_INV, NT>); break;
-        case CM_CONV: go(k_colpass<E, CM_CONV, NT>); break;
-        case CM_CONVJ: go(k_colpass<E, CM_CONVJ, NT>); break;
+        case CM_FWD: go(k_colpass<E, CM_FWD, NT, MIXED>); break;
+        case CM_INV: go(k_colpass<E, CM_INV, NT, MIXED>); break;
+        case CM_CONV: go(k_colpass<E, CM_CONV, NT, MIXED>); break;
+        case CM_CONVJ: go(k_colpass<E, CM_CONVJ, NT, MIXED>); break;
         default: break;
     }
 }
@@ -53,7 +53,7 @@ inline void launch_modes_col(dim3 grid, dim3 block, size_t lds, hipStream_t s, c
 hipError_t launch_xpass(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const XArgs& a);
 hipError_t launch_xpass_batch(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const XChunk* ch,
                               const int64_t* starts, int n);
-hipError_t launch_colpass_batch(int E, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode,
+hipError_t launch_colpass_batch(int E, int nt, bool mixed, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode,
                                 const ColChunk* ch, const int64_t* starts, int n);
 hipError_t launch_colpass(const FftPlan& pl, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s,
                           const ColArgs& a);
@@ -120,7 +120,7 @@ inline void launch_modes_xb(dim3 grid, dim3 block, size_t lds, hipStream_t s, in
         default: break;
     }
 }
-template <int E, int NT = kMaxThreads>
+template <int E, int NT = kMaxThreads, bool MIXED = false>
 inline void launch_modes_colb(dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch,
                               const int64_t* starts, int n) {
     auto go = [&](auto kernel) {
@@ -128,13 +128,19 @@ inline void launch_modes_colb(dim3 grid, dim3 block, size_t lds, hipStream_t s, 
         hipLaunchKernelGGL(kernel, grid, block, lds, s, ch, starts, n);
     };
     switch (mode) {
-        case CM_FWD: go(k_colpass_batch<E, CM_FWD, NT>); break;
-        case CM_INV: go(k_colpass_batch<E, CM_INV, NT>); break;
-        case CM_CONV: go(k_colpass_batch<E, CM_CONV, NT>); break;
-        case CM_CONVJ: go(k_colpass_batch<E, CM_CONVJ, NT>); break;
+        case CM_FWD: go(k_colpass_batch<E, CM_FWD, NT, MIXED>); break;
+        case CM_INV: go(k_colpass_batch<E, CM_INV, NT, MIXED>); break;
+        case CM_CONV: go(k_colpass_batch<E, CM_CONV, NT, MIXED>); break;
+        case CM_CONVJ: go(k_colpass_batch<E, CM_CONVJ, NT, MIXED>); break;
         default: break;
     }
 }
+// Column kernels for plans with LDS-fed stages (pk_kern_col_mixed.hip):
+// 256-thread blocks, values-per-thread counts listed by col_mixed_available.
+bool col_mixed_available(int E, int nt);
+bool launch_col_mixed(const FftPlan& pl, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);
+bool launch_colb_mixed(int E, dim3 grid, dim3 block, size_t lds, hipStream_t s, int mode, const ColChunk* ch,
+                       const int64_t* starts, int n);
 // Column kernels compiled for 512- / 1024-thread blocks (pk_kern_col_wide.hip).
 bool col_wide_available(int E, int nt);
 bool launch_col_wide(const FftPlan& pl, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s, const ColArgs& a);

@@ -257,6 +257,53 @@ def test_row_lengths_vs_oracle(oracle, n1):
     assert rel_l2(Jr, Rr) <= TOL
 
 
+# (shape, launch shapes {xE, yE, yNT, yT, zE, zNT, zT, conv axis}): strided-axis
+# plans whose values per thread miss a prime of the length, so one or two stages
+# (radix 2-5) run from LDS with round-robin butterflies
+MIXED_COL = [
+    ((64, 60, 45), (8, 6, 256, 16, 9, 256, 16, 3)),    # y: 6, 2, then 5 from LDS; z: 9, then 5 from LDS
+    ((64, 60, 45), (8, 4, 256, 16, 5, 256, 16, 2)),    # two LDS stages per axis: y 4 | 5, 3; z 5 | 3, 3
+    ((64, 60, 45), (8, 12, 256, 8, 15, 256, 16, 3)),   # y: 12, then 5 from LDS (P = 5); z all-register
+    ((48, 360, 40), (6, 24, 256, 16, 8, 256, 16, 3)),  # y: 24, 3, then 5 from LDS; z: 8, then 5 from LDS
+    ((48, 360, 40), (6, 12, 256, 8, 5, 256, 16, 2)),   # y: 12, 6, then 5 from LDS; z: 5, then 4, 2 from LDS
+]
+
+
+@pytest.mark.parametrize("shape,shapes", MIXED_COL, ids=lambda v: "x".join(map(str, v)))
+def test_mixed_column_plans_vs_oracle(oracle, shape, shapes):
+    """Strided-axis transforms with stages fed from LDS (their radix does not
+    divide the values per thread), set through pk_tune_set: OMW-2 and Biggs-4
+    against the oracle, and the batched entry point bitwise equal to the
+    single call under the same shapes."""
+    import petakit5d_amd as pk
+    print(f"[mixed cols] {shape} {shapes}", flush=True)
+    ctx = pk.Context(0)
+    ctx.tune_set(shape, shapes)
+    I, psf = _case(shape, (9, 9, 11), seed=7)
+    pb = _flip(psf)
+    J, _ = pk.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False, ctx=ctx)
+    R, _ = oracle.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False)
+    assert rel_l2(J, R) <= TOL
+    B = pk.decon_batch([I], psf, pb, 2, method="omw", Background=100, save16bit=False, ctx=ctx)
+    np.testing.assert_array_equal(B[0], J)
+    Jr, _, _ = pk.decon_lucy_function(I, psf, 4, Background=100, save16bit=False, ctx=ctx)
+    Rr, _, _ = oracle.decon_lucy_function(I, psf, 4, Background=100, save16bit=False)
+    assert rel_l2(Jr, Rr) <= TOL
+    ctx.close()
+
+
+def test_tune_set_rejects_infeasible_shapes():
+    import petakit5d_amd as pk
+    ctx = pk.Context(0)
+    with pytest.raises(pk.PkDeconError):
+        ctx.tune_set((64, 60, 45), (8, 7, 256, 16, 9, 256, 16, 3))  # 7 does not divide 60
+    with pytest.raises(pk.PkDeconError):
+        ctx.tune_set((64, 60, 45), (8, 6, 256, 32, 9, 256, 16, 3))  # 10 threads x 32 columns > 256
+    with pytest.raises(pk.PkDeconError):
+        ctx.tune_set((48, 360, 40), (6, 24, 512, 32, 8, 256, 16, 3))  # LDS-fed stages: 256-thread kernels only
+    ctx.close()
+
+
 EPILOGUE_PASS = 8  # PK_FLAG_EPILOGUE_PASS
 
 
