@@ -328,11 +328,33 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
         }
     };
 
-    Pass<XChunk> prep, biggs_y, ratio, update;
-    Pass<ColChunk> yf, zc1, yi, zc2;
+    // Chunk sets, one per stream: chunks are independent, so each set runs
+    // its whole pass sequence on its own stream and the launch tails of one
+    // set overlap the work of the others.  Sets are balanced by voxels
+    // (largest chunk first onto the lightest set); one stream when profiling.
+    constexpr int kSets = kSideStreams + 1;
+    const int nset = prof_on ? 1 : (int)std::min<size_t>(kSets, act.size());
+    std::vector<int> set_of(act.size(), 0);
+    {
+        std::vector<size_t> order(act.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return act[a]->vp->N > act[b]->vp->N; });
+        int64_t load[kSets] = {};
+        for (size_t i : order) {
+            int best = 0;
+            for (int k = 1; k < nset; ++k)
+                if (load[k] < load[best]) best = k;
+            set_of[i] = best;
+            load[best] += act[i]->vp->N;
+        }
+    }
+    Pass<XChunk> prep[kSets], biggs_y[kSets], ratio[kSets], update[kSets];
+    Pass<ColChunk> yf[kSets], zc1[kSets], yi[kSets], zc2[kSets];
     std::vector<LamChunk> lam;
-    for (Chunk* cp : act) {
-        Chunk& c = *cp;
+    for (size_t ci = 0; ci < act.size(); ++ci) {
+        Chunk& c = *act[ci];
+        const int si = set_of[ci];
         if (c.k == 0) {
             XArgs a = base_x(c, c.u16 ? XM_PREP_U16 : XM_PREP_F32);
             a.raw16 = c.raw16;
@@ -347,7 +369,7 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
             a.bg = c.u16 ? (float)std::min(65535.0, std::max(0.0, std::round(c.o.background))) : (float)c.o.background;
             a.I = c.I;
             a.J = c.J;
-            add_x(prep, c, a);
+            add_x(prep[si], c, a);
             if (c.method == 0) {
                 hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)std::min<int64_t>((c.vp->N + 255) / 256, 65536)), dim3(256),
                                    0, stream, c.J3, c.vp->N, 0.f);
@@ -365,25 +387,25 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
             a.J3 = c.par ? c.J : c.J3;
             a.Y = c.Y;
             a.lambda = c.lam;
-            add_x(biggs_y, c, a);
+            add_x(biggs_y[si], c, a);
         }
         const int cax = c.vp->conv_axis, oax = cax == 3 ? 2 : 3;  // pass order of the chunk's plan
-        add_col(yf, c, oax, CM_FWD, nullptr);
-        add_col(zc1, c, cax, CM_CONV, c.Hf);
-        add_col(yi, c, oax, CM_INV, nullptr);
+        add_col(yf[si], c, oax, CM_FWD, nullptr);
+        add_col(zc1[si], c, cax, CM_CONV, c.Hf);
+        add_col(yi[si], c, oax, CM_INV, nullptr);
         {
             XArgs a = base_x(c, XM_RATIO);
             a.I = c.I;
-            add_x(ratio, c, a);
+            add_x(ratio[si], c, a);
         }
-        add_col(zc2, c, cax, c.method == 1 ? CM_CONV : CM_CONVJ, c.method == 1 ? c.Hb : c.Hf);
+        add_col(zc2[si], c, cax, c.method == 1 ? CM_CONV : CM_CONVJ, c.method == 1 ? c.Hb : c.Hf);
         const bool last = kn == c.numit;  // the last update writes the output through the epilogue
         if (c.method == 1) {
             XArgs a = base_x(c, last ? XM_OMW_FINAL : XM_OMW_UPDATE);
             a.J = c.J;
             a.r2c_out = !last;
             if (last) a.ep = chunk_epi(c);
-            add_x(update, c, a);
+            add_x(update[si], c, a);
         } else {
             XArgs a = base_x(c, last ? XM_BIGGS_FINAL : XM_BIGGS_UPDATE);
             a.Y = c.Y;
@@ -393,17 +415,19 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
             a.write_j4 = !last;
             a.partials = c.part;
             if (last) a.ep = chunk_epi(c);
-            add_x(update, c, a);
+            add_x(update[si], c, a);
         }
     }
-    seal(prep);
-    seal(biggs_y);
-    seal(ratio);
-    seal(update);
-    seal(yf);
-    seal(zc1);
-    seal(yi);
-    seal(zc2);
+    for (int k = 0; k < nset; ++k) {
+        seal(prep[k]);
+        seal(biggs_y[k]);
+        seal(ratio[k]);
+        seal(update[k]);
+        seal(yf[k]);
+        seal(zc1[k]);
+        seal(yi[k]);
+        seal(zc2[k]);
+    }
     size_t lam_off = 0;
     if (!lam.empty()) {
         lam_off = pos(sizeof(LamChunk) * lam.size());
@@ -430,47 +454,64 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
     PK_HIP(hipEventRecord(ws.tab_ev[slot], stream));
     char* tab = ws.tab_dev[slot].as<char>();
 
-    auto run_x = [&](const Pass<XChunk>& ps, const char* name) {
+    auto run_x = [&](const Pass<XChunk>& ps, const char* name, hipStream_t st) {
         for (auto& kv : ps) {
             const auto& g = kv.second;
             hipEvent_t e0 = prof_mark();
             const XChunk* chd = reinterpret_cast<const XChunk*>(tab + g.dev_off);
             const int64_t* std_ = reinterpret_cast<const int64_t*>(tab + g.start_off);
-            if (g.L && launch_xrow_batch(g.L, g.E, dim3((unsigned)g.starts.back()), g.lds, stream, g.mode, chd, std_,
+            if (g.L && launch_xrow_batch(g.L, g.E, dim3((unsigned)g.starts.back()), g.lds, st, g.mode, chd, std_,
                                          (int)g.chunks.size()))
                 PK_HIP(hipGetLastError());
             else
-                PK_HIP(launch_xpass_batch(g.E, dim3((unsigned)g.starts.back()), dim3(g.block), g.lds, stream, g.mode,
+                PK_HIP(launch_xpass_batch(g.E, dim3((unsigned)g.starts.back()), dim3(g.block), g.lds, st, g.mode,
                                           chd, std_, (int)g.chunks.size()));
             prof_done(name, 0, e0);
         }
     };
-    auto run_c = [&](const Pass<ColChunk>& ps, const char* name) {
+    auto run_c = [&](const Pass<ColChunk>& ps, const char* name, hipStream_t st) {
         for (auto& kv : ps) {
             const auto& g = kv.second;
             hipEvent_t e0 = prof_mark();
             PK_HIP(launch_colpass_batch(g.E, g.nt, g.L != 0, dim3((unsigned)g.starts.back()), dim3(g.block), g.lds,
-                                        stream, g.mode,
-                                        reinterpret_cast<const ColChunk*>(tab + g.dev_off),
+                                        st, g.mode, reinterpret_cast<const ColChunk*>(tab + g.dev_off),
                                         reinterpret_cast<const int64_t*>(tab + g.start_off), (int)g.chunks.size()));
             prof_done(name, 0, e0);
         }
     };
-    run_x(prep, "b_x_prep");
     if (!lam.empty()) {
         hipLaunchKernelGGL(k_lambda_batch, dim3((unsigned)lam.size()), dim3(256), 0, stream,
                            reinterpret_cast<const LamChunk*>(tab + lam_off), kEpsF);
         PK_HIP(hipGetLastError());
     }
-    run_x(biggs_y, "b_x_biggs_y");
-    run_c(yf, "b_y_fwd");
-    run_c(zc1, "b_z_conv");
-    run_c(yi, "b_y_inv");
-    run_x(ratio, "b_x_ratio");
-    run_c(yf, "b_y_fwd");
-    run_c(zc2, "b_z_conv");
-    run_c(yi, "b_y_inv");
-    run_x(update, "b_x_update");
+    // fork: the side streams start after the table upload and the step lengths
+    if (nset > 1) {
+        if (!ev_fork) PK_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        PK_HIP(hipEventRecord(ev_fork, stream));
+        for (int k = 1; k < nset; ++k) {
+            if (!side[k - 1]) PK_HIP(hipStreamCreateWithFlags(&side[k - 1], hipStreamNonBlocking));
+            PK_HIP(hipStreamWaitEvent(side[k - 1], ev_fork, 0));
+        }
+    }
+    for (int k = 0; k < nset; ++k) {
+        hipStream_t st = k == 0 ? stream : side[k - 1];
+        run_x(prep[k], "b_x_prep", st);
+        run_x(biggs_y[k], "b_x_biggs_y", st);
+        run_c(yf[k], "b_y_fwd", st);
+        run_c(zc1[k], "b_z_conv", st);
+        run_c(yi[k], "b_y_inv", st);
+        run_x(ratio[k], "b_x_ratio", st);
+        run_c(yf[k], "b_y_fwd", st);
+        run_c(zc2[k], "b_z_conv", st);
+        run_c(yi[k], "b_y_inv", st);
+        run_x(update[k], "b_x_update", st);
+    }
+    // join: later work on the engine's stream sees every set's results
+    for (int k = 1; k < nset; ++k) {
+        if (!ev_join[k - 1]) PK_HIP(hipEventCreateWithFlags(&ev_join[k - 1], hipEventDisableTiming));
+        PK_HIP(hipEventRecord(ev_join[k - 1], side[k - 1]));
+        PK_HIP(hipStreamWaitEvent(stream, ev_join[k - 1], 0));
+    }
     for (Chunk* cp : act) {
         Chunk& c = *cp;
         if (c.k < c.numit) {

@@ -125,6 +125,14 @@ void pk_ctx_destroy(pk_ctx* c) {
             c->reset();
         } catch (...) {
         }
+        for (int i = 0; i < pk_ctx::kSideStreams; ++i) {
+            if (c->side[i]) {
+                (void)hipStreamSynchronize(c->side[i]);
+                (void)hipStreamDestroy(c->side[i]);
+            }
+            if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
+        }
+        if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
         if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     }
     delete c;

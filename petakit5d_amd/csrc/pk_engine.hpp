@@ -168,6 +168,13 @@ struct pk_ctx_impl {
     size_t otf_max_entries = 8;
     Workspace ws;
     bool prof_on = false;
+    // batched steps: chunk sets run their passes on the engine's stream and up
+    // to kSideStreams more (default priority), forked after the step's table
+    // upload and joined at its end
+    static constexpr int kSideStreams = 3;
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[kSideStreams] = {};
     std::vector<ProfPending> prof_pending;
     std::vector<hipEvent_t> ev_pool;
     std::map<std::string, ProfAgg> prof;
