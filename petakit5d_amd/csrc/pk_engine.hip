@@ -59,11 +59,13 @@ bool smooth235(int64_t n) {
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release() {
     if (p) {
-        (void)hipFree(p);
+        if (astream) (void)hipFreeAsync(p, astream);
+        else (void)hipFree(p);
         if (owner) owner->live_bytes -= bytes;
     }
     p = nullptr;
     bytes = 0;
+    astream = nullptr;
 }
 
 void pk_ctx_impl::alloc(DevBuf& b, size_t bytes) {
@@ -81,6 +83,25 @@ void pk_ctx_impl::alloc(DevBuf& b, size_t bytes) {
     b.p = p;
     b.bytes = bytes;
     b.owner = this;
+    live_bytes += bytes;
+}
+
+void pk_ctx_impl::alloc_async(DevBuf& b, size_t bytes, hipStream_t s) {
+    b.release();
+    if (bytes == 0) return;
+    if (budget && live_bytes + bytes > budget)
+        fail(PK_ERR_BUDGET, "device allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
+                                std::to_string(live_bytes) + " of " + std::to_string(budget) + " bytes in use)");
+    void* p = nullptr;
+    if (hipMallocAsync(&p, bytes, s) != hipSuccess) {
+        (void)hipGetLastError();
+        alloc(b, bytes);
+        return;
+    }
+    b.p = p;
+    b.bytes = bytes;
+    b.owner = this;
+    b.astream = s;
     live_bytes += bytes;
 }
 

@@ -38,6 +38,7 @@ struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
     pk_ctx_impl* owner = nullptr;
+    hipStream_t astream = nullptr;  // stream-ordered allocation: freed on this stream (no device sync)
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
@@ -185,6 +186,10 @@ struct pk_ctx_impl {
 
     // device memory helpers
     void alloc(DevBuf& b, size_t bytes);
+    // stream-ordered allocation on stream s (hipMallocAsync; plain hipMalloc
+    // where the runtime has no pool): its release is ordered after the work
+    // queued on s instead of synchronising the device
+    void alloc_async(DevBuf& b, size_t bytes, hipStream_t s);
     void ensure(DevBuf& b, size_t bytes);
 
     const VolPlan& plan(int64_t n1, int64_t n2, int64_t n3);

@@ -56,6 +56,11 @@ struct Server {
     std::deque<std::shared_ptr<ServeReq>> queue;
     std::map<uint64_t, std::shared_ptr<ServeReq>> reqs;
     std::vector<std::shared_ptr<ServeReq>> active;  // scheduler thread only
+    // finished on the host side, output copy in flight: retired when the
+    // event recorded after their last step and output copy completes
+    // (scheduler thread only)
+    std::vector<std::pair<std::shared_ptr<ServeReq>, hipEvent_t>> draining;
+    std::vector<hipEvent_t> ev_free;
     bool stop = false;
     uint64_t next_id = 1;
     int max_active = 64;
@@ -75,13 +80,15 @@ static void scheduler(pk_ctx_impl* ctx) {
         std::vector<std::shared_ptr<ServeReq>> cand;
         {
             std::unique_lock<std::mutex> lk(S.mq);
-            S.cv.wait(lk, [&] { return S.stop || !S.queue.empty() || !S.active.empty(); });
-            if (S.stop && S.active.empty()) {
+            S.cv.wait(lk, [&] { return S.stop || !S.queue.empty() || !S.active.empty() || !S.draining.empty(); });
+            if (S.stop && S.active.empty() && S.draining.empty()) {
                 for (auto& r : S.queue) {
                     r->state = PK_REQ_CANCELLED;
                     r->st = PK_ERR_CANCELLED;
                 }
                 S.queue.clear();
+                for (hipEvent_t ev : S.ev_free) (void)hipEventDestroy(ev);
+                S.ev_free.clear();
                 S.cv_done.notify_all();
                 return;
             }
@@ -112,7 +119,7 @@ static void scheduler(pk_ctx_impl* ctx) {
                     const size_t need = ctx->chunk_prepare(r->c, r->in, r->out, r->pf.data(), r->fd, r->hf,
                                                            r->pb.empty() ? nullptr : r->pb.data(), r->bd, r->hb);
                     r->pinned = true;
-                    ctx->alloc(r->mem, need);
+                    ctx->alloc_async(r->mem, need, ctx->stream);
                     ctx->chunk_place(r->c, r->mem.as<char>());
                     r->t_adm = Clock::now();
                     S.active.push_back(r);
@@ -133,7 +140,7 @@ static void scheduler(pk_ctx_impl* ctx) {
                                                                        r->pb.empty() ? nullptr : r->pb.data(), r->bd,
                                                                        r->hb);
                                 r->pinned = true;
-                                ctx->alloc(r->mem, need);
+                                ctx->alloc_async(r->mem, need, ctx->stream);
                                 ctx->chunk_place(r->c, r->mem.as<char>());
                                 r->t_adm = Clock::now();
                                 S.active.push_back(r);
@@ -158,8 +165,11 @@ static void scheduler(pk_ctx_impl* ctx) {
                                ctx->otf.end());
                 if (ctx->otf.size() == before) reject_head = true;
             }
-            // ---- one iteration for every request in flight
+            // ---- one iteration for every request in flight.  Nothing waits for
+            // the device here: the step-table double buffer keeps the host at
+            // most two steps ahead, and finished requests retire by event.
             if (!S.active.empty()) {
+                std::vector<std::shared_ptr<ServeReq>> ending;
                 try {
                     std::vector<Chunk*> act;
                     for (auto& r : S.active) act.push_back(&r->c);
@@ -167,26 +177,71 @@ static void scheduler(pk_ctx_impl* ctx) {
                     for (auto& r : S.active)
                         if (r->c.done()) {
                             ctx->chunk_finish(r->c);
-                            finished.push_back(r);
+                            ending.push_back(r);
                         }
-                    PK_HIP(hipStreamSynchronize(ctx->stream));
-                    ctx->prof_flush();
+                    if (!ending.empty()) {
+                        hipEvent_t ev = nullptr;
+                        if (!S.ev_free.empty()) {
+                            ev = S.ev_free.back();
+                            S.ev_free.pop_back();
+                        } else {
+                            PK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                        }
+                        PK_HIP(hipEventRecord(ev, ctx->stream));
+                        for (auto& r : ending) S.draining.emplace_back(r, ev);
+                    }
+                    if (ctx->prof_on) {
+                        PK_HIP(hipStreamSynchronize(ctx->stream));
+                        ctx->prof_flush();
+                    }
                 } catch (const Error& e) {
-                    finished.clear();  // a device error fails every request in flight
+                    ending.clear();  // a device error fails every request in flight
                     for (auto& r : S.active) failed.emplace_back(r, e);
                 }
                 std::vector<std::shared_ptr<ServeReq>> keep;
                 for (auto& r : S.active) {
-                    bool gone = std::find(finished.begin(), finished.end(), r) != finished.end();
-                    for (auto& f : failed) gone |= f.first == r;
-                    if (gone) {
+                    bool failed_r = false;
+                    for (auto& f : failed) failed_r |= f.first == r;
+                    if (failed_r) {
                         r->mem.release();
                         unpin(ctx, *r);
-                    } else {
+                    } else if (std::find(ending.begin(), ending.end(), r) == ending.end()) {
                         keep.push_back(r);
                     }
                 }
                 S.active.swap(keep);
+            }
+            // ---- retire requests whose last step and output copy completed;
+            // with nothing else to run, wait for the oldest
+            if (!S.draining.empty()) {
+                const bool idle = S.active.empty() && admitted.empty();
+                std::vector<std::pair<std::shared_ptr<ServeReq>, hipEvent_t>> still;
+                std::vector<hipEvent_t> seen;
+                for (size_t i = 0; i < S.draining.size(); ++i) {
+                    auto& d = S.draining[i];
+                    hipError_t q = hipEventQuery(d.second);
+                    if (q == hipErrorNotReady && idle && i == 0) q = hipEventSynchronize(d.second);
+                    if (q == hipErrorNotReady) {
+                        still.push_back(d);
+                        continue;
+                    }
+                    if (q == hipSuccess) {
+                        finished.push_back(d.first);
+                    } else {
+                        (void)hipGetLastError();
+                        failed.emplace_back(d.first, Error{PK_ERR_HIP, std::string("device error: ") +
+                                                                           hipGetErrorString(q)});
+                    }
+                    d.first->mem.release();
+                    unpin(ctx, *d.first);
+                    if (std::find(seen.begin(), seen.end(), d.second) == seen.end()) seen.push_back(d.second);
+                }
+                S.draining.swap(still);
+                for (hipEvent_t ev : seen) {  // an event shared by requests still draining stays out of the pool
+                    bool used = false;
+                    for (auto& d : S.draining) used |= d.second == ev;
+                    if (!used) S.ev_free.push_back(ev);
+                }
             }
         }
         {
@@ -227,6 +282,14 @@ static void scheduler(pk_ctx_impl* ctx) {
 
 void serve_start(pk_ctx_impl* ctx, int max_active) {
     if (ctx->srv) fail(PK_ERR_BUSY, "server already running");
+    {  // request arenas come from the device's default pool: keep freed memory there for the next requests
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess && pool) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        (void)hipGetLastError();
+    }
     ctx->srv = std::make_unique<Server>();
     ctx->srv->max_active = max_active > 0 ? max_active : 64;
     // many request shapes and channels: keep their OTFs (the HBM budget still bounds them)

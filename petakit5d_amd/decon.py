@@ -123,7 +123,7 @@ def _psf_array(P):
     if a.size == 0:
         raise ValueError("empty PSF")
     d = _dims3(a.shape)
-    return np.asfortranarray(a.astype(np.float32).reshape(d, order="F")), d
+    return np.asfortranarray(a.astype(np.float32, copy=False).reshape(d, order="F")), d
 
 
 def _smooth235(n: int) -> bool:

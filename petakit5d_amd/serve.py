@@ -50,6 +50,19 @@ class PkServeStats(ctypes.Structure):
     ]
 
 
+def _pinned(shape_rev, dtype):
+    """A pinned host tensor of the reversed (C-order) shape, or None when no
+    GPU runtime is there to pin memory."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        tdt = {np.dtype(np.uint16): torch.uint16, np.dtype(np.float32): torch.float32}[np.dtype(dtype)]
+        return torch.empty(shape_rev, dtype=tdt, pin_memory=True)
+    except Exception:  # noqa: BLE001 -- pinning is an optimisation only
+        return None
+
+
 def _bind(L):
     vp = ctypes.c_void_p
     if getattr(L, "_pk_serve_bound", False):
@@ -78,6 +91,8 @@ class Request:
         i = PkReqInfo()
         self.server.ctx.check(self.server.L.pk_serve_wait(self.server.ctx.handle, self.id, float(wait_ms),
                                                           ctypes.byref(i)))
+        if i.state >= DONE:
+            self._keep = None  # the server is done with the input (pinned staging goes back to the cache)
         return {"state": i.state, "status": i.status, "iterations_done": i.iterations_done,
                 "latency_ms": i.latency_ms, "queue_ms": i.queue_ms}
 
@@ -113,20 +128,39 @@ class Server:
             import torch
             out = torch.empty((d[2], d[1], d[0]), dtype=torch.uint16 if o["save16bit"] else torch.float32,
                               device=obj.device).permute(2, 1, 0)
-            torch.cuda.synchronize()
+            # the input's producers and the output block's earlier users ran on
+            # torch's stream; the server's stream runs after them (a device-wide
+            # sync would also wait for the requests already in flight)
+            torch.cuda.current_stream(obj.device).synchronize()
             vout = PkVol(ctypes.c_void_p(out.data_ptr()), 1 if o["save16bit"] else 2,
                          _lib.PK_MEM_DEVICE, (ctypes.c_int64 * 3)(*d))
         else:
-            out = np.empty(d, dtype=np.uint16 if o["save16bit"] else np.float32, order="F")
-            vout = PkVol(ctypes.c_void_p(out.ctypes.data), 1 if o["save16bit"] else 2, _lib.PK_MEM_HOST,
-                         (ctypes.c_int64 * 3)(*d))
+            # host volumes are staged in pinned memory (torch's caching host
+            # allocator): the server's copies then run asynchronously, beside
+            # the steps, instead of blocking the scheduler thread
+            pin_in = _pinned((d[2], d[1], d[0]), obj.dtype)
+            if pin_in is not None:
+                import torch
+                pin_in.permute(2, 1, 0).copy_(torch.from_numpy(obj))  # same layout; torch's parallel copy
+                obj = pin_in
+                vin = PkVol(ctypes.c_void_p(pin_in.data_ptr()), vin.dtype, _lib.PK_MEM_HOST, (ctypes.c_int64 * 3)(*d))
+            odt = np.uint16 if o["save16bit"] else np.float32
+            pin_out = _pinned((d[2], d[1], d[0]), odt)
+            if pin_out is not None:
+                out = pin_out.numpy().T  # a column-major (d0, d1, d2) view of the pinned buffer
+                optr = pin_out.data_ptr()
+                obj = (obj, pin_out)
+            else:
+                out = np.empty(d, dtype=odt, order="F")
+                optr = out.ctypes.data
+            vout = PkVol(ctypes.c_void_p(optr), 1 if o["save16bit"] else 2, _lib.PK_MEM_HOST, (ctypes.c_int64 * 3)(*d))
         pf, fd = _psf_array(PSF_f)
         pb, bd = _psf_array(PSF_b if m == 1 else PSF_f)
         q = PkRequest(m, int(NUMIT), vin, vout, ctypes.c_void_p(pf.ctypes.data), (ctypes.c_int64 * 3)(*fd),
                       ctypes.c_void_p(pb.ctypes.data), (ctypes.c_int64 * 3)(*bd), opts)
         rid = ctypes.c_uint64()
         self.ctx.check(self.L.pk_serve_submit(self.ctx.handle, ctypes.byref(q), ctypes.byref(rid)))
-        return Request(self, rid.value, (obj, pf, pb), out)
+        return Request(self, rid.value, (obj, pf, pb, out), out)
 
     def stats(self) -> dict:
         s = PkServeStats()
