@@ -554,7 +554,8 @@ def wl_large(args, torch, pk, L):
         Jg, _ = pk.decon_lucy_omw_function(sub, psf, psf_b, 2, Background=100, save16bit=False, ctx=ctx)
         J64, _ = po.decon_lucy_omw_function(sub, psf, psf_b, 2, Background=100, save16bit=False, precision="f64")
         J32, _ = po.decon_lucy_omw_function(sub, psf, psf_b, 2, Background=100, save16bit=False)
-        res["parity_256cube"] = {"rel_l2_vs_fp64": rel_l2(Jg, J64), "rel_l2_vs_cpu_fp32": rel_l2(Jg, J32)}
+        res["parity_256cube"] = {"rel_l2_vs_fp64": rel_l2(Jg, J64), "rel_l2_vs_cpu_fp32": rel_l2(Jg, J32),
+                                 "cpu_fp32_vs_fp64": rel_l2(J32, J64)}
     return res
 
 
