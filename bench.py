@@ -31,9 +31,14 @@ import time
 import numpy as np
 
 # Hardware queues per process, read when the HIP runtime initialises: batched
-# and served steps run four chunk-set streams, and HIP's default of 4 queues
-# leaves them two.  Only a default; an explicit setting wins.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# and served steps run eight chunk-set streams, and HIP's default of 4 queues
+# leaves them two.  Raised to 12 when lower (a setting above 12 is kept).
+try:
+    _q = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+except ValueError:
+    _q = 4
+if _q < 12:
+    os.environ["GPU_MAX_HW_QUEUES"] = "12"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E

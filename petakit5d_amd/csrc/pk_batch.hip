@@ -333,7 +333,7 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
     // set overlap the work of the others.  Sets are balanced by voxels
     // (largest chunk first onto the lightest set); one stream when profiling.
     constexpr int kSets = kSideStreams + 1;
-    const int nset = prof_on ? 1 : (int)std::min<size_t>(kSets, act.size());
+    const int nset = prof_on ? 1 : (int)std::min<size_t>(std::min(kSets, chunk_streams()), act.size());
     std::vector<int> set_of(act.size(), 0);
     {
         std::vector<size_t> order(act.size());

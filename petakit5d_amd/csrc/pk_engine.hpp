@@ -172,7 +172,7 @@ struct pk_ctx_impl {
     // batched steps: chunk sets run their passes on the engine's stream and up
     // to kSideStreams more (default priority), forked after the step's table
     // upload and joined at its end
-    static constexpr int kSideStreams = 3;
+    static constexpr int kSideStreams = 7;  // at most 8 sets; chunk_streams() picks how many
     hipStream_t side[kSideStreams] = {};
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join[kSideStreams] = {};
@@ -208,6 +208,7 @@ struct pk_ctx_impl {
     void run_col(const VolPlan& vp, int axis, float2* data, int mode, const float2* otf, float scale, int64_t z0 = 0,
                  int64_t nz = 0);
     int64_t slab_planes(const VolPlan& vp) const;
+    int chunk_streams() const;  // streams a batched step may use (PK_CHUNK_STREAMS, default 8)
     std::string autotune(int64_t n1, int64_t n2, int64_t n3);
     void set_shapes(int64_t n1, int64_t n2, int64_t n3, const std::array<int, 8>& v);
     bool apply_shapes(VolPlan& vp, const std::array<int, 8>& v);
