@@ -722,9 +722,6 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
 // Compile-time row kernels load their real-space operands after the inverse
 // transform when true (fewer live registers, exposed load latency).
 constexpr bool kXrowLateOperands = false;
-#ifndef XROW_LATE_E
-#define XROW_LATE_E 100
-#endif
 
 // Rows per 256-thread block of the compile-time row kernels.  A row of
 // P <= 64 threads never straddles a wavefront: each wavefront holds
@@ -816,8 +813,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
     // real-space operands: loaded before the inverse transform (their latency
     // overlaps it) unless kXrowLateOperands, which trades that overlap for
     // fewer live registers across the transform
-    constexpr bool LATE = kXrowLateOperands || E >= XROW_LATE_E;
-    if constexpr (!(IN && LATE)) load_operands();
+    if constexpr (!(IN && kXrowLateOperands)) load_operands();
     float2 v[E];
     if constexpr (IN) {
 #pragma unroll
@@ -826,7 +822,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
             v[e] = c2r_pre(sp[m], cconj(sp[L - m]), rt[e * P]);
         }
         ct_fft<L, E, true>(v, p, lds, lbase, tw);
-        if constexpr (LATE) load_operands();
+        if constexpr (kXrowLateOperands) load_operands();
     }
 
     double s_num = 0.0, s_den = 0.0;
