@@ -74,6 +74,13 @@ static void unpin(pk_ctx_impl* ctx, ServeReq& r) {
     r.pinned = false;
 }
 
+// Once admitted, a request's OTFs are pinned in the cache: its PSF copies are
+// no longer read.
+static void drop_psfs(ServeReq& r) {
+    std::vector<float>().swap(r.pf);
+    std::vector<float>().swap(r.pb);
+}
+
 static void scheduler(pk_ctx_impl* ctx) {
     Server& S = *ctx->srv;
     for (;;) {
@@ -122,6 +129,7 @@ static void scheduler(pk_ctx_impl* ctx) {
                     ctx->alloc_async(r->mem, need, ctx->stream);
                     ctx->chunk_place(r->c, r->mem.as<char>());
                     r->t_adm = Clock::now();
+                    drop_psfs(*r);
                     S.active.push_back(r);
                     admitted.push_back(r);
                 } catch (const Error& e) {
@@ -143,6 +151,7 @@ static void scheduler(pk_ctx_impl* ctx) {
                                 ctx->alloc_async(r->mem, need, ctx->stream);
                                 ctx->chunk_place(r->c, r->mem.as<char>());
                                 r->t_adm = Clock::now();
+                                drop_psfs(*r);
                                 S.active.push_back(r);
                                 admitted.push_back(r);
                                 ok = true;

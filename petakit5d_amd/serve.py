@@ -50,6 +50,24 @@ class PkServeStats(ctypes.Structure):
     ]
 
 
+def _pinned_host_vol(x):
+    """(keepalive, PkVol, 'numpy') for a pinned, column-major torch CPU tensor
+    (the server reads it in place: no staging copy), else None."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    if not isinstance(x, torch.Tensor) or x.is_cuda or x.dim() != 3 or not x.is_pinned():
+        return None
+    if x.dtype not in (torch.uint16, torch.float32):
+        return None
+    d = tuple(int(n) for n in x.shape)
+    if tuple(x.stride()) != (1, d[0], d[0] * d[1]):
+        return None
+    code = 1 if x.dtype == torch.uint16 else 2
+    return x.numpy(), PkVol(ctypes.c_void_p(x.data_ptr()), code, _lib.PK_MEM_HOST, (ctypes.c_int64 * 3)(*d)), "numpy"
+
+
 def _pinned(shape_rev, dtype):
     """A pinned host tensor of the reversed (C-order) shape, or None when no
     GPU runtime is there to pin memory."""
@@ -86,6 +104,7 @@ class Request:
 
     def __init__(self, server, rid, keep, out):
         self.server, self.id, self._keep, self.out = server, rid, keep, out
+        self._done = False
 
     def info(self, wait_ms=0.0) -> dict:
         i = PkReqInfo()
@@ -93,6 +112,7 @@ class Request:
                                                           ctypes.byref(i)))
         if i.state >= DONE:
             self._keep = None  # the server is done with the input (pinned staging goes back to the cache)
+            self._done = True
         return {"state": i.state, "status": i.status, "iterations_done": i.iterations_done,
                 "latency_ms": i.latency_ms, "queue_ms": i.queue_ms}
 
@@ -108,6 +128,16 @@ class Request:
     def cancel(self):
         self.server.ctx.check(self.server.L.pk_serve_cancel(self.server.ctx.handle, self.id))
 
+    def __del__(self):
+        # a finished request's record is dropped from the server (pk_serve_release);
+        # one not seen finished stays until the server stops
+        try:
+            srv = self.server
+            if self._done and srv is not None and srv.ctx is not None and srv.ctx.handle and not srv.stopped:
+                srv.L.pk_serve_release(srv.ctx.handle, self.id)
+        except Exception:  # noqa: BLE001 -- interpreter shutdown or a stopped server
+            pass
+
 
 class Server:
     """Continuous-batching decon server on one device.  The context's HBM
@@ -116,13 +146,15 @@ class Server:
     def __init__(self, ctx=None, max_active: int = 64):
         self.ctx = ctx or _lib.default_context()
         self.L = _bind(_lib.load())
+        self.stopped = False
         self.ctx.check(self.L.pk_serve_start(self.ctx.handle, int(max_active)))
 
     def submit(self, I, PSF_f, PSF_b=None, NUMIT=2, method="omw", **kwargs) -> Request:
         o = _parse(kwargs)
         m = {"omw": 1, "rl": 0, "simplified": 0}[method]
         opts, _ = _opts_struct(o, (1, 1, 1), False)
-        obj, vin, kind = _vol_of(I)
+        pinned_in = _pinned_host_vol(I)
+        obj, vin, kind = pinned_in if pinned_in is not None else _vol_of(I)
         d = tuple(vin.dims)
         if kind == "torch":
             import torch
@@ -138,7 +170,7 @@ class Server:
             # host volumes are staged in pinned memory (torch's caching host
             # allocator): the server's copies then run asynchronously, beside
             # the steps, instead of blocking the scheduler thread
-            pin_in = _pinned((d[2], d[1], d[0]), obj.dtype)
+            pin_in = None if pinned_in is not None else _pinned((d[2], d[1], d[0]), obj.dtype)
             if pin_in is not None:
                 import torch
                 pin_in.permute(2, 1, 0).copy_(torch.from_numpy(obj))  # same layout; torch's parallel copy
@@ -170,3 +202,4 @@ class Server:
     def stop(self):
         if self.ctx is not None and self.ctx.handle:
             self.ctx.check(self.L.pk_serve_stop(self.ctx.handle))
+        self.stopped = True
