@@ -462,41 +462,48 @@ def wl_served(args, torch, pk, L):
     del reqs
     cap_rps = len(jobs) / cap_s
     qps = 0.7 * cap_rps
-    # open-loop Poisson arrivals at 0.7x capacity
-    arr = []
-    t = 0.0
-    while t < args.duration:
-        t += g.exponential(1.0 / qps)
-        arr.append((t, job(len(arr))))
-    # a client thread consumes results in submission order and drops them, so
-    # pinned staging and outputs return to the host allocator's cache
     import queue as _queue
-    pending = _queue.Queue()
-    lat, detail = [], []
 
-    def consumer():
-        while True:
-            r = pending.get()
-            if r is None:
-                return
-            r, sh, meth = r
-            r.result()
-            inf = r.info()
-            lat.append(inf["latency_ms"])
-            detail.append((round(inf["latency_ms"], 1), round(inf["queue_ms"], 1), meth, list(sh)))
+    def open_loop(rate, seconds):
+        """Poisson arrivals at `rate` for `seconds`; a client thread consumes
+        results in submission order and drops them (pinned staging and outputs
+        return to the host allocator's cache).  Returns (trace, wall s,
+        latencies, per-request detail)."""
+        arr, t = [], 0.0
+        while t < seconds:
+            t += g.exponential(1.0 / rate)
+            arr.append((t, job(len(arr))))
+        pending = _queue.Queue()
+        lat, detail = [], []
 
-    cth = threading.Thread(target=consumer)
-    cth.start()
-    t0 = time.perf_counter()
-    for at, j in arr:
-        d = at - (time.perf_counter() - t0)
-        if d > 0:
-            time.sleep(d)
-        pending.put((submit(*j), j[0].shape, j[1]))
-    pending.put(None)
-    cth.join()
-    wall = time.perf_counter() - t0
+        def consumer():
+            while True:
+                item = pending.get()
+                if item is None:
+                    return
+                r, sh, meth = item
+                r.result()
+                inf = r.info()
+                lat.append(inf["latency_ms"])
+                detail.append((round(inf["latency_ms"], 1), round(inf["queue_ms"], 1), meth, list(sh)))
+
+        cth = threading.Thread(target=consumer)
+        cth.start()
+        t0 = time.perf_counter()
+        for at, j in arr:
+            d = at - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            pending.put((submit(*j), j[0].shape, j[1]))
+        pending.put(None)
+        cth.join()
+        return arr, time.perf_counter() - t0, lat, detail
+
+    # open-loop Poisson arrivals at 0.7x capacity (the reported load), then at 0.5x
+    arr, wall, lat, detail = open_loop(qps, args.duration)
     vox = sum(float(np.prod(j[0].shape)) for _, j in arr)
+    arr_h, wall_h, lat_h, _ = open_loop(0.5 * cap_rps, args.duration / 2)
+    vox_h = sum(float(np.prod(j[0].shape)) for _, j in arr_h)
     srv.stop()
     res = base_result("served decon Mvoxel/s + p99 latency (OMW-2 / Biggs-15 mix)", vox / 1e6 / wall, "Mvoxel/s", args,
                       float(np.mean(lat)), {
@@ -509,6 +516,11 @@ def wl_served(args, torch, pk, L):
                          "p99": round(float(np.percentile(lat, 99)), 2)}
     detail.sort(key=lambda d: -d[0])
     res["slowest"] = [{"latency_ms": d[0], "queue_ms": d[1], "method": d[2], "shape": d[3]} for d in detail[:5]]
+    res["half_load"] = {"req_per_s": round(0.5 * cap_rps, 2), "seconds": args.duration / 2, "requests": len(arr_h),
+                        "Mvoxel_s": round(vox_h / 1e6 / wall_h, 1),
+                        "latency_ms": {"p50": round(float(np.percentile(lat_h, 50)), 2),
+                                       "p90": round(float(np.percentile(lat_h, 90)), 2),
+                                       "p99": round(float(np.percentile(lat_h, 99)), 2)}}
     res["capacity"] = {"req_per_s": round(cap_rps, 2), "Mvoxel_s": round(cap_vox / 1e6 / cap_s, 1),
                        "how": "48 requests submitted at once"}
     if not args.no_plain:

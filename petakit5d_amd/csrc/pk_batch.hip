@@ -180,7 +180,7 @@ size_t pk_ctx_impl::chunk_prepare(Chunk& c, const pk_vol& in, const pk_vol& out,
 }
 
 // Carve the chunk's buffers from base and stage its input (copy, zero pad).
-void pk_ctx_impl::chunk_place(Chunk& c, char* base) {
+void pk_ctx_impl::chunk_place(Chunk& c, char* base, hipStream_t copy_stream) {
     const VolPlan& vp = *c.vp;
     char* p = base;
     auto take = [&](size_t bytes) {
@@ -207,7 +207,7 @@ void pk_ctx_impl::chunk_place(Chunk& c, char* base) {
     if (c.in_host) {
         const size_t ub = esz * (size_t)(c.d[0] * c.d[1] * c.d[2]);
         void* dst = take(ub);
-        PK_HIP(hipMemcpyAsync(dst, c.raw_user, ub, hipMemcpyHostToDevice, stream));
+        PK_HIP(hipMemcpyAsync(dst, c.raw_user, ub, hipMemcpyHostToDevice, copy_stream ? copy_stream : stream));
         raw = dst;
     }
     c.raw16 = c.u16 ? static_cast<const uint16_t*>(raw) : nullptr;
@@ -526,14 +526,23 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
 bool Chunk::done() const { return k >= std::max(numit, 1); }
 
 // Epilogue (decon_lucy_function.m:177-197) and output copy of a finished chunk.
-void pk_ctx_impl::chunk_finish(Chunk& c) {
+void pk_ctx_impl::chunk_finish(Chunk& c, hipStream_t copy_stream) {
     // NUMIT > 0: the last update wrote the output (fused epilogue)
     if (c.numit == 0) {
         const EpiArgs e = chunk_epi(c);
         hipLaunchKernelGGL(k_epilogue, dim3(epi_grid(e)), dim3(256), 0, stream, e);
         PK_HIP(hipGetLastError());
     }
-    if (c.out_host) PK_HIP(hipMemcpyAsync(c.out_host, c.dout, c.out_bytes, hipMemcpyDeviceToHost, stream));
+    if (c.out_host) {
+        if (copy_stream) copy_after_stream(copy_stream);  // after the chunk's last step (and epilogue)
+        PK_HIP(hipMemcpyAsync(c.out_host, c.dout, c.out_bytes, hipMemcpyDeviceToHost, copy_stream ? copy_stream : stream));
+    }
+}
+
+void pk_ctx_impl::copy_after_stream(hipStream_t copy_stream) {
+    if (!ev_copy) PK_HIP(hipEventCreateWithFlags(&ev_copy, hipEventDisableTiming));
+    PK_HIP(hipEventRecord(ev_copy, stream));
+    PK_HIP(hipStreamWaitEvent(copy_stream, ev_copy, 0));
 }
 
 // ------------------------------------------------------------------ batch

@@ -133,6 +133,12 @@ void pk_ctx_destroy(pk_ctx* c) {
             if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
         }
         if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+        if (c->ev_copy) (void)hipEventDestroy(c->ev_copy);
+        for (hipStream_t* cs : {&c->copy_stream, &c->h2d_stream})
+            if (*cs) {
+                (void)hipStreamSynchronize(*cs);
+                (void)hipStreamDestroy(*cs);
+            }
         if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     }
     delete c;

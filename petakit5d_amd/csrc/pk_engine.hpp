@@ -174,6 +174,9 @@ struct pk_ctx_impl {
     // upload and joined at its end
     static constexpr int kSideStreams = 7;  // at most 8 sets; chunk_streams() picks how many
     hipStream_t side[kSideStreams] = {};
+    hipStream_t copy_stream = nullptr;  // served requests' output copies (created on first use)
+    hipStream_t h2d_stream = nullptr;   // served requests' input copies and their arenas (created on first use)
+    hipEvent_t ev_copy = nullptr;       // orders a host copy after the engine stream's work
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join[kSideStreams] = {};
     std::vector<ProfPending> prof_pending;
@@ -236,9 +239,14 @@ struct pk_ctx_impl {
     // hf, hb: psf_key of psf_f / psf_b (hashed once per batch or request)
     size_t chunk_prepare(Chunk& c, const pk_vol& in, const pk_vol& out, const float* psf_f, const int64_t fd[3],
                          uint64_t hf, const float* psf_b, const int64_t bd[3], uint64_t hb);
-    void chunk_place(Chunk& c, char* base);
+    // copy_stream (optional): the stream of the chunk's host copy, so it
+    // overlaps the steps of other chunks.  chunk_place: the arena at base must
+    // be allocated in copy_stream's order (stream-ordered allocation on it);
+    // chunk_finish: the copy waits for the work queued on the engine stream.
+    void chunk_place(Chunk& c, char* base, hipStream_t copy_stream = nullptr);
     void run_step(std::vector<Chunk*>& act);
-    void chunk_finish(Chunk& c);
+    void chunk_finish(Chunk& c, hipStream_t copy_stream = nullptr);
+    void copy_after_stream(hipStream_t copy_stream);  // copy_stream waits for the work queued on stream so far
     void otf_pin(const float2* spec, int delta);
     // MATLAB-structured comparison path (pk_plain.hip); leaves the estimate in J
     void decon_unfused(int method, const VolPlan& vp, const void* raw, bool u16, float bg, const float2* Hf,

@@ -60,7 +60,20 @@ struct Server {
     // event recorded after their last step and output copy completes
     // (scheduler thread only)
     std::vector<std::pair<std::shared_ptr<ServeReq>, hipEvent_t>> draining;
+    // admitted, host input still being copied on the copy stream: a request
+    // joins the steps once its copy's event completes (scheduler thread only)
+    std::vector<std::pair<std::shared_ptr<ServeReq>, hipEvent_t>> staging;
     std::vector<hipEvent_t> ev_free;
+    hipEvent_t take_event() {
+        hipEvent_t ev = nullptr;
+        if (!ev_free.empty()) {
+            ev = ev_free.back();
+            ev_free.pop_back();
+        } else {
+            PK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        }
+        return ev;
+    }
     bool stop = false;
     uint64_t next_id = 1;
     int max_active = 64;
@@ -81,14 +94,42 @@ static void drop_psfs(ServeReq& r) {
     std::vector<float>().swap(r.pb);
 }
 
+// Stream whose order a request's arena is allocated in: the input-copy
+// stream for a host volume (its copy then starts without waiting for the
+// steps queued on the engine stream), else the engine stream.
+static hipStream_t arena_stream(pk_ctx_impl* ctx, const ServeReq& r) {
+    if (r.in.mem != PK_MEM_HOST) return ctx->stream;
+    if (!ctx->h2d_stream) PK_HIP(hipStreamCreateWithFlags(&ctx->h2d_stream, hipStreamNonBlocking));
+    return ctx->h2d_stream;
+}
+
+// Stage an admitted request's input: a host volume is copied on the input-copy
+// stream and the request waits in S.staging until that copy completes; a
+// device volume is read in place and the request joins the steps at once.
+static void place(pk_ctx_impl* ctx, Server& S, const std::shared_ptr<ServeReq>& r) {
+    const bool host_in = r->in.mem == PK_MEM_HOST;
+    ctx->chunk_place(r->c, r->mem.as<char>(), host_in ? arena_stream(ctx, *r) : nullptr);
+    r->t_adm = Clock::now();
+    drop_psfs(*r);
+    if (host_in) {
+        hipEvent_t ev = S.take_event();
+        PK_HIP(hipEventRecord(ev, ctx->h2d_stream));
+        S.staging.emplace_back(r, ev);
+    } else {
+        S.active.push_back(r);
+    }
+}
+
 static void scheduler(pk_ctx_impl* ctx) {
     Server& S = *ctx->srv;
     for (;;) {
         std::vector<std::shared_ptr<ServeReq>> cand;
         {
             std::unique_lock<std::mutex> lk(S.mq);
-            S.cv.wait(lk, [&] { return S.stop || !S.queue.empty() || !S.active.empty() || !S.draining.empty(); });
-            if (S.stop && S.active.empty() && S.draining.empty()) {
+            S.cv.wait(lk, [&] {
+                return S.stop || !S.queue.empty() || !S.active.empty() || !S.draining.empty() || !S.staging.empty();
+            });
+            if (S.stop && S.active.empty() && S.draining.empty() && S.staging.empty()) {
                 for (auto& r : S.queue) {
                     r->state = PK_REQ_CANCELLED;
                     r->st = PK_ERR_CANCELLED;
@@ -126,11 +167,8 @@ static void scheduler(pk_ctx_impl* ctx) {
                     const size_t need = ctx->chunk_prepare(r->c, r->in, r->out, r->pf.data(), r->fd, r->hf,
                                                            r->pb.empty() ? nullptr : r->pb.data(), r->bd, r->hb);
                     r->pinned = true;
-                    ctx->alloc_async(r->mem, need, ctx->stream);
-                    ctx->chunk_place(r->c, r->mem.as<char>());
-                    r->t_adm = Clock::now();
-                    drop_psfs(*r);
-                    S.active.push_back(r);
+                    ctx->alloc_async(r->mem, need, arena_stream(ctx, *r));
+                    place(ctx, S, r);
                     admitted.push_back(r);
                 } catch (const Error& e) {
                     unpin(ctx, *r);
@@ -148,11 +186,8 @@ static void scheduler(pk_ctx_impl* ctx) {
                                                                        r->pb.empty() ? nullptr : r->pb.data(), r->bd,
                                                                        r->hb);
                                 r->pinned = true;
-                                ctx->alloc_async(r->mem, need, ctx->stream);
-                                ctx->chunk_place(r->c, r->mem.as<char>());
-                                r->t_adm = Clock::now();
-                                drop_psfs(*r);
-                                S.active.push_back(r);
+                                ctx->alloc_async(r->mem, need, arena_stream(ctx, *r));
+                                place(ctx, S, r);
                                 admitted.push_back(r);
                                 ok = true;
                             } catch (const Error&) {
@@ -174,6 +209,31 @@ static void scheduler(pk_ctx_impl* ctx) {
                                ctx->otf.end());
                 if (ctx->otf.size() == before) reject_head = true;
             }
+            // ---- staged requests whose input copy completed join the steps; with
+            // nothing else to run, wait for the oldest copy
+            if (!S.staging.empty()) {
+                std::vector<std::pair<std::shared_ptr<ServeReq>, hipEvent_t>> still;
+                for (size_t i = 0; i < S.staging.size(); ++i) {
+                    auto& d = S.staging[i];
+                    hipError_t q = hipEventQuery(d.second);
+                    if (q == hipErrorNotReady && i == 0 && S.active.empty()) q = hipEventSynchronize(d.second);
+                    if (q == hipErrorNotReady) {
+                        still.push_back(d);
+                        continue;
+                    }
+                    if (q == hipSuccess) {
+                        S.active.push_back(d.first);
+                    } else {
+                        (void)hipGetLastError();
+                        failed.emplace_back(d.first, Error{PK_ERR_HIP, std::string("device error: ") +
+                                                                           hipGetErrorString(q)});
+                        d.first->mem.release();
+                        unpin(ctx, *d.first);
+                    }
+                    S.ev_free.push_back(d.second);
+                }
+                S.staging.swap(still);
+            }
             // ---- one iteration for every request in flight.  Nothing waits for
             // the device here: the step-table double buffer keeps the host at
             // most two steps ahead, and finished requests retire by event.
@@ -185,18 +245,22 @@ static void scheduler(pk_ctx_impl* ctx) {
                     ctx->run_step(act);
                     for (auto& r : S.active)
                         if (r->c.done()) {
-                            ctx->chunk_finish(r->c);
+                            // host outputs are copied on the copy stream, after the request's last step,
+                            // so the next step does not wait for them
+                            const bool host_out = r->c.out_host != nullptr;
+                            if (host_out && !ctx->copy_stream)
+                                PK_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+                            ctx->chunk_finish(r->c, host_out ? ctx->copy_stream : nullptr);
                             ending.push_back(r);
                         }
                     if (!ending.empty()) {
-                        hipEvent_t ev = nullptr;
-                        if (!S.ev_free.empty()) {
-                            ev = S.ev_free.back();
-                            S.ev_free.pop_back();
+                        hipEvent_t ev = S.take_event();
+                        if (ctx->copy_stream) {  // retire after both the steps and the copies
+                            ctx->copy_after_stream(ctx->copy_stream);
+                            PK_HIP(hipEventRecord(ev, ctx->copy_stream));
                         } else {
-                            PK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                            PK_HIP(hipEventRecord(ev, ctx->stream));
                         }
-                        PK_HIP(hipEventRecord(ev, ctx->stream));
                         for (auto& r : ending) S.draining.emplace_back(r, ev);
                     }
                     if (ctx->prof_on) {
@@ -223,7 +287,7 @@ static void scheduler(pk_ctx_impl* ctx) {
             // ---- retire requests whose last step and output copy completed;
             // with nothing else to run, wait for the oldest
             if (!S.draining.empty()) {
-                const bool idle = S.active.empty() && admitted.empty();
+                const bool idle = S.active.empty() && S.staging.empty();
                 std::vector<std::pair<std::shared_ptr<ServeReq>, hipEvent_t>> still;
                 std::vector<hipEvent_t> seen;
                 for (size_t i = 0; i < S.draining.size(); ++i) {
