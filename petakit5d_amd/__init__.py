@@ -7,6 +7,7 @@ reference's MATLAB call signatures.
 from ._lib import Context, PkDeconError, default_context, load  # noqa: F401
 from .backprojector import BackProjector, omw_backprojector_generation  # noqa: F401
 from .batch import decon_batch  # noqa: F401
+from .chunks import XR_zarrChunkCoordinatesExtraction, decon_large_in_memory  # noqa: F401
 from .serve import Server  # noqa: F401
 from .decon import (  # noqa: F401
     crop_psf_to_data,
@@ -30,6 +31,8 @@ __all__ = [
     "omw_backprojector_generation",
     "BackProjector",
     "decon_batch",
+    "XR_zarrChunkCoordinatesExtraction",
+    "decon_large_in_memory",
     "Server",
     "crop_psf_to_data",
     "fft_r2c",

@@ -59,18 +59,60 @@ bool smooth235(int64_t n) {
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release() {
     if (p) {
-        if (astream) (void)hipFreeAsync(p, astream);
+        if (pooled && owner) owner->block_put(p, bytes);
         else (void)hipFree(p);
         if (owner) owner->live_bytes -= bytes;
     }
     p = nullptr;
     bytes = 0;
-    astream = nullptr;
+    pooled = false;
+}
+
+// Size class of a cached block: the size rounded up to 1/8 of its highest
+// power of two (at most 12.5 % slack), 1 MiB at least.
+static size_t block_class(size_t b) {
+    if (b <= (size_t(1) << 20)) return size_t(1) << 20;
+    size_t p = size_t(1) << (63 - __builtin_clzll((unsigned long long)b));
+    const size_t step = p / 8;
+    return (b + step - 1) / step * step;
+}
+
+void pk_ctx_impl::block_put(void* p, size_t bytes) {
+    block_cache.emplace(bytes, p);
+    block_cache_bytes += bytes;
+}
+
+void pk_ctx_impl::block_cache_clear() {
+    if (block_cache.empty()) return;
+    for (auto& kv : block_cache) (void)hipFree(kv.second);
+    block_cache.clear();
+    block_cache_bytes = 0;
+}
+
+void pk_ctx_impl::alloc_pooled(DevBuf& b, size_t bytes) {
+    b.release();
+    if (bytes == 0) return;
+    const size_t cls = block_class(bytes);
+    auto it = block_cache.find(cls);
+    if (it == block_cache.end()) {
+        if (budget && live_bytes + block_cache_bytes + cls > budget) block_cache_clear();
+        alloc(b, cls);  // budget check and hipMalloc
+        b.pooled = true;
+        return;
+    }
+    b.p = it->second;
+    b.bytes = cls;
+    b.owner = this;
+    b.pooled = true;
+    block_cache.erase(it);
+    block_cache_bytes -= cls;
+    live_bytes += cls;
 }
 
 void pk_ctx_impl::alloc(DevBuf& b, size_t bytes) {
     b.release();
     if (bytes == 0) return;
+    if (budget && live_bytes + block_cache_bytes + bytes > budget) block_cache_clear();
     if (budget && live_bytes + bytes > budget)
         fail(PK_ERR_BUDGET, "device allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
                                 std::to_string(live_bytes) + " of " + std::to_string(budget) + " bytes in use)");
@@ -83,25 +125,6 @@ void pk_ctx_impl::alloc(DevBuf& b, size_t bytes) {
     b.p = p;
     b.bytes = bytes;
     b.owner = this;
-    live_bytes += bytes;
-}
-
-void pk_ctx_impl::alloc_async(DevBuf& b, size_t bytes, hipStream_t s) {
-    b.release();
-    if (bytes == 0) return;
-    if (budget && live_bytes + bytes > budget)
-        fail(PK_ERR_BUDGET, "device allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
-                                std::to_string(live_bytes) + " of " + std::to_string(budget) + " bytes in use)");
-    void* p = nullptr;
-    if (hipMallocAsync(&p, bytes, s) != hipSuccess) {
-        (void)hipGetLastError();
-        alloc(b, bytes);
-        return;
-    }
-    b.p = p;
-    b.bytes = bytes;
-    b.owner = this;
-    b.astream = s;
     live_bytes += bytes;
 }
 
@@ -1037,6 +1060,7 @@ void pk_ctx_impl::reset() {
     ws.I.release(); ws.J.release(); ws.A.release(); ws.J3.release(); ws.Y.release(); ws.J4.release();
     ws.partials.release(); ws.lambda.release(); ws.raw.release(); ws.out.release(); ws.mask.release();
     ws.mask2.release(); ws.count.release(); ws.tmp.release(); ws.batch.release();
+    block_cache_clear();
     for (int i = 0; i < 2; ++i) {
         ws.tab_dev[i].release();
         if (ws.tab_host[i].p) (void)hipHostFree(ws.tab_host[i].p);

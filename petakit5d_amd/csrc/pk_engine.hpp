@@ -38,7 +38,7 @@ struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
     pk_ctx_impl* owner = nullptr;
-    hipStream_t astream = nullptr;  // stream-ordered allocation: freed on this stream (no device sync)
+    bool pooled = false;  // from the context's block cache: release returns it there
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
@@ -189,10 +189,17 @@ struct pk_ctx_impl {
 
     // device memory helpers
     void alloc(DevBuf& b, size_t bytes);
-    // stream-ordered allocation on stream s (hipMallocAsync; plain hipMalloc
-    // where the runtime has no pool): its release is ordered after the work
-    // queued on s instead of synchronising the device
-    void alloc_async(DevBuf& b, size_t bytes, hipStream_t s);
+    // Device blocks kept for reuse by size class (request arenas of the
+    // server): allocation takes a cached block of the size's class when there
+    // is one, release puts the block back; nothing synchronises the device.
+    // The caller releases a block only when no queued work still uses it.
+    // Cached bytes count against the HBM budget; an allocation that would
+    // exceed it first frees the cache.
+    std::multimap<size_t, void*> block_cache;
+    size_t block_cache_bytes = 0;
+    void alloc_pooled(DevBuf& b, size_t bytes);
+    void block_put(void* p, size_t bytes);
+    void block_cache_clear();
     void ensure(DevBuf& b, size_t bytes);
 
     const VolPlan& plan(int64_t n1, int64_t n2, int64_t n3);
@@ -241,8 +248,9 @@ struct pk_ctx_impl {
                          uint64_t hf, const float* psf_b, const int64_t bd[3], uint64_t hb);
     // copy_stream (optional): the stream of the chunk's host copy, so it
     // overlaps the steps of other chunks.  chunk_place: the arena at base must
-    // be allocated in copy_stream's order (stream-ordered allocation on it);
-    // chunk_finish: the copy waits for the work queued on the engine stream.
+    // have no queued work on other streams (a cached block whose users have
+    // completed, or fresh memory); chunk_finish: the copy waits for the work
+    // queued on the engine stream.
     void chunk_place(Chunk& c, char* base, hipStream_t copy_stream = nullptr);
     void run_step(std::vector<Chunk*>& act);
     void chunk_finish(Chunk& c, hipStream_t copy_stream = nullptr);

@@ -94,13 +94,20 @@ static void drop_psfs(ServeReq& r) {
     std::vector<float>().swap(r.pb);
 }
 
-// Stream whose order a request's arena is allocated in: the input-copy
-// stream for a host volume (its copy then starts without waiting for the
-// steps queued on the engine stream), else the engine stream.
-static hipStream_t arena_stream(pk_ctx_impl* ctx, const ServeReq& r) {
-    if (r.in.mem != PK_MEM_HOST) return ctx->stream;
+// The input-copy stream (created on first use).  Request arenas come from the
+// context's block cache, whose blocks are free of queued work, so a copy on
+// this stream need not wait for the steps queued on the engine stream.
+static hipStream_t h2d_stream_of(pk_ctx_impl* ctx) {
     if (!ctx->h2d_stream) PK_HIP(hipStreamCreateWithFlags(&ctx->h2d_stream, hipStreamNonBlocking));
     return ctx->h2d_stream;
+}
+
+// Before a request's arena goes back to the cache on a failure path: wait for
+// whatever may still use it (a no-op when the device is already in error).
+static void quiesce(pk_ctx_impl* ctx) {
+    for (hipStream_t st : {ctx->h2d_stream, ctx->copy_stream, ctx->stream})
+        if (st) (void)hipStreamSynchronize(st);
+    (void)hipGetLastError();
 }
 
 // Stage an admitted request's input: a host volume is copied on the input-copy
@@ -108,7 +115,7 @@ static hipStream_t arena_stream(pk_ctx_impl* ctx, const ServeReq& r) {
 // device volume is read in place and the request joins the steps at once.
 static void place(pk_ctx_impl* ctx, Server& S, const std::shared_ptr<ServeReq>& r) {
     const bool host_in = r->in.mem == PK_MEM_HOST;
-    ctx->chunk_place(r->c, r->mem.as<char>(), host_in ? arena_stream(ctx, *r) : nullptr);
+    ctx->chunk_place(r->c, r->mem.as<char>(), host_in ? h2d_stream_of(ctx) : nullptr);
     r->t_adm = Clock::now();
     drop_psfs(*r);
     if (host_in) {
@@ -167,11 +174,12 @@ static void scheduler(pk_ctx_impl* ctx) {
                     const size_t need = ctx->chunk_prepare(r->c, r->in, r->out, r->pf.data(), r->fd, r->hf,
                                                            r->pb.empty() ? nullptr : r->pb.data(), r->bd, r->hb);
                     r->pinned = true;
-                    ctx->alloc_async(r->mem, need, arena_stream(ctx, *r));
+                    ctx->alloc_pooled(r->mem, need);
                     place(ctx, S, r);
                     admitted.push_back(r);
                 } catch (const Error& e) {
                     unpin(ctx, *r);
+                    if (r->mem.p) quiesce(ctx);
                     r->mem.release();
                     if (e.st == PK_ERR_BUDGET) {
                         // cached OTFs nobody reads are the first memory to give back; then wait
@@ -186,12 +194,13 @@ static void scheduler(pk_ctx_impl* ctx) {
                                                                        r->pb.empty() ? nullptr : r->pb.data(), r->bd,
                                                                        r->hb);
                                 r->pinned = true;
-                                ctx->alloc_async(r->mem, need, arena_stream(ctx, *r));
+                                ctx->alloc_pooled(r->mem, need);
                                 place(ctx, S, r);
                                 admitted.push_back(r);
                                 ok = true;
                             } catch (const Error&) {
                                 unpin(ctx, *r);
+                                if (r->mem.p) quiesce(ctx);
                                 r->mem.release();
                             }
                         }
@@ -270,6 +279,7 @@ static void scheduler(pk_ctx_impl* ctx) {
                 } catch (const Error& e) {
                     ending.clear();  // a device error fails every request in flight
                     for (auto& r : S.active) failed.emplace_back(r, e);
+                    quiesce(ctx);
                 }
                 std::vector<std::shared_ptr<ServeReq>> keep;
                 for (auto& r : S.active) {
@@ -355,14 +365,6 @@ static void scheduler(pk_ctx_impl* ctx) {
 
 void serve_start(pk_ctx_impl* ctx, int max_active) {
     if (ctx->srv) fail(PK_ERR_BUSY, "server already running");
-    {  // request arenas come from the device's default pool: keep freed memory there for the next requests
-        hipMemPool_t pool = nullptr;
-        if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess && pool) {
-            uint64_t keep = UINT64_MAX;
-            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-        }
-        (void)hipGetLastError();
-    }
     ctx->srv = std::make_unique<Server>();
     ctx->srv->max_active = max_active > 0 ? max_active : 64;
     // many request shapes and channels: keep their OTFs (the HBM budget still bounds them)
