@@ -482,12 +482,33 @@ __device__ __forceinline__ RawRow raw_row(const XArgs& a, int64_t row, int64_t n
     const int64_t y = row % n2, z = row / n2;
     return RawRow{(y + a.s1 * z) * a.s0, y < a.s1 && z < a.s2};
 }
+// A pair inside the source row whose first value sits on a 2-value boundary
+// is read as one word (4 bytes for uint16, 8 for float); the values are the
+// same either way.
 template <class T>
 __device__ __forceinline__ float2 ld_raw_padded(const T* __restrict__ base, const XArgs& a, const RawRow& r,
                                                 int64_t x0, bool pair) {
     float2 v = make_float2(0.f, 0.f);
-    if (r.in && x0 < a.s0) v.x = (float)base[r.src + x0];
-    if (pair && r.in && x0 + 1 < a.s0) v.y = (float)base[r.src + x0 + 1];
+    if (!r.in) return v;
+    const T* __restrict__ q = base + (r.src + x0);
+    if (pair && x0 + 1 < a.s0) {
+        if ((reinterpret_cast<uintptr_t>(q) & (2 * sizeof(T) - 1)) == 0) {
+            if constexpr (sizeof(T) == 2) {
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(q);
+                v.x = (float)(w & 0xffffu);
+                v.y = (float)(w >> 16);
+            } else {
+                const float2 f = *reinterpret_cast<const float2*>(q);
+                v.x = (float)f.x;
+                v.y = (float)f.y;
+            }
+        } else {
+            v.x = (float)q[0];
+            v.y = (float)q[1];
+        }
+    } else if (x0 < a.s0) {
+        v.x = (float)q[0];
+    }
     return v;
 }
 
