@@ -8,6 +8,7 @@ from ._lib import Context, PkDeconError, default_context, load  # noqa: F401
 from .backprojector import BackProjector, omw_backprojector_generation  # noqa: F401
 from .batch import decon_batch  # noqa: F401
 from .chunks import XR_zarrChunkCoordinatesExtraction, decon_large_in_memory  # noqa: F401
+from .rldecon import RLdecon_volume  # noqa: F401
 from .serve import Server  # noqa: F401
 from .decon import (  # noqa: F401
     crop_psf_to_data,
@@ -33,6 +34,7 @@ __all__ = [
     "decon_batch",
     "XR_zarrChunkCoordinatesExtraction",
     "decon_large_in_memory",
+    "RLdecon_volume",
     "Server",
     "crop_psf_to_data",
     "fft_r2c",

@@ -1,0 +1,101 @@
+"""Per-volume decon driver with the semantics of RLdecon.m for in-memory data
+(rawdata given, PSF given as an array): PSF crop to the data, OMW back
+projector generation with an on-disk cache, method dispatch, background,
+scale, offset, damp, bbox and edge erosion passed to the decon functions.
+
+Files and deviations: the back projector is cached as .npy under the
+reference's file-name stem (RLdecon.m:292, '.tif' there: TIFF I/O is not part
+of this engine); the engine pads non-2/3/5-smooth sizes with zeros and crops
+back (pad=True)."""
+from __future__ import annotations
+
+import hashlib
+import os
+
+import numpy as np
+
+from .backprojector import omw_backprojector_generation
+from .decon import _is_torch, crop_psf_to_data, decon_lucy_function, decon_lucy_omw_function
+
+__all__ = ["RLdecon_volume", "omw_bp_filename"]
+
+_BP_CACHE: dict = {}
+
+
+def omw_bp_filename(psfgenPath: str, psfFsn: str, wienerAlpha: float, OTFCumThresh: float) -> str:
+    """Back projector cache name: sprintf('%s/%s_back_projector_alpha_%0.6f_otf_cum_thresh_%0.6f', ...)
+    (RLdecon.m:292), with the .npy suffix of this engine's cache."""
+    return os.path.join(psfgenPath, f"{psfFsn}_back_projector_alpha_{wienerAlpha:0.6f}_otf_cum_thresh_"
+                                    f"{OTFCumThresh:0.6f}.npy")
+
+
+def _omw_bp(psf, wienerAlpha, skewed, OTFCumThresh, hannWinBounds, psfgenPath, psfFsn, ctx):
+    """OMW back projector of psf (RLdecon.m:290-325): loaded from the cache file
+    when it exists, else generated (omw_backprojector_generation) and written;
+    also kept in memory per (PSF bytes, alpha, threshold, window, skewed)."""
+    key = (hashlib.sha1(np.ascontiguousarray(psf).tobytes()).hexdigest(), psf.shape, float(wienerAlpha),
+           float(OTFCumThresh), tuple(float(x) for x in hannWinBounds), None if skewed is None else bool(skewed))
+    if key in _BP_CACHE:
+        return _BP_CACHE[key]
+    fn = omw_bp_filename(psfgenPath, psfFsn, wienerAlpha, OTFCumThresh) if psfgenPath and psfFsn else None
+    if fn and os.path.exists(fn):
+        b = np.load(fn, allow_pickle=False)
+    else:
+        b = omw_backprojector_generation(psf, wienerAlpha, skewed, OTFCumThresh=OTFCumThresh,
+                                         hannWinBounds=list(hannWinBounds), ctx=ctx)[0]
+        b = np.asarray(b, dtype=np.float32)
+        if fn:
+            os.makedirs(psfgenPath, exist_ok=True)
+            tmp = fn[:-4] + f"_{os.getpid()}.npy"  # written aside, then moved in place (RLdecon.m:315-316)
+            np.save(tmp, b, allow_pickle=False)
+            os.replace(tmp, fn)
+    _BP_CACHE[key] = b
+    return b
+
+
+def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, wienerAlpha=0.005,
+                   OTFCumThresh=0.9, hannWinBounds=(0.8, 1.0), skewed=None, flipZstack=False, save16bit=True,
+                   dampFactor=1.0, scaleFactor=None, deconOffset=0.0, deconBbox=None, EdgeErosion=0,
+                   psfgenPath=None, psfFsn=None, ctx=None):
+    """deconvolved = RLdecon(..., 'rawdata', rawdata, ...) for one in-memory volume.
+
+    Follows RLdecon.m: PSF as single (:235), centre-cropped when larger than the
+    data (:264-270); for 'omw' the back projector from the cache or
+    omw_backprojector_generation (:290-325), cropped to the data (:326-332);
+    flipZstack (:346-348); with nIter > 0 and any non-zero voxel, scaleFactor
+    [] -> 1 and PSF ./ sum(PSF) (:367-373), then decon_lucy_function
+    ('simplified') or decon_lucy_omw_function ('omw') with Background,
+    save16bit, dampFactor, scaleFactor, deconOffset, bbox = deconBbox and
+    EdgeErosion (:387-404); otherwise the raw data, cropped to deconBbox
+    (:415-420).  Other methods ('original' deconvlucy, 'cudagen') are not this
+    engine's."""
+    if RLMethod not in ("simplified", "omw"):
+        raise ValueError(f"RLMethod '{RLMethod}': this engine serves 'simplified' and 'omw'")
+    psf = np.asarray(psf.detach().cpu().numpy() if _is_torch(psf) else psf, dtype=np.float32)
+    imSize = tuple(int(n) for n in (rawdata.shape if _is_torch(rawdata) else np.shape(rawdata)))[:3]
+    psf = np.asfortranarray(crop_psf_to_data(psf, imSize))
+    psf_b = None
+    if RLMethod == "omw":
+        psf_b = np.asfortranarray(crop_psf_to_data(
+            _omw_bp(psf, wienerAlpha, skewed, OTFCumThresh, hannWinBounds, psfgenPath, psfFsn, ctx), imSize))
+    if flipZstack:
+        rawdata = rawdata.flip(2) if _is_torch(rawdata) else np.asfortranarray(np.asarray(rawdata)[:, :, ::-1])
+    has_signal = bool((rawdata != 0).any()) if _is_torch(rawdata) else bool(np.any(rawdata))
+    if not (nIter > 0 and has_signal):
+        out = rawdata
+        if deconBbox is not None and len(deconBbox) == 6:
+            b = [int(v) for v in deconBbox]
+            out = out[b[0] - 1:b[3], b[1] - 1:b[4], b[2] - 1:b[5]]
+        return out
+    if scaleFactor is None:
+        scaleFactor = 1.0
+    psf = np.asfortranarray(psf / np.float32(psf.sum(dtype=np.float32)))
+    kw = dict(Background=Background, save16bit=save16bit, dampFactor=dampFactor, scaleFactor=scaleFactor,
+              deconOffset=deconOffset, EdgeErosion=EdgeErosion, pad=True, ctx=ctx)
+    if deconBbox is not None and len(deconBbox) == 6:
+        kw["bbox"] = list(deconBbox)
+    if RLMethod == "simplified":
+        J, _, _ = decon_lucy_function(rawdata, psf, int(nIter), **kw)
+    else:
+        J, _ = decon_lucy_omw_function(rawdata, psf, psf_b, int(nIter), **kw)
+    return J

@@ -1,0 +1,50 @@
+"""RLdecon.m per-volume semantics on the device: 'simplified' and 'omw' dispatch
+to the decon functions with a PSF normalised to unit sum and cropped to the
+data, the OMW back projector generated with the given alpha / threshold and
+cached on disk (a second call loads it and gives the same bits), deconBbox
+and a non-smooth volume size (zero-padded by the engine)."""
+import numpy as np
+import pytest
+
+from petakit5d_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(psf_shape=(15, 15, 21), sig=(1.2, 3.0), shear=20.0, shape=(60, 49, 18)):
+    psf = synth.gaussian_psf(psf_shape, sig[0], sig[1], shear) * 7.0  # not unit sum: the driver normalises
+    g0, g1 = synth.rng(61, 0), synth.rng(61, 1)
+    img = synth._fft_convolve_circular(synth.point_emitters(shape, 30, (500, 3000), g0), psf / psf.sum()) + 100
+    return synth.poisson_u16(img, g1), psf.astype(np.float32)
+
+
+def test_simplified_dispatch_equals_decon_lucy_function():
+    import petakit5d_amd as pk
+    raw, psf = _case()  # z = 18 < PSF z = 21: the PSF is centre-cropped; 49 is not 2/3/5-smooth
+    J = pk.RLdecon_volume(raw, psf, 4, RLMethod="simplified", Background=100, save16bit=False,
+                          deconBbox=[3, 2, 2, 50, 40, 16])
+    pc = pk.crop_psf_to_data(psf, raw.shape)
+    R, _, _ = pk.decon_lucy_function(raw, pc / pc.sum(dtype=np.float32), 4, Background=100, save16bit=False,
+                                     bbox=[3, 2, 2, 50, 40, 16], pad=True)
+    np.testing.assert_array_equal(J, R)
+
+
+def test_omw_dispatch_and_back_projector_cache(tmp_path):
+    import petakit5d_amd as pk
+    from petakit5d_amd import rldecon
+    raw, psf = _case((24, 24, 31), (1.0, 2.5), 0.0, (60, 49, 36))
+    rldecon._BP_CACHE.clear()
+    J1 = pk.RLdecon_volume(raw, psf, 2, RLMethod="omw", Background=100, wienerAlpha=0.01, OTFCumThresh=0.85,
+                           psfgenPath=str(tmp_path), psfFsn="psfA")
+    fn = rldecon.omw_bp_filename(str(tmp_path), "psfA", 0.01, 0.85)
+    b = np.load(fn)
+    pc = pk.crop_psf_to_data(psf, raw.shape)
+    b_ref = pk.omw_backprojector_generation(pc, 0.01, None, OTFCumThresh=0.85)[0]
+    np.testing.assert_array_equal(b, b_ref)
+    R, _ = pk.decon_lucy_omw_function(raw, pc / pc.sum(dtype=np.float32), pk.crop_psf_to_data(b, raw.shape), 2,
+                                      Background=100, pad=True)
+    np.testing.assert_array_equal(J1, R)
+    rldecon._BP_CACHE.clear()  # second call: the back projector comes from the file
+    J2 = pk.RLdecon_volume(raw, psf, 2, RLMethod="omw", Background=100, wienerAlpha=0.01, OTFCumThresh=0.85,
+                           psfgenPath=str(tmp_path), psfFsn="psfA")
+    np.testing.assert_array_equal(J2, J1)

@@ -1,0 +1,27 @@
+"""RLdecon.m per-volume semantics without the device: the back projector cache
+name (RLdecon.m:292) and the no-decon branch (nIter = 0 or an all-zero volume
+returns the raw data, cropped to deconBbox; :415-420)."""
+import numpy as np
+import pytest
+
+from petakit5d_amd.rldecon import RLdecon_volume, omw_bp_filename
+
+
+def test_bp_cache_name_matches_reference_format():
+    # sprintf('%s/%s_back_projector_alpha_%0.6f_otf_cum_thresh_%0.6f.tif', psfgenPath, psfFsn, 0.005, 0.9)
+    fn = omw_bp_filename("/data/psfgen", "psf_488", 0.005, 0.9)
+    assert fn == "/data/psfgen/psf_488_back_projector_alpha_0.005000_otf_cum_thresh_0.900000.npy"
+
+
+def test_no_iterations_returns_raw_cropped():
+    raw = np.asfortranarray(np.arange(6 * 5 * 4, dtype=np.uint16).reshape((6, 5, 4), order="F"))
+    psf = np.ones((3, 3, 3), np.float32)
+    out = RLdecon_volume(raw, psf, 0, RLMethod="simplified", deconBbox=[2, 1, 2, 5, 4, 3])
+    np.testing.assert_array_equal(out, raw[1:5, 0:4, 1:3])
+    zero = np.zeros((6, 5, 4), np.uint16)
+    assert RLdecon_volume(zero, psf, 5, RLMethod="simplified") is zero
+
+
+def test_unserved_method_rejected():
+    with pytest.raises(ValueError):
+        RLdecon_volume(np.ones((4, 4, 4), np.uint16), np.ones((3, 3, 3), np.float32), 2, RLMethod="cudagen")
