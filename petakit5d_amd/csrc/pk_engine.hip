@@ -188,8 +188,12 @@ struct Tune {
     // PK_CHUNK_STREAMS: streams of a batched step (1..8); measured on 64 chunks
     // (OMW-2 / Biggs-10 Gvoxel/s): 4 -> 17.8 / 3.28, 6 -> 18.0 / 3.38, 8 -> 18.3 / 3.42
     int streams = 8;
+    // PK_TUNE_COL_CANDS: strided-axis shapes the autotuner times per axis, best-ranked first
+    // (64 chunks, OMW-2 Gvoxel/s: 4 -> 17.5, 8 -> 17.0)
+    int col_cands = 4;
     Tune() {
         if (const char* v = std::getenv("PK_CHUNK_STREAMS")) streams = std::max(1, std::min(8, std::atoi(v)));
+        if (const char* v = std::getenv("PK_TUNE_COL_CANDS")) col_cands = std::max(1, std::min(16, std::atoi(v)));
         if (const char* v = std::getenv("PK_SLAB_PLANES")) slab = std::atoi(v);
         if (const char* v = std::getenv("PK_GRAPHS")) graphs = std::atoi(v) != 0;
         if (const char* v = std::getenv("PK_XCD_REMAP")) xcd = std::atoi(v);
@@ -683,7 +687,7 @@ std::string pk_ctx_impl::autotune(int64_t n1, int64_t n2, int64_t n3) {
         tf[i] = tc[i] = 1e30f;
         fwd[i] = conv[i] = axis == 2 ? vp.y : vp.z;
         std::vector<AxisLaunch> cands = col_candidates(L, axis - 1);
-        if (cands.size() > 4) cands.resize(4);
+        if (cands.size() > (size_t)tune().col_cands) cands.resize(tune().col_cands);
         bool first = true;
         for (AxisLaunch c : cands) {
             c.fp.tw = tw;
