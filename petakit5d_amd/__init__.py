@@ -9,6 +9,7 @@ from .backprojector import BackProjector, omw_backprojector_generation  # noqa: 
 from .batch import decon_batch  # noqa: F401
 from .chunks import XR_zarrChunkCoordinatesExtraction, decon_large_in_memory  # noqa: F401
 from .rldecon import RLdecon_volume  # noqa: F401
+from .zarr_io import decon_zarr, read_zarr, write_zarr  # noqa: F401
 from .serve import Server  # noqa: F401
 from .decon import (  # noqa: F401
     crop_psf_to_data,
@@ -35,6 +36,9 @@ __all__ = [
     "XR_zarrChunkCoordinatesExtraction",
     "decon_large_in_memory",
     "RLdecon_volume",
+    "read_zarr",
+    "write_zarr",
+    "decon_zarr",
     "Server",
     "crop_psf_to_data",
     "fft_r2c",
