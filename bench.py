@@ -611,8 +611,9 @@ def wl_copy(args, torch, pk, L):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(volumes are independent units; see DESIGN.md on sharding)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported. Volumes and chunks are "
+              "independent units: a multi-GPU deployment runs one process and context per GPU, each "
+              "with its own server, and assigns whole requests or chunks to GPUs", file=sys.stderr)
         sys.exit(2)
     import torch
 
