@@ -8,6 +8,7 @@ from ._lib import Context, PkDeconError, default_context, load  # noqa: F401
 from .backprojector import BackProjector, omw_backprojector_generation  # noqa: F401
 from .batch import decon_batch  # noqa: F401
 from .chunks import XR_zarrChunkCoordinatesExtraction, decon_large_in_memory  # noqa: F401
+from .psfgen import psf_gen_new  # noqa: F401
 from .rldecon import RLdecon_volume  # noqa: F401
 from .zarr_io import decon_zarr, read_zarr, write_zarr  # noqa: F401
 from .serve import Server  # noqa: F401
@@ -36,6 +37,7 @@ __all__ = [
     "XR_zarrChunkCoordinatesExtraction",
     "decon_large_in_memory",
     "RLdecon_volume",
+    "psf_gen_new",
     "read_zarr",
     "write_zarr",
     "decon_zarr",

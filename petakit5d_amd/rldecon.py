@@ -15,9 +15,10 @@ import os
 import numpy as np
 
 from .backprojector import omw_backprojector_generation
+from .psfgen import psf_gen_new
 from .decon import _is_torch, crop_psf_to_data, decon_lucy_function, decon_lucy_omw_function
 
-__all__ = ["RLdecon_volume", "omw_bp_filename"]
+__all__ = ["RLdecon_volume", "omw_bp_filename", "psfgen_crop"]
 
 _BP_CACHE: dict = {}
 
@@ -53,10 +54,28 @@ def _omw_bp(psf, wienerAlpha, skewed, OTFCumThresh, hannWinBounds, psfgenPath, p
     return b
 
 
+def psfgen_crop(psf):
+    """Crop of a generated PSF to its non-zero extent plus a one-voxel margin,
+    the same on both sides of every axis so the centre stays put
+    (RLdecon.m:250-260)."""
+    p = np.asarray(psf)
+    ps = np.array(p.shape[:3])
+    crop = []
+    for ax in range(3):
+        other = tuple(i for i in range(3) if i != ax)
+        nzi = np.nonzero(p.sum(axis=other))[0]
+        if nzi.size == 0:
+            crop.append(0)
+            continue
+        crop.append(min(int(nzi[0]), int(ps[ax] - 1 - nzi[-1])) - 1)
+    c = np.maximum(0, np.array(crop))
+    return p[c[0]:ps[0] - c[0], c[1]:ps[1] - c[1], c[2]:ps[2] - c[2]]
+
+
 def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, wienerAlpha=0.005,
                    OTFCumThresh=0.9, hannWinBounds=(0.8, 1.0), skewed=None, flipZstack=False, save16bit=True,
                    dampFactor=1.0, scaleFactor=None, deconOffset=0.0, deconBbox=None, EdgeErosion=0,
-                   psfgenPath=None, psfFsn=None, ctx=None):
+                   psfgenPath=None, psfFsn=None, psfGen=False, dzPSF=None, dz=None, ctx=None):
     """deconvolved = RLdecon(..., 'rawdata', rawdata, ...) for one in-memory volume.
 
     Follows RLdecon.m: PSF as single (:235), centre-cropped when larger than the
@@ -67,11 +86,19 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     ('simplified') or decon_lucy_omw_function ('omw') with Background,
     save16bit, dampFactor, scaleFactor, deconOffset, bbox = deconBbox and
     EdgeErosion (:387-404); otherwise the raw data, cropped to deconBbox
-    (:415-420).  Other methods ('original' deconvlucy, 'cudagen') are not this
+    (:415-420).  psfGen: the PSF first goes through psf_gen_new(psf, dzPSF,
+    dz, 1.5, 'masked') and, for methods other than 'omw', psfgen_crop
+    (:240-260).  Other methods ('original' deconvlucy, 'cudagen') are not this
     engine's."""
     if RLMethod not in ("simplified", "omw"):
         raise ValueError(f"RLMethod '{RLMethod}': this engine serves 'simplified' and 'omw'")
     psf = np.asarray(psf.detach().cpu().numpy() if _is_torch(psf) else psf, dtype=np.float32)
+    if psfGen:
+        if dzPSF is None or dz is None:
+            raise ValueError("psfGen needs dzPSF and dz")
+        psf = psf_gen_new(psf, dzPSF, dz, 1.5, "masked")
+        if RLMethod != "omw":
+            psf = np.ascontiguousarray(psfgen_crop(psf))
     imSize = tuple(int(n) for n in (rawdata.shape if _is_torch(rawdata) else np.shape(rawdata)))[:3]
     psf = np.asfortranarray(crop_psf_to_data(psf, imSize))
     psf_b = None

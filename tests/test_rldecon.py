@@ -4,7 +4,7 @@ returns the raw data, cropped to deconBbox; :415-420)."""
 import numpy as np
 import pytest
 
-from petakit5d_amd.rldecon import RLdecon_volume, omw_bp_filename
+from petakit5d_amd.rldecon import RLdecon_volume, omw_bp_filename, psfgen_crop
 
 
 def test_bp_cache_name_matches_reference_format():
@@ -25,3 +25,13 @@ def test_no_iterations_returns_raw_cropped():
 def test_unserved_method_rejected():
     with pytest.raises(ValueError):
         RLdecon_volume(np.ones((4, 4, 4), np.uint16), np.ones((3, 3, 3), np.float32), 2, RLMethod="cudagen")
+
+
+def test_psfgen_crop_keeps_the_centre():
+    # non-zero extent y 3..8 of 12 (0-based), x 5..6 of 9, z all of 7: crop min(first, n - 1 - last) - 1
+    # per axis on both sides -> y 2, x 1, z 0
+    p = np.zeros((12, 9, 7), np.float32)
+    p[3:9, 5:7, :] = 1
+    c = psfgen_crop(p)
+    assert c.shape == (8, 7, 7)
+    np.testing.assert_array_equal(c, p[2:10, 1:8, :])
