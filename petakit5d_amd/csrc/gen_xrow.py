@@ -1,6 +1,8 @@
 #!/usr/bin/env python3
 """Writes pk_kern_xrow_*.hip: the (row length L, values per thread E) pairs
-that get a compile-time row kernel, one per 2/3/5-smooth L in [32, 1024] that a rule below covers.
+that get a compile-time row kernel, one per 2/3/5-smooth L in [32, 1024] that a rule below covers
+(even rows of n1 = 2L), and one per odd 2/3/5-smooth n1 in [15, 1023] (odd rows, key -n1,
+transformed at full length).
 
 The first rule keeps every stage on home registers: E holds every prime of L
 and P = L / E is a power of two dividing 256 (whole 256-thread blocks of
@@ -98,6 +100,12 @@ def main():
             if E:
                 pairs.append((L, E))
     pairs += [p for p in EXTRA if p not in pairs]
+    # odd rows (n1 = L odd, full-length complex transform): keyed -L
+    for L in range(15, 1024, 2):
+        if smooth(L):
+            E = pick(L)
+            if E:
+                pairs.append((-L, E))
     here = os.path.dirname(os.path.abspath(__file__))
     for f in os.listdir(here):
         if f.startswith("pk_kern_xrow_") and f.endswith(".hip"):
@@ -108,10 +116,12 @@ def main():
     for gi, grp in enumerate(groups):
         tag = f"{gi:02d}"
         names.append(tag)
-        body = "\n".join(f"    if (L == {L} && E == {E}) {{\n        launch_modes_xrow<{L}, {E}>(grid, lds, s, a, tw);\n"
+        def targs(L, E):
+            return f"{L}, {E}" if L > 0 else f"{-L}, {E}, true"
+        body = "\n".join(f"    if (L == {L} && E == {E}) {{\n        launch_modes_xrow<{targs(L, E)}>(grid, lds, s, a, tw);\n"
                          f"        return true;\n    }}" for L, E in grp)
-        bodyb = "\n".join(f"    if (L == {L} && E == {E}) {{\n        launch_modes_xrowb<{L}, {E}>(grid, lds, s, mode, ch, "
-                          f"starts, n);\n        return true;\n    }}" for L, E in grp)
+        bodyb = "\n".join(f"    if (L == {L} && E == {E}) {{\n        launch_modes_xrowb<{targs(L, E)}>(grid, lds, s, mode, "
+                          f"ch, starts, n);\n        return true;\n    }}" for L, E in grp)
         avail = " || ".join(f"(L == {L} && E == {E})" for L, E in grp)
         lst = ", ".join(f"{L}/{E}" for L, E in grp)
         with open(os.path.join(here, f"pk_kern_xrow_{tag}.hip"), "w") as fh:

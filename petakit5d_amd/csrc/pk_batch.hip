@@ -271,7 +271,8 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
     auto add_x = [&](Pass<XChunk>& ps, const Chunk& c, const XArgs& a) {
         const VolPlan& vp = *c.vp;
         // chunks whose rows have a compile-time kernel are grouped by row length
-        const int ctL = (vp.packed && vp.x.block.x == 256 && xrow_available(vp.x.fp.L, vp.x.fp.E)) ? vp.x.fp.L : 0;
+        const int ctk = vp.packed ? vp.x.fp.L : -vp.x.fp.L;  // compile-time kernel key (odd rows negative)
+        const int ctL = (vp.x.block.x == 256 && xrow_available(ctk, vp.x.fp.E)) ? ctk : 0;
         Group<XChunk>& g = ps[{vp.x.fp.E, a.mode, 256, ctL, (int64_t)vp.x.block.x, (int64_t)vp.x.lds}];
         g.E = vp.x.fp.E;
         g.mode = a.mode;

@@ -321,7 +321,7 @@ static std::vector<AxisLaunch> x_candidates(int Lx, bool packed) {
     std::vector<std::pair<int, FftPlan>> ct, gen;
     for (int E : kEs) {
         if (Lx % E || Lx / E > kMaxThreads) continue;
-        if (packed && !tune().generic_x && xrow_available(Lx, E)) {
+        if (!tune().generic_x && xrow_available(packed ? Lx : -Lx, E)) {
             const FftPlan fp = ct_row_plan(Lx, E);
             ct.emplace_back(fp.nst * 10 + 4 * std::abs(E - 8), fp);
         }
@@ -507,7 +507,8 @@ struct Wisdom {
 // applied to a plan; false (plan unchanged) when a shape is infeasible.
 bool pk_ctx_impl::apply_shapes(VolPlan& vp, const std::array<int, 8>& v) {
     const int Lx = (int)(vp.packed ? vp.n1 / 2 : vp.n1);
-    const bool ctx = vp.packed && !tune().generic_x && v[0] > 0 && Lx % v[0] == 0 && xrow_available(Lx, v[0]);
+    const bool ctx =
+        !tune().generic_x && v[0] > 0 && Lx % v[0] == 0 && xrow_available(vp.packed ? Lx : -Lx, v[0]);
     if (!ctx && (v[0] < 1 || Lx % v[0] != 0)) return false;
     const FftPlan fx = ctx ? ct_row_plan(Lx, v[0]) : plan_1d(Lx, v[0], kMaxThreads, v[0], 10, 4);
     AxisLaunch ay, az;
@@ -827,8 +828,9 @@ void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a, int64_t z0, int64_t nz) {
     hipEvent_t e0 = prof_mark();
     // compile-time kernel when one exists for this (row length, values per
     // thread) and the tile is the full 256-thread block it assumes
-    if (vp.packed && vp.x.block.x == (unsigned)kMaxThreads && !tune().generic_x &&
-        launch_xrow(vp.x.fp.L, vp.x.fp.E, dim3((unsigned)nblk), vp.x.lds, stream, a, vp.x.fp.tw))
+    if (vp.x.block.x == (unsigned)kMaxThreads && !tune().generic_x &&
+        launch_xrow(vp.packed ? vp.x.fp.L : -vp.x.fp.L, vp.x.fp.E, dim3((unsigned)nblk), vp.x.lds, stream, a,
+                    vp.x.fp.tw))
         PK_HIP(hipGetLastError());
     else
         PK_HIP(launch_xpass(vp.x.fp, dim3((unsigned)nblk), vp.x.block, vp.x.lds, stream, a));

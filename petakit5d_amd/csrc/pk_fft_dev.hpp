@@ -756,11 +756,13 @@ __host__ __device__ constexpr int xrow_lds_rows(int P) {
     return xrow_tile(P) + ((P <= 64 && 64 % P != 0) ? kMaxThreads / 64 : 0);
 }
 
-// The same pass for an even row of n1 = 2L real values with L and E fixed at
+// The same pass for an even row of n1 = 2L real values (ODD: an odd row of
+// n1 = L real values, transformed at full length as complex values with zero
+// imaginary parts, as the runtime-plan kernel does) with L and E fixed at
 // compile time and 256-thread blocks of xrow_tile(P) rows.  Rows past the end
 // of the volume (and the left-over lanes) read the last row (in bounds) and
 // store nothing; every wave runs the same instruction stream.
-template <int L, int E, int MODE_>
+template <int L, int E, int MODE_, bool ODD = false>
 __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restrict__ tw, float2* lds, int64_t blk) {
     constexpr int MODE = xm_base(MODE_);
     constexpr bool EPI = MODE_ != MODE;  // fused epilogue
@@ -771,7 +773,8 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
     constexpr int P = L / E;
     constexpr int RT = xrow_tile(P);
     constexpr int LP = row_pitch(L);
-    constexpr int HP = (L + 1 + 15) / 16 * 16;  // spectrum row pitch for h1 = L + 1
+    constexpr int H1 = ODD ? (L + 1) / 2 : L + 1;  // half-spectrum length of the row
+    constexpr int HP = (H1 + 15) / 16 * 16;       // its pitch
     static_assert(P * E == L && (P <= 64 || RT * P == kMaxThreads), "row tile");
     const int t = threadIdx.x;
     int rr, p;
@@ -793,7 +796,16 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
     const int lbase = rr * LP;
     float2* __restrict__ sp = a.spec + rowc * HP;
     const float2* __restrict__ rt = a.rtw + p;
-    const int64_t r2 = rowc * L;  // real row start in float2 (pairs)
+    const int64_t r2 = rowc * L;  // real row start: float2 pairs (even rows) or floats (ODD)
+    // real-space value(s) m of the row: a pair, or one value in .x (ODD)
+    auto ldr = [&](const float* __restrict__ base, int m) -> float2 {
+        if constexpr (ODD) return make_float2(base[r2 + m], 0.f);
+        else return reinterpret_cast<const float2*>(base)[r2 + m];
+    };
+    auto str = [&](float* __restrict__ base, int m, float2 o) {
+        if constexpr (ODD) base[r2 + m] = o.x;
+        else reinterpret_cast<float2*>(base)[r2 + m] = o;
+    };
 
     RawRow rr_src{0, false};
     if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
@@ -806,28 +818,29 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
             const int m = p + e * P;
             opA[e] = make_float2(0.f, 0.f);
             opB[e] = make_float2(0.f, 0.f);
-            if constexpr (MODE == XM_R2C) opA[e] = reinterpret_cast<const float2*>(a.src)[r2 + m];
+            if constexpr (MODE == XM_R2C) opA[e] = ldr(a.src, m);
             if constexpr (MODE == XM_PREP_U16) {
                 if (a.s0) {
-                    opA[e] = ld_raw_padded(a.raw16, a, rr_src, 2 * (int64_t)m, true);
+                    opA[e] = ld_raw_padded(a.raw16, a, rr_src, ODD ? (int64_t)m : 2 * (int64_t)m, !ODD);
+                } else if constexpr (ODD) {
+                    opA[e] = make_float2((float)a.raw16[r2 + m], 0.f);
                 } else {
                     const uint32_t w = reinterpret_cast<const uint32_t*>(a.raw16)[r2 + m];
                     opA[e] = make_float2((float)(w & 0xffffu), (float)(w >> 16));
                 }
             }
             if constexpr (MODE == XM_PREP_F32)
-                opA[e] = a.s0 ? ld_raw_padded(a.rawf, a, rr_src, 2 * (int64_t)m, true)
-                              : reinterpret_cast<const float2*>(a.rawf)[r2 + m];
-            if constexpr (MODE == XM_RATIO) opA[e] = reinterpret_cast<const float2*>(a.I)[r2 + m];
-            if constexpr (MODE == XM_OMW_UPDATE)
-                opA[e] = reinterpret_cast<const float2*>(a.Jin ? a.Jin : a.J)[r2 + m];
+                opA[e] = a.s0 ? ld_raw_padded(a.rawf, a, rr_src, ODD ? (int64_t)m : 2 * (int64_t)m, !ODD)
+                              : ldr(a.rawf, m);
+            if constexpr (MODE == XM_RATIO) opA[e] = ldr(a.I, m);
+            if constexpr (MODE == XM_OMW_UPDATE) opA[e] = ldr(a.Jin ? a.Jin : a.J, m);
             if constexpr (MODE == XM_BIGGS_Y) {
-                opA[e] = reinterpret_cast<const float2*>(a.J)[r2 + m];
-                opB[e] = reinterpret_cast<const float2*>(a.J3)[r2 + m];
+                opA[e] = ldr(a.J, m);
+                opB[e] = ldr(a.J3, m);
             }
             if constexpr (MODE == XM_BIGGS_UPDATE) {
-                opA[e] = reinterpret_cast<const float2*>(a.Y)[r2 + m];
-                if (!EPI && a.acc) opB[e] = reinterpret_cast<const float2*>(a.J4)[r2 + m];
+                opA[e] = ldr(a.Y, m);
+                if (!EPI && a.acc) opB[e] = ldr(a.J4, m);
             }
         }
     };
@@ -840,7 +853,8 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int m = p + e * P;
-            v[e] = c2r_pre(sp[m], cconj(sp[L - m]), rt[e * P]);
+            if constexpr (ODD) v[e] = m < H1 ? sp[m] : cconj(sp[L - m]);  // the Hermitian half supplies the rest
+            else v[e] = c2r_pre(sp[m], cconj(sp[L - m]), rt[e * P]);
         }
         ct_fft<L, E, true>(v, p, lds, lbase, tw);
         if constexpr (kXrowLateOperands) load_operands();
@@ -858,46 +872,56 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
         const int m = p + e * P;
         float2 o = make_float2(0.f, 0.f);
         if constexpr (MODE == XM_C2R) {
-            if (valid) reinterpret_cast<float2*>(a.dst)[r2 + m] = v[e];
+            if (valid) str(a.dst, m, v[e]);
         } else {
             o = x_elem<MODE>(v[e], opA[e], opB[e], a.bg, a.eps, lam);
         }
         if (valid) {
             if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
-                reinterpret_cast<float2*>(a.I)[r2 + m] = o;
-                if (a.J) reinterpret_cast<float2*>(a.J)[r2 + m] = o;
+                str(a.I, m, o);
+                if (a.J) str(a.J, m, o);
             } else if constexpr (MODE == XM_OMW_UPDATE) {
                 if constexpr (EPI) {
-                    if (er.keep) epi_put(a.ep, er, 2 * (int64_t)m, true, o);
+                    if (er.keep) epi_put(a.ep, er, ODD ? (int64_t)m : 2 * (int64_t)m, !ODD, o);
                 } else {
-                    reinterpret_cast<float2*>(a.J)[r2 + m] = o;
+                    str(a.J, m, o);
                 }
             } else if constexpr (MODE == XM_BIGGS_Y) {
-                reinterpret_cast<float2*>(a.Y)[r2 + m] = o;
+                str(a.Y, m, o);
             } else if constexpr (MODE == XM_BIGGS_UPDATE) {
                 const float2 y = opA[e];
                 if constexpr (EPI) {
-                    if (er.keep) epi_put(a.ep, er, 2 * (int64_t)m, true, o);
+                    if (er.keep) epi_put(a.ep, er, ODD ? (int64_t)m : 2 * (int64_t)m, !ODD, o);
                 } else {
-                    reinterpret_cast<float2*>(a.Jn)[r2 + m] = o;
+                    str(a.Jn, m, o);
                 }
                 const float2 d = make_float2(o.x - y.x, o.y - y.y);
                 if (!EPI && a.acc) {
                     const float2 j4 = opB[e];
                     s_num += (double)(d.x * j4.x);
                     s_den += (double)(j4.x * j4.x);
-                    s_num += (double)(d.y * j4.y);
-                    s_den += (double)(j4.y * j4.y);
+                    if constexpr (!ODD) {
+                        s_num += (double)(d.y * j4.y);
+                        s_den += (double)(j4.y * j4.y);
+                    }
                 }
-                if (!EPI && a.write_j4) reinterpret_cast<float2*>(a.J4)[r2 + m] = d;
+                if (!EPI && a.write_j4) str(a.J4, m, d);
             }
         }
+        if constexpr (ODD) o.y = 0.f;  // the forward transform input is real
         v[e] = o;
     }
 
     bool out_spec = OUT_ALWAYS;
     if constexpr (OUT_MAYBE) out_spec = !EPI && a.r2c_out != 0;
-    if (out_spec) {
+    if (out_spec && ODD) {
+        ct_fft<L, E, false>(v, p, lds, lbase, tw);
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (p + e * P < H1) sp[p + e * P] = v[e];
+        }
+    } else if (out_spec) {
         ct_fft<L, E, false>(v, p, lds, lbase, tw);
 #pragma unroll
         for (int e = 0; e < E; ++e) lds[lds_slot<true>(lbase, p + e * P, 1)] = v[e];
@@ -940,10 +964,10 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
     }
 }
 
-template <int L, int E, int MODE>
+template <int L, int E, int MODE, bool ODD = false>
 __global__ void __launch_bounds__(kMaxThreads) k_xrow(XArgs a, const float2* __restrict__ tw) {
     extern __shared__ float2 lds[];
-    xrow_body<L, E, MODE>(a, tw, lds, blockIdx.x);
+    xrow_body<L, E, MODE, ODD>(a, tw, lds, blockIdx.x);
 }
 
 // One kernel per (E, mode) so each mode gets its own register allocation.
@@ -984,13 +1008,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_xpass_batch(const XChunk* __res
 
 // Batched launch of the compile-time row kernel: every chunk of the group has
 // rows of length 2L (same instantiation, same arithmetic as a single call).
-template <int L, int E, int MODE>
+template <int L, int E, int MODE, bool ODD = false>
 __global__ void __launch_bounds__(kMaxThreads) k_xrow_batch(const XChunk* __restrict__ ch,
                                                             const int64_t* __restrict__ starts, int n) {
     extern __shared__ float2 lds[];
     const int c = chunk_of(starts, n, blockIdx.x);
     const XArgs a = ch[c].a;
-    xrow_body<L, E, MODE>(a, ch[c].pl.tw, lds, (int64_t)blockIdx.x - starts[c]);
+    xrow_body<L, E, MODE, ODD>(a, ch[c].pl.tw, lds, (int64_t)blockIdx.x - starts[c]);
 }
 
 // ------------------------------------------------------------------ strided axes

@@ -1,5 +1,5 @@
 // pk_kern_xrow_09.hip -- compile-time row-transform kernels, single-volume and batched
-// launches, for (row length / values per thread) 1024/8, 256/4, 256/16, 128/4, 512/16.  Written by gen_xrow.py.
+// launches, for (row length / values per thread) 1024/8, 256/4, 256/16, 128/4, 512/16, -15/15.  Written by gen_xrow.py.
 #include "pk_launch.hpp"
 
 namespace pk {
@@ -22,6 +22,10 @@ bool launch_xrow_09(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XA
     }
     if (L == 512 && E == 16) {
         launch_modes_xrow<512, 16>(grid, lds, s, a, tw);
+        return true;
+    }
+    if (L == -15 && E == 15) {
+        launch_modes_xrow<15, 15, true>(grid, lds, s, a, tw);
         return true;
     }
     return false;
@@ -48,7 +52,11 @@ bool launch_xrowb_09(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mod
         launch_modes_xrowb<512, 16>(grid, lds, s, mode, ch, starts, n);
         return true;
     }
+    if (L == -15 && E == 15) {
+        launch_modes_xrowb<15, 15, true>(grid, lds, s, mode, ch, starts, n);
+        return true;
+    }
     return false;
 }
-bool xrow_available_09(int L, int E) { return (L == 1024 && E == 8) || (L == 256 && E == 4) || (L == 256 && E == 16) || (L == 128 && E == 4) || (L == 512 && E == 16); }
+bool xrow_available_09(int L, int E) { return (L == 1024 && E == 8) || (L == 256 && E == 4) || (L == 256 && E == 16) || (L == 128 && E == 4) || (L == 512 && E == 16) || (L == -15 && E == 15); }
 }  // namespace pk

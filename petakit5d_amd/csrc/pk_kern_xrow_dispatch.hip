@@ -1,5 +1,5 @@
 // pk_kern_xrow_dispatch.hip -- dispatch over the compile-time row kernels listed in
-// pk_kern_xrow_NN.hip (59 (L, E) pairs).  Written by gen_xrow.py.
+// pk_kern_xrow_NN.hip (71 (L, E) pairs).  Written by gen_xrow.py.
 #include "pk_launch.hpp"
 
 namespace pk {
@@ -43,6 +43,14 @@ bool launch_xrow_09(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XA
 bool launch_xrowb_09(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
                       const int64_t* starts, int n);
 bool xrow_available_09(int L, int E);
+bool launch_xrow_10(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
+bool launch_xrowb_10(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                      const int64_t* starts, int n);
+bool xrow_available_10(int L, int E);
+bool launch_xrow_11(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
+bool launch_xrowb_11(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
+                      const int64_t* starts, int n);
+bool xrow_available_11(int L, int E);
 
 bool launch_xrow(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw) {
     return launch_xrow_00(L, E, grid, lds, s, a, tw) ||
@@ -54,7 +62,9 @@ bool launch_xrow(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs
            launch_xrow_06(L, E, grid, lds, s, a, tw) ||
            launch_xrow_07(L, E, grid, lds, s, a, tw) ||
            launch_xrow_08(L, E, grid, lds, s, a, tw) ||
-           launch_xrow_09(L, E, grid, lds, s, a, tw);
+           launch_xrow_09(L, E, grid, lds, s, a, tw) ||
+           launch_xrow_10(L, E, grid, lds, s, a, tw) ||
+           launch_xrow_11(L, E, grid, lds, s, a, tw);
 }
 bool launch_xrow_batch(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
                        const int64_t* starts, int n) {
@@ -67,7 +77,9 @@ bool launch_xrow_batch(int L, int E, dim3 grid, size_t lds, hipStream_t s, int m
            launch_xrowb_06(L, E, grid, lds, s, mode, ch, starts, n) ||
            launch_xrowb_07(L, E, grid, lds, s, mode, ch, starts, n) ||
            launch_xrowb_08(L, E, grid, lds, s, mode, ch, starts, n) ||
-           launch_xrowb_09(L, E, grid, lds, s, mode, ch, starts, n);
+           launch_xrowb_09(L, E, grid, lds, s, mode, ch, starts, n) ||
+           launch_xrowb_10(L, E, grid, lds, s, mode, ch, starts, n) ||
+           launch_xrowb_11(L, E, grid, lds, s, mode, ch, starts, n);
 }
 bool xrow_available(int L, int E) {
     return xrow_available_00(L, E) ||
@@ -79,6 +91,8 @@ bool xrow_available(int L, int E) {
            xrow_available_06(L, E) ||
            xrow_available_07(L, E) ||
            xrow_available_08(L, E) ||
-           xrow_available_09(L, E);
+           xrow_available_09(L, E) ||
+           xrow_available_10(L, E) ||
+           xrow_available_11(L, E);
 }
 }  // namespace pk

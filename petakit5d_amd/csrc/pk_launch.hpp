@@ -58,41 +58,41 @@ hipError_t launch_colpass_batch(int E, int nt, bool mixed, dim3 grid, dim3 block
 hipError_t launch_colpass(const FftPlan& pl, int nt, dim3 grid, dim3 block, size_t lds, hipStream_t s,
                           const ColArgs& a);
 bool e_supported(int E);
-template <int L, int E>
+template <int L, int E, bool ODD = false>
 inline void launch_modes_xrow(dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw) {
     const dim3 b(kMaxThreads);
     switch (a.mode) {
-        case XM_R2C: hipLaunchKernelGGL((k_xrow<L, E, XM_R2C>), grid, b, lds, s, a, tw); break;
-        case XM_C2R: hipLaunchKernelGGL((k_xrow<L, E, XM_C2R>), grid, b, lds, s, a, tw); break;
-        case XM_PREP_U16: hipLaunchKernelGGL((k_xrow<L, E, XM_PREP_U16>), grid, b, lds, s, a, tw); break;
-        case XM_PREP_F32: hipLaunchKernelGGL((k_xrow<L, E, XM_PREP_F32>), grid, b, lds, s, a, tw); break;
-        case XM_RATIO: hipLaunchKernelGGL((k_xrow<L, E, XM_RATIO>), grid, b, lds, s, a, tw); break;
-        case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xrow<L, E, XM_OMW_UPDATE>), grid, b, lds, s, a, tw); break;
-        case XM_BIGGS_Y: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_Y>), grid, b, lds, s, a, tw); break;
-        case XM_BIGGS_UPDATE: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_UPDATE>), grid, b, lds, s, a, tw); break;
-        case XM_OMW_FINAL: hipLaunchKernelGGL((k_xrow<L, E, XM_OMW_FINAL>), grid, b, lds, s, a, tw); break;
-        case XM_BIGGS_FINAL: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_FINAL>), grid, b, lds, s, a, tw); break;
+        case XM_R2C: hipLaunchKernelGGL((k_xrow<L, E, XM_R2C, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_C2R: hipLaunchKernelGGL((k_xrow<L, E, XM_C2R, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_PREP_U16: hipLaunchKernelGGL((k_xrow<L, E, XM_PREP_U16, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_PREP_F32: hipLaunchKernelGGL((k_xrow<L, E, XM_PREP_F32, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_RATIO: hipLaunchKernelGGL((k_xrow<L, E, XM_RATIO, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xrow<L, E, XM_OMW_UPDATE, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_BIGGS_Y: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_Y, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_BIGGS_UPDATE: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_UPDATE, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_OMW_FINAL: hipLaunchKernelGGL((k_xrow<L, E, XM_OMW_FINAL, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_BIGGS_FINAL: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_FINAL, ODD>), grid, b, lds, s, a, tw); break;
         default: break;
     }
 }
-template <int L, int E>
+template <int L, int E, bool ODD = false>
 inline void launch_modes_xrowb(dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch, const int64_t* starts,
                                int n) {
     const dim3 b(kMaxThreads);
     switch (mode) {
-        case XM_R2C: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_R2C>), grid, b, lds, s, ch, starts, n); break;
-        case XM_C2R: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_C2R>), grid, b, lds, s, ch, starts, n); break;
-        case XM_PREP_U16: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_PREP_U16>), grid, b, lds, s, ch, starts, n); break;
-        case XM_PREP_F32: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_PREP_F32>), grid, b, lds, s, ch, starts, n); break;
-        case XM_RATIO: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_RATIO>), grid, b, lds, s, ch, starts, n); break;
-        case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_OMW_UPDATE>), grid, b, lds, s, ch, starts, n); break;
-        case XM_BIGGS_Y: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_Y>), grid, b, lds, s, ch, starts, n); break;
+        case XM_R2C: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_R2C, ODD>), grid, b, lds, s, ch, starts, n); break;
+        case XM_C2R: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_C2R, ODD>), grid, b, lds, s, ch, starts, n); break;
+        case XM_PREP_U16: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_PREP_U16, ODD>), grid, b, lds, s, ch, starts, n); break;
+        case XM_PREP_F32: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_PREP_F32, ODD>), grid, b, lds, s, ch, starts, n); break;
+        case XM_RATIO: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_RATIO, ODD>), grid, b, lds, s, ch, starts, n); break;
+        case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_OMW_UPDATE, ODD>), grid, b, lds, s, ch, starts, n); break;
+        case XM_BIGGS_Y: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_Y, ODD>), grid, b, lds, s, ch, starts, n); break;
         case XM_BIGGS_UPDATE:
-            hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_UPDATE>), grid, b, lds, s, ch, starts, n);
+            hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_UPDATE, ODD>), grid, b, lds, s, ch, starts, n);
             break;
-        case XM_OMW_FINAL: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_OMW_FINAL>), grid, b, lds, s, ch, starts, n); break;
+        case XM_OMW_FINAL: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_OMW_FINAL, ODD>), grid, b, lds, s, ch, starts, n); break;
         case XM_BIGGS_FINAL:
-            hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_FINAL>), grid, b, lds, s, ch, starts, n);
+            hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_FINAL, ODD>), grid, b, lds, s, ch, starts, n);
             break;
         default: break;
     }
@@ -101,7 +101,8 @@ inline void launch_modes_xrowb(dim3 grid, size_t lds, hipStream_t s, int mode, c
 bool launch_xrow_batch(int L, int E, dim3 grid, size_t lds, hipStream_t s, int mode, const XChunk* ch,
                        const int64_t* starts, int n);
 // Compile-time row kernels for the (L, E) pairs in pk_kern_xrow_*.hip; false
-// when the pair has none (the caller then uses the generic kernel).
+// when the pair has none (the caller then uses the generic kernel).  L is the
+// transform length: n1 / 2 for even rows, -n1 for odd rows (ODD kernels).
 bool launch_xrow(int L, int E, dim3 grid, size_t lds, hipStream_t s, const XArgs& a, const float2* tw);
 bool xrow_available(int L, int E);
 template <int E>

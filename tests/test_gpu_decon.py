@@ -233,7 +233,8 @@ def test_graph_replays_match_eager(method):
 
 
 @pytest.mark.parametrize("n1", [72, 96, 120, 144, 160, 192, 240, 288, 480, 640, 1920,
-                                90, 100, 150, 180, 216, 250, 270, 300, 324, 360, 432, 576, 1000])
+                                90, 100, 150, 180, 216, 250, 270, 300, 324, 360, 432, 576, 1000,
+                                15, 45, 75, 125, 135, 225, 405, 729])
 def test_row_lengths_vs_oracle(oracle, n1):
     """Every family of compile-time row kernels (row length n1 = 2L, L = P * E)
     against the oracle, OMW-2 and Biggs-4, through the single and the batched
@@ -241,7 +242,9 @@ def test_row_lengths_vs_oracle(oracle, n1):
     160, 192, 640, 1920) or rows packed per wavefront with idle lanes (72: P = 6,
     100: 5, 150: 5, 216: 9, 250: 25, 1000: 50); one stage fed from LDS because
     its radix does not divide E (90: E = 9, 120: 5, 144: 8, 180: 10, 240: 8,
-    270: 9, 288: 9, 300: 10, 324: 9, 360: 9, 432: 8, 480: 8, 576: 9)."""
+    270: 9, 288: 9, 300: 10, 324: 9, 360: 9, 432: 8, 480: 8, 576: 9).  Odd rows
+    (full-length transform of n1 real values): 15 (P = 1), 45, 75, 125, 135,
+    225, 405 with compile-time kernels, 729 on the runtime-plan kernel."""
     import petakit5d_amd as pk
     shape = (n1, 12, 10)
     print(f"[rows] n1={n1}", flush=True)
