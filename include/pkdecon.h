@@ -277,6 +277,18 @@ int32_t pk_profile_read(pk_ctx *ctx, pk_prof_entry *out, int32_t max);
  * max_prime = 7 is the reference rule; 5 gives the 2/3/5-smooth sizes the GPU transforms need. */
 int64_t pk_find_good_factor_number(int64_t n, int32_t direction, int32_t allow_odd_seven, int32_t max_prime);
 
+/* TIFF LZW strip codec (host code; no context, no device).  Decodes one LZW
+ * strip as libtiff's codec does for the reference reader's
+ * TIFFReadEncodedStrip calls (io/cpp-tiff/src/parallelreadtiff.cpp:283-300),
+ * and encodes one strip as the reference writer's encoder
+ * (io/cpp-tiff/src/lzwencode.cpp, used per strip by parallelwritetiff.cpp:
+ * 52-60).  *out_len receives the bytes written; PK_ERR_OUT_SIZE when cap is too
+ * small, PK_ERR_INVALID_ARG on a malformed strip.  Thread-safe: strips are
+ * independent.  pk_lzw_encode_bound(n) is a capacity that always suffices. */
+pk_status pk_lzw_decode(const uint8_t *src, size_t n, uint8_t *dst, size_t cap, size_t *out_len);
+pk_status pk_lzw_encode(const uint8_t *src, size_t n, uint8_t *dst, size_t cap, size_t *out_len);
+size_t pk_lzw_encode_bound(size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@ from .batch import decon_batch  # noqa: F401
 from .chunks import XR_zarrChunkCoordinatesExtraction, decon_large_in_memory  # noqa: F401
 from .psfgen import psf_gen_new  # noqa: F401
 from .rldecon import RLdecon_volume  # noqa: F401
+from .tiff_io import readtiff, writetiff  # noqa: F401
 from .zarr_io import decon_zarr, read_zarr, write_zarr  # noqa: F401
 from .serve import Server  # noqa: F401
 from .decon import (  # noqa: F401
@@ -41,6 +42,8 @@ __all__ = [
     "read_zarr",
     "write_zarr",
     "decon_zarr",
+    "readtiff",
+    "writetiff",
     "Server",
     "crop_psf_to_data",
     "fft_r2c",

@@ -3,10 +3,11 @@
 projector generation with an on-disk cache, method dispatch, background,
 scale, offset, damp, bbox and edge erosion passed to the decon functions.
 
-Files and deviations: the back projector is cached as .npy under the
-reference's file-name stem (RLdecon.m:292, '.tif' there: TIFF I/O is not part
-of this engine); the engine pads non-2/3/5-smooth sizes with zeros and crops
-back (pad=True)."""
+Files: the back projector is cached as a TIFF under the reference's file name
+(RLdecon.m:292), with the OTF magnitude and OTF mask beside it (:313-323);
+rawdata may be a TIFF or Zarr path, read as readtiff / readzarr would
+(RLdecon.m:160-185).  Deviation: the engine pads non-2/3/5-smooth sizes with
+zeros and crops back (pad=True)."""
 from __future__ import annotations
 
 import hashlib
@@ -16,6 +17,7 @@ import numpy as np
 
 from .backprojector import omw_backprojector_generation
 from .psfgen import psf_gen_new
+from .tiff_io import readtiff, writetiff
 from .decon import _is_torch, crop_psf_to_data, decon_lucy_function, decon_lucy_omw_function
 
 __all__ = ["RLdecon_volume", "omw_bp_filename", "psfgen_crop"]
@@ -24,34 +26,51 @@ _BP_CACHE: dict = {}
 
 
 def omw_bp_filename(psfgenPath: str, psfFsn: str, wienerAlpha: float, OTFCumThresh: float) -> str:
-    """Back projector cache name: sprintf('%s/%s_back_projector_alpha_%0.6f_otf_cum_thresh_%0.6f', ...)
-    (RLdecon.m:292), with the .npy suffix of this engine's cache."""
+    """Back projector cache name:
+    sprintf('%s/%s_back_projector_alpha_%0.6f_otf_cum_thresh_%0.6f.tif', ...) (RLdecon.m:292)."""
     return os.path.join(psfgenPath, f"{psfFsn}_back_projector_alpha_{wienerAlpha:0.6f}_otf_cum_thresh_"
-                                    f"{OTFCumThresh:0.6f}.npy")
+                                    f"{OTFCumThresh:0.6f}.tif")
 
 
 def _omw_bp(psf, wienerAlpha, skewed, OTFCumThresh, hannWinBounds, psfgenPath, psfFsn, ctx):
     """OMW back projector of psf (RLdecon.m:290-325): loaded from the cache file
-    when it exists, else generated (omw_backprojector_generation) and written;
-    also kept in memory per (PSF bytes, alpha, threshold, window, skewed)."""
+    when it exists, else generated (omw_backprojector_generation) and written
+    with the OTF magnitude and mask files; also kept in memory per (PSF bytes,
+    alpha, threshold, window, skewed)."""
     key = (hashlib.sha1(np.ascontiguousarray(psf).tobytes()).hexdigest(), psf.shape, float(wienerAlpha),
            float(OTFCumThresh), tuple(float(x) for x in hannWinBounds), None if skewed is None else bool(skewed))
     if key in _BP_CACHE:
         return _BP_CACHE[key]
     fn = omw_bp_filename(psfgenPath, psfFsn, wienerAlpha, OTFCumThresh) if psfgenPath and psfFsn else None
     if fn and os.path.exists(fn):
-        b = np.load(fn, allow_pickle=False)
+        b = readtiff(fn).astype(np.float32, copy=False)
     else:
-        b = omw_backprojector_generation(psf, wienerAlpha, skewed, OTFCumThresh=OTFCumThresh,
-                                         hannWinBounds=list(hannWinBounds), ctx=ctx)[0]
+        b, _, absc, mask = omw_backprojector_generation(psf, wienerAlpha, skewed, OTFCumThresh=OTFCumThresh,
+                                                        hannWinBounds=list(hannWinBounds), ctx=ctx)
         b = np.asarray(b, dtype=np.float32)
         if fn:
             os.makedirs(psfgenPath, exist_ok=True)
-            tmp = fn[:-4] + f"_{os.getpid()}.npy"  # written aside, then moved in place (RLdecon.m:315-316)
-            np.save(tmp, b, allow_pickle=False)
-            os.replace(tmp, fn)
+            tag = f"{os.getpid()}"
+            # each file written aside, then moved in place (RLdecon.m:310-323)
+            for final, data in ((fn, b),
+                                (os.path.join(psfgenPath, f"{psfFsn}_OTF.tif"), np.asarray(absc, np.float32)),
+                                (os.path.join(psfgenPath, f"{psfFsn}_OTF_mask_otf_cum_thresh_{OTFCumThresh:0.6f}.tif"),
+                                 np.asarray(mask).astype(np.uint8))):
+                tmp = final[:-4] + f"_{tag}.tif"
+                writetiff(data, tmp)
+                os.replace(tmp, final)
     _BP_CACHE[key] = b
     return b
+
+
+def _read_volume(path):
+    ext = os.path.splitext(path.rstrip("/"))[1].lower()
+    if ext in (".tif", ".tiff"):
+        return readtiff(path)
+    if ext == ".zarr":
+        from .zarr_io import read_zarr
+        return read_zarr(path)
+    raise ValueError(f"input file '{path}': .tif, .tiff or .zarr expected")
 
 
 def psfgen_crop(psf):
@@ -76,7 +95,8 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
                    OTFCumThresh=0.9, hannWinBounds=(0.8, 1.0), skewed=None, flipZstack=False, save16bit=True,
                    dampFactor=1.0, scaleFactor=None, deconOffset=0.0, deconBbox=None, EdgeErosion=0,
                    psfgenPath=None, psfFsn=None, psfGen=False, dzPSF=None, dz=None, ctx=None):
-    """deconvolved = RLdecon(..., 'rawdata', rawdata, ...) for one in-memory volume.
+    """deconvolved = RLdecon(..., 'rawdata', rawdata, ...) for one volume
+    (rawdata an array, or a .tif / .zarr path; psf an array or a .tif path).
 
     Follows RLdecon.m: PSF as single (:235), centre-cropped when larger than the
     data (:264-270); for 'omw' the back projector from the cache or
@@ -92,6 +112,12 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     engine's."""
     if RLMethod not in ("simplified", "omw"):
         raise ValueError(f"RLMethod '{RLMethod}': this engine serves 'simplified' and 'omw'")
+    if isinstance(rawdata, (str, os.PathLike)):  # input file (RLdecon.m:335-343)
+        rawdata = _read_volume(str(rawdata))
+    if isinstance(psf, (str, os.PathLike)):  # PSF file (RLdecon.m:214-236)
+        if psfFsn is None:
+            psfFsn = os.path.splitext(os.path.basename(str(psf)))[0]
+        psf = readtiff(str(psf))
     psf = np.asarray(psf.detach().cpu().numpy() if _is_torch(psf) else psf, dtype=np.float32)
     if psfGen:
         if dzPSF is None or dz is None:

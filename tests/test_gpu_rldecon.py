@@ -3,6 +3,8 @@ to the decon functions with a PSF normalised to unit sum and cropped to the
 data, the OMW back projector generated with the given alpha / threshold and
 cached on disk (a second call loads it and gives the same bits), deconBbox
 and a non-smooth volume size (zero-padded by the engine)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -37,7 +39,10 @@ def test_omw_dispatch_and_back_projector_cache(tmp_path):
     J1 = pk.RLdecon_volume(raw, psf, 2, RLMethod="omw", Background=100, wienerAlpha=0.01, OTFCumThresh=0.85,
                            psfgenPath=str(tmp_path), psfFsn="psfA")
     fn = rldecon.omw_bp_filename(str(tmp_path), "psfA", 0.01, 0.85)
-    b = np.load(fn)
+    b = pk.readtiff(fn)
+    assert os.path.exists(os.path.join(str(tmp_path), "psfA_OTF.tif"))
+    mask = pk.readtiff(os.path.join(str(tmp_path), "psfA_OTF_mask_otf_cum_thresh_0.850000.tif"))
+    assert mask.dtype == np.uint8 and mask.any()
     pc = pk.crop_psf_to_data(psf, raw.shape)
     b_ref = pk.omw_backprojector_generation(pc, 0.01, None, OTFCumThresh=0.85)[0]
     np.testing.assert_array_equal(b, b_ref)
@@ -48,3 +53,18 @@ def test_omw_dispatch_and_back_projector_cache(tmp_path):
     J2 = pk.RLdecon_volume(raw, psf, 2, RLMethod="omw", Background=100, wienerAlpha=0.01, OTFCumThresh=0.85,
                            psfgenPath=str(tmp_path), psfFsn="psfA")
     np.testing.assert_array_equal(J2, J1)
+
+
+def test_file_inputs_match_arrays(tmp_path):
+    """rawdata as a .tif or .zarr path and the PSF as a .tif path (RLdecon.m
+    reads them with readtiff / readzarr) give the bits of the in-memory call."""
+    import petakit5d_amd as pk
+    raw, psf = _case((24, 24, 31), (1.0, 2.5), 0.0, (60, 49, 36))
+    print("[rldecon files] 60x49x36", flush=True)
+    pk.writetiff(raw, str(tmp_path / "raw.tif"))
+    pk.writetiff(psf, str(tmp_path / "psf.tif"))
+    pk.write_zarr(str(tmp_path / "raw.zarr"), raw, chunks=(32, 32, 16))
+    J = pk.RLdecon_volume(raw, psf, 3, Background=100)
+    for src in ("raw.tif", "raw.zarr"):
+        Jf = pk.RLdecon_volume(str(tmp_path / src), str(tmp_path / "psf.tif"), 3, Background=100)
+        np.testing.assert_array_equal(Jf, J)

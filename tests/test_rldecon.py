@@ -10,7 +10,7 @@ from petakit5d_amd.rldecon import RLdecon_volume, omw_bp_filename, psfgen_crop
 def test_bp_cache_name_matches_reference_format():
     # sprintf('%s/%s_back_projector_alpha_%0.6f_otf_cum_thresh_%0.6f.tif', psfgenPath, psfFsn, 0.005, 0.9)
     fn = omw_bp_filename("/data/psfgen", "psf_488", 0.005, 0.9)
-    assert fn == "/data/psfgen/psf_488_back_projector_alpha_0.005000_otf_cum_thresh_0.900000.npy"
+    assert fn == "/data/psfgen/psf_488_back_projector_alpha_0.005000_otf_cum_thresh_0.900000.tif"
 
 
 def test_no_iterations_returns_raw_cropped():
