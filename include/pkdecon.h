@@ -289,6 +289,15 @@ pk_status pk_lzw_decode(const uint8_t *src, size_t n, uint8_t *dst, size_t cap, 
 pk_status pk_lzw_encode(const uint8_t *src, size_t n, uint8_t *dst, size_t cap, size_t *out_len);
 size_t pk_lzw_encode_bound(size_t n);
 
+/* LZ4 block codec (host code; no context, no device): the streams inside the
+ * blosc-compressed Zarr chunks the reference reads and writes by default
+ * (blosc + lz4, io/cpp-zarr/src/zarr.cpp:15-18, through c-blosc in
+ * parallelreadzarr.cpp:148-157).  Same return conventions as the LZW codec;
+ * pk_lz4_encode needs cap >= pk_lz4_encode_bound(n). */
+pk_status pk_lz4_decode(const uint8_t *src, size_t n, uint8_t *dst, size_t cap, size_t *out_len);
+pk_status pk_lz4_encode(const uint8_t *src, size_t n, uint8_t *dst, size_t cap, size_t *out_len);
+size_t pk_lz4_encode_bound(size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,9 @@ EXPORTS = [
     "pk_lzw_decode",
     "pk_lzw_encode",
     "pk_lzw_encode_bound",
+    "pk_lz4_decode",
+    "pk_lz4_encode",
+    "pk_lz4_encode_bound",
 ]
 
 
@@ -182,7 +185,8 @@ def load():
         L.pk_find_good_factor_number.restype = ctypes.c_int64
         for name in EXPORTS:
             if name not in ("pk_opts_default", "pk_bp_opts_default", "pk_ctx_destroy", "pk_last_error", "pk_device_bytes",
-                            "pk_find_good_factor_number", "pk_profile_read", "pk_lzw_encode_bound"):
+                            "pk_find_good_factor_number", "pk_profile_read", "pk_lzw_encode_bound",
+                            "pk_lz4_encode_bound"):
                 getattr(L, name).restype = ctypes.c_int
         _lib = L
         return _lib

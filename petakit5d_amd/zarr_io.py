@@ -6,19 +6,170 @@ Metadata follow the reference's reader and writer (io/cpp-zarr/src/zarr.cpp:
 16-18 defaults: dtype '<u2', order 'F', dimension_separator '.', fill_value 0;
 :51-96 parsing: chunks, compressor, dimension_separator '.' or '/', fill_value
 null / NaN -> 0).  Arrays use MATLAB's size order (shape [y x z], dim 1
-contiguous).  Compressors: none, zlib and gzip; blosc (the reference's default
-codec) is not available in this build and raises NotImplementedError.  Edge
-chunks are stored full size, as the Zarr v2 format requires."""
+contiguous).  Compressors: none, zlib, gzip and blosc (the reference's default:
+lz4, byte shuffle, clevel 5, zarr.cpp:15-18) with its lz4 / lz4hc or zlib
+streams; blosc's blosclz, snappy and zstd streams and bit shuffle raise
+NotImplementedError.  Edge chunks are stored full size, as the Zarr v2 format
+requires.
+
+Blosc chunks follow the published blosc 1 format: a 16-byte header (format
+versions, flags, type size, uncompressed bytes, block size, compressed bytes),
+then, unless the flags say the chunk is a plain copy, one 32-bit start offset
+per block; a block holds type-size streams (one stream when the flags forbid
+splitting, when the type size exceeds 16, when a stream would be shorter than
+128 bytes, and for the last, short block), each a 32-bit length and that many
+bytes, stored raw when the length equals the stream's size.  Byte shuffle
+groups byte k of every element of a block together.  The LZ4 streams go
+through libpkdecon.so's host codec (pk_lz4_decode / pk_lz4_encode)."""
 from __future__ import annotations
 
+import ctypes
 import gzip
 import json
 import os
+import struct
 import zlib
 
 import numpy as np
 
-__all__ = ["read_zarr", "write_zarr", "decon_zarr"]
+__all__ = ["read_zarr", "write_zarr", "decon_zarr", "blosc_decode", "blosc_encode"]
+
+_BLOSC_MAX_SPLITS, _BLOSC_MIN_STREAM = 16, 128
+_BLOSC_CODES = {"blosclz": 0, "lz4": 1, "lz4hc": 1, "snappy": 2, "zlib": 3, "zstd": 4}
+
+
+def _lz4():
+    from ._lib import load
+    L = load()
+    for name in ("pk_lz4_decode", "pk_lz4_encode"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                      ctypes.POINTER(ctypes.c_size_t)]
+    L.pk_lz4_encode_bound.restype = ctypes.c_size_t
+    L.pk_lz4_encode_bound.argtypes = [ctypes.c_size_t]
+    return L
+
+
+def _lz4_decode(L, src: bytes, size: int) -> bytes:
+    a = np.frombuffer(src, dtype=np.uint8)
+    out = np.empty(max(size, 1), dtype=np.uint8)
+    n = ctypes.c_size_t(0)
+    rc = L.pk_lz4_decode(a.ctypes.data if a.size else None, a.size, out.ctypes.data, size, ctypes.byref(n))
+    if rc != 0 or n.value != size:
+        raise ValueError("corrupt lz4 stream in a blosc chunk")
+    return out[:size].tobytes()
+
+
+def _lz4_encode(L, src: bytes) -> bytes:
+    a = np.frombuffer(src, dtype=np.uint8)
+    cap = int(L.pk_lz4_encode_bound(a.size))
+    out = np.empty(cap, dtype=np.uint8)
+    n = ctypes.c_size_t(0)
+    if L.pk_lz4_encode(a.ctypes.data if a.size else None, a.size, out.ctypes.data, cap, ctypes.byref(n)) != 0:
+        raise RuntimeError("pk_lz4_encode failed")
+    return out[:n.value].tobytes()
+
+
+def _splits(flags, typesize, bsize, last_short):
+    if (flags & 0x10) or last_short or typesize > _BLOSC_MAX_SPLITS or bsize // typesize < _BLOSC_MIN_STREAM:
+        return 1
+    return typesize
+
+
+def _unshuffle(b: bytes, typesize: int) -> bytes:
+    q = len(b) // typesize
+    if typesize <= 1 or q == 0:
+        return b
+    a = np.frombuffer(b, dtype=np.uint8)
+    body = a[:q * typesize].reshape(typesize, q).T.reshape(-1)
+    return body.tobytes() + a[q * typesize:].tobytes()
+
+
+def _shuffle(b: bytes, typesize: int) -> bytes:
+    q = len(b) // typesize
+    if typesize <= 1 or q == 0:
+        return b
+    a = np.frombuffer(b, dtype=np.uint8)
+    body = a[:q * typesize].reshape(q, typesize).T.reshape(-1)
+    return body.tobytes() + a[q * typesize:].tobytes()
+
+
+def blosc_decode(raw: bytes) -> bytes:
+    """One blosc 1 chunk -> its bytes."""
+    if len(raw) < 16:
+        raise ValueError("blosc chunk shorter than its header")
+    flags, typesize = raw[2], max(raw[3], 1)
+    nbytes, blocksize, cbytes = struct.unpack_from("<III", raw, 4)
+    if flags & 0x2:  # plain copy
+        return bytes(raw[16:16 + nbytes])
+    if flags & 0x4:
+        raise NotImplementedError("blosc bit shuffle is not available in this build")
+    code = flags >> 5
+    if code not in (1, 3):
+        name = {0: "blosclz", 2: "snappy", 4: "zstd"}.get(code, str(code))
+        raise NotImplementedError(f"blosc '{name}' streams are not available in this build (lz4, lz4hc, zlib are)")
+    if nbytes == 0:
+        return b""
+    if blocksize == 0:
+        raise ValueError("blosc chunk with a zero block size")
+    L = _lz4() if code == 1 else None
+    nblocks = -(-nbytes // blocksize)
+    starts = struct.unpack_from(f"<{nblocks}i", raw, 16)
+    out = []
+    for j, pos in enumerate(starts):
+        bsize = min(blocksize, nbytes - j * blocksize)
+        ns = _splits(flags, typesize, bsize, bsize < blocksize)
+        neb = bsize // ns
+        parts = []
+        for _ in range(ns):
+            (cs,) = struct.unpack_from("<i", raw, pos)
+            pos += 4
+            seg = raw[pos:pos + cs]
+            pos += cs
+            if cs == neb:
+                parts.append(bytes(seg))
+            elif code == 1:
+                parts.append(_lz4_decode(L, seg, neb))
+            else:
+                parts.append(zlib.decompress(seg))
+        blk = b"".join(parts)
+        if len(blk) != bsize:
+            raise ValueError("corrupt blosc block")
+        out.append(_unshuffle(blk, typesize) if flags & 0x1 else blk)
+    return b"".join(out)
+
+
+def blosc_encode(data: bytes, typesize: int, shuffle: int = 1, blocksize: int = 1 << 18) -> bytes:
+    """Bytes -> one blosc 1 chunk with lz4 streams (byte shuffle when shuffle=1)."""
+    if shuffle not in (0, 1):
+        raise NotImplementedError("blosc bit shuffle is not available in this build")
+    L = _lz4()
+    nbytes = len(data)
+    typesize = max(1, int(typesize))
+    bsz = max(typesize, min(blocksize, nbytes) // typesize * typesize) if nbytes else 0
+    flags = (1 << 5) | (0x1 if shuffle and typesize > 1 else 0)
+    nblocks = -(-nbytes // bsz) if nbytes else 0
+    body, starts = [], []
+    off = 16 + 4 * nblocks
+    for j in range(nblocks):
+        blk = data[j * bsz:(j + 1) * bsz]
+        if flags & 0x1:
+            blk = _shuffle(blk, typesize)
+        ns = _splits(flags, typesize, len(blk), len(blk) < bsz)
+        neb = len(blk) // ns
+        starts.append(off)
+        for k in range(ns):
+            seg = blk[k * neb:(k + 1) * neb]
+            c = _lz4_encode(L, seg)
+            if len(c) >= neb:
+                c = seg
+            body.append(struct.pack("<i", len(c)) + c)
+            off += 4 + len(c)
+    if off > 16 + nbytes:  # incompressible: a plain copy
+        return struct.pack("<BBBBIII", 2, 1, 0x2 | (flags & 0xE0), min(typesize, 255), nbytes, bsz, 16 + nbytes) + data
+    head = struct.pack("<BBBBIII", 2, 1, flags, min(typesize, 255), nbytes, bsz, off)
+    return head + struct.pack(f"<{nblocks}i", *starts) + b"".join(body)
 
 
 def _meta(path):
@@ -43,7 +194,9 @@ def _decode(raw, comp):
         return zlib.decompress(raw)
     if cid == "gzip":
         return gzip.decompress(raw)
-    raise NotImplementedError(f"chunk compressor '{cid}' is not available in this build (none, zlib, gzip are)")
+    if cid == "blosc":
+        return blosc_decode(raw)
+    raise NotImplementedError(f"chunk compressor '{cid}' is not available in this build (none, zlib, gzip, blosc are)")
 
 
 def _chunk_path(path, idx, sep):
@@ -90,14 +243,17 @@ def read_zarr(path, bbox=None):
 def write_zarr(path, data, chunks=(256, 256, 256), compressor="zlib", level=1, dimension_separator=".", order="F",
                fill_value=0):
     """Write data (shape = MATLAB size) as a Zarr v2 store: zlib-compressed
-    (compressor='zlib') or raw (None) chunks, full-size edge chunks."""
+    (compressor='zlib'), blosc lz4 with byte shuffle (compressor='blosc', the
+    reference writer's default, zarr.cpp:15-18) or raw (None) chunks,
+    full-size edge chunks."""
     a = np.asarray(data)
     if dimension_separator not in (".", "/"):
         raise ValueError("dimension_separator must be '.' or '/'")
     chunks = [int(min(c, n)) if n > 0 else int(c) for c, n in zip(chunks, a.shape)]
-    comp = None if compressor is None else {"id": "zlib", "level": int(level)}
-    if compressor not in (None, "zlib"):
-        raise NotImplementedError("write_zarr: compressor must be 'zlib' or None")
+    if compressor not in (None, "zlib", "blosc"):
+        raise NotImplementedError("write_zarr: compressor must be 'zlib', 'blosc' or None")
+    comp = {None: None, "zlib": {"id": "zlib", "level": int(level)},
+            "blosc": {"id": "blosc", "cname": "lz4", "clevel": 5, "shuffle": 1, "blocksize": 0}}[compressor]
     os.makedirs(path, exist_ok=True)
     meta = {"zarr_format": 2, "shape": list(a.shape), "chunks": chunks, "dtype": a.dtype.str,
             "compressor": comp, "fill_value": fill_value, "order": order, "filters": None,
@@ -111,8 +267,10 @@ def write_zarr(path, data, chunks=(256, 256, 256), compressor="zlib", level=1, d
         src = tuple(slice(s, min(s + k, n)) for s, k, n in zip(c0, chunks, a.shape))
         blk[tuple(slice(0, x.stop - x.start) for x in src)] = a[src]
         raw = blk.tobytes(order=order)
-        if comp is not None:
+        if compressor == "zlib":
             raw = zlib.compress(raw, int(level))
+        elif compressor == "blosc":
+            raw = blosc_encode(raw, a.dtype.itemsize)
         fn = _chunk_path(path, ci, dimension_separator)
         os.makedirs(os.path.dirname(fn), exist_ok=True)
         with open(fn, "wb") as f:
@@ -120,14 +278,18 @@ def write_zarr(path, data, chunks=(256, 256, 256), compressor="zlib", level=1, d
 
 
 def decon_zarr(in_path, out_path, PSF_f, PSF_b, NUMIT, method="omw", batchSize=(512, 512, 512), chunks=None,
-               **kwargs):
+               compressor=None, **kwargs):
     """Deconvolve the store at in_path into a store at out_path: the volume is
     read, deconvolved in batches of batchSize through one batched engine call
     (decon_large_in_memory), and written with the input's chunk shape unless
-    chunks is given."""
+    chunks is given, and with the input's compressor (blosc or zlib; zlib for
+    other inputs) unless compressor is given."""
     from .chunks import decon_large_in_memory
     m, _, _ = _meta(in_path)
+    if compressor is None:
+        cid = (m.get("compressor") or {}).get("id")
+        compressor = "blosc" if cid == "blosc" else "zlib"
     im = read_zarr(in_path)
     J = decon_large_in_memory(im, PSF_f, PSF_b, NUMIT, method=method, batchSize=batchSize, **kwargs)
-    write_zarr(out_path, J, chunks=chunks or m["chunks"])
+    write_zarr(out_path, J, chunks=chunks or m["chunks"], compressor=compressor)
     return J
