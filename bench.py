@@ -18,7 +18,8 @@ on the host cores for the same call.
 Other workloads (--workload): rl128 (Biggs RL 15 it. on 128x128x64), batched
 (64 random-size chunks, OMW-2 and RL-10 in batched launches), served
 (continuous-batching server, Poisson arrivals, p50/p99 vs the plain path),
-large (2048x2048x512, OMW-2 and Biggs-10).
+large (2048x2048x512, OMW-2 and Biggs-10), deskew (deskewFrame3D of a
+512x512x256 decon result).
 """
 import argparse
 import ctypes
@@ -50,7 +51,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="omw", choices=["omw", "rl128", "batched", "served", "large", "copy"])
+    ap.add_argument("--workload", default="omw", choices=["omw", "rl128", "batched", "served", "large", "copy",
+                                                                  "deskew"])
     ap.add_argument("--shape", type=int, nargs=3, default=None)
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--duration", type=float, default=20.0, help="served: seconds of Poisson arrivals")
@@ -608,6 +610,69 @@ def wl_copy(args, torch, pk, L):
     return res
 
 
+def wl_deskew(args, torch, pk, L):
+    """deskewFrame3D of a decon result (single, 512x512x256, the default
+    workload's size) at the reference defaults SkewAngle 32.45, dz 0.3,
+    xyPixelSize 0.108: device in, device out, one pk_deskew_frame3d per step;
+    kernel time from HIP events on the engine's stream; CPU baseline the
+    oracle's numpy resampling on a 64-plane slab of the same volume."""
+    import ctypes
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    shape, geo = (512, 512, 256), (32.45, 0.3, 0.108)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    vol = torch.rand((shape[2], shape[1], shape[0]), device="cuda", generator=g).permute(2, 1, 0)
+    od = pk.deskew.deskew_out_size(shape, *geo)
+    out = torch.empty((od[2], od[1], od[0]), device="cuda").permute(2, 1, 0)
+    ctx = pk.default_context()
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+
+    def call():
+        ctx.check(L.pk_deskew_frame3d(ctx.handle, ctypes.c_void_p(vol.data_ptr()), _lib_i64x3(shape),
+                                      ctypes.c_double(geo[0]), ctypes.c_double(geo[1]), ctypes.c_double(geo[2]),
+                                      0, 0, ctypes.c_void_p(out.data_ptr())))
+
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in evs:
+        a.record(stream)
+        call()
+        b.record(stream)
+    torch.cuda.synchronize()
+    kms = sum(a.elapsed_time(b) for a, b in evs) / len(evs)
+    ctx.set_stream(None)
+    nin, nout = vol.numel(), out.numel()
+    nbytes = 4.0 * (nin + nout)
+    res = base_result("deskewed Mvoxel/s (deskewFrame3D, linear)", nout / ms / 1e3, "Mvoxel/s", args, ms, {
+        "workload": "deskewFrame3D of a 512x512x256 single volume, SkewAngle 32.45, dz 0.3, xyPixelSize 0.108, "
+                    "linear, no crop; device-resident in and out",
+        "in": list(shape), "out": list(od)})
+    res["roofline"] = {"bound": "hbm", "kernel": "k_deskew", "achieved": round(nbytes / kms / 1e6, 1),
+                       "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(nbytes / kms / 1e6 / HBM_PEAK_GBPS, 4),
+                       "traffic": None, "algorithmic_bytes": nbytes, "kernel_ms": round(kms, 4)}
+    if not args.no_cpu_baseline:
+        import deskew_oracle
+        slab = vol[:, :, :64].permute(2, 1, 0).cpu().numpy().transpose(2, 1, 0)
+        t0 = time.perf_counter()
+        ref = deskew_oracle.deskew(slab, *geo)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(ref.size / dt / 1e6, 2), "unit": "Mvoxel/s", "cores": 1, "kind": "port",
+                               "sample": f"512x512x64 slab, numpy float64, {dt:.2f} s"}
+    return res
+
+
+def _lib_i64x3(v):
+    from petakit5d_amd._lib import i64x3
+    return i64x3(v)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -622,7 +687,7 @@ def main():
 
     L = _lib.load()
     res = {"omw": wl_omw, "rl128": wl_rl128, "batched": wl_batched, "served": wl_served, "large": wl_large,
-           "copy": wl_copy}[
+           "copy": wl_copy, "deskew": wl_deskew}[
         args.workload](args, torch, pk, L)
     print(json.dumps(res), flush=True)
 

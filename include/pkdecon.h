@@ -277,6 +277,18 @@ int32_t pk_profile_read(pk_ctx *ctx, pk_prof_entry *out, int32_t max);
  * max_prime = 7 is the reference rule; 5 gives the 2/3/5-smooth sizes the GPU transforms need. */
 int64_t pk_find_good_factor_number(int64_t n, int32_t direction, int32_t allow_odd_seven, int32_t max_prime);
 
+/* deskewFrame3D(vol, angle, dz, xyPixelSize, reverse, 'Crop', crop) with the
+ * default linear interpolation (pointDetection/llsmtools/transforms/
+ * deskewFrame3D.m:26-70, applied to the deconvolved volume at
+ * microscopeDataProcessing/decon/RLdecon.m:448-449).  pk_deskew_out_dims gives
+ * the output size [ny, nxOut, nz] (host only); pk_deskew_frame3d writes it
+ * from the single-precision volume at d_vol into d_out (both device memory,
+ * column-major) and returns when done. */
+pk_status pk_deskew_out_dims(const int64_t dims[3], double angle, double dz, double xy_pixel_size, int32_t reverse,
+                             int32_t crop, int64_t out_dims[3]);
+pk_status pk_deskew_frame3d(pk_ctx *ctx, const float *d_vol, const int64_t dims[3], double angle, double dz,
+                            double xy_pixel_size, int32_t reverse, int32_t crop, float *d_out);
+
 /* TIFF LZW strip codec (host code; no context, no device).  Decodes one LZW
  * strip as libtiff's codec does for the reference reader's
  * TIFFReadEncodedStrip calls (io/cpp-tiff/src/parallelreadtiff.cpp:283-300),

@@ -70,6 +70,8 @@ EXPORTS = [
     "pk_lz4_decode",
     "pk_lz4_encode",
     "pk_lz4_encode_bound",
+    "pk_deskew_out_dims",
+    "pk_deskew_frame3d",
 ]
 
 

@@ -12,6 +12,15 @@
 
 struct pk_ctx : pk::pk_ctx_impl {};
 
+namespace pk {
+struct DeskewGeom {
+    int64_t nx_out;
+    double xstep, xshift;
+};
+DeskewGeom deskew_geometry(const int64_t dims[3], double angle, double dz, double xy, bool reverse, bool crop);
+void launch_deskew(const float* vol, float* out, const int64_t dims[3], const DeskewGeom& g, hipStream_t s);
+}  // namespace pk
+
 namespace {
 template <class Fn>
 pk_status guarded(pk_ctx* c, Fn&& fn) {
@@ -267,6 +276,36 @@ pk_status pk_fft_c2r(pk_ctx* c, const float* d_spec, float* d_real, const int64_
         PK_HIP(hipMemcpy2DAsync(c->ws.A.p, sizeof(float2) * vp.hp, d_spec, sizeof(float2) * vp.h1,
                                 sizeof(float2) * vp.h1, (size_t)(vp.n2 * vp.n3), hipMemcpyDeviceToDevice, c->stream));
         c->c2r3d(vp, c->ws.A.as<float2>(), d_real);
+        PK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+static void deskew_check(const int64_t dims[3], double dz, double xy) {
+    if (!dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) pk::fail(PK_ERR_INVALID_ARG, "volume dimensions must be >= 1");
+    if (!(dz >= 0) || !(xy > 0)) pk::fail(PK_ERR_INVALID_ARG, "dz must be >= 0 and xyPixelSize > 0");
+}
+
+pk_status pk_deskew_out_dims(const int64_t dims[3], double angle, double dz, double xy_pixel_size, int32_t reverse,
+                             int32_t crop, int64_t out_dims[3]) {
+    return host_guarded(nullptr, [&] {
+        deskew_check(dims, dz, xy_pixel_size);
+        if (!out_dims) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        const pk::DeskewGeom g = pk::deskew_geometry(dims, angle, dz, xy_pixel_size, reverse != 0, crop != 0);
+        if (g.nx_out < 1) pk::fail(PK_ERR_INVALID_ARG, "deskewed width below one column");
+        out_dims[0] = dims[0];
+        out_dims[1] = g.nx_out;
+        out_dims[2] = dims[2];
+    });
+}
+
+pk_status pk_deskew_frame3d(pk_ctx* c, const float* d_vol, const int64_t dims[3], double angle, double dz,
+                            double xy_pixel_size, int32_t reverse, int32_t crop, float* d_out) {
+    return guarded(c, [&] {
+        deskew_check(dims, dz, xy_pixel_size);
+        if (!d_vol || !d_out) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        const pk::DeskewGeom g = pk::deskew_geometry(dims, angle, dz, xy_pixel_size, reverse != 0, crop != 0);
+        if (g.nx_out < 1) pk::fail(PK_ERR_INVALID_ARG, "deskewed width below one column");
+        pk::launch_deskew(d_vol, d_out, dims, g, c->stream);
         PK_HIP(hipStreamSynchronize(c->stream));
     });
 }

@@ -94,7 +94,8 @@ def psfgen_crop(psf):
 def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, wienerAlpha=0.005,
                    OTFCumThresh=0.9, hannWinBounds=(0.8, 1.0), skewed=None, flipZstack=False, save16bit=True,
                    dampFactor=1.0, scaleFactor=None, deconOffset=0.0, deconBbox=None, EdgeErosion=0,
-                   psfgenPath=None, psfFsn=None, psfGen=False, dzPSF=None, dz=None, ctx=None):
+                   psfgenPath=None, psfFsn=None, psfGen=False, dzPSF=None, dz=None, Deskew=False, SkewAngle=32.45,
+                   xyPixelSize=0.108, Reverse=False, Crop=False, ctx=None):
     """deconvolved = RLdecon(..., 'rawdata', rawdata, ...) for one volume
     (rawdata an array, or a .tif / .zarr path; psf an array or a .tif path).
 
@@ -108,10 +109,20 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     EdgeErosion (:387-404); otherwise the raw data, cropped to deconBbox
     (:415-420).  psfGen: the PSF first goes through psf_gen_new(psf, dzPSF,
     dz, 1.5, 'masked') and, for methods other than 'omw', psfgen_crop
-    (:240-260).  Other methods ('original' deconvlucy, 'cudagen') are not this
-    engine's."""
+    (:240-260).  Deskew=True also returns the deskewed result,
+    deskewFrame3D(single(deconvolved), SkewAngle, dz, xyPixelSize, Reverse,
+    'Crop', Crop) (:448-449), as (deconvolved, ds).  Other methods ('original'
+    deconvlucy, 'cudagen') and rotation are not this engine's."""
     if RLMethod not in ("simplified", "omw"):
         raise ValueError(f"RLMethod '{RLMethod}': this engine serves 'simplified' and 'omw'")
+    if Deskew:
+        if dz is None:
+            raise ValueError("Deskew needs dz")
+        J = RLdecon_volume(rawdata, psf, nIter, RLMethod, Background, wienerAlpha, OTFCumThresh, hannWinBounds,
+                           skewed, flipZstack, save16bit, dampFactor, scaleFactor, deconOffset, deconBbox,
+                           EdgeErosion, psfgenPath, psfFsn, psfGen, dzPSF, dz, ctx=ctx)
+        from .deskew import deskewFrame3D
+        return J, deskewFrame3D(J, SkewAngle, dz, xyPixelSize, Reverse, Crop=Crop, ctx=ctx)
     if isinstance(rawdata, (str, os.PathLike)):  # input file (RLdecon.m:335-343)
         rawdata = _read_volume(str(rawdata))
     if isinstance(psf, (str, os.PathLike)):  # PSF file (RLdecon.m:214-236)

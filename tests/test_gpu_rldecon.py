@@ -68,3 +68,16 @@ def test_file_inputs_match_arrays(tmp_path):
     for src in ("raw.tif", "raw.zarr"):
         Jf = pk.RLdecon_volume(str(tmp_path / src), str(tmp_path / "psf.tif"), 3, Background=100)
         np.testing.assert_array_equal(Jf, J)
+
+
+def test_deskew_after_decon():
+    """Deskew=True returns the decon result and deskewFrame3D of it
+    (RLdecon.m:448-449)."""
+    import petakit5d_amd as pk
+    raw, psf = _case()
+    print("[rldecon deskew]", flush=True)
+    J = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=False)
+    J2, ds = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=False, Deskew=True, SkewAngle=32.45, dz=0.3,
+                               xyPixelSize=0.108, Crop=True)
+    np.testing.assert_array_equal(J2, J)
+    np.testing.assert_array_equal(ds, pk.deskewFrame3D(J, 32.45, 0.3, 0.108, Crop=True))
