@@ -240,6 +240,54 @@ def read_zarr(path, bbox=None):
     return out
 
 
+def _create_store(path, shape, dtype, chunks, compressor, level, dimension_separator, order, fill_value):
+    if dimension_separator not in (".", "/"):
+        raise ValueError("dimension_separator must be '.' or '/'")
+    if compressor not in (None, "zlib", "blosc"):
+        raise NotImplementedError("write_zarr: compressor must be 'zlib', 'blosc' or None")
+    chunks = [int(min(c, n)) if n > 0 else int(c) for c, n in zip(chunks, shape)]
+    comp = {None: None, "zlib": {"id": "zlib", "level": int(level)},
+            "blosc": {"id": "blosc", "cname": "lz4", "clevel": 5, "shuffle": 1, "blocksize": 0}}[compressor]
+    os.makedirs(path, exist_ok=True)
+    meta = {"zarr_format": 2, "shape": [int(n) for n in shape], "chunks": chunks, "dtype": np.dtype(dtype).str,
+            "compressor": comp, "fill_value": fill_value, "order": order, "filters": None,
+            "dimension_separator": dimension_separator}
+    with open(os.path.join(path, ".zarray"), "w") as f:
+        json.dump(meta, f)
+    return meta
+
+
+def _write_chunks(path, meta, a, origin=(0, 0, 0)):
+    """Write the chunks covered by a, placed at origin (0-based, a multiple
+    of the chunk shape); chunks past the array's end are stored full size."""
+    chunks, shape = meta["chunks"], meta["shape"]
+    comp = meta["compressor"]
+    cid = None if comp is None else comp["id"]
+    sep, order = meta["dimension_separator"], meta["order"]
+    if any(o % c for o, c in zip(origin, chunks)):
+        raise ValueError("region origin not on a chunk boundary")
+    c_first = [o // c for o, c in zip(origin, chunks)]
+    c_count = [-(-n // c) for n, c in zip(a.shape, chunks)]
+    for rel in np.ndindex(*c_count):
+        ci = [f + r for f, r in zip(c_first, rel)]
+        c0 = [r * k for r, k in zip(rel, chunks)]
+        src = tuple(slice(s0, min(s0 + k, n)) for s0, k, n in zip(c0, chunks, a.shape))
+        g_end = [o + x.stop for o, x in zip(origin, src)]
+        if any(e < min((i + 1) * k, n) for e, i, k, n in zip(g_end, ci, chunks, shape)):
+            raise ValueError("region ends inside a chunk that is not the array's last")
+        blk = np.full(chunks, meta["fill_value"], dtype=a.dtype)
+        blk[tuple(slice(0, x.stop - x.start) for x in src)] = a[src]
+        raw = blk.tobytes(order=order)
+        if cid == "zlib":
+            raw = zlib.compress(raw, int(comp["level"]))
+        elif cid == "blosc":
+            raw = blosc_encode(raw, a.dtype.itemsize)
+        fn = _chunk_path(path, ci, sep)
+        os.makedirs(os.path.dirname(fn), exist_ok=True)
+        with open(fn, "wb") as f:
+            f.write(raw)
+
+
 def write_zarr(path, data, chunks=(256, 256, 256), compressor="zlib", level=1, dimension_separator=".", order="F",
                fill_value=0):
     """Write data (shape = MATLAB size) as a Zarr v2 store: zlib-compressed
@@ -247,49 +295,46 @@ def write_zarr(path, data, chunks=(256, 256, 256), compressor="zlib", level=1, d
     reference writer's default, zarr.cpp:15-18) or raw (None) chunks,
     full-size edge chunks."""
     a = np.asarray(data)
-    if dimension_separator not in (".", "/"):
-        raise ValueError("dimension_separator must be '.' or '/'")
-    chunks = [int(min(c, n)) if n > 0 else int(c) for c, n in zip(chunks, a.shape)]
-    if compressor not in (None, "zlib", "blosc"):
-        raise NotImplementedError("write_zarr: compressor must be 'zlib', 'blosc' or None")
-    comp = {None: None, "zlib": {"id": "zlib", "level": int(level)},
-            "blosc": {"id": "blosc", "cname": "lz4", "clevel": 5, "shuffle": 1, "blocksize": 0}}[compressor]
-    os.makedirs(path, exist_ok=True)
-    meta = {"zarr_format": 2, "shape": list(a.shape), "chunks": chunks, "dtype": a.dtype.str,
-            "compressor": comp, "fill_value": fill_value, "order": order, "filters": None,
-            "dimension_separator": dimension_separator}
-    with open(os.path.join(path, ".zarray"), "w") as f:
-        json.dump(meta, f)
-    grid = [-(-n // c) for n, c in zip(a.shape, chunks)]
-    for ci in np.ndindex(*grid):
-        c0 = [c * k for c, k in zip(ci, chunks)]
-        blk = np.full(chunks, fill_value, dtype=a.dtype)
-        src = tuple(slice(s, min(s + k, n)) for s, k, n in zip(c0, chunks, a.shape))
-        blk[tuple(slice(0, x.stop - x.start) for x in src)] = a[src]
-        raw = blk.tobytes(order=order)
-        if compressor == "zlib":
-            raw = zlib.compress(raw, int(level))
-        elif compressor == "blosc":
-            raw = blosc_encode(raw, a.dtype.itemsize)
-        fn = _chunk_path(path, ci, dimension_separator)
-        os.makedirs(os.path.dirname(fn), exist_ok=True)
-        with open(fn, "wb") as f:
-            f.write(raw)
+    meta = _create_store(path, a.shape, a.dtype, chunks, compressor, level, dimension_separator, order, fill_value)
+    _write_chunks(path, meta, a)
 
 
 def decon_zarr(in_path, out_path, PSF_f, PSF_b, NUMIT, method="omw", batchSize=(512, 512, 512), chunks=None,
-               compressor=None, **kwargs):
-    """Deconvolve the store at in_path into a store at out_path: the volume is
-    read, deconvolved in batches of batchSize through one batched engine call
-    (decon_large_in_memory), and written with the input's chunk shape unless
-    chunks is given, and with the input's compressor (blosc or zlib; zlib for
-    other inputs) unless compressor is given."""
-    from .chunks import decon_large_in_memory
+               compressor=None, groupSize=16, **kwargs):
+    """Deconvolve the store at in_path into a store at out_path, streaming:
+    batches of batchSize (rounded up to whole output chunks) with a border of
+    round((size(psf) + 10) / 2) (RLdecon_large_in_memory.m:171-175, as
+    RLdecon_for_zarr_block.m:101-171 takes them from the store), read
+    groupSize at a time, each group deconvolved by ONE batched engine call,
+    and each batch's region written straight into the output store.  Only one
+    group's batches are held in host memory.  The output has the input's chunk
+    shape unless chunks is given, and the input's compressor (blosc or zlib;
+    zlib for other inputs) unless compressor is given.  Returns None."""
+    from .batch import decon_batch
+    from .chunks import XR_zarrChunkCoordinatesExtraction, _vec3, matlab_round
     m, _, _ = _meta(in_path)
+    shape = [int(n) for n in m["shape"]]
+    if len(shape) != 3:
+        raise ValueError("a 3-D store is required")
     if compressor is None:
         cid = (m.get("compressor") or {}).get("id")
         compressor = "blosc" if cid == "blosc" else "zlib"
-    im = read_zarr(in_path)
-    J = decon_large_in_memory(im, PSF_f, PSF_b, NUMIT, method=method, batchSize=batchSize, **kwargs)
-    write_zarr(out_path, J, chunks=chunks or m["chunks"], compressor=compressor)
-    return J
+    ochunks = [int(min(c, n)) for c, n in zip(chunks or m["chunks"], shape)]
+    bs = _vec3(batchSize, "batchSize")
+    bs = np.array([-(-int(b) // c) * c for b, c in zip(bs, ochunks)])  # whole output chunks per batch
+    psf_shape = np.shape(PSF_f)
+    border = matlab_round((np.asarray(psf_shape, dtype=np.float64) + 10) / 2)
+    batch, region, _ = XR_zarrChunkCoordinatesExtraction(shape, batchSize=bs, BlockSize=bs, sameBatchSize=False,
+                                                         BorderSize=border)
+    meta_out = None
+    for g0 in range(0, len(batch), max(1, int(groupSize))):
+        bb, rr = batch[g0:g0 + groupSize], region[g0:g0 + groupSize]
+        ims = [read_zarr(in_path, b) for b in bb]
+        outs = decon_batch(ims, PSF_f, PSF_b, NUMIT, method=method, **kwargs)
+        if meta_out is None:
+            meta_out = _create_store(out_path, shape, outs[0].dtype, ochunks, compressor, 1, ".", "F", 0)
+        for b, r, o in zip(bb, rr, outs):
+            lo = r[:3] - b[:3]
+            hi = lo + (r[3:] - r[:3]) + 1
+            _write_chunks(out_path, meta_out, np.asarray(o)[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]],
+                          origin=tuple(int(v) - 1 for v in r[:3]))

@@ -57,19 +57,22 @@ def test_large_in_memory_device_volume_matches_host():
 
 @pytest.mark.parametrize("compressor", ["zlib", "blosc"])
 def test_decon_zarr_equals_in_memory(tmp_path, compressor):
-    """A Zarr store deconvolved chunk-wise (read, one batched call, write)
-    equals the in-memory driver on the same array, bitwise; blosc stores (the
-    reference's default codec) are written back as blosc."""
+    """A Zarr store deconvolved batch group by batch group (read, one batched
+    call per group, regions written straight into the output store) equals
+    the in-memory driver on the same array, bitwise, whatever the group size;
+    blosc stores (the reference's default codec) are written back as blosc."""
     import json
     import petakit5d_amd as pk
     print(f"[decon_zarr] {compressor}", flush=True)
     im = _vol((80, 72, 36), 8)
     psf = synth.gaussian_psf((7, 7, 9), 1.0, 2.5)
     psf_b = np.ascontiguousarray(psf[::-1, ::-1, ::-1])
-    src, dst = str(tmp_path / "in.zarr"), str(tmp_path / "out.zarr")
+    src = str(tmp_path / "in.zarr")
     pk.write_zarr(src, im, chunks=(32, 32, 16), compressor=compressor)
-    J = pk.decon_zarr(src, dst, psf, psf_b, 2, batchSize=(40, 40, 20), Background=100)
-    R = pk.decon_large_in_memory(im, psf, psf_b, 2, batchSize=(40, 40, 20), Background=100)
-    np.testing.assert_array_equal(J, R)
-    np.testing.assert_array_equal(pk.read_zarr(dst), R)
-    assert json.load(open(f"{dst}/.zarray"))["compressor"]["id"] == compressor
+    R = pk.decon_large_in_memory(im, psf, psf_b, 2, batchSize=(64, 64, 32), Background=100)
+    for g in (16, 1):
+        dst = str(tmp_path / f"out{g}.zarr")
+        # batchSize (40, 40, 20) rounds up to whole 32 x 32 x 16 chunks: (64, 64, 32)
+        assert pk.decon_zarr(src, dst, psf, psf_b, 2, batchSize=(40, 40, 20), groupSize=g, Background=100) is None
+        np.testing.assert_array_equal(pk.read_zarr(dst), R)
+        assert json.load(open(f"{dst}/.zarray"))["compressor"]["id"] == compressor
