@@ -11,6 +11,7 @@ zeros and crops back (pad=True)."""
 from __future__ import annotations
 
 import hashlib
+import math
 import os
 
 import numpy as np
@@ -20,7 +21,7 @@ from .psfgen import psf_gen_new
 from .tiff_io import readtiff, writetiff
 from .decon import _is_torch, crop_psf_to_data, decon_lucy_function, decon_lucy_omw_function
 
-__all__ = ["RLdecon_volume", "omw_bp_filename", "psfgen_crop"]
+__all__ = ["RLdecon_volume", "omw_bp_filename", "psfgen_crop", "psfgen_dz_ratio"]
 
 _BP_CACHE: dict = {}
 
@@ -63,6 +64,16 @@ def _omw_bp(psf, wienerAlpha, skewed, OTFCumThresh, hannWinBounds, psfgenPath, p
     return b
 
 
+def psfgen_dz_ratio(SkewAngle, objectiveScan=False):
+    """dz_ratio = sind(abs(SkewAngle)) for stage scans, 1 for objective scans
+    (RLdecon.m:101-106); sind is exact at multiples of 90 degrees."""
+    if objectiveScan:
+        return 1.0
+    a = abs(float(SkewAngle)) % 360.0
+    exact = {0.0: 0.0, 90.0: 1.0, 180.0: 0.0, 270.0: -1.0}
+    return exact[a] if a in exact else math.sin(math.radians(a))
+
+
 def _read_volume(path):
     ext = os.path.splitext(path.rstrip("/"))[1].lower()
     if ext in (".tif", ".tiff"):
@@ -95,7 +106,7 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
                    OTFCumThresh=0.9, hannWinBounds=(0.8, 1.0), skewed=None, flipZstack=False, save16bit=True,
                    dampFactor=1.0, scaleFactor=None, deconOffset=0.0, deconBbox=None, EdgeErosion=0,
                    psfgenPath=None, psfFsn=None, psfGen=False, dzPSF=None, dz=None, Deskew=False, SkewAngle=32.45,
-                   xyPixelSize=0.108, Reverse=False, Crop=False, ctx=None):
+                   xyPixelSize=0.108, Reverse=False, Crop=False, objectiveScan=False, ctx=None):
     """deconvolved = RLdecon(..., 'rawdata', rawdata, ...) for one volume
     (rawdata an array, or a .tif / .zarr path; psf an array or a .tif path).
 
@@ -108,8 +119,12 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     save16bit, dampFactor, scaleFactor, deconOffset, bbox = deconBbox and
     EdgeErosion (:387-404); otherwise the raw data, cropped to deconBbox
     (:415-420).  psfGen: the PSF first goes through psf_gen_new(psf, dzPSF,
-    dz, 1.5, 'masked') and, for methods other than 'omw', psfgen_crop
-    (:240-260).  Deskew=True also returns the deskewed result,
+    dz * dz_ratio, 1.5, 'masked'), dz_ratio = sind(|SkewAngle|) for stage
+    scans and 1 for objectiveScan (:101-106, 247), and, for methods other than
+    'omw', psfgen_crop (:250-260); with psfgenPath and psfFsn the generated
+    PSF is loaded from psfgenPath/psfFsn_RLMethod.tif when that exists
+    (:219-229) and written there after generation for file inputs
+    (:272-279).  Deskew=True also returns the deskewed result,
     deskewFrame3D(single(deconvolved), SkewAngle, dz, xyPixelSize, Reverse,
     'Crop', Crop) (:448-449), as (deconvolved, ds).  Other methods ('original'
     deconvlucy, 'cudagen') and rotation are not this engine's."""
@@ -123,21 +138,31 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
                            EdgeErosion, psfgenPath, psfFsn, psfGen, dzPSF, dz, ctx=ctx)
         from .deskew import deskewFrame3D
         return J, deskewFrame3D(J, SkewAngle, dz, xyPixelSize, Reverse, Crop=Crop, ctx=ctx)
-    if isinstance(rawdata, (str, os.PathLike)):  # input file (RLdecon.m:335-343)
+    file_input = isinstance(rawdata, (str, os.PathLike))
+    if file_input:  # input file (RLdecon.m:335-343)
         rawdata = _read_volume(str(rawdata))
     if isinstance(psf, (str, os.PathLike)):  # PSF file (RLdecon.m:214-236)
         if psfFsn is None:
             psfFsn = os.path.splitext(os.path.basename(str(psf)))[0]
         psf = readtiff(str(psf))
     psf = np.asarray(psf.detach().cpu().numpy() if _is_torch(psf) else psf, dtype=np.float32)
+    gen_fn = os.path.join(psfgenPath, f"{psfFsn}_{RLMethod}.tif") if (psfGen and psfgenPath and psfFsn) else None
+    if gen_fn and os.path.exists(gen_fn):  # generated before (RLdecon.m:224-229)
+        psf = readtiff(gen_fn).astype(np.float32, copy=False)
+        psfGen = False
     if psfGen:
         if dzPSF is None or dz is None:
             raise ValueError("psfGen needs dzPSF and dz")
-        psf = psf_gen_new(psf, dzPSF, dz, 1.5, "masked")
+        psf = psf_gen_new(psf, dzPSF, dz * psfgen_dz_ratio(SkewAngle, objectiveScan), 1.5, "masked")
         if RLMethod != "omw":
             psf = np.ascontiguousarray(psfgen_crop(psf))
     imSize = tuple(int(n) for n in (rawdata.shape if _is_torch(rawdata) else np.shape(rawdata)))[:3]
     psf = np.asfortranarray(crop_psf_to_data(psf, imSize))
+    if gen_fn and psfGen and file_input and not os.path.exists(gen_fn):  # RLdecon.m:272-279
+        os.makedirs(psfgenPath, exist_ok=True)
+        tmp = gen_fn[:-4] + f"_{os.getpid()}.tif"
+        writetiff(psf, tmp)
+        os.replace(tmp, gen_fn)
     psf_b = None
     if RLMethod == "omw":
         psf_b = np.asfortranarray(crop_psf_to_data(

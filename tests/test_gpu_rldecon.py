@@ -81,3 +81,25 @@ def test_deskew_after_decon():
                                xyPixelSize=0.108, Crop=True)
     np.testing.assert_array_equal(J2, J)
     np.testing.assert_array_equal(ds, pk.deskewFrame3D(J, 32.45, 0.3, 0.108, Crop=True))
+
+
+def test_psfgen_file_cache(tmp_path):
+    """psfGen with a file input: the PSF goes through psf_gen_new with
+    dz * sind(SkewAngle), is written to psfgenPath/psfFsn_simplified.tif, and a
+    second call loads it from there (same bits out)."""
+    import petakit5d_amd as pk
+    raw, psf = _case((24, 24, 41), (1.0, 2.5), 0.0, (60, 49, 36))
+    psf = psf * np.float32(1000) + np.float32(100)  # a measured-looking PSF: background to remove
+    print("[rldecon psfGen cache]", flush=True)
+    pk.writetiff(raw, str(tmp_path / "raw.tif"))
+    pk.writetiff(psf, str(tmp_path / "psf.tif"))
+    gp = str(tmp_path / "psfgen")
+    kw = dict(Background=100, psfGen=True, dzPSF=0.1, dz=0.3, SkewAngle=32.45, psfgenPath=gp)
+    J1 = pk.RLdecon_volume(str(tmp_path / "raw.tif"), str(tmp_path / "psf.tif"), 3, **kw)
+    fn = os.path.join(gp, "psf_simplified.tif")
+    assert os.path.exists(fn)
+    from petakit5d_amd.rldecon import psfgen_crop, psfgen_dz_ratio
+    g = psfgen_crop(pk.psf_gen_new(psf, 0.1, 0.3 * psfgen_dz_ratio(32.45), 1.5, "masked"))
+    np.testing.assert_array_equal(pk.readtiff(fn), pk.crop_psf_to_data(np.ascontiguousarray(g), raw.shape))
+    J2 = pk.RLdecon_volume(str(tmp_path / "raw.tif"), str(tmp_path / "psf.tif"), 3, **kw)
+    np.testing.assert_array_equal(J2, J1)

@@ -35,3 +35,13 @@ def test_psfgen_crop_keeps_the_centre():
     c = psfgen_crop(p)
     assert c.shape == (8, 7, 7)
     np.testing.assert_array_equal(c, p[2:10, 1:8, :])
+
+
+def test_psfgen_dz_ratio():
+    """dz_ratio = sind(|SkewAngle|) for stage scans, 1 for objective scans (RLdecon.m:101-106)."""
+    import math
+    from petakit5d_amd.rldecon import psfgen_dz_ratio
+    assert psfgen_dz_ratio(32.45) == math.sin(32.45 * math.pi / 180)
+    assert psfgen_dz_ratio(-32.45) == psfgen_dz_ratio(32.45)
+    assert psfgen_dz_ratio(32.45, objectiveScan=True) == 1.0
+    assert psfgen_dz_ratio(90) == 1.0 and psfgen_dz_ratio(0) == 0.0
