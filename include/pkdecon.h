@@ -289,6 +289,17 @@ pk_status pk_deskew_out_dims(const int64_t dims[3], double angle, double dz, dou
 pk_status pk_deskew_frame3d(pk_ctx *ctx, const float *d_vol, const int64_t dims[3], double angle, double dz,
                             double xy_pixel_size, int32_t reverse, int32_t crop, float *d_out);
 
+/* Affine resampling as imwarp(vol, affine3d(T), 'linear', 'FillValues', 0,
+ * 'OutputView', imref3d(out_dims, 1, 1, 1)) does for rotateFrame3D
+ * (pointDetection/llsmtools/transforms/rotateFrame3D.m:46-100, used after
+ * deskew at microscopeDataProcessing/decon/RLdecon.m:483-485): inv_tform is
+ * inv(T), row-major 4x4 in the row-vector convention ([x y z 1] = [u v w 1] *
+ * inv_tform, x = column, y = row, z = page, 1-based); trilinear over the
+ * input padded with one voxel of zeros.  d_vol / d_out are device memory,
+ * column-major single; returns when done. */
+pk_status pk_affine_warp3d(pk_ctx *ctx, const float *d_vol, const int64_t dims[3], const double inv_tform[16],
+                           const int64_t out_dims[3], float *d_out);
+
 /* TIFF LZW strip codec (host code; no context, no device).  Decodes one LZW
  * strip as libtiff's codec does for the reference reader's
  * TIFFReadEncodedStrip calls (io/cpp-tiff/src/parallelreadtiff.cpp:283-300),

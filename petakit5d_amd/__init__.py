@@ -14,6 +14,7 @@ from .tiff_io import readtiff, writetiff  # noqa: F401
 from .zarr_io import decon_zarr, read_zarr, write_zarr  # noqa: F401
 from .serve import Server  # noqa: F401
 from .deskew import deskewFrame3D  # noqa: F401
+from .rotate import rotateFrame3D  # noqa: F401
 from .decon import (  # noqa: F401
     crop_psf_to_data,
     decon_lucy_function,
@@ -45,6 +46,7 @@ __all__ = [
     "decon_zarr",
     "readtiff",
     "deskewFrame3D",
+    "rotateFrame3D",
     "writetiff",
     "Server",
     "crop_psf_to_data",

@@ -72,6 +72,7 @@ EXPORTS = [
     "pk_lz4_encode_bound",
     "pk_deskew_out_dims",
     "pk_deskew_frame3d",
+    "pk_affine_warp3d",
 ]
 
 

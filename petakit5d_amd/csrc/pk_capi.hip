@@ -3,6 +3,7 @@
 // exceptions into a pk_status plus pk_last_error() text, and never aborts.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -19,6 +20,11 @@ struct DeskewGeom {
 };
 DeskewGeom deskew_geometry(const int64_t dims[3], double angle, double dz, double xy, bool reverse, bool crop);
 void launch_deskew(const float* vol, float* out, const int64_t dims[3], const DeskewGeom& g, hipStream_t s);
+struct Affine3 {
+    double a[4][4];
+};
+void launch_affine3d(const float* vol, float* out, const int64_t dims[3], const int64_t odims[3], const Affine3& m,
+                     hipStream_t s);
 }  // namespace pk
 
 namespace {
@@ -306,6 +312,22 @@ pk_status pk_deskew_frame3d(pk_ctx* c, const float* d_vol, const int64_t dims[3]
         const pk::DeskewGeom g = pk::deskew_geometry(dims, angle, dz, xy_pixel_size, reverse != 0, crop != 0);
         if (g.nx_out < 1) pk::fail(PK_ERR_INVALID_ARG, "deskewed width below one column");
         pk::launch_deskew(d_vol, d_out, dims, g, c->stream);
+        PK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+pk_status pk_affine_warp3d(pk_ctx* c, const float* d_vol, const int64_t dims[3], const double inv_tform[16],
+                           const int64_t out_dims[3], float* d_out) {
+    return guarded(c, [&] {
+        if (!d_vol || !d_out || !dims || !out_dims || !inv_tform) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        for (int i = 0; i < 3; ++i)
+            if (dims[i] < 1 || out_dims[i] < 1) pk::fail(PK_ERR_INVALID_ARG, "volume dimensions must be >= 1");
+        pk::Affine3 m;
+        for (int i = 0; i < 16; ++i) {
+            if (!std::isfinite(inv_tform[i])) pk::fail(PK_ERR_INVALID_ARG, "transform entries must be finite");
+            m.a[i / 4][i % 4] = inv_tform[i];
+        }
+        pk::launch_affine3d(d_vol, d_out, dims, out_dims, m, c->stream);
         PK_HIP(hipStreamSynchronize(c->stream));
     });
 }

@@ -26,6 +26,10 @@
 
 namespace pk {
 
+struct Affine3 {
+    double a[4][4];  // row-vector convention: [x y z 1] = [u v w 1] * a
+};
+
 struct DeskewGeom {
     int64_t nx_out;
     double xstep, xshift;
@@ -104,6 +108,68 @@ void launch_deskew(const float* vol, float* out, const int64_t dims[3], const De
     else
         hipLaunchKernelGGL(k_deskew<false>, dim3((unsigned)cols), dim3(nt), 0, s, vol, out, dims[0], dims[1],
                            g.nx_out, g.xstep, g.xshift);
+    PK_HIP(hipGetLastError());
+}
+
+}  // namespace pk
+
+namespace pk {
+
+// Affine resampling of a column-major volume, imwarp-style: output voxel
+// (row r, column c, page p), 1-based world point (u, v, w) = (c, r, p), reads
+// the input at [x y z 1] = [u v w 1] * inv_m (row-vector convention, x the
+// column, y the row, z the page), trilinear over the input padded with one
+// voxel of the fill value 0 on every side (rotateFrame3D.m:46-100 resamples
+// through imwarp this way).  One block per output column; threads walk down it.
+__global__ void __launch_bounds__(256) k_affine3d(const float* __restrict__ vol, float* __restrict__ out, int64_t ny,
+                                                  int64_t nx, int64_t nz, int64_t oy, int64_t ox, Affine3 m) {
+    const int64_t col = blockIdx.x;
+    const int64_t c = col % ox, p = col / ox;
+    const double u = (double)(c + 1), w = (double)(p + 1);
+    const double bx = u * m.a[0][0] + w * m.a[2][0] + m.a[3][0];
+    const double by = u * m.a[0][1] + w * m.a[2][1] + m.a[3][1];
+    const double bz = u * m.a[0][2] + w * m.a[2][2] + m.a[3][2];
+    float* __restrict__ o = out + col * oy;
+    for (int64_t r = threadIdx.x; r < oy; r += blockDim.x) {
+        const double v = (double)(r + 1);
+        const double x = bx + v * m.a[1][0], y = by + v * m.a[1][1], z = bz + v * m.a[1][2];
+        float val = 0.f;
+        if (x > -1.0 && x < (double)(nx + 2) && y > -1.0 && y < (double)(ny + 2) && z > -1.0 && z < (double)(nz + 2)) {
+            const double fx0 = std::floor(x), fy0 = std::floor(y), fz0 = std::floor(z);
+            const int64_t ix = (int64_t)fx0, iy = (int64_t)fy0, iz = (int64_t)fz0;  // 1-based lower corner
+            const float fx = (float)(x - fx0), fy = (float)(y - fy0), fz = (float)(z - fz0);
+            float cval[2][2][2];
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+                    for (int dy = 0; dy < 2; ++dy) {
+                        const int64_t jx = ix + dx, jy = iy + dy, jz = iz + dz;
+                        const bool in = jx >= 1 && jx <= nx && jy >= 1 && jy <= ny && jz >= 1 && jz <= nz;
+                        cval[dz][dx][dy] = in ? vol[(jy - 1) + ny * ((jx - 1) + nx * (jz - 1))] : 0.f;
+                    }
+            float pl[2];
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz) {
+                const float a = cval[dz][0][0] + fy * (cval[dz][0][1] - cval[dz][0][0]);
+                const float b = cval[dz][1][0] + fy * (cval[dz][1][1] - cval[dz][1][0]);
+                pl[dz] = a + fx * (b - a);
+            }
+            val = pl[0] + fz * (pl[1] - pl[0]);
+        }
+        o[r] = val;
+    }
+}
+
+void launch_affine3d(const float* vol, float* out, const int64_t dims[3], const int64_t odims[3], const Affine3& m,
+                     hipStream_t s) {
+    const int64_t cols = odims[1] * odims[2];
+    if (cols <= 0 || odims[0] <= 0) return;
+    const unsigned nt = odims[0] >= 256 ? 256u : (unsigned)((odims[0] + 63) / 64 * 64);
+    if (cols * (int64_t)nt > 0xFFFFFFFFll) fail(PK_ERR_UNSUPPORTED_SIZE, "output volume has too many columns");
+    hipLaunchKernelGGL(k_affine3d, dim3((unsigned)cols), dim3(nt), 0, s, vol, out, dims[0], dims[1], dims[2], odims[0],
+                       odims[1], m);
     PK_HIP(hipGetLastError());
 }
 
