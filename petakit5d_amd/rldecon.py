@@ -74,6 +74,13 @@ def psfgen_dz_ratio(SkewAngle, objectiveScan=False):
     return exact[a] if a in exact else math.sin(math.radians(a))
 
 
+def _to_uint16(a):
+    """MATLAB uint16(single): round half away from zero, saturate to [0, 65535]."""
+    a = np.asarray(a, dtype=np.float32)
+    r = np.sign(a) * np.floor(np.abs(a) + np.float32(0.5))
+    return np.asfortranarray(np.clip(np.nan_to_num(r), 0, 65535).astype(np.uint16))
+
+
 def _read_volume(path):
     ext = os.path.splitext(path.rstrip("/"))[1].lower()
     if ext in (".tif", ".tiff"):
@@ -106,7 +113,8 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
                    OTFCumThresh=0.9, hannWinBounds=(0.8, 1.0), skewed=None, flipZstack=False, save16bit=True,
                    dampFactor=1.0, scaleFactor=None, deconOffset=0.0, deconBbox=None, EdgeErosion=0,
                    psfgenPath=None, psfFsn=None, psfGen=False, dzPSF=None, dz=None, Deskew=False, SkewAngle=32.45,
-                   xyPixelSize=0.108, Reverse=False, Crop=False, objectiveScan=False, ctx=None):
+                   xyPixelSize=0.108, Reverse=False, Crop=False, objectiveScan=False, Rotate=False,
+                   DSRCombined=True, ctx=None):
     """deconvolved = RLdecon(..., 'rawdata', rawdata, ...) for one volume
     (rawdata an array, or a .tif / .zarr path; psf an array or a .tif path).
 
@@ -124,20 +132,42 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     'omw', psfgen_crop (:250-260); with psfgenPath and psfFsn the generated
     PSF is loaded from psfgenPath/psfFsn_RLMethod.tif when that exists
     (:219-229) and written there after generation for file inputs
-    (:272-279).  Deskew=True also returns the deskewed result,
-    deskewFrame3D(single(deconvolved), SkewAngle, dz, xyPixelSize, Reverse,
-    'Crop', Crop) (:448-449), as (deconvolved, ds).  Other methods ('original'
-    deconvlucy, 'cudagen') and rotation are not this engine's."""
+    (:272-279).  Deskew / Rotate return (deconvolved, ds, dsr) (None for a
+    step not run): ds = deskewFrame3D(single(deconvolved), SkewAngle, dz,
+    xyPixelSize, Reverse, 'Crop', Crop) (:448-449), dsr = rotateFrame3D(ds,
+    SkewAngle, zAniso, Reverse, 'Crop', true) with zAniso = dz_ratio dz /
+    xyPixelSize (:107, 483-486), or of the deconvolved volume itself for
+    objective scans (:494-497); uint16 when save16bit (:464-468, 498-501).
+    objectiveScan turns Deskew off (:147-149); Rotate of a stage scan needs
+    DSRCombined=False (the combined deskewRotateFrame3D, :488-493, is not this
+    engine's).  Other methods ('original' deconvlucy, 'cudagen') are not this
+    engine's either."""
     if RLMethod not in ("simplified", "omw"):
         raise ValueError(f"RLMethod '{RLMethod}': this engine serves 'simplified' and 'omw'")
-    if Deskew:
+    if objectiveScan:
+        Deskew = False
+    if Deskew or Rotate:
         if dz is None:
-            raise ValueError("Deskew needs dz")
+            raise ValueError("Deskew / Rotate need dz")
+        if Rotate and not objectiveScan and (DSRCombined or not Deskew):
+            raise NotImplementedError("Rotate of a stage scan runs deskewRotateFrame3D (DSRCombined) in the "
+                                      "reference; pass Deskew=True, DSRCombined=False for deskew then rotate")
         J = RLdecon_volume(rawdata, psf, nIter, RLMethod, Background, wienerAlpha, OTFCumThresh, hannWinBounds,
                            skewed, flipZstack, save16bit, dampFactor, scaleFactor, deconOffset, deconBbox,
-                           EdgeErosion, psfgenPath, psfFsn, psfGen, dzPSF, dz, ctx=ctx)
+                           EdgeErosion, psfgenPath, psfFsn, psfGen, dzPSF, dz, SkewAngle=SkewAngle,
+                           objectiveScan=objectiveScan, ctx=ctx)
         from .deskew import deskewFrame3D
-        return J, deskewFrame3D(J, SkewAngle, dz, xyPixelSize, Reverse, Crop=Crop, ctx=ctx)
+        from .rotate import rotateFrame3D
+        zAniso = psfgen_dz_ratio(SkewAngle, objectiveScan) * dz / xyPixelSize
+        ds = deskewFrame3D(J, SkewAngle, dz, xyPixelSize, Reverse, Crop=Crop, ctx=ctx) if Deskew else None
+        dsr = None
+        if Rotate:
+            dsr = rotateFrame3D(J if objectiveScan else ds, SkewAngle, zAniso, Reverse, Crop=True,
+                                ObjectiveScan=objectiveScan, ctx=ctx)
+        if save16bit:
+            ds = None if ds is None else _to_uint16(ds)
+            dsr = None if dsr is None else _to_uint16(dsr)
+        return J, ds, dsr
     file_input = isinstance(rawdata, (str, os.PathLike))
     if file_input:  # input file (RLdecon.m:335-343)
         rawdata = _read_volume(str(rawdata))

@@ -77,10 +77,25 @@ def test_deskew_after_decon():
     raw, psf = _case()
     print("[rldecon deskew]", flush=True)
     J = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=False)
-    J2, ds = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=False, Deskew=True, SkewAngle=32.45, dz=0.3,
-                               xyPixelSize=0.108, Crop=True)
+    J2, ds, dsr = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=False, Deskew=True, SkewAngle=32.45,
+                                    dz=0.3, xyPixelSize=0.108, Crop=True)
     np.testing.assert_array_equal(J2, J)
+    assert dsr is None
     np.testing.assert_array_equal(ds, pk.deskewFrame3D(J, 32.45, 0.3, 0.108, Crop=True))
+    # deskew then rotate (DSRCombined false), uint16 outputs (RLdecon.m:438-501)
+    J3, ds3, dsr3 = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=False, Deskew=True, Rotate=True,
+                                      DSRCombined=False, SkewAngle=32.45, dz=0.3, xyPixelSize=0.108)
+    zan = np.sin(32.45 * np.pi / 180) * 0.3 / 0.108
+    np.testing.assert_array_equal(dsr3, pk.rotateFrame3D(ds3, 32.45, zan, False, Crop=True))
+    # objective scan: no deskew, the deconvolved volume rotated with zAniso = dz / xyPixelSize
+    _, ds4, dsr4 = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=True, Rotate=True, objectiveScan=True,
+                                     dz=0.3, xyPixelSize=0.108)
+    Ju = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=True)
+    assert ds4 is None and dsr4.dtype == np.uint16
+    r = pk.rotateFrame3D(Ju, 32.45, 0.3 / 0.108, False, Crop=True, ObjectiveScan=True)
+    np.testing.assert_array_equal(dsr4, np.clip(np.floor(r + 0.5), 0, 65535).astype(np.uint16))
+    with pytest.raises(NotImplementedError):
+        pk.RLdecon_volume(raw, psf, 1, Rotate=True, dz=0.3)
 
 
 def test_psfgen_file_cache(tmp_path):
