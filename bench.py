@@ -19,11 +19,12 @@ Other workloads (--workload): rl128 (Biggs RL 15 it. on 128x128x64), batched
 (64 random-size chunks, OMW-2 and RL-10 in batched launches), served
 (continuous-batching server, Poisson arrivals, p50/p99 vs the plain path),
 large (2048x2048x512, OMW-2 and Biggs-10), deskew (deskewFrame3D of a
-512x512x256 decon result).
+512x512x256 decon result), rotate (rotateFrame3D of the deskewed volume).
 """
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import threading
@@ -52,7 +53,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="omw", choices=["omw", "rl128", "batched", "served", "large", "copy",
-                                                                  "deskew"])
+                                                                  "deskew", "rotate"])
     ap.add_argument("--shape", type=int, nargs=3, default=None)
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--duration", type=float, default=20.0, help="served: seconds of Poisson arrivals")
@@ -668,6 +669,57 @@ def wl_deskew(args, torch, pk, L):
     return res
 
 
+def wl_rotate(args, torch, pk, L):
+    """rotateFrame3D of a deskewed 512x512x256 stage-scan volume (the deskew
+    workload's output, 512x1110x256) at SkewAngle 32.45, zAniso = sind(32.45)
+    0.3 / 0.108, automatic crop: one pk_affine_warp3d per step on device
+    buffers, kernel time from HIP events on the engine's stream."""
+    import ctypes
+    from petakit5d_amd.rotate import rotate_transform
+    shape, geo = (512, 512, 256), (32.45, 0.3, 0.108)
+    g = torch.Generator(device="cuda").manual_seed(6)
+    vol = torch.rand((shape[2], shape[1], shape[0]), device="cuda", generator=g).permute(2, 1, 0)
+    ds = pk.deskewFrame3D(vol, *geo)
+    dsh = tuple(int(n) for n in ds.shape)
+    zan = math.sin(geo[0] * math.pi / 180) * geo[1] / geo[2]
+    T, od = rotate_transform(dsh, geo[0], zan, proj=ds.amax(dim=0).cpu().numpy())
+    inv = np.ascontiguousarray(np.linalg.inv(T))
+    out = torch.empty((od[2], od[1], od[0]), device="cuda").permute(2, 1, 0)
+    ctx = pk.default_context()
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+
+    def call():
+        ctx.check(L.pk_affine_warp3d(ctx.handle, ctypes.c_void_p(ds.data_ptr()), _lib_i64x3(dsh),
+                                     inv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _lib_i64x3(od),
+                                     ctypes.c_void_p(out.data_ptr())))
+
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in evs:
+        a.record(stream)
+        call()
+        b.record(stream)
+    torch.cuda.synchronize()
+    kms = sum(a.elapsed_time(b) for a, b in evs) / len(evs)
+    ctx.set_stream(None)
+    nbytes = 4.0 * (ds.numel() + out.numel())
+    res = base_result("rotated Mvoxel/s (rotateFrame3D, linear)", out.numel() / ms / 1e3, "Mvoxel/s", args, ms, {
+        "workload": "rotateFrame3D of a deskewed 512x1110x256 single volume, SkewAngle 32.45, zAniso 1.49, "
+                    "automatic crop; device-resident in and out", "in": list(dsh), "out": list(od)})
+    res["roofline"] = {"bound": "hbm", "kernel": "k_affine3d", "achieved": round(nbytes / kms / 1e6, 1),
+                       "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(nbytes / kms / 1e6 / HBM_PEAK_GBPS, 4),
+                       "traffic": None, "algorithmic_bytes": nbytes, "kernel_ms": round(kms, 4)}
+    return res
+
+
 def _lib_i64x3(v):
     from petakit5d_amd._lib import i64x3
     return i64x3(v)
@@ -687,7 +739,7 @@ def main():
 
     L = _lib.load()
     res = {"omw": wl_omw, "rl128": wl_rl128, "batched": wl_batched, "served": wl_served, "large": wl_large,
-           "copy": wl_copy, "deskew": wl_deskew}[
+           "copy": wl_copy, "deskew": wl_deskew, "rotate": wl_rotate}[
         args.workload](args, torch, pk, L)
     print(json.dumps(res), flush=True)
 
