@@ -75,7 +75,13 @@ def psfgen_dz_ratio(SkewAngle, objectiveScan=False):
 
 
 def _to_uint16(a):
-    """MATLAB uint16(single): round half away from zero, saturate to [0, 65535]."""
+    """MATLAB uint16(single): round half away from zero, saturate to [0, 65535],
+    NaN -> 0.  A CUDA tensor stays on its device (torch uint16 result)."""
+    if _is_torch(a):
+        import torch
+        t = a.to(torch.float32)
+        r = torch.sign(t) * torch.floor(t.abs() + 0.5)
+        return torch.nan_to_num(r, nan=0.0).clamp_(0, 65535).to(torch.int32).to(torch.uint16)
     a = np.asarray(a, dtype=np.float32)
     r = np.sign(a) * np.floor(np.abs(a) + np.float32(0.5))
     return np.asfortranarray(np.clip(np.nan_to_num(r), 0, 65535).astype(np.uint16))
@@ -144,6 +150,7 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     engine's either."""
     if RLMethod not in ("simplified", "omw"):
         raise ValueError(f"RLMethod '{RLMethod}': this engine serves 'simplified' and 'omw'")
+    SkewAngle = abs(float(SkewAngle))  # RLdecon.m:101
     if objectiveScan:
         Deskew = False
     if Deskew or Rotate:

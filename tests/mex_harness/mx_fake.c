@@ -69,7 +69,70 @@ static int call(int nlhs, mxArray **plhs, int nrhs, const mxArray **prhs) {
     return 0;
 }
 
-int main(void) {
+static void *slurp(const char *dir, const char *name, size_t bytes) {
+    char path[1024];
+    snprintf(path, sizeof(path), "%s/%s", dir, name);
+    FILE *f = fopen(path, "rb");
+    if (!f) { printf("missing %s\n", path); exit(2); }
+    void *p = malloc(bytes);
+    if (fread(p, 1, bytes, f) != bytes) { printf("short %s\n", path); exit(2); }
+    fclose(f);
+    return p;
+}
+
+static void spill(const char *dir, const char *name, const void *p, size_t bytes) {
+    char path[1024];
+    snprintf(path, sizeof(path), "%s/%s", dir, name);
+    FILE *f = fopen(path, "wb");
+    if (!f || fwrite(p, 1, bytes, f) != bytes) { printf("cannot write %s\n", path); exit(2); }
+    fclose(f);
+}
+
+static mxArray *opts_struct(double background, double save16bit) {
+    mxArray *s = calloc(1, sizeof(*s));
+    s->cls = mxSTRUCT_CLASS; s->nd = 2; s->dims[0] = s->dims[1] = 1;
+    s->fname[0] = "Background"; s->fval[0] = dbl(background);
+    s->fname[1] = "save16bit"; s->fval[1] = dbl(save16bit);
+    s->nf = 2;
+    return s;
+}
+
+/* files <dir> n1 n2 n3 p1 p2 p3 numit_rl numit_omw background: reads I.u16,
+ * psf.f32 and psfb.f32 from dir, runs 'rl' and 'omw' through mexFunction with
+ * single outputs, writes J_rl.f32 and J_omw.f32 (column-major) and
+ * err_rl.f64 (the err_mat output). */
+static int files_mode(int argc, char **argv) {
+    if (argc != 12) { printf("usage: files dir n1 n2 n3 p1 p2 p3 numit_rl numit_omw background\n"); return 2; }
+    const char *dir = argv[2];
+    mwSize d[3], pd[3];
+    for (int i = 0; i < 3; ++i) { d[i] = (mwSize)atol(argv[3 + i]); pd[i] = (mwSize)atol(argv[6 + i]); }
+    const double it_rl = atof(argv[9]), it_omw = atof(argv[10]), bg = atof(argv[11]);
+    const size_t n = (size_t)d[0] * d[1] * d[2], np = (size_t)pd[0] * pd[1] * pd[2];
+    mxArray *I = mxCreateNumericArray(3, d, mxUINT16_CLASS, mxREAL);
+    mxArray *P = mxCreateNumericArray(3, pd, mxSINGLE_CLASS, mxREAL);
+    mxArray *PB = mxCreateNumericArray(3, pd, mxSINGLE_CLASS, mxREAL);
+    void *t = slurp(dir, "I.u16", n * 2); memcpy(I->data, t, n * 2); free(t);
+    t = slurp(dir, "psf.f32", np * 4); memcpy(P->data, t, np * 4); free(t);
+    t = slurp(dir, "psfb.f32", np * 4); memcpy(PB->data, t, np * 4); free(t);
+    mxArray *s = opts_struct(bg, 0);
+    mxArray *out[4] = {0};
+    const mxArray *a1[] = {str("rl"), I, P, dbl(it_rl), s};
+    if (call(2, out, 5, a1)) return 1;
+    spill(dir, "J_rl.f32", out[0]->data, n * 4);
+    spill(dir, "err_rl.f64", out[1]->data, (size_t)it_rl * 4 * 8);
+    printf("rl ok %zu %zu %zu\n", out[0]->dims[0], out[0]->dims[1], out[0]->dims[2]);
+    const mxArray *a2[] = {str("omw"), I, P, PB, dbl(it_omw), s};
+    if (call(1, out, 6, a2)) return 1;
+    spill(dir, "J_omw.f32", out[0]->data, n * 4);
+    printf("omw ok %zu %zu %zu\n", out[0]->dims[0], out[0]->dims[1], out[0]->dims[2]);
+    const mxArray *a3[] = {str("reset")};
+    if (call(0, out, 1, a3)) return 1;
+    printf("reset ok\n");
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    if (argc > 1 && !strcmp(argv[1], "files")) return files_mode(argc, argv);
     mxArray *out[4] = {0};
     const mxArray *a1[] = {str("fgfn"), dbl(101), dbl(1), dbl(0), dbl(7)};
     if (!call(1, out, 5, a1)) printf("fgfn %g\n", ((double *)out[0]->data)[0]);

@@ -17,7 +17,7 @@ def _vol(shape, seed):
 
 
 @pytest.mark.parametrize("method,numit", [("omw", 2), ("rl", 3)])
-def test_large_in_memory_equals_per_batch_calls(method, numit):
+def test_large_in_memory_equals_per_batch_calls(oracle, method, numit):
     import petakit5d_amd as pk
     from petakit5d_amd.chunks import XR_zarrChunkCoordinatesExtraction, matlab_round
     shape, bs = (150, 130, 60), (64, 64, 32)
@@ -40,6 +40,23 @@ def test_large_in_memory_equals_per_batch_calls(method, numit):
             R, _, _ = pk.decon_lucy_function(c, pf, numit, Background=100, save16bit=False, pad=True)
         want = R[lo[0] - 1:lo[3], lo[1] - 1:lo[4], lo[2] - 1:lo[5]]
         np.testing.assert_array_equal(J[r[0] - 1:r[3], r[1] - 1:r[4], r[2] - 1:r[5]], want)
+    # the first and the last batch against the oracle (batch zero-padded to
+    # smooth sizes, result cropped back, region taken)
+    from conftest import rel_l2
+    for b, r, lo in (zip(batch, region, local).__next__(), (batch[-1], region[-1], local[-1])):
+        c = np.asfortranarray(im[b[0] - 1:b[3], b[1] - 1:b[4], b[2] - 1:b[5]])
+        pd = tuple(pk.findGoodFactorNumber(n, 1, False, maxPrime=5) for n in c.shape)
+        P = np.zeros(pd, c.dtype, order="F")
+        P[:c.shape[0], :c.shape[1], :c.shape[2]] = c
+        pf, pb = pk.crop_psf_to_data(psf, c.shape), pk.crop_psf_to_data(psf_b, c.shape)
+        if method == "omw":
+            R, _ = oracle.decon_lucy_omw_function(P, pf, pb, numit, Background=100, save16bit=False)
+        else:
+            R, _, _ = oracle.decon_lucy_function(P, pf, numit, Background=100, save16bit=False)
+        want = R[lo[0] - 1:lo[3], lo[1] - 1:lo[4], lo[2] - 1:lo[5]]
+        e = rel_l2(J[r[0] - 1:r[3], r[1] - 1:r[4], r[2] - 1:r[5]], want)
+        print(f"  batch {b}: region rel-L2 vs oracle FP32 {e:.3e}", flush=True)
+        assert e <= 1e-4
 
 
 def test_large_in_memory_device_volume_matches_host():

@@ -124,6 +124,28 @@ def test_epilogue_options_vs_oracle(oracle, kw):
         assert rel_l2(J, R) <= TOL
 
 
+@pytest.mark.parametrize("method", ["rl", "omw"])
+@pytest.mark.parametrize("bg", [99.37, 0.0])
+def test_single_input_fractional_background_vs_oracle(oracle, method, bg):
+    """A single-class input takes max(I - Background, 0) in single precision
+    (decon_lucy_function.m:93-94, decon_lucy_omw_function.m:86-87), so a
+    non-integer Background is subtracted as is, not rounded to uint16."""
+    import petakit5d_amd as pk
+    print(f"[single input] {method} Background={bg}", flush=True)
+    I16, psf = _case((48, 40, 30), (9, 9, 11), seed=17)
+    rng = np.random.default_rng(3)
+    I = (I16.astype(np.float32) + rng.random(I16.shape, dtype=np.float32)).astype(np.float32, order="F")
+    if method == "rl":
+        J, _, _ = pk.decon_lucy_function(I, psf, 5, Background=bg, save16bit=False)
+        R, _, _ = oracle.decon_lucy_function(I, psf, 5, Background=bg, save16bit=False)
+    else:
+        J, _ = pk.decon_lucy_omw_function(I, psf, _flip(psf), 2, Background=bg, save16bit=False)
+        R, _ = oracle.decon_lucy_omw_function(I, psf, _flip(psf), 2, Background=bg, save16bit=False)
+    e = rel_l2(J, R)
+    print(f"  rel-L2 vs CPU FP32 {e:.3e}")
+    assert e <= TOL
+
+
 def test_device_resident_matches_host(oracle):
     import torch
     import petakit5d_amd as pk

@@ -94,8 +94,6 @@ def test_deskew_after_decon():
     assert ds4 is None and dsr4.dtype == np.uint16
     r = pk.rotateFrame3D(Ju, 32.45, 0.3 / 0.108, False, Crop=True, ObjectiveScan=True)
     np.testing.assert_array_equal(dsr4, np.clip(np.floor(r + 0.5), 0, 65535).astype(np.uint16))
-    with pytest.raises(NotImplementedError):
-        pk.RLdecon_volume(raw, psf, 1, Rotate=True, dz=0.3)
 
 
 def test_psfgen_file_cache(tmp_path):
@@ -118,3 +116,61 @@ def test_psfgen_file_cache(tmp_path):
     np.testing.assert_array_equal(pk.readtiff(fn), pk.crop_psf_to_data(np.ascontiguousarray(g), raw.shape))
     J2 = pk.RLdecon_volume(str(tmp_path / "raw.tif"), str(tmp_path / "psf.tif"), 3, **kw)
     np.testing.assert_array_equal(J2, J1)
+
+
+def _oracle_padded(oracle, method, raw, psf, psf_b, numit, **kw):
+    """The oracle on the zero-padded (2/3/5-smooth) volume, cropped back: the
+    engine's treatment of a non-smooth size."""
+    import petakit5d_amd as pk
+    d = raw.shape
+    pd = tuple(pk.findGoodFactorNumber(n, 1, False, maxPrime=5) for n in d)
+    P = np.zeros(pd, raw.dtype, order="F")
+    P[:d[0], :d[1], :d[2]] = raw
+    if method == "omw":
+        R, _ = oracle.decon_lucy_omw_function(P, psf, psf_b, numit, **kw)
+    else:
+        R, _, _ = oracle.decon_lucy_function(P, psf, numit, **kw)
+    return R[:d[0], :d[1], :d[2]]
+
+
+@pytest.mark.parametrize("method", ["simplified", "omw"])
+def test_rldecon_vs_oracle(oracle, method):
+    """RLdecon_volume on a 60x49x36 volume (49 is padded to 50) with an
+    un-normalised PSF against the oracle on the same PSF crop / normalisation /
+    back projector (RLdecon.m:264-270, 367-373, 387-404)."""
+    import petakit5d_amd as pk
+    from conftest import rel_l2
+    raw, psf = _case((24, 24, 31), (1.0, 2.5), 15.0, (60, 49, 36))
+    print(f"[rldecon vs oracle] {method}", flush=True)
+    J = pk.RLdecon_volume(raw, psf, 3, RLMethod=method, Background=100, save16bit=False)
+    pc = pk.crop_psf_to_data(psf, raw.shape)
+    pn = np.asfortranarray(pc / pc.sum(dtype=np.float32))
+    pb = None
+    if method == "omw":
+        pb = pk.crop_psf_to_data(pk.omw_backprojector_generation(pc, 0.005, None, OTFCumThresh=0.9)[0], raw.shape)
+    R = _oracle_padded(oracle, "omw" if method == "omw" else "rl", raw, pn, pb, 3, Background=100, save16bit=False)
+    e = rel_l2(J, R)
+    print(f"  rel-L2 vs CPU FP32 {e:.3e}")
+    assert e <= 1e-4
+
+
+def test_rotate_negative_angle_and_device_tensor():
+    """SkewAngle is taken as abs(SkewAngle) before deskew and rotation
+    (RLdecon.m:101); a CUDA tensor input gives device outputs whose uint16
+    conversion (round half away, saturate) equals the host path's."""
+    import torch
+    import petakit5d_amd as pk
+    raw, psf = _case()
+    print("[rldecon negative angle / tensor]", flush=True)
+    kw = dict(Background=100, Deskew=True, Rotate=True, DSRCombined=False, dz=0.3, xyPixelSize=0.108)
+    Jp, dsp, dsrp = pk.RLdecon_volume(raw, psf, 2, SkewAngle=32.45, **kw)
+    Jn, dsn, dsrn = pk.RLdecon_volume(raw, psf, 2, SkewAngle=-32.45, **kw)
+    np.testing.assert_array_equal(dsn, dsp)
+    np.testing.assert_array_equal(dsrn, dsrp)
+    assert dsp.dtype == np.uint16 and dsrp.dtype == np.uint16
+    rt = torch.from_numpy(np.ascontiguousarray(raw.transpose(2, 1, 0))).cuda().permute(2, 1, 0)
+    Jt, dst, dsrt = pk.RLdecon_volume(rt, psf, 2, SkewAngle=32.45, **kw)
+    assert dst.is_cuda and dst.dtype == torch.uint16 and dsrt.dtype == torch.uint16
+    np.testing.assert_array_equal(Jt.cpu().numpy(), Jp)
+    np.testing.assert_array_equal(dst.cpu().numpy(), dsp)
+    np.testing.assert_array_equal(dsrt.cpu().numpy(), dsrp)

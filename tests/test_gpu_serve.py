@@ -13,7 +13,7 @@ def _vol(shape, seed):
     return synth.rng(seed, 0).integers(80, 400, size=shape).astype(np.uint16)
 
 
-def test_mixed_stream_equals_single_calls():
+def test_mixed_stream_equals_single_calls(oracle):
     import petakit5d_amd as pk
     print("[serve] mixed stream", flush=True)
     psf1 = synth.gaussian_psf((9, 9, 11), 1.0, 2.5)
@@ -38,6 +38,18 @@ def test_mixed_stream_equals_single_calls():
         np.testing.assert_array_equal(J, R)
     st = srv.stats()
     assert st["completed"] == 6 and st["failed"] == 0
+    # served results against the oracle (one request per method)
+    import pk_oracle
+    from conftest import rel_l2
+    for req, I, psf, pb, meth, it in (jobs[0], jobs[1]):
+        J = req.result(timeout_s=60)
+        if meth == "omw":
+            R, _ = pk_oracle.decon_lucy_omw_function(I, psf, pb, it, Background=100, save16bit=False)
+        else:
+            R, _, _ = pk_oracle.decon_lucy_function(I, psf, it, Background=100, save16bit=False)
+        e = rel_l2(J, R)
+        print(f"  served {meth}-{it} {I.shape}: rel-L2 vs oracle FP32 {e:.3e}", flush=True)
+        assert e <= 1e-4
     srv.stop()
     ctx.close()
 
