@@ -115,7 +115,36 @@ def bwconvhull(img2):
 
 
 def knn2(edge_pts, query_pts, chunk=4096):
-    """Two nearest edge points per query (Euclidean, double), ties -> lower index."""
+    """Two nearest edge points per query (Euclidean, double), ties -> lower index.
+
+    Candidates come from a k-d tree (scipy cKDTree, 8 nearest); their squared
+    distances are recomputed with the same formula as the exhaustive search
+    and ranked by (distance, index).  A query whose 8th candidate is not
+    clearly farther than its 2nd falls back to the exhaustive search, so the
+    result is that of the exhaustive search in every case."""
+    E = np.asarray(edge_pts, np.float64)
+    Qp = np.asarray(query_pts, np.float64)
+    if E.shape[0] <= 8 or Qp.shape[0] < 4096:
+        return _knn2_exhaustive(E, Qp, chunk)
+    from scipy.spatial import cKDTree
+    dk, ik = cKDTree(E).query(Qp, k=8)
+    d2 = ((Qp[:, None, 0] - E[ik, 0]) ** 2 + (Qp[:, None, 1] - E[ik, 1]) ** 2 + (Qp[:, None, 2] - E[ik, 2]) ** 2)
+    order = np.lexsort((ik, d2), axis=1)[:, :2]
+    inds = np.take_along_axis(ik, order, axis=1)
+    dsq = np.take_along_axis(d2, order, axis=1)
+    # exact ranking is only guaranteed when every point outside the 8 is farther than the 2nd
+    unsure = dk[:, 7] <= np.sqrt(dsq[:, 1]) * (1 + 1e-9) + 1e-12
+    if unsure.any():
+        ui, ud = _knn2_exhaustive(E, Qp[unsure], chunk)
+        inds[unsure] = ui
+        dsq[unsure] = ud ** 2
+        d = np.sqrt(dsq)
+        d[unsure] = ud
+        return inds, d
+    return inds, np.sqrt(dsq)
+
+
+def _knn2_exhaustive(edge_pts, query_pts, chunk=4096):
     nq = query_pts.shape[0]
     inds = np.zeros((nq, 2), np.int64)
     d = np.zeros((nq, 2), np.float64)
@@ -127,6 +156,82 @@ def knn2(edge_pts, query_pts, chunk=4096):
         inds[s:s + chunk] = o
         d[s:s + chunk] = np.sqrt(np.take_along_axis(d2, o, axis=1))
     return inds, d
+
+
+def otf_mask_morphology(mask, area_thresh):
+    """omw_backprojector_generation.m:44-48 on a thresholded OTF mask: imopen,
+    bwareaopen, imclose, imopen, bwareaopen with strel('sphere', 2)."""
+    b2 = ball(2)
+    mask = imdilate(imerode(mask, b2), b2)  # :44 imopen
+    mask = bwareaopen(mask, area_thresh)  # :45
+    mask = imerode(imdilate(mask, b2), b2)  # :46 imclose
+    mask = imdilate(imerode(mask, b2), b2)  # :47 imopen
+    return bwareaopen(mask, area_thresh)  # :48
+
+
+def skew_detect(mask):
+    """omw_backprojector_generation.m:57-71: skewed when the x-z projection has
+    more than one component, covers every z, and peaks more than one plane
+    from the centre."""
+    xz = mask.sum(axis=0) > 0  # squeeze(sum(OTF_mask, 1)) > 0 : (n2, n3)
+    _, ncc, _ = label_colmajor(xz)
+    if ncc > 1:
+        line = xz.sum(axis=0)  # sum(OTF_mask_xz, 1): along z
+        pind = int(np.argmax(line)) + 1
+        if np.all(line > 0) and abs(pind - (line.size + 1) / 2) > 1:
+            return True
+    return False
+
+
+def skewed_components(mask):
+    """omw_backprojector_generation.m:73-88: label the mask (bwlabeln, labels
+    by first voxel in column-major order); more than three components: keep
+    the three largest (maxk) and relabel; one component: it becomes label 2;
+    then cat(3, L == 3, L == 2, L == 1) -- right, centre, left along dim 3."""
+    L, n, sizes = label_colmajor(mask)
+    if n + 1 > 4:
+        top = np.argsort(-sizes, kind="stable")[:3] + 1
+        mask = np.isin(L, top)
+        L, n, sizes = label_colmajor(mask)
+    elif n + 1 == 2:
+        L = (L + 1) * mask
+    return np.concatenate([L == 3, L == 2, L == 1], axis=2)
+
+
+def largest_component(mask):
+    """omw_backprojector_generation.m:90-93: the largest 26-connected
+    component (the first in label order among equals)."""
+    L, n, sizes = label_colmajor(mask)
+    return L == (int(np.argmax(sizes)) + 1)
+
+
+def slice_hulls(mask):
+    """omw_backprojector_generation.m:98-109: bwconvhull of every z-slice, then
+    every y-slice, then every x-slice that held mask voxels before the pass
+    (index sets taken first)."""
+    mask = mask.copy()
+    zinds = np.nonzero(mask.sum(axis=(0, 1)))[0]
+    xinds = np.nonzero(mask.sum(axis=(0, 2)))[0]
+    yinds = np.nonzero(mask.sum(axis=(1, 2)))[0]
+    for z in zinds:
+        mask[:, :, z] = bwconvhull(mask[:, :, z])
+    for y in yinds:
+        mask[y, :, :] = bwconvhull(mask[y, :, :])
+    for x in xinds:
+        mask[:, x, :] = bwconvhull(mask[:, x, :])
+    return mask
+
+
+def edge_radius(E, Q, Rn):
+    """omw_backprojector_generation.m:135-139 literally: [inds, d_mat] =
+    knnsearch(E, Q, 'K', 2); d_1 = d_mat(inds(:,1), 1); d_2 = d_mat(inds(:,2), 2)
+    -- d_mat's rows indexed by EDGE index, as the reference writes it;
+    rn = (Rn(inds(:,1)) .* d_2 + Rn(inds(:,2)) .* d_1) ./ (d_1 + d_2)."""
+    inds, dmat = knn2(E, Q)  # :135
+    d1 = dmat[inds[:, 0], 0]  # :137
+    d2 = dmat[inds[:, 1], 1]  # :138
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return (Rn[inds[:, 0]] * d2 + Rn[inds[:, 1]] * d1) / (d1 + d2)  # :139
 
 
 def omw_backprojector_generation(psf, alpha, skewed=None, OTFCumThresh=0.9, OTFAreaThresh=100,
@@ -148,50 +253,15 @@ def omw_backprojector_generation(psf, alpha, skewed=None, OTFCumThresh=0.9, OTFA
     tind = int(np.argmax(cs > cs[-1] * OTFCumThresh))  # :38 (sequential sum)
     dc = np.float32(np.float32(vals[tind]) / abs_OTF[0, 0, 0])  # :39 (single)
     dc = max(dc, np.float32(minIntThrsh))  # :41
-    mask = abs_OTF_c > abs_OTF[0, 0, 0] * dc  # :43
-    b2 = ball(2)
-    mask = imdilate(imerode(mask, b2), b2)  # :44 imopen
-    mask = bwareaopen(mask, OTFAreaThresh)  # :45
-    mask = imerode(imdilate(mask, b2), b2)  # :46 imclose
-    mask = imdilate(imerode(mask, b2), b2)  # :47 imopen
-    mask = bwareaopen(mask, OTFAreaThresh)  # :48
+    mask = otf_mask_morphology(abs_OTF_c > abs_OTF[0, 0, 0] * dc, OTFAreaThresh)  # :43-48
     if not mask.any():  # :50-52
         raise ValueError("The OTF mask is empty, check the PSF and OTF-related parameters!")
 
     if skewed is None:  # :57-71
-        skewed = False
-        xz = mask.sum(axis=0) > 0  # squeeze(sum(OTF_mask, 1)) > 0 : (n2, n3)
-        _, ncc, _ = label_colmajor(xz)
-        if ncc > 1:
-            line = xz.sum(axis=0)  # sum(OTF_mask_xz, 1): along z
-            pind = int(np.argmax(line)) + 1
-            if np.all(line > 0) and abs(pind - (line.size + 1) / 2) > 1:
-                skewed = True
-
-    if skewed:  # :73-88
-        L, n, sizes = label_colmajor(mask)
-        if n + 1 > 4:
-            top = np.argsort(-sizes, kind="stable")[:3] + 1
-            mask = np.isin(L, top)
-            L, n, sizes = label_colmajor(mask)
-        elif n + 1 == 2:
-            L = (L + 1) * mask
-        mask = np.concatenate([L == 3, L == 2, L == 1], axis=2)
-    else:  # :90-93
-        L, n, sizes = label_colmajor(mask)
-        mask = L == (int(np.argmax(sizes)) + 1)
+        skewed = skew_detect(mask)
+    mask = skewed_components(mask) if skewed else largest_component(mask)  # :73-94
     psz3 = mask.shape
-
-    # convex hulls per slice along z, y, x (:98-109); index sets taken first
-    zinds = np.nonzero(mask.sum(axis=(0, 1)))[0]
-    xinds = np.nonzero(mask.sum(axis=(0, 2)))[0]
-    yinds = np.nonzero(mask.sum(axis=(1, 2)))[0]
-    for z in zinds:
-        mask[:, :, z] = bwconvhull(mask[:, :, z])
-    for y in yinds:
-        mask[y, :, :] = bwconvhull(mask[y, :, :])
-    for x in xinds:
-        mask[:, x, :] = bwconvhull(mask[:, x, :])
+    mask = slice_hulls(mask)  # :98-109
 
     edge = mask & ~imerode(mask, ball(1))  # :122
     c = (np.array(psz3, np.float64) + 1) / 2  # :123
@@ -227,12 +297,8 @@ def omw_backprojector_generation(psf, alpha, skewed=None, OTFCumThresh=0.9, OTFA
     ef = f(edge)
     E = np.stack([f(Yn)[ef], f(Xn)[ef], f(Zn)[ef]], axis=1)
     Q = np.stack([f(Yn), f(Xn), f(Zn)], axis=1)
-    inds, dmat = knn2(E, Q)  # :135
-    Rn = f(rs)[ef]  # :136
-    d1 = dmat[inds[:, 0], 0]  # :137 (indexes d_mat rows by edge index, as the reference does)
-    d2 = dmat[inds[:, 1], 1]  # :138
+    rn = edge_radius(E, Q, f(rs)[ef])  # :135-139
     with np.errstate(invalid="ignore", divide="ignore"):
-        rn = (Rn[inds[:, 0]] * d2 + Rn[inds[:, 1]] * d1) / (d1 + d2)  # :139
         bw = f(rs) / rn  # :141
     lo, hi = float(hannWinBounds[0]), float(hannWinBounds[1])
     with np.errstate(invalid="ignore"):

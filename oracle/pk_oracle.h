@@ -76,6 +76,10 @@ int64_t pko_find_good_factor_number(int64_t n, int dir, int allow_odd_seven, int
 /* MATLAB uint16(x): round half away from zero, saturate, NaN -> 0. */
 uint16_t pko_to_u16(double x);
 
+/* Neighbourhood of strel('disk', r) (default N = 4): half-width w[dy + R] of
+ * every row dy in [-R, R]; returns R (w holds 2r + 1 ints), -1 for r < 0. */
+int pko_strel_disk(int r, int *w);
+
 #ifdef __cplusplus
 }
 #endif

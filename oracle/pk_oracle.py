@@ -12,6 +12,7 @@ Functions mirror the reference (paths relative to the PetaKit5D tree):
   decon_lucy_function     microscopeDataProcessing/decon/decon_lucy_function.m:1-215 (useGPU=false)
   decon_lucy_omw_function microscopeDataProcessing/decon/decon_lucy_omw_function.m:1-178 (useGPU=false)
   decon_mask_edge_erosion microscopeDataProcessing/decon/decon_mask_edge_erosion.m:1-30
+  strel_disk              strel('disk', r) as decon_mask_edge_erosion.m:22-27 uses it
   findGoodFactorNumber    utils/findGoodFactorNumber.m:1-26
 """
 from __future__ import annotations
@@ -60,6 +61,8 @@ def lib():
             getattr(L, f"pko_decon_omw_{suf}").argtypes = [p, i64p, p, i64p, p, i64p, ctypes.c_int,
                                                            ctypes.POINTER(_Opts), p, p]
         L.pko_mask_edge_erosion.argtypes = [p, i64p, ctypes.c_int]
+        L.pko_strel_disk.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.pko_strel_disk.restype = ctypes.c_int
         L.pko_find_good_factor_number.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.pko_find_good_factor_number.restype = ctypes.c_int64
         L.pko_to_u16.argtypes = [ctypes.c_double]
@@ -185,6 +188,19 @@ def decon_mask_edge_erosion(mask, EdgeErosion):
     d = _dims3(m.shape)
     _check(lib().pko_mask_edge_erosion(_ptr(m), _i64(d), int(EdgeErosion)))
     return m.astype(bool)
+
+
+def strel_disk(r):
+    """getnhood(strel('disk', r)) (default N = 4) as a boolean (2R+1) x (2R+1)
+    array, rows = dim 1."""
+    w = (ctypes.c_int * (2 * int(r) + 1))()
+    R = lib().pko_strel_disk(int(r), w)
+    if R < 0:
+        raise OracleError("invalid argument")
+    nh = np.zeros((2 * R + 1, 2 * R + 1), bool)
+    for i in range(2 * R + 1):
+        nh[i, R - w[i]:R + w[i] + 1] = True
+    return nh
 
 
 def findGoodFactorNumber(givenNum, directionFlag=1, allowOddSeven=False, maxPrime=7):

@@ -39,13 +39,20 @@ def test_backprojector_vs_oracle(bo, size, sig, shear, skewed):
     assert rel_l2(omw, romw) < 1e-5
 
 
-def test_config2_psf_backprojector_invariants():
+def test_config2_psf_backprojector_vs_oracle(bo):
+    """The 64x64x101 sheared config-2 PSF (413,696 voxels): OTF mask equal,
+    b_omw and OTF_bp_omw within 1e-5 of the oracle, DC gain 1 / (1 + alpha)."""
     import petakit5d_amd as pk
     psf = synth.config2_psf()
+    print("[bp config2] 64x64x101", flush=True)
     b, omw, absc, mask = pk.omw_backprojector_generation(psf, 0.005, None)
+    rb, romw, rabsc, rmask, rsk = bo.omw_backprojector_generation(psf, 0.005, None)
     assert b.shape == (64, 64, 101) and np.isfinite(b).all()
     assert abs(float(b.astype(np.float64).sum()) - 1 / 1.005) < 1e-4
-    print(f"[bp config2] mask voxels {float(np.sum(mask)):.0f}, b_omw peak {b.max():.4g}")
+    np.testing.assert_array_equal(np.asarray(mask, np.float64), np.asarray(rmask, np.float64))
+    e = rel_l2(b, rb)
+    print(f"  skewed={rsk} mask voxels {float(np.sum(mask)):.0f}, rel-L2(b_omw) {e:.2e}")
+    assert e < 1e-5 and rel_l2(omw, romw) < 1e-5
 
 
 def test_backprojector_errors():

@@ -193,6 +193,88 @@ def test_edge_erosion_irregular(oracle):
     np.testing.assert_array_equal(e, ref)
 
 
+def test_strel_disk_small_radii_exact(oracle):
+    """strel('disk', r) for r < 3 is the exact disk dx^2 + dy^2 <= r^2."""
+    for r in (0, 1, 2):
+        g = np.arange(-r, r + 1)
+        np.testing.assert_array_equal(oracle.strel_disk(r), g[:, None] ** 2 + g[None, :] ** 2 <= r * r)
+
+
+def test_strel_disk_radius5_documented_neighbourhood(oracle):
+    """The MATLAB documentation shows strel('disk', 5) as a 9 x 9 neighbourhood
+    (smaller than the radius-5 disk: the periodic-line approximation)."""
+    want = np.array([[0, 0, 1, 1, 1, 1, 1, 0, 0],
+                     [0, 1, 1, 1, 1, 1, 1, 1, 0],
+                     [1, 1, 1, 1, 1, 1, 1, 1, 1],
+                     [1, 1, 1, 1, 1, 1, 1, 1, 1],
+                     [1, 1, 1, 1, 1, 1, 1, 1, 1],
+                     [1, 1, 1, 1, 1, 1, 1, 1, 1],
+                     [1, 1, 1, 1, 1, 1, 1, 1, 1],
+                     [0, 1, 1, 1, 1, 1, 1, 1, 0],
+                     [0, 0, 1, 1, 1, 1, 1, 0, 0]], bool)
+    np.testing.assert_array_equal(oracle.strel_disk(5), want)
+
+
+def _periodic_line_sum(r):
+    """strel('disk', r), r >= 3, by brute force: dilate a point by each periodic
+    line {i v : |i| <= floor(k / |v|)} (scipy), then by the two compensating
+    lines."""
+    from scipy import ndimage as ndi
+    G = 4 * r + 3
+    img = np.zeros((G, G), bool)
+    img[G // 2, G // 2] = True
+    t = np.pi / 8
+    k = 2 * r / (1 / np.tan(t) + 1 / np.sin(t))
+    for v in ((1, 0), (1, 1), (0, 1), (-1, 1)):
+        rp = int(np.floor(k / np.hypot(*v)))
+        se = np.zeros((2 * rp + 1,) * 2, bool) if rp else np.ones((1, 1), bool)
+        for i in range(-rp, rp + 1):
+            se[rp + i * v[0], rp + i * v[1]] = True
+        img = ndi.binary_dilation(img, structure=se)
+    rows = np.nonzero(img.any(axis=1))[0]
+    e = rows.max() - G // 2
+    ln = 2 * (r - e - 1) + 1
+    if ln >= 3:
+        img = ndi.binary_dilation(img, structure=np.ones((1, ln), bool))
+        img = ndi.binary_dilation(img, structure=np.ones((ln, 1), bool))
+    rr, cc = np.nonzero(img)
+    R = max(np.abs(rr - G // 2).max(), np.abs(cc - G // 2).max())
+    return img[G // 2 - R:G // 2 + R + 1, G // 2 - R:G // 2 + R + 1]
+
+
+@pytest.mark.parametrize("r", [3, 4, 6, 7, 9, 12, 17, 25])
+def test_strel_disk_is_the_periodic_line_decomposition(oracle, r):
+    nh = oracle.strel_disk(r)
+    np.testing.assert_array_equal(nh, _periodic_line_sum(r))
+    assert r - 2 <= nh.shape[0] // 2 <= r  # an approximation from inside the radius-r extent
+    np.testing.assert_array_equal(nh, nh[::-1, ::-1])
+    np.testing.assert_array_equal(nh, nh.T)
+
+
+@pytest.mark.parametrize("E", [4, 8])
+def test_edge_erosion_irregular_large(oracle, E):
+    """EdgeErosion 4 and 8 (the frame driver's default, XR_RLdeconFrame3D.m:41)
+    on an irregular mask: faces cleared, every slice eroded by strel('disk',
+    E - 1), then dim 3 by a (2E - 1)-long line (decon_mask_edge_erosion.m:
+    17-27); out-of-volume neighbours count as true.  Checked against scipy's
+    binary erosion with the same neighbourhoods."""
+    from scipy import ndimage as ndi
+    g = np.random.default_rng(E)
+    m = np.ones((40, 36, 24), bool)
+    for _ in range(6):  # holes of zeros (an irregular I ~= 0 mask)
+        c = g.integers(0, [40, 36, 24])
+        m[c[0]:c[0] + 5, c[1]:c[1] + 4, c[2]:c[2] + 3] = False
+    e = oracle.decon_mask_edge_erosion(m, E)
+    ref = m.copy()
+    ref[[0, -1], :, :] = False
+    ref[:, [0, -1], :] = False
+    ref[:, :, [0, -1]] = False
+    se = oracle.strel_disk(E - 1)
+    ref = ndi.binary_erosion(ref, structure=se[:, :, None], border_value=1)
+    ref = ndi.binary_erosion(ref, structure=np.ones((1, 1, 2 * E - 1), bool), border_value=1)
+    np.testing.assert_array_equal(e, ref)
+
+
 @pytest.mark.parametrize("n,d,odd7,mp,exp", [
     (101, 1, False, 7, 108), (257, 1, False, 7, 270), (448, 1, False, 7, 448), (448, 1, False, 5, 450),
     (7, 1, False, 7, 8), (14, 1, False, 7, 14), (21, 1, False, 7, 24), (21, 1, True, 7, 21),

@@ -152,3 +152,150 @@ def test_wb_gaussian_back_projector_normalised(bo):
     assert abs(out.sum() - 1) < 0.02
     f = bo.wb_fwhm_psf(out.astype(np.float32))
     assert all(abs(a / b - 1) < 0.05 for a, b in zip(f, info["FWHM"]))
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# The toolbox steps of omw_backprojector_generation.m, one at a time, on hand-made masks whose answers follow
+# from the documented semantics (labels numbered by first voxel in column-major order, maxk, cat along dim 3,
+# imopen / imclose with strel('sphere', 2), bwareaopen's "fewer than P voxels"), not from the oracle's code.
+def _comp(shape, voxels):
+    m = np.zeros(shape, bool)
+    for v in voxels:
+        m[v] = True
+    return m
+
+
+def test_skewed_components_roles_and_order(bo):
+    """:73-88 -- three components: the one holding the earliest voxel in
+    column-major order is L == 1 (left), then L == 2 (centre), L == 3 (right);
+    the output is cat(3, right, centre, left)."""
+    sh = (6, 5, 4)
+    A = _comp(sh, [(0, 0, 0), (1, 0, 0)])            # first voxel: linear index 0
+    B = _comp(sh, [(3, 2, 1), (3, 2, 2)])            # first voxel: 3 + 6*2 + 30*1 = 45
+    C = _comp(sh, [(4, 4, 3), (5, 4, 3), (5, 3, 3)])  # first voxel: 5 + 6*3 + 30*3 = 113
+    out = bo.skewed_components(A | B | C)
+    np.testing.assert_array_equal(out, np.concatenate([C, B, A], axis=2))
+
+
+def test_skewed_components_keep_three_largest(bo):
+    """:75-81 -- more than three components: the three largest (maxk) are kept
+    and relabelled in scan order."""
+    sh = (8, 8, 3)
+    S1 = _comp(sh, [(0, 0, 0)])                                     # size 1, scanned first
+    S4 = _comp(sh, [(3, 0, 0), (4, 0, 0), (5, 0, 0), (6, 0, 0)])   # size 4
+    S2 = _comp(sh, [(0, 3, 0), (0, 4, 0)])                          # size 2
+    S3 = _comp(sh, [(4, 4, 1), (5, 4, 1), (6, 4, 1)])               # size 3
+    S5 = _comp(sh, [(0, 7, 2), (1, 7, 2), (2, 7, 2), (3, 7, 2), (4, 7, 2)])  # size 5, scanned last
+    out = bo.skewed_components(S1 | S4 | S2 | S3 | S5)
+    # kept: S5, S4, S3; scan order of their first voxels: S4 (3), S3 (4+32+64=100), S5 (56+128=184)
+    np.testing.assert_array_equal(out, np.concatenate([S5, S3, S4], axis=2))
+
+
+def test_skewed_components_one_or_two(bo):
+    """:82-84 -- a single component becomes label 2 (centre); with two the
+    right block stays empty."""
+    sh = (5, 5, 2)
+    A = _comp(sh, [(0, 0, 0), (1, 1, 0)])
+    B = _comp(sh, [(4, 4, 1)])
+    z = np.zeros(sh, bool)
+    np.testing.assert_array_equal(bo.skewed_components(A), np.concatenate([z, A, z], axis=2))
+    np.testing.assert_array_equal(bo.skewed_components(A | B), np.concatenate([z, B, A], axis=2))
+
+
+def test_largest_component(bo):
+    sh = (6, 6, 3)
+    A = _comp(sh, [(0, 0, 0), (1, 0, 0)])
+    B = _comp(sh, [(4, 4, 2), (5, 5, 2), (3, 3, 2)])  # diagonal: one component under 26-connectivity
+    np.testing.assert_array_equal(bo.largest_component(A | B), B)
+
+
+def _brute_rn(E, Q, Rn, literal=True):
+    """knnsearch K = 2 by exhaustive search (squared distances, ties to the lower
+    edge index), then line 137-139 of the reference literally (d_mat's rows
+    indexed by edge index) or as the query's own distances."""
+    import math
+    d, idx = [], []
+    for q in Q:
+        d2 = [sum((q[k] - e[k]) ** 2 for k in range(3)) for e in E]
+        o = sorted(range(len(E)), key=lambda i: (d2[i], i))[:2]
+        idx.append(o)
+        d.append([math.sqrt(d2[o[0]]), math.sqrt(d2[o[1]])])
+    out = []
+    for qi, (i1, i2) in enumerate(idx):
+        d1 = d[i1][0] if literal else d[qi][0]
+        d2 = d[i2][1] if literal else d[qi][1]
+        out.append((Rn[i1] * d2 + Rn[i2] * d1) / (d1 + d2))
+    return np.array(out)
+
+
+def test_edge_radius_reproduces_the_indexing_quirk(bo):
+    """:135-139 -- d_1 = d_mat(inds(:,1), 1) reads row inds(q,1) of d_mat (an
+    edge index) instead of row q; the oracle keeps that literally.  The data
+    are chosen so that the literal and the per-query formulas differ."""
+    g = np.random.default_rng(5)
+    E = g.standard_normal((6, 3))
+    Q = g.standard_normal((15, 3))
+    Rn = g.uniform(1, 3, 6)
+    got = bo.edge_radius(E, Q, Rn)
+    lit = _brute_rn(E, Q, Rn, literal=True)
+    per_query = _brute_rn(E, Q, Rn, literal=False)
+    np.testing.assert_allclose(got, lit, rtol=1e-13)
+    assert np.abs(lit - per_query).max() > 1e-3
+
+
+def _ball(shape, c, r):
+    g = np.indices(shape)
+    return ((g[0] - c[0]) ** 2 + (g[1] - c[1]) ** 2 + (g[2] - c[2]) ** 2) <= r * r
+
+
+def test_otf_mask_morphology_known_answers(bo):
+    """:44-48 -- sets that are dilations by strel('sphere', 2) survive the
+    openings; a one-voxel hole inside one is filled by the closing; a lone
+    voxel does not survive the opening; bwareaopen drops components with fewer
+    than OTFAreaThresh voxels (a 98-voxel component: kept at 98, dropped at 99;
+    a bare sphere of 33 voxels: dropped)."""
+    from scipy import ndimage as ndi
+    sh = (48, 40, 40)
+    b2 = _ball((5, 5, 5), (2, 2, 2), 2)
+    dil = lambda s: ndi.binary_dilation(s, structure=b2)  # noqa: E731
+    seed = np.zeros(sh, bool)
+    seed[8:15, 8:15, 8:15] = True
+    big = dil(seed)                                      # rounded 11^3 box
+    line = np.zeros(sh, bool)
+    line[32, 30, 20:26] = True
+    rod = dil(line)                                      # 98 voxels
+    sph = _ball(sh, (32, 8, 30), 2)                      # 33 voxels
+    assert rod.sum() == 98 and sph.sum() == 33
+    m = big | rod | sph
+    m[11, 11, 11] = False                                # hole inside the box
+    m[40, 35, 5] = True                                  # lone voxel
+    np.testing.assert_array_equal(bo.otf_mask_morphology(m, 98), big | rod)
+    np.testing.assert_array_equal(bo.otf_mask_morphology(m, 99), big)
+
+
+def test_hull_of_pixel_corners(bo):
+    """bwconvhull(BW, 'union') fills the convex hull of the foreground pixels'
+    corners (regionprops' ConvexHull / ConvexImage).  Pixels (0,0), (0,8),
+    (4,0): the hull's slanted edge runs from corner (0.5, 8.5) to (4.5, 0.5),
+    i.e. 2r + c = 9.5, which passes through no pixel centre, so the hull image
+    is {0 <= r <= 4, 0 <= c <= 8, 2r + c <= 9}."""
+    img = np.zeros((7, 11), bool)
+    img[0, 0] = img[0, 8] = img[4, 0] = True
+    r, c = np.indices(img.shape)
+    np.testing.assert_array_equal(bo.bwconvhull(img), (r <= 4) & (c <= 8) & (2 * r + c <= 9))
+    np.testing.assert_array_equal(bo.bwconvhull(img.T), ((r <= 4) & (c <= 8) & (2 * r + c <= 9)).T)
+
+
+def test_knn2_tree_equals_exhaustive_search(bo):
+    """The k-d tree candidates plus exact re-ranking give the exhaustive
+    search's answer, ties included (unit vectors of lattice directions repeat
+    exactly)."""
+    g = np.random.default_rng(0)
+    E = g.integers(-4, 5, (2500, 3)).astype(float)
+    E /= np.maximum(np.linalg.norm(E, axis=1, keepdims=True), 1e-9)
+    Q = g.integers(-7, 8, (9000, 3)).astype(float)
+    Q /= np.maximum(np.linalg.norm(Q, axis=1, keepdims=True), 1e-9)
+    i1, d1 = bo.knn2(E, Q)
+    i2, d2 = bo._knn2_exhaustive(E, Q)
+    np.testing.assert_array_equal(i1, i2)
+    np.testing.assert_array_equal(d1, d2)
