@@ -119,6 +119,24 @@ pk_status pk_decon_lucy_omw(pk_ctx *ctx, const pk_vol *I, const float *psf_f, co
                             const float *psf_b, const int64_t b_dims[3], int32_t numit, const pk_opts *opts,
                             pk_vol *out, double *err_mat);
 
+/* No counterpart in the reference: pk_decon_lucy_omw for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * pk_decon_lucy_omw, with these limits: I and out must be device memory
+ * (PK_MEM_DEVICE); PK_FLAG_UNFUSED and PK_FLAG_AUTOTUNE are rejected; the
+ * call allocates nothing and caches nothing -- its plan tables, OTFs,
+ * estimate, spectrum and masks live in one static device array of
+ * PK_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), and a size
+ * needing more -- 8 bytes per voxel plus three half spectra of
+ * 8 * roundup(n1/2 + 1, 16) * n2 * n3 bytes, 2 bytes more per voxel with
+ * EdgeErosion (about 20 and 22 bytes per voxel; 192x192x160 fits) -- returns
+ * PK_ERR_UNSUPPORTED_SIZE.  Calls that overlap on one GPU share that array
+ * and race: serialise them. */
+#define PK_NOALLOC_SCRATCH_BYTES (128ull << 20)
+pk_status pk_decon_lucy_omw_noalloc(pk_ctx *ctx, const pk_vol *I, const float *psf_f, const int64_t f_dims[3],
+                                    const float *psf_b, const int64_t b_dims[3], int32_t numit, const pk_opts *opts,
+                                    pk_vol *out, double *err_mat);
+
 /* One chunk of a batch: input and output volumes of the same (any) size. */
 typedef struct {
     pk_vol in;

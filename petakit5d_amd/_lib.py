@@ -45,6 +45,7 @@ EXPORTS = [
     "pk_device_bytes",
     "pk_decon_lucy",
     "pk_decon_lucy_omw",
+    "pk_decon_lucy_omw_noalloc",
     "pk_decon_batch",
     "pk_serve_start",
     "pk_serve_stop",
@@ -167,6 +168,8 @@ def load():
                                     ctypes.POINTER(PkVol), vp]
         L.pk_decon_lucy_omw.argtypes = [vp, ctypes.POINTER(PkVol), vp, i64p, vp, i64p, ctypes.c_int32,
                                         ctypes.POINTER(PkOpts), ctypes.POINTER(PkVol), vp]
+        L.pk_decon_lucy_omw_noalloc.argtypes = [vp, ctypes.POINTER(PkVol), vp, i64p, vp, i64p, ctypes.c_int32,
+                                                ctypes.POINTER(PkOpts), ctypes.POINTER(PkVol), vp]
         L.pk_psf2otf.argtypes = [vp, vp, i64p, i64p, vp]
         L.pk_decon_batch.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, vp, i64p, vp, i64p, ctypes.c_int32,
                                      ctypes.POINTER(PkOpts)]

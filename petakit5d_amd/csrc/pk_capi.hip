@@ -125,6 +125,7 @@ pk_status pk_ctx_create(int device, uint64_t hbm_budget_bytes, pk_ctx** out) {
         return PK_ERR_HIP;
     }
     c->stream = c->own_stream;
+    pk::noalloc_setup(device);  // the no-allocation entry point's scratch array on this device
     *out = c;
     return PK_OK;
 }

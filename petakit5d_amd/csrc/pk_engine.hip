@@ -539,10 +539,21 @@ void pk_ctx_impl::set_shapes(int64_t n1, int64_t n2, int64_t n3, const std::arra
     w.table[key] = v;
 }
 
-const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
-    std::vector<int64_t> key{n1, n2, n3};
-    auto it = plans.find(key);
-    if (it != plans.end()) return it->second;
+// W_{n1}^k for k in [0, n1/2] (the split step of packed real rows), exact at
+// quarter and half turns, long double elsewhere, rounded once.
+static void fill_real_twiddles(std::vector<float2>& h, int64_t n1) {
+    const int M = (int)(n1 / 2);
+    h.resize(M + 1);
+    for (int k = 0; k <= M; ++k) {
+        long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n1;
+        float c = (float)cosl(a), s = (float)(-sinl(a));
+        if (4 * k == n1) { c = 0.f; s = -1.f; }
+        if (2 * k == n1) { c = -1.f; s = 0.f; }
+        h[k] = make_float2(c, s);
+    }
+}
+
+VolPlan pk_ctx_impl::plan_shapes(int64_t n1, int64_t n2, int64_t n3) const {
     for (int64_t n : {n1, n2, n3})
         if (!smooth235(n))
             fail(PK_ERR_UNSUPPORTED_SIZE, "axis length " + std::to_string(n) +
@@ -558,8 +569,38 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
     vp.S = vp.hp * n2 * n3;
     vp.Sa = vp.h1 * n2 * n3;
     vp.packed = (n1 % 2 == 0);
-    const int Lx = (int)(vp.packed ? n1 / 2 : n1);
+    // x (rows), y, z (columns): the first candidate is the built-in choice
+    vp.x = x_candidates(vp.row_len(), vp.packed)[0];
+    vp.y = col_candidates((int)n2, 1)[0];
+    vp.z = col_candidates((int)n3, 2)[0];
+    {  // shapes measured earlier for this size (this process or PK_TUNE_FILE)
+        Wisdom& w = Wisdom::get();
+        std::lock_guard<std::mutex> lk(w.mu);
+        w.load();
+        auto t = w.table.find({n1, n2, n3});
+        if (t != w.table.end()) {
+            VolPlan tuned = vp;
+            if (const_cast<pk_ctx_impl*>(this)->apply_shapes(tuned, t->second)) vp = tuned;
+        }
+    }
+    return vp;
+}
 
+void twiddle_host_tables(const VolPlan& vp, std::vector<float2>& tx, std::vector<float2>& ty,
+                         std::vector<float2>& tz, std::vector<float2>& rt) {
+    fill_twiddles(tx, vp.row_len());
+    fill_twiddles(ty, (int)vp.n2);
+    fill_twiddles(tz, (int)vp.n3);
+    rt.clear();
+    if (vp.packed) fill_real_twiddles(rt, vp.n1);
+}
+
+const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
+    std::vector<int64_t> key{n1, n2, n3};
+    auto it = plans.find(key);
+    if (it != plans.end()) return it->second;
+    VolPlan vp = plan_shapes(n1, n2, n3);
+    // device twiddle tables, one per transform length, shared by the plans
     auto table = [&](int L) -> const float2* {
         auto t = tw.find(L);
         if (t != tw.end()) return t->second->as<float2>();
@@ -572,41 +613,20 @@ const VolPlan& pk_ctx_impl::plan(int64_t n1, int64_t n2, int64_t n3) {
         tw[L] = std::move(b);
         return r;
     };
-
-    // x (rows): the first candidate is the built-in choice
-    vp.x = x_candidates(Lx, vp.packed)[0];
-    vp.x.fp.tw = table(Lx);
-    // y, z (columns)
-    vp.y = col_candidates((int)n2, 1)[0];
+    vp.x.fp.tw = table(vp.row_len());
     vp.y.fp.tw = table((int)n2);
-    vp.z = col_candidates((int)n3, 2)[0];
     vp.z.fp.tw = table((int)n3);
-
     if (vp.packed) {
         auto r = rtw.find(n1);
         if (r == rtw.end()) {
-            const int M = (int)(n1 / 2);
-            std::vector<float2> h(M + 1);
-            for (int k = 0; k <= M; ++k) {
-                long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n1;
-                float c = (float)cosl(a), s = (float)(-sinl(a));
-                if (4 * k == n1) { c = 0.f; s = -1.f; }
-                if (2 * k == n1) { c = -1.f; s = 0.f; }
-                h[k] = make_float2(c, s);
-            }
+            std::vector<float2> h;
+            fill_real_twiddles(h, n1);
             auto b = std::make_unique<DevBuf>();
-            alloc(*b, sizeof(float2) * (size_t)(M + 1));
-            PK_HIP(hipMemcpy(b->p, h.data(), sizeof(float2) * (size_t)(M + 1), hipMemcpyHostToDevice));
+            alloc(*b, sizeof(float2) * h.size());
+            PK_HIP(hipMemcpy(b->p, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
             r = rtw.emplace(n1, std::move(b)).first;
         }
         vp.rtw = r->second->as<float2>();
-    }
-    {  // shapes measured earlier for this size (this process or PK_TUNE_FILE)
-        Wisdom& w = Wisdom::get();
-        std::lock_guard<std::mutex> lk(w.mu);
-        w.load();
-        auto t = w.table.find(key);
-        if (t != w.table.end()) apply_shapes(vp, t->second);
     }
     return plans.emplace(key, vp).first->second;
 }
@@ -1037,20 +1057,25 @@ void pk_ctx_impl::build_otf(const VolPlan& vp, const float* q, const int64_t pd[
     ensure(ws.tmp, sizeof(float) * (size_t)std::max<int64_t>(vp.N, P));
     DevBuf dq;
     alloc(dq, sizeof(float) * (size_t)P);
-    PK_HIP(hipMemcpyAsync(dq.p, q, sizeof(float) * (size_t)P, hipMemcpyHostToDevice, stream));
-    float* real = ws.tmp.as<float>();
+    build_otf_into(vp, q, pd, dst, ws.tmp.as<float>(), dq.as<float>());
+    PK_HIP(hipStreamSynchronize(stream));  // dq is freed on return
+    prof_flush();
+}
+
+void pk_ctx_impl::build_otf_into(const VolPlan& vp, const float* q, const int64_t pd[3], float2* dst, float* real,
+                                 float* dq) {
+    const int64_t P = pd[0] * pd[1] * pd[2];
+    PK_HIP(hipMemcpyAsync(dq, q, sizeof(float) * (size_t)P, hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(vp.N)), dim3(256), 0, stream, real, vp.N, 0.f);
     PK_HIP(hipGetLastError());
     bool allzero = true;
     for (int64_t i = 0; i < P && allzero; ++i) allzero = (q[i] == 0.f);
     if (!allzero) {  // an all-zero PSF gives a zero OTF (decon_psf2otf.m:42-44)
-        hipLaunchKernelGGL(k_scatter_psf, dim3(grid_for(P)), dim3(256), 0, stream, dq.as<float>(), (int)pd[0],
-                           (int)pd[1], (int)pd[2], real, vp.n1, vp.n2, vp.n3);
+        hipLaunchKernelGGL(k_scatter_psf, dim3(grid_for(P)), dim3(256), 0, stream, dq, (int)pd[0], (int)pd[1],
+                           (int)pd[2], real, vp.n1, vp.n2, vp.n3);
         PK_HIP(hipGetLastError());
     }
     r2c3d(vp, real, dst);
-    PK_HIP(hipStreamSynchronize(stream));  // dq is freed on return
-    prof_flush();
 }
 
 void pk_ctx_impl::reset() {
@@ -1084,8 +1109,8 @@ static double matlab_u16(double x) {
     return std::round(x);
 }
 
-void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const int64_t fd[3], const float* psf_b,
-                        const int64_t bd[3], int numit, const pk_opts* o, pk_vol* out, double* err_mat) {
+DeconGeom decon_check(int method, const pk_vol* Iv, const float* psf_f, const int64_t fd[3], const float* psf_b,
+                      const int64_t bd[3], int numit, const pk_opts* o, const pk_vol* out) {
     if (!Iv || !out || !o || !psf_f || numit < 0) fail(PK_ERR_INVALID_ARG, "null argument or negative NUMIT");
     if (method == 1 && !psf_b) fail(PK_ERR_INVALID_ARG, "PSF_b missing");
     if (Iv->dtype != PK_UINT16 && Iv->dtype != PK_SINGLE)
@@ -1098,20 +1123,162 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
             fail(PK_ERR_OUT_SIZE, "outSizeIsSmallerThanPsfSize: PSF larger than the volume");
     }
     // output geometry (bbox, 1-based inclusive)
-    int64_t b0[3] = {0, 0, 0}, od[3] = {n1, n2, n3};
+    DeconGeom g{{0, 0, 0}, {n1, n2, n3}};
     if (o->has_bbox) {
         for (int d = 0; d < 3; ++d) {
             const int64_t s = o->bbox[d], e = o->bbox[d + 3];
             if (s < 1 || e > Iv->dims[d] || e < s) fail(PK_ERR_INVALID_ARG, "bbox outside the volume");
-            b0[d] = s - 1;
-            od[d] = e - s + 1;
+            g.b0[d] = s - 1;
+            g.od[d] = e - s + 1;
         }
     }
-    if (out->dims[0] != od[0] || out->dims[1] != od[1] || out->dims[2] != od[2])
+    if (out->dims[0] != g.od[0] || out->dims[1] != g.od[1] || out->dims[2] != g.od[2])
         fail(PK_ERR_INVALID_ARG, "output dims do not match the (bbox) result size");
     if (out->dtype != (o->save16bit ? PK_UINT16 : PK_SINGLE))
         fail(PK_ERR_INVALID_ARG, "output class must be uint16 when save16bit, single otherwise");
     if (!out->data || !Iv->data) fail(PK_ERR_INVALID_ARG, "null data pointer");
+    return g;
+}
+
+// decon_mask_edge_erosion(I ~= 0, E) (decon_mask_edge_erosion.m:1-30) on the
+// stream: m1, m2 (N bytes each) and cnt (two counters) are the caller's;
+// returns the buffer holding the eroded mask.
+const uint8_t* pk_ctx_impl::edge_mask(const uint16_t* raw16, const float* rawf, int64_t n1, int64_t n2, int64_t n3,
+                                      int E, uint8_t* m1, uint8_t* m2, unsigned long long* cnt) {
+    const int64_t N = n1 * n2 * n3;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, stream, reinterpret_cast<float*>(cnt), (int64_t)4, 0.f);
+    const int is3d = n3 > 1;
+    hipLaunchKernelGGL(k_mask_init, dim3(grid_for(N)), dim3(256), 0, stream, raw16, rawf, N, m1, cnt);
+    hipLaunchKernelGGL(k_mask_faces, dim3(grid_for(N)), dim3(256), 0, stream, m1, n1, n2, n3, is3d, cnt, E);
+    hipLaunchKernelGGL(k_mask_erode_disk, dim3(grid_for(N)), dim3(256), 0, stream, m1, m2, n1, n2, n3, E - 1, cnt);
+    const uint8_t* mask = m2;
+    if (is3d) {
+        hipLaunchKernelGGL(k_mask_erode_z, dim3(grid_for(N)), dim3(256), 0, stream, m2, m1, n1, n2, n3, E - 1, cnt);
+        mask = m1;
+    }
+    PK_HIP(hipGetLastError());
+    return mask;
+}
+
+EpiArgs epi_args(const VolPlan& vp, const pk_opts* o, const DeconGeom& g, const float* I, const uint8_t* mask,
+                 const uint16_t* raw16, const float* rawf, void* dout) {
+    const size_t osz = o->save16bit ? 2 : 4;
+    EpiArgs e{};
+    e.I = I;
+    e.mask = mask;
+    e.raw16 = raw16;
+    e.rawf = rawf;
+    e.out16 = o->save16bit ? static_cast<uint16_t*>(dout) : nullptr;
+    e.outf = o->save16bit ? nullptr : static_cast<float*>(dout);
+    e.n1 = vp.n1;
+    e.n2 = vp.n2;
+    e.o1 = g.od[0];
+    e.o2 = g.od[1];
+    e.o3 = g.od[2];
+    e.b1 = g.b0[0];
+    e.b2 = g.b0[1];
+    e.b3 = g.b0[2];
+    e.damp = (float)o->damp_factor;
+    e.scale = (float)o->scale_factor;
+    e.offset = (float)o->decon_offset;
+    e.apply_scale = (o->scale_factor != 1 || o->decon_offset != 0 || o->edge_erosion > 0);
+    e.use_mask = o->decon_offset != 0 || o->edge_erosion > 0;
+    if (!(o->damp_factor > 1)) e.damp = 0.f;
+    // pair stores in the fused form: even dim-1 crop start and size, aligned output
+    e.pair = vp.packed && g.b0[0] % 2 == 0 && g.od[0] % 2 == 0 &&
+             reinterpret_cast<uintptr_t>(dout) % (2 * osz) == 0;
+    return e;
+}
+
+// prep (decon_lucy_function.m:85-100, GPU branch): uint16 I - uint16(Background)
+// saturates at 0; single I gives max(I - Background, 0).  Writes I, its
+// half spectrum into A and (Biggs) J_2 = I.
+static XArgs prep_args(int method, float2* A, float* I, float* J, const uint16_t* raw16, const float* rawf,
+                       double background) {
+    XArgs pa{};
+    pa.spec = A;
+    pa.I = I;
+    pa.J = method == 1 ? nullptr : J;  // OMW: the first update reads I (J_1 = I) instead of a copy
+    if (raw16) {
+        pa.mode = XM_PREP_U16;
+        pa.raw16 = raw16;
+        pa.bg = (float)matlab_u16(background);
+    } else {
+        pa.mode = XM_PREP_F32;
+        pa.rawf = rawf;
+        pa.bg = (float)background;
+    }
+    return pa;
+}
+
+// OMW passes (decon_lucy_omw_function.m:105-112) after nothing but the OTFs:
+// prep, then per iteration J_{k+1} = max(J_k .* real(ifftn(fftn(I ./
+// max(real(ifftn(fftn(J_k) .* OTF_f)), eps)) .* OTF_b)), 0); with fuse_epi
+// the last update writes the output through the epilogue e.  The dim-1 and
+// dim-2 passes run slab by slab (kz planes; 0 = the whole volume).  loop_key:
+// the iteration loop is replayed as a graph for a repeated key (null: eager).
+void pk_ctx_impl::omw_passes(const VolPlan& vp, const XArgs& pa, float* I, float* J, float2* A, const float2* Hf,
+                             const float2* Hb, int numit, const EpiArgs& e, bool fuse_epi, int64_t kz,
+                             const std::vector<uint64_t>* loop_key) {
+    const int64_t step = kz > 0 ? kz : vp.n3;
+    const float scale = (float)(1.0 / (double)vp.N);  // ifftn's 1/N
+    auto slabs = [&](auto&& body) {
+        for (int64_t z0 = 0; z0 < vp.n3; z0 += step) body(z0, std::min(step, vp.n3 - z0));
+    };
+    // the strided pass pair around the convolution (dim 2 unless the plan
+    // convolves along dim 2); slabs only split dim-2 passes
+    const int outer = vp.conv_axis == 3 ? 2 : 3;
+    auto y_fwd = [&](int64_t z0, int64_t nz) { run_col(vp, outer, A, CM_FWD, nullptr, 1.f, z0, nz); };
+    auto y_inv = [&](int64_t z0, int64_t nz) { run_col(vp, outer, A, CM_INV, nullptr, 1.f, z0, nz); };
+    auto z_conv = [&](const float2* H) { run_col(vp, vp.conv_axis, A, CM_CONV, H, scale); };
+    slabs([&](int64_t z0, int64_t nz) {
+        run_x(vp, pa, z0, nz);
+        if (numit > 0) y_fwd(z0, nz);
+    });
+    auto loop = [&] {
+        for (int k = 1; k <= numit; ++k) {
+            z_conv(Hf);
+            slabs([&](int64_t z0, int64_t nz) {
+                y_inv(z0, nz);
+                XArgs r{};
+                r.mode = XM_RATIO;
+                r.spec = A;
+                r.I = I;
+                run_x(vp, r, z0, nz);
+                y_fwd(z0, nz);
+            });
+            z_conv(Hb);
+            slabs([&](int64_t z0, int64_t nz) {
+                y_inv(z0, nz);
+                XArgs u{};
+                u.mode = XM_OMW_UPDATE;
+                u.spec = A;
+                u.J = J;
+                u.Jin = k == 1 ? I : nullptr;
+                u.r2c_out = k < numit;
+                if (!(fuse_epi && k == numit)) run_x(vp, u, z0, nz);
+                if (k < numit) y_fwd(z0, nz);
+            });
+        }
+    };
+    if (loop_key) run_graphed(*loop_key, loop);
+    else loop();
+    if (fuse_epi) {  // last update + epilogue
+        XArgs u{};
+        u.mode = XM_OMW_FINAL;
+        u.spec = A;
+        u.J = J;
+        u.Jin = numit == 1 ? I : nullptr;
+        u.ep = e;
+        run_x(vp, u);
+    }
+}
+
+void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const int64_t fd[3], const float* psf_b,
+                        const int64_t bd[3], int numit, const pk_opts* o, pk_vol* out, double* err_mat) {
+    const DeconGeom g = decon_check(method, Iv, psf_f, fd, psf_b, bd, numit, o, out);
+    const int64_t n1 = Iv->dims[0], n2 = Iv->dims[1], n3 = Iv->dims[2];
+    const int64_t* od = g.od;
 
     if ((o->flags & PK_FLAG_AUTOTUNE) && !plan(n1, n2, n3).tuned) autotune(n1, n2, n3);
     const VolPlan& vp = plan(n1, n2, n3);
@@ -1149,29 +1316,13 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
     float2* A = ws.A.as<float2>();
 
     // epilogue (decon_lucy_function.m:177-203)
-    const bool need_mask = o->decon_offset != 0 || o->edge_erosion > 0;
     const uint8_t* mask = nullptr;
     if (o->edge_erosion > 0) {
         ensure(ws.mask, (size_t)N);
         ensure(ws.mask2, (size_t)N);
-        ensure(ws.count, 16);
-        PK_HIP(hipMemsetAsync(ws.count.p, 0, 16, stream));
-        auto* cnt = ws.count.as<unsigned long long>();
-        uint8_t* m1 = ws.mask.as<uint8_t>();
-        uint8_t* m2 = ws.mask2.as<uint8_t>();
-        const int is3d = n3 > 1;
-        hipLaunchKernelGGL(k_mask_init, dim3(grid_for(N)), dim3(256), 0, stream, raw16, rawf, N, m1, cnt);
-        hipLaunchKernelGGL(k_mask_faces, dim3(grid_for(N)), dim3(256), 0, stream, m1, n1, n2, n3, is3d, cnt,
-                           (int)o->edge_erosion);
-        hipLaunchKernelGGL(k_mask_erode_disk, dim3(grid_for(N)), dim3(256), 0, stream, m1, m2, n1, n2, n3,
-                           (int)o->edge_erosion - 1, cnt);
-        mask = m2;
-        if (is3d) {
-            hipLaunchKernelGGL(k_mask_erode_z, dim3(grid_for(N)), dim3(256), 0, stream, m2, m1, n1, n2, n3,
-                               (int)o->edge_erosion - 1, cnt);
-            mask = m1;
-        }
-        PK_HIP(hipGetLastError());
+        ensure(ws.count, 32);
+        mask = edge_mask(raw16, rawf, n1, n2, n3, (int)o->edge_erosion, ws.mask.as<uint8_t>(), ws.mask2.as<uint8_t>(),
+                         ws.count.as<unsigned long long>());
     }
     const int64_t on = od[0] * od[1] * od[2];
     const size_t osz = o->save16bit ? 2 : 4;
@@ -1180,30 +1331,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         ensure(ws.out, osz * on);
         dout = ws.out.p;
     }
-    EpiArgs e{};
-    e.I = I;
-    e.mask = mask;
-    e.raw16 = raw16;
-    e.rawf = rawf;
-    e.out16 = o->save16bit ? static_cast<uint16_t*>(dout) : nullptr;
-    e.outf = o->save16bit ? nullptr : static_cast<float*>(dout);
-    e.n1 = n1;
-    e.n2 = n2;
-    e.o1 = od[0];
-    e.o2 = od[1];
-    e.o3 = od[2];
-    e.b1 = b0[0];
-    e.b2 = b0[1];
-    e.b3 = b0[2];
-    e.damp = (float)o->damp_factor;
-    e.scale = (float)o->scale_factor;
-    e.offset = (float)o->decon_offset;
-    e.apply_scale = (o->scale_factor != 1 || o->decon_offset != 0 || o->edge_erosion > 0);
-    e.use_mask = need_mask;
-    if (!(o->damp_factor > 1)) e.damp = 0.f;
-    // pair stores in the fused form: even dim-1 crop start and size, aligned output
-    e.pair = vp.packed && b0[0] % 2 == 0 && od[0] % 2 == 0 &&
-             reinterpret_cast<uintptr_t>(dout) % (2 * osz) == 0;
+    EpiArgs e = epi_args(vp, o, g, I, mask, raw16, rawf, dout);
     // the last update writes the output itself (fused epilogue) when the row
     // pass covers the whole volume in one launch
     const int64_t kz = (o->flags & PK_FLAG_UNFUSED) ? 0 : slab_planes(vp);
@@ -1221,8 +1349,6 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         auto slabs = [&](auto&& body) {
             for (int64_t z0 = 0; z0 < vp.n3; z0 += step) body(z0, std::min(step, vp.n3 - z0));
         };
-        // the strided pass pair around the convolution (dim 2 unless the plan
-        // convolves along dim 2); slabs only split dim-2 passes
         const int outer = vp.conv_axis == 3 ? 2 : 3;
         auto y_fwd = [&](int64_t z0, int64_t nz) { run_col(vp, outer, A, CM_FWD, nullptr, 1.f, z0, nz); };
         auto y_inv = [&](int64_t z0, int64_t nz) { run_col(vp, outer, A, CM_INV, nullptr, 1.f, z0, nz); };
@@ -1237,64 +1363,10 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
             (uint64_t)method, (uint64_t)numit, u64(&vp), (uint64_t)vp.N, u64(Hf), u64(Hb), u64(ws.I.p), u64(ws.J.p),
             u64(ws.A.p), u64(ws.J3.p), u64(ws.Y.p), u64(ws.J4.p), u64(ws.lambda.p), u64(ws.partials.p),
             (uint64_t)kz, u64(stream), (uint64_t)fuse_epi};
-
-        // prep (decon_lucy_function.m:85-100, GPU branch): uint16 I - uint16(Background)
-        // saturates at 0; single I gives max(I - Background, 0)
-        XArgs pa{};
-        pa.spec = A;
-        pa.I = I;
-        pa.J = method == 1 ? nullptr : J;  // OMW: the first update reads I (J_1 = I) instead of a copy
-        if (raw16) {
-            pa.mode = XM_PREP_U16;
-            pa.raw16 = raw16;
-            pa.bg = (float)matlab_u16(o->background);
-        } else {
-            pa.mode = XM_PREP_F32;
-            pa.rawf = rawf;
-            pa.bg = (float)o->background;
-        }
+        const XArgs pa = prep_args(method, A, I, J, raw16, rawf, o->background);
 
         if (method == 1) {
-            // J_{k+1} = max(J_k .* real(ifftn(fftn(I ./ max(real(ifftn(fftn(J_k) .* OTF_f)), eps)) .* OTF_b)), 0)
-            slabs([&](int64_t z0, int64_t nz) {
-                run_x(vp, pa, z0, nz);
-                if (numit > 0) y_fwd(z0, nz);
-            });
-            run_graphed(loop_key, [&] {
-            for (int k = 1; k <= numit; ++k) {
-                z_conv(Hf, false);
-                slabs([&](int64_t z0, int64_t nz) {
-                    y_inv(z0, nz);
-                    XArgs r{};
-                    r.mode = XM_RATIO;
-                    r.spec = A;
-                    r.I = I;
-                    run_x(vp, r, z0, nz);
-                    y_fwd(z0, nz);
-                });
-                z_conv(Hb, false);
-                slabs([&](int64_t z0, int64_t nz) {
-                    y_inv(z0, nz);
-                    XArgs u{};
-                    u.mode = XM_OMW_UPDATE;
-                    u.spec = A;
-                    u.J = J;
-                    u.Jin = k == 1 ? I : nullptr;
-                    u.r2c_out = k < numit;
-                    if (!(fuse_epi && k == numit)) run_x(vp, u, z0, nz);
-                    if (k < numit) y_fwd(z0, nz);
-                });
-            }
-            });
-            if (fuse_epi) {  // last update + epilogue
-                XArgs u{};
-                u.mode = XM_OMW_FINAL;
-                u.spec = A;
-                u.J = J;
-                u.Jin = numit == 1 ? I : nullptr;
-                u.ep = e;
-                run_x(vp, u);
-            }
+            omw_passes(vp, pa, I, J, A, Hf, Hb, numit, e, fuse_epi, kz, &loop_key);
         } else {
             run_x(vp, pa);
             float* J2 = J;
@@ -1376,6 +1448,134 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         PK_HIP(hipMemcpyAsync(out->data, dout, osz * on, hipMemcpyDeviceToHost, stream));
     PK_HIP(hipStreamSynchronize(stream));
     prof_flush();
+    if (err_mat) std::memset(err_mat, 0, sizeof(double) * 4 * (size_t)numit);
+}
+
+// ------------------------------------------------------------------ no-allocation OMW
+namespace {
+// Carves 256-byte aligned pieces off a device region (base may be null to
+// measure: the pieces' addresses are then offsets).
+struct Carver {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t bytes) {
+        off = (off + 255) / 256 * 256;
+        T* p = reinterpret_cast<T*>(base + off);
+        off += bytes;
+        return p;
+    }
+};
+
+struct ProfOff {  // per-pass events stay off for the call (an event is a resource)
+    bool& flag;
+    bool was;
+    explicit ProfOff(bool& f) : flag(f), was(f) { flag = false; }
+    ~ProfOff() { flag = was; }
+};
+
+// PSF ./ sum(PSF(:)) with the sum accumulated in double and the division in
+// single, as the cached path's otf_build does
+std::vector<float> unit_sum(const float* psf, const int64_t pd[3]) {
+    const int64_t P = pd[0] * pd[1] * pd[2];
+    double s = 0;
+    for (int64_t i = 0; i < P; ++i) s += (double)psf[i];
+    const float sf = (float)s;
+    std::vector<float> q((size_t)P);
+    for (int64_t i = 0; i < P; ++i) q[(size_t)i] = psf[i] / sf;
+    return q;
+}
+
+struct NoallocLayout {
+    float2 *tx, *ty, *tz, *rt;
+    float *I, *J;
+    float2 *A, *Hf, *Hb;
+    uint8_t *m1, *m2;
+    unsigned long long* cnt;
+    size_t bytes;
+};
+
+NoallocLayout noalloc_layout(char* base, const int64_t d[3], int edge_erosion) {
+    const int64_t n1 = d[0], n2 = d[1], n3 = d[2], N = n1 * n2 * n3;
+    const int64_t h1 = n1 / 2 + 1, hp = (h1 + 15) / 16 * 16, S = hp * n2 * n3;
+    const bool packed = n1 % 2 == 0;
+    Carver c{base};
+    NoallocLayout L{};
+    L.tx = c.take<float2>(sizeof(float2) * (size_t)(packed ? n1 / 2 : n1));
+    L.ty = c.take<float2>(sizeof(float2) * (size_t)n2);
+    L.tz = c.take<float2>(sizeof(float2) * (size_t)n3);
+    L.rt = packed ? c.take<float2>(sizeof(float2) * (size_t)(n1 / 2 + 1)) : nullptr;
+    L.I = c.take<float>(sizeof(float) * (size_t)N);
+    L.J = c.take<float>(sizeof(float) * (size_t)N);
+    L.A = c.take<float2>(sizeof(float2) * (size_t)S);
+    L.Hf = c.take<float2>(sizeof(float2) * (size_t)S);
+    L.Hb = c.take<float2>(sizeof(float2) * (size_t)S);
+    if (edge_erosion > 0) {
+        L.m1 = c.take<uint8_t>((size_t)N);
+        L.m2 = c.take<uint8_t>((size_t)N);
+        L.cnt = c.take<unsigned long long>(32);
+    }
+    L.bytes = c.off;
+    return L;
+}
+}  // namespace
+
+size_t pk_ctx_impl::noalloc_bytes(const int64_t d[3], int edge_erosion) {
+    return noalloc_layout(nullptr, d, edge_erosion).bytes;
+}
+
+void pk_ctx_impl::decon_omw_noalloc(const pk_vol* Iv, const float* psf_f, const int64_t fd[3], const float* psf_b,
+                                    const int64_t bd[3], int numit, const pk_opts* o, pk_vol* out, double* err_mat,
+                                    void* scratch, size_t scratch_bytes) {
+    const DeconGeom g = decon_check(1, Iv, psf_f, fd, psf_b, bd, numit, o, out);
+    if (Iv->mem != PK_MEM_DEVICE || out->mem != PK_MEM_DEVICE)
+        fail(PK_ERR_INVALID_ARG, "the no-allocation entry point takes device volumes only (I and out)");
+    if (o->flags & (PK_FLAG_UNFUSED | PK_FLAG_AUTOTUNE))
+        fail(PK_ERR_INVALID_ARG, "PK_FLAG_UNFUSED and PK_FLAG_AUTOTUNE allocate: not accepted without allocation");
+    if (!scratch) fail(PK_ERR_HIP, "the device scratch array is not available on this device");
+    const int64_t n1 = Iv->dims[0], n2 = Iv->dims[1], n3 = Iv->dims[2];
+    const size_t need = noalloc_bytes(Iv->dims, o->edge_erosion);
+    if (need > scratch_bytes)
+        fail(PK_ERR_UNSUPPORTED_SIZE, "a " + std::to_string(n1) + "x" + std::to_string(n2) + "x" + std::to_string(n3) +
+                                          " volume needs " + std::to_string(need) + " bytes of device scratch; the "
+                                          "no-allocation entry point holds " + std::to_string(scratch_bytes));
+    ProfOff no_events(prof_on);
+    VolPlan vp = plan_shapes(n1, n2, n3);  // host only: shapes and pass order, no tables
+    const NoallocLayout L = noalloc_layout(static_cast<char*>(scratch), Iv->dims, o->edge_erosion);
+    // plan tables into the scratch (the host vectors live until the stream sync below)
+    std::vector<float2> tx, ty, tz, rt;
+    twiddle_host_tables(vp, tx, ty, tz, rt);
+    PK_HIP(hipMemcpyAsync(L.tx, tx.data(), sizeof(float2) * tx.size(), hipMemcpyHostToDevice, stream));
+    PK_HIP(hipMemcpyAsync(L.ty, ty.data(), sizeof(float2) * ty.size(), hipMemcpyHostToDevice, stream));
+    PK_HIP(hipMemcpyAsync(L.tz, tz.data(), sizeof(float2) * tz.size(), hipMemcpyHostToDevice, stream));
+    vp.x.fp.tw = L.tx;
+    vp.y.fp.tw = L.ty;
+    vp.z.fp.tw = L.tz;
+    if (vp.packed) {
+        PK_HIP(hipMemcpyAsync(L.rt, rt.data(), sizeof(float2) * rt.size(), hipMemcpyHostToDevice, stream));
+        vp.rtw = L.rt;
+    }
+    // OTF_f and OTF_b (decon_lucy_omw_function.m:70-71) built in place: J (not
+    // yet live) is the zero-padded PSF volume, A (not yet live) holds the
+    // uploaded PSF values
+    const std::vector<float> qf = unit_sum(psf_f, fd), qb = unit_sum(psf_b, bd);
+    build_otf_into(vp, qf.data(), fd, L.Hf, L.J, reinterpret_cast<float*>(L.A));
+    build_otf_into(vp, qb.data(), bd, L.Hb, L.J, reinterpret_cast<float*>(L.A));
+
+    const uint16_t* raw16 = Iv->dtype == PK_UINT16 ? static_cast<const uint16_t*>(Iv->data) : nullptr;
+    const float* rawf = Iv->dtype == PK_SINGLE ? static_cast<const float*>(Iv->data) : nullptr;
+    const uint8_t* mask = nullptr;
+    if (o->edge_erosion > 0) mask = edge_mask(raw16, rawf, n1, n2, n3, (int)o->edge_erosion, L.m1, L.m2, L.cnt);
+    EpiArgs e = epi_args(vp, o, g, L.I, mask, raw16, rawf, out->data);
+    const bool fuse_epi = !(o->flags & PK_FLAG_EPILOGUE_PASS) && numit > 0;
+    const XArgs pa = prep_args(1, L.A, L.I, L.J, raw16, rawf, o->background);
+    omw_passes(vp, pa, L.I, L.J, L.A, L.Hf, L.Hb, numit, e, fuse_epi, 0, nullptr);
+    if (!fuse_epi) {
+        e.J = numit == 0 ? L.I : L.J;  // OMW with no iteration: J = I
+        hipLaunchKernelGGL(k_epilogue, dim3(epi_grid(e)), dim3(256), 0, stream, e);
+        PK_HIP(hipGetLastError());
+    }
+    PK_HIP(hipStreamSynchronize(stream));
     if (err_mat) std::memset(err_mat, 0, sizeof(double) * 4 * (size_t)numit);
 }
 

@@ -71,7 +71,15 @@ struct VolPlan {
     // forward pass before it and an inverse pass after it
     int conv_axis = 3;
     bool tuned = false;  // launch shapes and pass order measured by autotune()
+    // complex points of a row transform: n1 / 2 for packed (even) rows, n1 for odd rows
+    int row_len() const { return (int)(packed ? n1 / 2 : n1); }
 };
+
+// Host twiddle tables of a plan: its row length, n2, n3 and (even n1) the
+// split step's W_{n1}^k, k in [0, n1/2] -- the contents of the device tables
+// plan() uploads.
+void twiddle_host_tables(const VolPlan& vp, std::vector<float2>& tx, std::vector<float2>& ty,
+                         std::vector<float2>& tz, std::vector<float2>& rt);
 
 struct OtfEntry {
     int64_t dims[3];
@@ -106,6 +114,21 @@ struct Chunk {
 // BackProjector.m (host): see pk_wb_backprojector in pkdecon.h
 void wb_backprojector(const float* psf, const int64_t d[3], int bp_type, double alpha, double beta, int n,
                       int res_flag, const double ires[3], double* psf_bp, double* otf_bp, double* info);
+
+// Output geometry of a decon call: 0-based bbox start and output size.
+struct DeconGeom {
+    int64_t b0[3], od[3];
+};
+// Argument checks of pk_decon_lucy / pk_decon_lucy_omw (method 0 / 1).
+DeconGeom decon_check(int method, const pk_vol* I, const float* psf_f, const int64_t fd[3], const float* psf_b,
+                      const int64_t bd[3], int numit, const pk_opts* o, const pk_vol* out);
+// Epilogue arguments (decon_lucy_function.m:177-203) writing to dout.
+EpiArgs epi_args(const struct VolPlan& vp, const pk_opts* o, const DeconGeom& g, const float* I,
+                 const uint8_t* mask, const uint16_t* raw16, const float* rawf, void* dout);
+
+// Resolves the no-allocation entry point's device scratch array on `device`
+// (the current device); called once per context creation (pk_noalloc.hip).
+void noalloc_setup(int device);
 
 struct Server;
 // request loop (pk_serve.hip)
@@ -203,12 +226,20 @@ struct pk_ctx_impl {
     void ensure(DevBuf& b, size_t bytes);
 
     const VolPlan& plan(int64_t n1, int64_t n2, int64_t n3);
+    // geometry, launch shapes (built-in or from the tuning table) and pass
+    // order of a padded size, without device tables (their pointers null)
+    VolPlan plan_shapes(int64_t n1, int64_t n2, int64_t n3) const;
     const float2* otf_for(const VolPlan& vp, const float* psf, const int64_t pdims[3]);
     // cache lookup by key (nullptr on a miss) and build of a missing entry
     const float2* otf_find(const VolPlan& vp, uint64_t key, const int64_t pdims[3]);
     const float2* otf_build(const VolPlan& vp, uint64_t key, const float* psf, const int64_t pdims[3]);
     // R2C of the zero-padded, circularly shifted PSF q (decon_psf2otf.m:19-27) into dst
     void build_otf(const VolPlan& vp, const float* q, const int64_t pdims[3], float2* dst);
+    // the same on the engine's stream with the caller's device buffers: real
+    // (N floats, zeroed here) and dq (the PSF's prod(pdims) floats, uploaded
+    // here from q, which must stay valid until the stream reaches the copy)
+    void build_otf_into(const VolPlan& vp, const float* q, const int64_t pdims[3], float2* dst, float* real,
+                        float* dq);
 
     // transforms (device pointers)
     void r2c3d(const VolPlan& vp, const float* real, float2* spec);
@@ -238,6 +269,22 @@ struct pk_ctx_impl {
 
     void decon(int method, const pk_vol* I, const float* psf_f, const int64_t fd[3], const float* psf_b,
                const int64_t bd[3], int numit, const pk_opts* o, pk_vol* out, double* err_mat);
+    // decon_lucy_omw_function with every device buffer (plan tables, OTFs,
+    // estimate, spectrum, masks) carved from the caller's device region
+    // [scratch, scratch + scratch_bytes): no allocation, no cache, no graph.
+    // I and out must be device memory.
+    void decon_omw_noalloc(const pk_vol* I, const float* psf_f, const int64_t fd[3], const float* psf_b,
+                           const int64_t bd[3], int numit, const pk_opts* o, pk_vol* out, double* err_mat,
+                           void* scratch, size_t scratch_bytes);
+    // bytes of scratch decon_omw_noalloc needs for a volume of size d
+    static size_t noalloc_bytes(const int64_t d[3], int edge_erosion);
+    // the OMW pass sequence shared by decon and decon_omw_noalloc
+    void omw_passes(const VolPlan& vp, const XArgs& prep, float* I, float* J, float2* A, const float2* Hf,
+                    const float2* Hb, int numit, const EpiArgs& e, bool fuse_epi, int64_t kz,
+                    const std::vector<uint64_t>* loop_key);
+    // decon_mask_edge_erosion of I ~= 0 into the caller's buffers; returns the eroded mask
+    const uint8_t* edge_mask(const uint16_t* raw16, const float* rawf, int64_t n1, int64_t n2, int64_t n3, int E,
+                             uint8_t* m1, uint8_t* m2, unsigned long long* cnt);
     void reset();
     void decon_batch(int method, const pk_batch_item* items, int n, const float* psf_f, const int64_t fd[3],
                      const float* psf_b, const int64_t bd[3], int numit, const pk_opts* o);
