@@ -13,7 +13,12 @@ is for the pass with the largest share of device time (algorithmic bytes per
 launch / mean launch time, HIP events on the engine's stream); `plain_gpu` is
 the same call on the MATLAB-structured unfused path with OTFs rebuilt per
 call; `cpu_baseline` times the oracle (the CPU single-precision restatement)
-on the host cores for the same call.
+on the host cores for the same call.  The same line carries the metric's
+other halves: `batched` (64 random-size chunks, OMW-2 and Biggs-10 in
+batched launches, Mvoxel/s against the chunks' HBM roofline and against the
+plain path) and `served` (the continuous-batching server under Poisson
+arrivals at 0.7x its measured capacity for --duration seconds: p50 / p99
+latency, with the plain path's on the same arrival trace).
 
 Other workloads (--workload): rl128 (Biggs RL 15 it. on 128x128x64), batched
 (64 random-size chunks, OMW-2 and RL-10 in batched launches), served
@@ -56,7 +61,11 @@ def parse():
                                                                   "deskew", "rotate"])
     ap.add_argument("--shape", type=int, nargs=3, default=None)
     ap.add_argument("--iters", type=int, default=None)
-    ap.add_argument("--duration", type=float, default=20.0, help="served: seconds of Poisson arrivals")
+    ap.add_argument("--duration", type=float, default=60.0, help="served: seconds of Poisson arrivals")
+    ap.add_argument("--plain-seconds", type=float, default=15.0,
+                    help="served: seconds of the same arrival trace replayed on the plain path")
+    ap.add_argument("--no-batched", action="store_true", help="default workload: skip the batched (config 3) section")
+    ap.add_argument("--no-served", action="store_true", help="default workload: skip the served (config 4) section")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check against the oracle")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
@@ -230,6 +239,11 @@ def roofline(prof, steps, workload=None):
             passes, sum(v[1] for v in prof.values()) / steps)
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def base_result(metric, value, unit, args, ms, config):
     return {"metric": metric, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -296,7 +310,17 @@ def wl_omw(args, torch, pk, L):
     for _ in range(3):
         pk.decon_lucy_omw_function(I, psf, psf_b, iters, Background=100, ctx=ctx)
     res["host_io_Mvoxel_s"] = round(nvox / 1e6 / ((time.perf_counter() - t) / 3), 1)
+    progress("cpu baseline and parity")
     cpu_and_parity(args, res, ctx, pk, I, psf, psf_b, iters, "omw", nvox, shape)
+    if shape == (512, 512, 256) and iters == 2:
+        # the headline metric's other halves: batched (config 3) and served at fixed load (config 4)
+        ctx.close()
+        del It, call
+        torch.cuda.empty_cache()
+        if not args.no_batched:
+            res["batched"] = batched_section(args, torch, pk, psf, psf_b, args.steps)
+        if not args.no_served:
+            res["served"] = served_section(args, torch, pk)
     return res
 
 
@@ -358,10 +382,24 @@ def _chunk_sizes(n, seed):
     return [(int(g.integers(128, 385)), int(g.integers(128, 385)), int(g.integers(64, 257))) for _ in range(n)]
 
 
-def wl_batched(args, torch, pk, L):
+def chunk_model_bytes(shape, iters, method):
+    """Algorithmic bytes of one decon call on the padded size of a chunk (the
+    mandated pass design: 24N + 176S per OMW iteration, 40N + 176S per Biggs
+    iteration, 12N set-up; N padded voxels, S half-spectrum elements)."""
+    from petakit5d_amd.decon import findGoodFactorNumber
+    n1, n2, n3 = (findGoodFactorNumber(n, 1, False, maxPrime=5) for n in shape)
+    N = n1 * n2 * n3
+    S = (n1 // 2 + 1) * n2 * n3
+    per = (24 * N + 176 * S) if method == "omw" else (40 * N + 176 * S)
+    return iters * per + 12 * N
+
+
+def batched_section(args, torch, pk, psf, psf_b, steps):
+    """Config 3: 64 chunks (x, y ~ U[128, 384], z ~ U[64, 256]) cropped from one
+    blob volume, OMW-2 and Biggs-10 through pk_decon_batch; Mvoxel/s, the
+    fraction of the chunks' HBM roofline, one fused call per chunk and the
+    plain path beside it."""
     from petakit5d_amd import synth
-    psf = synth.config2_psf()
-    psf_b, _ = back_projector(psf, args.bp)
     sizes = _chunk_sizes(64, 0)
     # chunks are crops of one blob volume (the config-2 generator at the same blob density)
     pool = synth._blob_volume_torch((384, 384, 256), psf, 11250, 3, 100.0, as_tensor=True)
@@ -374,17 +412,24 @@ def wl_batched(args, torch, pk, L):
     del pool
     ctx = pk.Context(0)
     nvox = float(sum(np.prod(s) for s in sizes))
-    out = {}
-    for method, iters in (("omw", args.iters or 2), ("rl", 10)):
+    out = {"chunks": 64, "voxels": nvox}
+    for method, iters in (("omw", 2), ("rl", 10)):
+        progress(f"batched {method}-{iters}: 64 chunks")
         fl = 0 if args.no_autotune else 4  # PK_FLAG_AUTOTUNE: shapes measured per chunk size in the warm-up batch
         fn = lambda: pk.decon_batch(chunks, psf, psf_b, iters, method=method, Background=100, ctx=ctx,  # noqa: E731
                                     flags=fl)
-        ms, _, prof = timed(fn, max(2, args.steps // 3), 1, torch, ctx, events_in_timed=args.events_in_timed)
+        nb = max(2, steps // 3)
+        ms, _, prof = timed(fn, nb, 1, torch, ctx, events_in_timed=args.events_in_timed)
         prof.pop("__ms_with_events", None)
-        nb = max(2, args.steps // 3)
-        rec = {"Mvoxel_s": round(nvox / 1e6 / (ms / 1e3), 1), "ms_per_batch": round(ms, 2),
+        mvs = nvox / 1e6 / (ms / 1e3)
+        model = float(sum(chunk_model_bytes(s, iters, method) for s in sizes))
+        roof = nvox / 1e6 / (model / (HBM_PEAK_GBPS * 1e9))
+        rec = {"Mvoxel_s": round(mvs, 1), "ms_per_batch": round(ms, 2),
+               "roofline_Mvoxel_s": round(roof, 1), "roofline_frac": round(mvs / roof, 4),
+               "model_bytes_per_batch": model,
                "launches_per_batch": int(sum(v[0] for v in prof.values()) / nb),
                "passes_ms_per_batch": {k: round(v[1] / nb, 3) for k, v in sorted(prof.items())}}
+
         # one fused call per chunk (chunks padded by the Python wrapper when not smooth)
         def seq(plain=False):
             for c in chunks:
@@ -395,25 +440,37 @@ def wl_batched(args, torch, pk, L):
                                                flags=PLAIN if plain else 0)
                 else:
                     pk.decon_lucy_function(c, pf, iters, Background=100, pad=True, ctx=ctx, flags=PLAIN if plain else 0)
-        sms, _, sprof = timed(seq, 1, 1, torch, ctx, events_in_timed=args.events_in_timed)
-        sprof.pop("__ms_with_events", None)
-        rec["one_call_per_chunk_passes_ms"] = {k: round(v[1], 3) for k, v in sorted(sprof.items())}
+        sms, _, _ = timed(seq, 1, 1, torch)
         rec["one_call_per_chunk_Mvoxel_s"] = round(nvox / 1e6 / (sms / 1e3), 1)
         if not args.no_plain:
             pms, _, _ = timed(lambda: seq(True), 1, 0, torch)
             rec["plain_gpu_Mvoxel_s"] = round(nvox / 1e6 / (pms / 1e3), 1)
             rec["speedup_vs_plain"] = round(rec["Mvoxel_s"] / rec["plain_gpu_Mvoxel_s"], 2)
         out[f"{method}{iters}"] = rec
-    v = out["omw2"]["Mvoxel_s"] if "omw2" in out else list(out.values())[0]["Mvoxel_s"]
-    res = base_result("deconvolved Mvoxel/s (batched OMW, 2 it)", v, "Mvoxel/s", args, out["omw2"]["ms_per_batch"], {
-        "workload": "64 chunks, x,y ~ U[128,384], z ~ U[64,256] (padded to 2/3/5-smooth), cropped from a blob volume, "
-                    "one shared 64x64x101 PSF (centre-cropped for chunks < 101 in z), device-resident uint16; "
-                    "one step = one batch", "chunks": 64, "voxels": nvox})
+    ctx.close()
+    return out
+
+
+def wl_batched(args, torch, pk, L):
+    from petakit5d_amd import synth
+    psf = synth.config2_psf()
+    psf_b, _ = back_projector(psf, args.bp)
+    out = batched_section(args, torch, pk, psf, psf_b, args.steps)
+    res = base_result("deconvolved Mvoxel/s (batched OMW, 2 it)", out["omw2"]["Mvoxel_s"], "Mvoxel/s", args,
+                      out["omw2"]["ms_per_batch"], {
+                          "workload": "64 chunks, x,y ~ U[128,384], z ~ U[64,256] (padded to 2/3/5-smooth), cropped "
+                                      "from a blob volume, one shared 64x64x101 PSF (centre-cropped for chunks < 101 "
+                                      "in z), device-resident uint16; one step = one batch", "chunks": 64,
+                          "voxels": out["voxels"]})
     res["batched"] = out
     return res
 
 
-def wl_served(args, torch, pk, L):
+def served_section(args, torch, pk, half_load=False):
+    """Config 4: a continuous-batching server fed Poisson arrivals at 0.7x its
+    measured capacity for args.duration seconds; p50 / p99 latency, Mvoxel/s,
+    and the plain path (one unfused call per request, FIFO) on the first
+    args.plain_seconds of the same arrival trace."""
     from petakit5d_amd import synth
     g = synth.rng(4, 0)
     psfs = [synth.gaussian_psf((64, 64, 101), 1.0, 2.5), synth.gaussian_psf((64, 64, 101), 1.3, 3.2)]
@@ -493,44 +550,49 @@ def wl_served(args, torch, pk, L):
         cth = threading.Thread(target=consumer)
         cth.start()
         t0 = time.perf_counter()
+        mark = 20.0
         for at, j in arr:
             d = at - (time.perf_counter() - t0)
             if d > 0:
                 time.sleep(d)
             pending.put((submit(*j), j[0].shape, j[1]))
+            if at >= mark:
+                progress(f"served: {at:.0f} s of arrivals, {len(lat)} requests done")
+                mark += 20.0
         pending.put(None)
         cth.join()
         return arr, time.perf_counter() - t0, lat, detail
 
-    # open-loop Poisson arrivals at 0.7x capacity (the reported load), then at 0.5x
+    # open-loop Poisson arrivals at 0.7x capacity (the reported load)
+    progress(f"served: capacity {cap_rps:.1f} req/s; Poisson arrivals at {qps:.1f} req/s for {args.duration:.0f} s")
     arr, wall, lat, detail = open_loop(qps, args.duration)
     vox = sum(float(np.prod(j[0].shape)) for _, j in arr)
-    arr_h, wall_h, lat_h, _ = open_loop(0.5 * cap_rps, args.duration / 2)
-    vox_h = sum(float(np.prod(j[0].shape)) for _, j in arr_h)
-    srv.stop()
-    res = base_result("served decon Mvoxel/s + p99 latency (OMW-2 / Biggs-15 mix)", vox / 1e6 / wall, "Mvoxel/s", args,
-                      float(np.mean(lat)), {
-                          "workload": f"Poisson arrivals at {qps:.2f}/s (0.7x measured capacity) for {args.duration:.0f} s; "
-                                      "sizes log-uniform 96x96x64..1024x1024x256 (24 shapes), 2 channels "
-                                      "(64x64x101 PSFs sigma 1.0/2.5, 1.3/3.2), 70% OMW-2 / 30% Biggs-15, HBM budget "
-                                      "200 GB, host uint16 in/out (client volumes in pinned memory)",
-                          "requests": len(arr)})
+    res = {"workload": f"Poisson arrivals at {qps:.2f}/s (0.7x measured capacity) for {args.duration:.0f} s; "
+                       "sizes log-uniform 96x96x64..1024x1024x256 (24 shapes), 2 channels "
+                       "(64x64x101 PSFs sigma 1.0/2.5, 1.3/3.2), 70% OMW-2 / 30% Biggs-15, HBM budget "
+                       "200 GB, host uint16 in/out (client volumes in pinned memory)",
+           "seconds": args.duration, "requests": len(arr), "req_per_s": round(qps, 2),
+           "Mvoxel_s": round(vox / 1e6 / wall, 1)}
     res["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 2), "p90": round(float(np.percentile(lat, 90)), 2),
                          "p99": round(float(np.percentile(lat, 99)), 2)}
     detail.sort(key=lambda d: -d[0])
-    res["slowest"] = [{"latency_ms": d[0], "queue_ms": d[1], "method": d[2], "shape": d[3]} for d in detail[:5]]
-    res["half_load"] = {"req_per_s": round(0.5 * cap_rps, 2), "seconds": args.duration / 2, "requests": len(arr_h),
-                        "Mvoxel_s": round(vox_h / 1e6 / wall_h, 1),
-                        "latency_ms": {"p50": round(float(np.percentile(lat_h, 50)), 2),
-                                       "p90": round(float(np.percentile(lat_h, 90)), 2),
-                                       "p99": round(float(np.percentile(lat_h, 99)), 2)}}
+    res["slowest"] = [{"latency_ms": d[0], "queue_ms": d[1], "method": d[2], "shape": d[3]} for d in detail[:3]]
+    if half_load:
+        arr_h, wall_h, lat_h, _ = open_loop(0.5 * cap_rps, args.duration / 2)
+        vox_h = sum(float(np.prod(j[0].shape)) for _, j in arr_h)
+        res["half_load"] = {"req_per_s": round(0.5 * cap_rps, 2), "seconds": args.duration / 2, "requests": len(arr_h),
+                            "Mvoxel_s": round(vox_h / 1e6 / wall_h, 1),
+                            "latency_ms": {"p50": round(float(np.percentile(lat_h, 50)), 2),
+                                           "p90": round(float(np.percentile(lat_h, 90)), 2),
+                                           "p99": round(float(np.percentile(lat_h, 99)), 2)}}
+    srv.stop()
     res["capacity"] = {"req_per_s": round(cap_rps, 2), "Mvoxel_s": round(cap_vox / 1e6 / cap_s, 1),
                        "how": "48 requests submitted at once"}
     if not args.no_plain:
         # the same trace on the plain path: one unfused call per request, FIFO, one worker
         lat_p = []
         done = threading.Event()
-        q = list(arr)
+        q = [a for a in arr if a[0] <= args.plain_seconds]  # the trace's first plain_seconds
 
         def worker():
             t00 = time.perf_counter()
@@ -548,17 +610,27 @@ def wl_served(args, torch, pk, L):
                 lat_p.append((time.perf_counter() - t00 - at) * 1e3)
             done.set()
 
+        progress(f"served: plain path on the first {args.plain_seconds:.0f} s of the trace ({len(q)} requests)")
         tp = time.perf_counter()
         th = threading.Thread(target=worker)
         th.start()
         th.join()
         pw = time.perf_counter() - tp
-        busy_vox = vox
-        res["plain_gpu"] = {"Mvoxel_s": round(busy_vox / 1e6 / pw, 1),
+        busy_vox = sum(float(np.prod(j[0].shape)) for _, j in q)
+        res["plain_gpu"] = {"Mvoxel_s": round(busy_vox / 1e6 / pw, 1), "seconds_of_trace": args.plain_seconds,
+                            "requests": len(q),
                             "latency_ms": {"p50": round(float(np.percentile(lat_p, 50)), 2),
                                            "p99": round(float(np.percentile(lat_p, 99)), 2)},
                             "path": "one MATLAB-structured unfused call per request, OTFs rebuilt per call, FIFO"}
         res["capacity_speedup_vs_plain"] = round(res["capacity"]["Mvoxel_s"] / res["plain_gpu"]["Mvoxel_s"], 2)
+    return res
+
+
+def wl_served(args, torch, pk, L):
+    out = served_section(args, torch, pk, half_load=True)
+    res = base_result("served decon Mvoxel/s + p99 latency (OMW-2 / Biggs-15 mix)", out["Mvoxel_s"], "Mvoxel/s", args,
+                      out["latency_ms"]["p50"], {"workload": out.pop("workload"), "requests": out["requests"]})
+    res["served"] = out
     return res
 
 

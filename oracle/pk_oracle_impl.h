@@ -404,7 +404,7 @@ done:
     return err;
 }
 
-/* decon_lucy_omw_function.m:265-381, CPU branch. */
+/* decon_lucy_omw_function.m:79-178 (prep :90-98, loop :105-112, epilogue :145-176), CPU branch. */
 int F(pko_decon_omw)(const REAL *Iin, const int64_t d[3], const REAL *psf_f, const int64_t pfd[3],
                      const REAL *psf_b, const int64_t pbd[3], int numit, const pko_opts *o, REAL *J_out,
                      uint16_t *u16)

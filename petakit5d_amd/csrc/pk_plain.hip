@@ -1,6 +1,6 @@
 // pk_plain.hip -- the MATLAB-structured comparison path (PK_FLAG_UNFUSED):
 // the iteration of decon_lucy_function.m:121-142 and
-// decon_lucy_omw_function.m:321-328 with one kernel per gpuArray statement --
+// decon_lucy_omw_function.m:105-112 with one kernel per gpuArray statement --
 // fftn / ifftn as separate 3-pass transforms, the OTF product, max, ./ and .*
 // each a full-array pass.  Same transforms, plans and arithmetic as the fused
 // engine; only the fusion differs.  Used as the "plain GPU path" baseline.

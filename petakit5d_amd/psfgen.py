@@ -67,7 +67,10 @@ def psf_gen_new(psf, dz_psf, dz_data, medFactor=1.5, PSFGenMethod="masked"):
     psf_raw = psf.copy()
     ny, nx, nz = psf.shape
     method = PSFGenMethod.lower()
-    ends = np.r_[0:10, ny - 10:ny] if ny >= 20 else np.arange(ny)  # rows [1:10, end-9:end]
+    if ny < 10:  # psf([1:10, end-9:end], ...) indexes past the end in the reference (psf_gen_new.m:50, 59)
+        raise ValueError("psf_gen_new needs at least 10 rows (dim 1)")
+    # rows [1:10, end-9:end]: for ny < 20 the two ranges overlap and MATLAB keeps the repeats
+    ends = np.r_[0:10, ny - 10:ny]
     if nz > 1:
         fl = psf_raw[:, :, np.r_[0:5, nz - 5:nz]]
         if np.any(fl > 0):

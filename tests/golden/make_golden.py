@@ -7,7 +7,7 @@ from an independent computation:
   * rl_*.npz    -- Richardson-Lucy written with explicit circular
                    convolution / correlation sums over the PSF support (no
                    FFT), following decon_lucy_function.m:116-143 and
-                   decon_lucy_omw_function.m:321-329 literally, in float64.
+                   decon_lucy_omw_function.m:105-112 literally, in float64.
 """
 import os
 
