@@ -43,7 +43,7 @@ typedef enum {
     PK_ERR_CANCELLED = 12
 } pk_status;
 
-typedef enum { PK_UINT16 = 1, PK_SINGLE = 2 } pk_dtype;
+typedef enum { PK_UINT16 = 1, PK_SINGLE = 2, PK_UINT8 = 3 /* masks only */ } pk_dtype;
 typedef enum { PK_MEM_HOST = 0, PK_MEM_DEVICE = 1 } pk_memspace;
 
 /* A volume: data pointer (host or device), class, MATLAB size. */
@@ -70,7 +70,7 @@ typedef struct {
 
 /* Comparison baselines (no counterpart in the reference's options):
  * PK_FLAG_UNFUSED runs the MATLAB-structured path -- one kernel per gpuArray
- * statement of decon_lucy_function.m:121-142 / decon_lucy_omw_function.m:321-328
+ * statement of decon_lucy_function.m:121-142 / decon_lucy_omw_function.m:105-112
  * (3-pass fftn / ifftn, separate OTF product, max, ./, .*) -- on the same
  * transforms; PK_FLAG_NO_OTF_CACHE rebuilds the OTFs on every call. */
 #define PK_FLAG_UNFUSED 1
@@ -82,6 +82,17 @@ typedef struct {
  * uint16, bbox) as a pass of its own after the last update instead of inside
  * it; the results are bitwise identical (comparison baseline). */
 #define PK_FLAG_EPILOGUE_PASS 8
+/* PK_FLAG_DEBUG: 'debug', true of decon_lucy_function / decon_lucy_omw_function
+ * (decon_lucy_function.m:144-168, decon_lucy_omw_function.m:114-139): after
+ * every iteration k the mean squared difference between the estimate and the
+ * background-subtracted input is measured (on the transform-size volume), and
+ * err_mat (NUMIT x 4, column-major) receives the reference's rows
+ * [k, MSE_k, MSE_k / MSE_1, criterion] for k = 1..NUMIT, before the
+ * reference's final trimming (RL: rows 1..istp, OMW: rows 1..floor(NUMIT /
+ * saveStep)), which the caller applies.  Every saveStep-th estimate goes to
+ * the iteration hook (pk_set_iter_hook; the reference writes it as
+ * Iter_%04d.tif).  The iteration loop then runs kernel by kernel. */
+#define PK_FLAG_DEBUG 16
 
 typedef struct pk_ctx pk_ctx;
 
@@ -101,6 +112,13 @@ pk_status pk_reset(pk_ctx *ctx);
  * context's own stream. */
 pk_status pk_set_stream(pk_ctx *ctx, void *hip_stream);
 const char *pk_last_error(const pk_ctx *ctx);
+/* Iteration hook of PK_FLAG_DEBUG calls (no counterpart in the reference,
+ * which writes the estimates itself): called on the calling thread with the
+ * estimate after iteration k (k a multiple of save_step), host single
+ * values valid during the call, at the transform size dims.  save_step is
+ * 'saveStep' (>= 1); hook may be NULL (metrics only). */
+typedef void (*pk_iter_hook)(void *user, int32_t k, const float *estimate, const int64_t dims[3]);
+pk_status pk_set_iter_hook(pk_ctx *ctx, int32_t save_step, pk_iter_hook hook, void *user);
 /* Bytes of device memory the context holds (workspace + OTF cache). */
 uint64_t pk_device_bytes(const pk_ctx *ctx);
 
@@ -136,6 +154,14 @@ pk_status pk_decon_lucy_omw(pk_ctx *ctx, const pk_vol *I, const float *psf_f, co
 pk_status pk_decon_lucy_omw_noalloc(pk_ctx *ctx, const pk_vol *I, const float *psf_f, const int64_t f_dims[3],
                                     const float *psf_b, const int64_t b_dims[3], int32_t numit, const pk_opts *opts,
                                     pk_vol *out, double *err_mat);
+
+/* The whole-volume edge-erosion mask of the large-volume driver
+ * (microscopeDataProcessing/decon/RLdecon_large_in_memory.m:150-155):
+ * bw = im > 0 with the six faces cleared, mask = bwdist(~bw) > EdgeErosion - 1
+ * (Euclidean distance in voxels to the nearest false voxel of the volume).
+ * im: uint16 or single, host or device; mask: PK_UINT8 (0 / 1), same dims,
+ * host or device.  1 <= edge_erosion <= 256. */
+pk_status pk_edge_erosion_mask(pk_ctx *ctx, const pk_vol *im, int32_t edge_erosion, pk_vol *mask);
 
 /* One chunk of a batch: input and output volumes of the same (any) size. */
 typedef struct {
@@ -306,6 +332,25 @@ pk_status pk_deskew_out_dims(const int64_t dims[3], double angle, double dz, dou
                              int32_t crop, int64_t out_dims[3]);
 pk_status pk_deskew_frame3d(pk_ctx *ctx, const float *d_vol, const int64_t dims[3], double angle, double dz,
                             double xy_pixel_size, int32_t reverse, int32_t crop, float *d_out);
+
+/* volout = deskewRotateFrame3D(vol, angle, dz, xyPixelSize, 'reverse', reverse,
+ * 'objectiveScan', objective_scan, 'xStepThresh', x_step_thresh, 'bbox', bbox)
+ * with the default linear interpolation and no resampling
+ * (microscopeDataProcessing/deskew_rotate/deskewRotateFrame3D.m:50-253, the
+ * combined deskew and rotation RLdecon.m:489-495 runs by default).  When the
+ * slice-to-slice shift cos(angle) dz / xyPixelSize exceeds x_step_thresh
+ * (2.0 by default) the slices are first interpolated in skewed space to a
+ * finer step (skewed_space_interp_defined_stepsize.m:24-100); a stage scan is
+ * then warped row by row as volume_deskew_rotate_warp_mex.cpp:217-282 does, an
+ * objective scan through imwarp's trilinear form.  in / out: device volumes;
+ * in uint16 or single; out single, or uint16 = uint16() of the single result;
+ * out->dims = the bbox size ([s1 s2 s3 e1 e2 e3], 1-based inclusive; NULL =
+ * the whole output, pk_deskew_rotate_out_dims).  Returns when done. */
+pk_status pk_deskew_rotate_out_dims(const int64_t dims[3], double angle, double dz, double xy_pixel_size,
+                                    int32_t reverse, int32_t objective_scan, double x_step_thresh, int64_t out_dims[3]);
+pk_status pk_deskew_rotate_frame3d(pk_ctx *ctx, const pk_vol *in, double angle, double dz, double xy_pixel_size,
+                                   int32_t reverse, int32_t objective_scan, double x_step_thresh, const int64_t bbox[6],
+                                   pk_vol *out);
 
 /* Affine resampling as imwarp(vol, affine3d(T), 'linear', 'FillValues', 0,
  * 'OutputView', imref3d(out_dims, 1, 1, 1)) does for rotateFrame3D

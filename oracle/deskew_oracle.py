@@ -49,3 +49,197 @@ def deskew(vol, angle, dz, xy, reverse=False, crop=False):
         b = pad[:, np.minimum(x0 + 1, nx + 1), p - 1] * (x0 + 1 <= nx + 1)
         out[:, np.nonzero(ok)[0], p - 1] = a + f * (b - a)
     return out
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# deskewRotateFrame3D (microscopeDataProcessing/deskew_rotate/deskewRotateFrame3D.m:50-253), linear interpolation, no
+# resampling: the reference's fast path -- skewed-space interpolation to a finer z step when the slice-to-slice
+# shift exceeds xStepThresh (skewed_space_interp_defined_stepsize.m:24-100), then the row-preserving bilinear warp of
+# volume_deskew_rotate_warp_mex.cpp:217-282 with the backward matrix of deskewRotateFrame3D.m:212-219.
+def _mround(x):
+    return math.floor(abs(x) + 0.5) * (1 if x >= 0 else -1)
+
+
+def dsr_geometry(shape, angle, dz, xy, reverse=False, objectiveScan=False, xStepThresh=2.0):
+    """(tmat, outSize, xstep, stepsize) -- tmat the 4x4 backward matrix over
+    0-based (dim1, dim2, dim3) coordinates, outSize [ny nx' nz'], xstep the
+    original |dx| and stepsize the interpolated slice step (1: none)."""
+    ny, nx, nz = (int(n) for n in shape)
+    theta = angle * math.pi / 180
+    dx = math.cos(theta) * dz / xy
+    zAniso = dz / xy if objectiveScan else math.sin(abs(theta)) * dz / xy
+    if not objectiveScan:
+        out = [ny, (nx - 1) * math.cos(theta) + (nz - 1) * zAniso / math.sin(abs(theta)),
+               (nx - 1) * math.sin(abs(theta)) - 4]
+    else:
+        out = [ny, nx * math.cos(theta) + nz * zAniso * math.sin(abs(theta)),
+               nz * zAniso * math.cos(theta) + nx * math.sin(abs(theta))]
+    out = [_mround(v) for v in out]
+    stepsize, xstep0 = 1.0, abs(dx)
+    if not objectiveScan and abs(dx) > xStepThresh:
+        if abs(dx) / xStepThresh < 1.5:
+            thr = xy * xStepThresh / math.cos(theta)
+            dzout = dz / math.ceil(dz / thr)
+            counter = 1
+            while dzout / thr < 0.4:
+                dzout = dz / (math.ceil((dz / thr) / (1 / (counter + 1))) * (1 / (counter + 1)))
+                counter += 1
+        else:
+            dzout = dz / math.ceil(abs(dx) / xStepThresh)
+        stepsize = dzout / dz
+        nz = math.floor(round((nz - 1) / stepsize * 100000) / 100000) + 1
+        dz = dzout
+        dx = math.cos(theta) * dz / xy
+        zAniso = math.sin(abs(theta)) * dz / xy
+    if not reverse:
+        xshift, xstp = -dx, dx
+    else:
+        xshift, xstp = dx + math.ceil((nz - 1) * dx), -dx
+    nxDs = math.ceil((nz - 1) * dx) + nx
+    if objectiveScan:
+        nxDs = nx
+        dsS = np.eye(4)
+    else:
+        dsS = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [xstp, 0, 1, 0], [xshift, 0, 0, 1]], dtype=np.float64)
+    if reverse:
+        theta = -theta
+    c = (np.array([ny, nxDs, nz], dtype=np.float64) + 1) / 2
+    T1 = np.eye(4)
+    T1[3, :3] = -c[[1, 0, 2]]
+    S = np.diag([1.0, 1.0, zAniso, 1.0])
+    ct, st = math.cos(theta), math.sin(theta)
+    R = np.array([[ct, 0, -st, 0], [0, 1, 0, 0], [st, 0, ct, 0], [0, 0, 0, 1]], dtype=np.float64)
+    T2 = np.eye(4)
+    T2[3, :3] = (np.array(out, dtype=np.float64)[[1, 0, 2]] + 1) / 2
+    T = dsS @ (T1 @ S @ R @ T2)  # imwarp's forward matrix (:210), 1-based [x y z 1] row vectors
+    off = np.zeros((4, 4))
+    off[3, :3] = 1
+    if not objectiveScan and reverse:
+        dsS[3, 0] -= dx
+    M = dsS @ ((T1 + off) @ S @ R @ (T2 - off))
+    tmat = np.linalg.inv(M.T)[[1, 0, 2, 3]][:, [1, 0, 2, 3]]
+    return tmat, tuple(out), xstep0, stepsize, T
+
+
+def row_structured(tmat, tol=1e-9):
+    """True when the backward matrix keeps dim-1 rows (source dim 1 = output
+    dim 1, source dim 2 a function of output dim 3 only, source dim 3 of
+    output dims 2 and 3): the form volume_deskew_rotate_warp_mex evaluates."""
+    t = np.asarray(tmat)
+    return (abs(t[0][0] - 1) < tol and abs(t[0][1]) < tol and abs(t[0][2]) < tol and abs(t[0][3]) < tol
+            and abs(t[1][0]) < tol and abs(t[1][1]) < tol and abs(t[2][0]) < tol)
+
+
+def general_warp(vol, T, out):
+    """imwarp(vol, affine3d(T), 'linear', 'FillValues', 0, 'OutputView',
+    imref3d(out, 1, 1, 1)): output voxel [x y z] (1-based, x = dim 2) reads
+    the input at [x y z 1] * inv(T), trilinear over one voxel of zero padding."""
+    from scipy import ndimage as ndi
+    v = np.asarray(vol, dtype=np.float64)
+    inv = np.linalg.inv(T)
+    Y, X, Z = np.meshgrid(np.arange(1, out[0] + 1), np.arange(1, out[1] + 1), np.arange(1, out[2] + 1),
+                          indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel(), np.ones(X.size)], axis=1) @ inv
+    coords = np.stack([pts[:, 1] - 1, pts[:, 0] - 1, pts[:, 2] - 1])
+    r = ndi.map_coordinates(v, coords, order=1, mode="grid-constant", cval=0.0, prefilter=False)
+    return r.reshape(out).astype(np.float32)
+
+
+def skewed_interp(vol, xstep, stepsize, reverse=False):
+    """skewed_space_interp_defined_stepsize(im, xstep, stepsize, 'Reverse', reverse)
+    (skewed_space_interp_defined_stepsize.m:24-100) in single precision: slice
+    zint at zout = 1 + (zint - 1) * stepsize blends slice floor(zout) sampled at
+    x + frac(zout) * xstep and the next slice at x - (1 - frac(zout)) * xstep
+    (signs swapped when reversed), each by linear interpolation over zero
+    padding; slices falling on an input slice are copied."""
+    im = np.asarray(vol, dtype=np.float32)
+    ny, nx, nz = im.shape
+    if stepsize == 1:
+        return im.copy()
+    xa = math.ceil(xstep)
+    n = math.floor(round((nz - 1) / stepsize * 100000) / 100000) + 1
+    out = np.zeros((ny, nx, n), np.float32)
+    for zi in range(n):
+        zout = 1 + zi * stepsize
+        zs = math.floor(zout)
+        zw = zout - zs
+        if zs >= nz:
+            out[:, :, zi] = im[:, :, nz - 1]
+            continue
+        if reverse:
+            szm = xa - zw * xstep + 1
+            tzm = szm - (xa - xstep)
+        else:
+            szm = zw * xstep + 1
+            tzm = szm + (xa - xstep)
+        s, t = math.floor(szm), math.floor(tzm)
+        sw, tw = szm - s, tzm - t
+        if (reverse and sw == 0) or (not reverse and tw == 0):
+            out[:, :, zi] = im[:, :, zs - 1]
+            continue
+        pad_s = np.zeros((ny, nx + xa + 1), np.float32)
+        pad_t = np.zeros((ny, nx + xa + 1), np.float32)
+        if reverse:
+            pad_s[:, xa:xa + nx] = im[:, :, zs - 1]
+            pad_t[:, 0:nx] = im[:, :, zs]
+        else:
+            pad_s[:, 0:nx] = im[:, :, zs - 1]
+            pad_t[:, xa:xa + nx] = im[:, :, zs]
+        sq = pad_s[:, s - 1:s - 1 + nx] * np.float32(1 - sw) + pad_s[:, s:s + nx] * np.float32(sw)
+        tq = pad_t[:, t - 1:t - 1 + nx] * np.float32(1 - tw) + pad_t[:, t:t + nx] * np.float32(tw)
+        out[:, :, zi] = sq * np.float32(1 - zw) + tq * np.float32(zw)
+    return out
+
+
+def row_warp(vol, tmat, bbox):
+    """volume_deskew_rotate_warp_mex.cpp:217-282: output (x, y, z) over the
+    1-based inclusive bbox reads source row (ty, tz), ty = t12 z + t13, tz =
+    t21 y + t22 z + t23 (0-based, z / y the output indices plus the bbox start),
+    blending the four rows around it bilinearly; rows with ty or tz outside
+    [0, n - 1) stay zero, except ty (tz) exactly n - 1, which reads the last row."""
+    v = np.asarray(vol, dtype=np.float32)
+    sx, sy, sz = v.shape
+    b = [int(x) for x in bbox]
+    ox, oy, oz = b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1
+    out = np.zeros((ox, oy, oz), np.float32)
+    x0 = b[0] - 1
+    for z in range(oz):
+        ty = tmat[1][2] * (z + b[2] - 1) + tmat[1][3]
+        if ty < 0 or ty + 1 >= sy:
+            if ty + 1 != sy:
+                continue
+            y0, wy = sy - 2, np.float32(1.0)
+        else:
+            y0 = math.floor(ty)
+            wy = np.float32(ty - y0)
+        for y in range(oy):
+            tz = tmat[2][1] * (y + b[1] - 1) + tmat[2][2] * (z + b[2] - 1) + tmat[2][3]
+            if tz < 0 or tz + 1 >= sz:
+                if tz + 1 != sz:
+                    continue
+                z0, wz = sz - 2, np.float32(1.0)
+            else:
+                z0 = math.floor(tz)
+                wz = np.float32(tz - z0)
+            one = np.float32(1)
+            w00, w01 = (one - wy) * (one - wz), (one - wy) * wz
+            w10, w11 = wy * (one - wz), wy * wz
+            a = v[x0:x0 + ox, y0, z0]
+            bb = v[x0:x0 + ox, y0, z0 + 1]
+            c = v[x0:x0 + ox, y0 + 1, z0]
+            d = v[x0:x0 + ox, y0 + 1, z0 + 1]
+            out[:, y, z] = ((a * w00 + bb * w01) + c * w10) + d * w11
+    return out
+
+
+def deskew_rotate(vol, angle, dz, xy, reverse=False, objectiveScan=False, xStepThresh=2.0, bbox=None):
+    """Single-precision result of deskewRotateFrame3D(single(vol), angle, dz,
+    xy, 'reverse', reverse, 'objectiveScan', objectiveScan, 'xStepThresh',
+    xStepThresh, 'bbox', bbox) on the linear path."""
+    tmat, out, xstep, stepsize, T = dsr_geometry(vol.shape, angle, dz, xy, reverse, objectiveScan, xStepThresh)
+    v = skewed_interp(vol, xstep, stepsize, reverse) if stepsize != 1 else np.asarray(vol, np.float32)
+    if not row_structured(tmat):  # objective scans: the reference's imwarp form (:210)
+        if bbox is not None:
+            raise ValueError("bbox with an objective-scan transform is not restated here")
+        return general_warp(v, T, out)
+    return row_warp(v, tmat, bbox if bbox is not None else [1, 1, 1, out[0], out[1], out[2]])

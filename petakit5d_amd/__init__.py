@@ -15,6 +15,7 @@ from .zarr_io import decon_zarr, read_zarr, write_zarr  # noqa: F401
 from .serve import Server  # noqa: F401
 from .deskew import deskewFrame3D  # noqa: F401
 from .rotate import rotateFrame3D  # noqa: F401
+from .deskew_rotate import deskewRotateFrame3D  # noqa: F401
 from .decon import (  # noqa: F401
     crop_psf_to_data,
     decon_lucy_function,
@@ -47,6 +48,7 @@ __all__ = [
     "readtiff",
     "deskewFrame3D",
     "rotateFrame3D",
+    "deskewRotateFrame3D",
     "writetiff",
     "Server",
     "crop_psf_to_data",

@@ -31,7 +31,7 @@ STATUS = {
     11: "PK_ERR_BUSY",
     12: "PK_ERR_CANCELLED",
 }
-PK_UINT16, PK_SINGLE = 1, 2
+PK_UINT16, PK_SINGLE, PK_UINT8 = 1, 2, 3
 PK_MEM_HOST, PK_MEM_DEVICE = 0, 1
 
 # every symbol include/pkdecon.h declares
@@ -42,11 +42,13 @@ EXPORTS = [
     "pk_reset",
     "pk_set_stream",
     "pk_last_error",
+    "pk_set_iter_hook",
     "pk_device_bytes",
     "pk_decon_lucy",
     "pk_decon_lucy_omw",
     "pk_decon_lucy_omw_noalloc",
     "pk_decon_batch",
+    "pk_edge_erosion_mask",
     "pk_serve_start",
     "pk_serve_stop",
     "pk_serve_submit",
@@ -73,6 +75,8 @@ EXPORTS = [
     "pk_lz4_encode_bound",
     "pk_deskew_out_dims",
     "pk_deskew_frame3d",
+    "pk_deskew_rotate_out_dims",
+    "pk_deskew_rotate_frame3d",
     "pk_affine_warp3d",
 ]
 

@@ -90,12 +90,53 @@ def _crop(x, b):
     return np.asfortranarray(x[s])
 
 
-def decon_large_in_memory(im, PSF_f, PSF_b, NUMIT, method="omw", batchSize=(512, 512, 512), **kwargs):
+def edge_erosion_mask(im, EdgeErosion, ctx=None):
+    """The whole-volume edge-erosion mask of RLdecon_large_in_memory.m:150-155:
+    bw = im > 0 with the six faces cleared, kept where bwdist(~bw) >
+    EdgeErosion - 1 (pk_edge_erosion_mask on the device).  Boolean numpy
+    array for a host volume, bool CUDA tensor for a device volume."""
+    import ctypes
+
+    from . import _lib
+    from ._lib import PK_MEM_DEVICE, PK_MEM_HOST, PK_SINGLE, PK_UINT8, PK_UINT16, PkVol
+    shape = tuple(int(n) for n in (im.shape if _is_torch(im) else np.shape(im)))
+    if len(shape) != 3:
+        raise ValueError("a 3-D volume is required")
+    ctx = ctx or _lib.default_context()
+    if _is_torch(im) and im.is_cuda:
+        import torch
+        t = im if im.dtype in (torch.uint16, torch.float32) else im.to(torch.float32)
+        if tuple(t.stride()) != (1, shape[0], shape[0] * shape[1]):
+            t = t.permute(2, 1, 0).contiguous().permute(2, 1, 0)
+        m = torch.empty((shape[2], shape[1], shape[0]), dtype=torch.uint8, device=t.device).permute(2, 1, 0)
+        vin = PkVol(ctypes.c_void_p(t.data_ptr()), PK_UINT16 if t.dtype == torch.uint16 else PK_SINGLE,
+                    PK_MEM_DEVICE, (ctypes.c_int64 * 3)(*shape))
+        vm = PkVol(ctypes.c_void_p(m.data_ptr()), PK_UINT8, PK_MEM_DEVICE, (ctypes.c_int64 * 3)(*shape))
+        torch.cuda.current_stream(t.device).synchronize()
+        ctx.check(_lib.load().pk_edge_erosion_mask(ctx.handle, ctypes.byref(vin), int(EdgeErosion), ctypes.byref(vm)))
+        return m.bool()
+    a = np.asarray(im.numpy() if _is_torch(im) else im)
+    if a.dtype not in (np.uint16, np.float32):
+        a = a.astype(np.float32)
+    a = np.asfortranarray(a)
+    m = np.empty(shape, dtype=np.uint8, order="F")
+    vin = PkVol(ctypes.c_void_p(a.ctypes.data), PK_UINT16 if a.dtype == np.uint16 else PK_SINGLE, PK_MEM_HOST,
+                (ctypes.c_int64 * 3)(*shape))
+    vm = PkVol(ctypes.c_void_p(m.ctypes.data), PK_UINT8, PK_MEM_HOST, (ctypes.c_int64 * 3)(*shape))
+    ctx.check(_lib.load().pk_edge_erosion_mask(ctx.handle, ctypes.byref(vin), int(EdgeErosion), ctypes.byref(vm)))
+    return m.astype(bool)
+
+
+def decon_large_in_memory(im, PSF_f, PSF_b, NUMIT, method="omw", batchSize=(512, 512, 512), EdgeErosion=8,
+                          ErodeMaskfile=None, **kwargs):
     """Deconvolve a volume larger than one chunk, as the in-memory large-volume
     driver does (decon/RLdecon_large_in_memory.m:171-262): batches of
     batchSize (BlockSize = batchSize, sameBatchSize false) with a border of
     round((size(psf) + 10) / 2) on every side (:171-175), each deconvolved and
-    its region written into the output (:189-256).
+    its region written into the output (:189-256).  With EdgeErosion > 0 (the
+    reference's default 8) the assembled volume is multiplied by
+    edge_erosion_mask(im, EdgeErosion) (:141-162, 271-274); an existing
+    ErodeMaskfile (TIFF) replaces that mask (:78-80, 276-280).
 
     Here every batch goes through ONE pk_decon_batch call (one launch per pass
     and block shape over all batches, the PSF centre-cropped for batches
@@ -103,8 +144,13 @@ def decon_large_in_memory(im, PSF_f, PSF_b, NUMIT, method="omw", batchSize=(512,
     kwargs are decon_batch's (Background, save16bit, dampFactor, ...; bbox is
     not accepted: the output is the whole volume).  Returns the volume
     (numpy for a host input, a column-major CUDA tensor for a device input)."""
+    import os
     if "bbox" in {k.lower() for k in kwargs}:
         raise ValueError("bbox: the large-volume driver writes the whole volume")
+    erode_file = ErodeMaskfile if ErodeMaskfile and os.path.exists(str(ErodeMaskfile)) else None
+    if erode_file:
+        EdgeErosion = 0
+    mask = edge_erosion_mask(im, EdgeErosion, kwargs.get("ctx")) if EdgeErosion and EdgeErosion > 0 else None
     shape = tuple(int(n) for n in (im.shape if _is_torch(im) else np.shape(im)))
     if len(shape) != 3:
         raise ValueError("a 3-D volume is required")
@@ -126,4 +172,15 @@ def decon_large_in_memory(im, PSF_f, PSF_b, NUMIT, method="omw", batchSize=(512,
         lo = r[:3] - b[:3]  # 0-based start of the region inside the batch
         hi = lo + (r[3:] - r[:3]) + 1
         out[r[0] - 1:r[3], r[1] - 1:r[4], r[2] - 1:r[5]] = o[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+    if erode_file:
+        from .tiff_io import readtiff
+        m = np.asarray(readtiff(erode_file))
+        if m.shape != shape:
+            raise ValueError(f"ErodeMaskfile size {m.shape} differs from the volume's {shape}")
+        mask = m != 0
+        if _is_torch(out):
+            import torch
+            mask = torch.from_numpy(np.ascontiguousarray(mask.transpose(2, 1, 0))).to(out.device).permute(2, 1, 0)
+    if mask is not None:  # im .* cast(im_bw_erode, class(im))
+        out[~mask] = 0
     return out

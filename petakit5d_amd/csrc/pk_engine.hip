@@ -4,7 +4,7 @@
 // strided passes; every elementwise step of the reference rides inside the
 // first or last pass of a transform):
 //
-//   OMW (decon_lucy_omw_function.m:321-329), per iteration:
+//   OMW (decon_lucy_omw_function.m:105-112), per iteration:
 //     [x PREP or previous x UPDATE]  J -> A = R2C_x(J)
 //     y FWD;  z FWD*OTF_f*INV;  y INV
 //     x RATIO      c = C2R_x(A); r = I ./ max(c, eps); A = R2C_x(r)
@@ -986,7 +986,7 @@ static uint64_t content_hash(const void* p, size_t n, uint64_t seed) {
 }
 
 // OTF of PSF ./ sum(PSF(:)) at the volume size (decon_lucy_function.m:76,
-// decon_lucy_omw_function.m:286-287), half spectrum, cached on the device.
+// decon_lucy_omw_function.m:70-71), half spectrum, cached on the device.
 uint64_t psf_key(const float* psf, const int64_t pd[3]) {
     const int64_t P = pd[0] * pd[1] * pd[2];
     return content_hash(psf, sizeof(float) * (size_t)P, content_hash(pd, sizeof(int64_t) * 3, 0x5eed));
@@ -1090,7 +1090,7 @@ void pk_ctx_impl::reset() {
     rtw.clear();
     ws.I.release(); ws.J.release(); ws.A.release(); ws.J3.release(); ws.Y.release(); ws.J4.release();
     ws.partials.release(); ws.lambda.release(); ws.raw.release(); ws.out.release(); ws.mask.release();
-    ws.mask2.release(); ws.count.release(); ws.tmp.release(); ws.batch.release();
+    ws.mask2.release(); ws.count.release(); ws.tmp.release(); ws.batch.release(); ws.dsr.release(); ws.dbg.release();
     block_cache_clear();
     for (int i = 0; i < 2; ++i) {
         ws.tab_dev[i].release();
@@ -1150,7 +1150,8 @@ const uint8_t* pk_ctx_impl::edge_mask(const uint16_t* raw16, const float* rawf, 
     const int is3d = n3 > 1;
     hipLaunchKernelGGL(k_mask_init, dim3(grid_for(N)), dim3(256), 0, stream, raw16, rawf, N, m1, cnt);
     hipLaunchKernelGGL(k_mask_faces, dim3(grid_for(N)), dim3(256), 0, stream, m1, n1, n2, n3, is3d, cnt, E);
-    hipLaunchKernelGGL(k_mask_erode_disk, dim3(grid_for(N)), dim3(256), 0, stream, m1, m2, n1, n2, n3, E - 1, cnt);
+    hipLaunchKernelGGL(k_mask_erode_disk, dim3(grid_for(N)), dim3(256), 0, stream, m1, m2, n1, n2, n3, disk_se(E - 1),
+                       cnt);
     const uint8_t* mask = m2;
     if (is3d) {
         hipLaunchKernelGGL(k_mask_erode_z, dim3(grid_for(N)), dim3(256), 0, stream, m2, m1, n1, n2, n3, E - 1, cnt);
@@ -1219,7 +1220,8 @@ static XArgs prep_args(int method, float2* A, float* I, float* J, const uint16_t
 // the iteration loop is replayed as a graph for a repeated key (null: eager).
 void pk_ctx_impl::omw_passes(const VolPlan& vp, const XArgs& pa, float* I, float* J, float2* A, const float2* Hf,
                              const float2* Hb, int numit, const EpiArgs& e, bool fuse_epi, int64_t kz,
-                             const std::vector<uint64_t>* loop_key) {
+                             const std::vector<uint64_t>* loop_key,
+                             const std::function<void(int, const float*)>* after) {
     const int64_t step = kz > 0 ? kz : vp.n3;
     const float scale = (float)(1.0 / (double)vp.N);  // ifftn's 1/N
     auto slabs = [&](auto&& body) {
@@ -1259,9 +1261,10 @@ void pk_ctx_impl::omw_passes(const VolPlan& vp, const XArgs& pa, float* I, float
                 if (!(fuse_epi && k == numit)) run_x(vp, u, z0, nz);
                 if (k < numit) y_fwd(z0, nz);
             });
+            if (after) (*after)(k, J);
         }
     };
-    if (loop_key) run_graphed(*loop_key, loop);
+    if (loop_key && !after) run_graphed(*loop_key, loop);
     else loop();
     if (fuse_epi) {  // last update + epilogue
         XArgs u{};
@@ -1334,8 +1337,31 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
     EpiArgs e = epi_args(vp, o, g, I, mask, raw16, rawf, dout);
     // the last update writes the output itself (fused epilogue) when the row
     // pass covers the whole volume in one launch
+    const bool dbg = (o->flags & PK_FLAG_DEBUG) != 0;
+    if (dbg && (o->flags & PK_FLAG_UNFUSED)) fail(PK_ERR_INVALID_ARG, "PK_FLAG_DEBUG runs the fused engine only");
     const int64_t kz = (o->flags & PK_FLAG_UNFUSED) ? 0 : slab_planes(vp);
-    const bool fuse_epi = !(o->flags & (PK_FLAG_UNFUSED | PK_FLAG_EPILOGUE_PASS)) && numit > 0 && kz == 0;
+    const bool fuse_epi = !(o->flags & (PK_FLAG_UNFUSED | PK_FLAG_EPILOGUE_PASS)) && numit > 0 && kz == 0 && !dbg;
+    // debug: MSE of every iteration's estimate against I (double sums), the hook every dbg_step iterations
+    std::vector<double> mse;
+    std::vector<float> snap;
+    const std::function<void(int, const float*)> dbg_after = [&](int k, const float* Jk) {
+        const int nb = 1024;
+        ensure(ws.dbg, sizeof(double) * (size_t)(nb + 1));
+        double* part = ws.dbg.as<double>();
+        hipLaunchKernelGGL(k_sqdiff_partial, dim3(nb), dim3(256), 0, stream, Jk, (const float*)I, N, part);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, stream, (const double*)part, nb, part + nb);
+        PK_HIP(hipGetLastError());
+        double sum = 0;
+        PK_HIP(hipMemcpyAsync(&sum, part + nb, sizeof(double), hipMemcpyDeviceToHost, stream));
+        PK_HIP(hipStreamSynchronize(stream));
+        mse.push_back(sum / (double)N);
+        if (dbg_hook && dbg_step > 0 && k % dbg_step == 0) {
+            snap.resize((size_t)N);
+            PK_HIP(hipMemcpy(snap.data(), Jk, sizeof(float) * (size_t)N, hipMemcpyDeviceToHost));
+            const int64_t dd[3] = {vp.n1, vp.n2, vp.n3};
+            dbg_hook(dbg_user, k, snap.data(), dd);
+        }
+    };
     if (o->flags & PK_FLAG_UNFUSED) {
         const float bgv = raw16 ? (float)matlab_u16(o->background) : (float)o->background;
         decon_unfused(method, vp, raw, raw16 != nullptr, bgv, Hf, Hb, numit, I, J);
@@ -1366,7 +1392,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         const XArgs pa = prep_args(method, A, I, J, raw16, rawf, o->background);
 
         if (method == 1) {
-            omw_passes(vp, pa, I, J, A, Hf, Hb, numit, e, fuse_epi, kz, &loop_key);
+            omw_passes(vp, pa, I, J, A, Hf, Hb, numit, e, fuse_epi, kz, &loop_key, dbg ? &dbg_after : nullptr);
         } else {
             run_x(vp, pa);
             float* J2 = J;
@@ -1375,7 +1401,7 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
             float* J4 = ws.J4.as<float>();
             float* lam = ws.lambda.as<float>();
             double* part = ws.partials.as<double>();
-            run_graphed(loop_key, [&] {
+            auto biggs_loop = [&] {
             J2 = J;
             J3 = ws.J3.as<float>();
             hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(N)), dim3(256), 0, stream, J3, N, 0.f);
@@ -1423,8 +1449,11 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
                     if (!(fuse_epi && k == numit)) run_x(vp, u, z0, nz);
                 });
                 std::swap(J2, J3);
+                if (dbg) dbg_after(k, J2);
             }
-            });
+            };
+            if (dbg) biggs_loop();
+            else run_graphed(loop_key, biggs_loop);
             if (fuse_epi) {  // last update + epilogue
                 XArgs u{};
                 u.mode = XM_BIGGS_FINAL;
@@ -1448,7 +1477,69 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         PK_HIP(hipMemcpyAsync(out->data, dout, osz * on, hipMemcpyDeviceToHost, stream));
     PK_HIP(hipStreamSynchronize(stream));
     prof_flush();
-    if (err_mat) std::memset(err_mat, 0, sizeof(double) * 4 * (size_t)numit);
+    if (err_mat) {
+        std::memset(err_mat, 0, sizeof(double) * 4 * (size_t)numit);
+        if (dbg && (int)mse.size() == numit) debug_err_mat(mse, dbg_step, err_mat);
+    }
+}
+
+// err_mat of decon_lucy_function.m:144-157 / decon_lucy_omw_function.m:114-127
+// (NUMIT x 4, column-major) from the per-iteration MSE, row for row as the
+// reference fills it -- including its use of row istp (not iteration istp *
+// estep) in the criterion, and of the latest istp for later rows.
+void debug_err_mat(const std::vector<double>& mse, int estep, double* e) {
+    const int n = (int)mse.size();
+    auto E = [&](int row, int col) -> double& { return e[(size_t)(row - 1) + (size_t)n * (size_t)(col - 1)]; };
+    int istp = 0;
+    for (int k = 1; k <= n; ++k) {
+        E(k, 1) = k;
+        E(k, 2) = mse[(size_t)k - 1];
+        E(k, 3) = E(k, 2) / E(1, 2);
+        if (k % estep == 0 && k > estep * 2) {
+            istp = k / estep;
+            E(k, 4) = std::min(std::fabs(E(istp, 3) - E(istp - 1, 3)) / 10,
+                               std::fabs(E(istp, 3) + E(istp - 2, 3) - 2 * E(istp - 1, 3)) / 2);
+        } else if (k < 3 * estep) {
+            E(k, 4) = 1;
+        } else {
+            E(k, 4) = E(istp * estep, 4);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ whole-volume edge erosion
+void pk_ctx_impl::edge_erosion_mask(const pk_vol* im, int E, pk_vol* mask) {
+    if (!im || !mask || !im->data || !mask->data) fail(PK_ERR_INVALID_ARG, "null volume");
+    if (im->dtype != PK_UINT16 && im->dtype != PK_SINGLE) fail(PK_ERR_UNSUPPORTED_TYPE, "im must be uint16 or single");
+    if (mask->dtype != PK_UINT8) fail(PK_ERR_INVALID_ARG, "the mask is PK_UINT8");
+    if (E < 1 || E > 256) fail(PK_ERR_INVALID_ARG, "EdgeErosion must lie in [1, 256]");
+    for (int d = 0; d < 3; ++d)
+        if (im->dims[d] < 1 || mask->dims[d] != im->dims[d]) fail(PK_ERR_INVALID_ARG, "mask dims must equal im dims");
+    const int64_t n1 = im->dims[0], n2 = im->dims[1], n3 = im->dims[2], N = n1 * n2 * n3;
+    const size_t esz = im->dtype == PK_UINT16 ? 2 : 4;
+    const void* raw = im->data;
+    if (im->mem == PK_MEM_HOST) {
+        ensure(ws.raw, esz * (size_t)N);
+        PK_HIP(hipMemcpyAsync(ws.raw.p, im->data, esz * (size_t)N, hipMemcpyHostToDevice, stream));
+        raw = ws.raw.p;
+    }
+    const size_t dbytes = (sizeof(uint16_t) * (size_t)N + 255) / 256 * 256;
+    ensure(ws.tmp, 2 * dbytes + (mask->mem == PK_MEM_HOST ? (size_t)N : 0));
+    uint16_t* d1 = ws.tmp.as<uint16_t>();
+    uint16_t* d2 = reinterpret_cast<uint16_t*>(ws.tmp.as<char>() + dbytes);
+    uint8_t* m = mask->mem == PK_MEM_HOST ? reinterpret_cast<uint8_t*>(ws.tmp.as<char>() + 2 * dbytes)
+                                          : static_cast<uint8_t*>(mask->data);
+    const int R = E - 1;
+    const uint16_t* r16 = im->dtype == PK_UINT16 ? static_cast<const uint16_t*>(raw) : nullptr;
+    const float* rf = im->dtype == PK_SINGLE ? static_cast<const float*>(raw) : nullptr;
+    hipLaunchKernelGGL(k_edt_mask_pass1, dim3(grid_for(N)), dim3(256), 0, stream, r16, rf, n1, n2, n3, R, d1);
+    hipLaunchKernelGGL(k_edt_mask_pass, dim3(grid_for(N)), dim3(256), 0, stream, d1, d2, n1, n2, n3, 2, R);
+    hipLaunchKernelGGL(k_edt_mask_pass, dim3(grid_for(N)), dim3(256), 0, stream, d2, d1, n1, n2, n3, 3, R);
+    hipLaunchKernelGGL(k_edt_mask_final, dim3(grid_for(N)), dim3(256), 0, stream, d1, m, N, R);
+    PK_HIP(hipGetLastError());
+    if (mask->mem == PK_MEM_HOST)
+        PK_HIP(hipMemcpyAsync(mask->data, m, (size_t)N, hipMemcpyDeviceToHost, stream));
+    PK_HIP(hipStreamSynchronize(stream));
 }
 
 // ------------------------------------------------------------------ no-allocation OMW

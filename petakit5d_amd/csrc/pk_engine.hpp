@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <array>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -126,6 +127,13 @@ DeconGeom decon_check(int method, const pk_vol* I, const float* psf_f, const int
 EpiArgs epi_args(const struct VolPlan& vp, const pk_opts* o, const DeconGeom& g, const float* I,
                  const uint8_t* mask, const uint16_t* raw16, const float* rawf, void* dout);
 
+// outSize [ny nx' nz'] of deskewRotateFrame3D (deskewRotateFrame3D.m:61-69)
+void dsr_output_size(const int64_t dims[3], double angle, double dz, double xy, bool reverse, bool objective,
+                     double x_step_thresh, int64_t out[3]);
+
+// err_mat of the reference's debug branch from the per-iteration MSE (NUMIT x 4, column-major)
+void debug_err_mat(const std::vector<double>& mse, int estep, double* err_mat);
+
 // Resolves the no-allocation entry point's device scratch array on `device`
 // (the current device); called once per context creation (pk_noalloc.hip).
 void noalloc_setup(int device);
@@ -155,6 +163,8 @@ struct HostBuf {
 struct Workspace {
     DevBuf I, J, A, J3, Y, J4, partials, lambda, raw, out, mask, mask2, count, tmp;
     DevBuf batch;  // batched decon: per-chunk arena
+    DevBuf dsr;    // deskew-rotate: skewed-space interpolated volume / conversions
+    DevBuf dbg;    // debug metrics: block partials of the squared differences and their sum
     // per-step descriptor tables, double-buffered: step s stages its table in
     // tab_host[s&1] and copies it to tab_dev[s&1]; tab_ev[s&1] marks the copy's
     // completion, so the host can build step s+1 while step s runs
@@ -279,13 +289,25 @@ struct pk_ctx_impl {
     // bytes of scratch decon_omw_noalloc needs for a volume of size d
     static size_t noalloc_bytes(const int64_t d[3], int edge_erosion);
     // the OMW pass sequence shared by decon and decon_omw_noalloc
+    // after (optional): called with k and the estimate after every iteration
     void omw_passes(const VolPlan& vp, const XArgs& prep, float* I, float* J, float2* A, const float2* Hf,
                     const float2* Hb, int numit, const EpiArgs& e, bool fuse_epi, int64_t kz,
-                    const std::vector<uint64_t>* loop_key);
+                    const std::vector<uint64_t>* loop_key,
+                    const std::function<void(int, const float*)>* after = nullptr);
+    // debug ('debug', true): per-iteration MSE and the iteration hook
+    int32_t dbg_step = 1;
+    pk_iter_hook dbg_hook = nullptr;
+    void* dbg_user = nullptr;
     // decon_mask_edge_erosion of I ~= 0 into the caller's buffers; returns the eroded mask
     const uint8_t* edge_mask(const uint16_t* raw16, const float* rawf, int64_t n1, int64_t n2, int64_t n3, int E,
                              uint8_t* m1, uint8_t* m2, unsigned long long* cnt);
     void reset();
+    // deskewRotateFrame3D, linear, no resampling (pk_dsr.hip); bbox: 1-based
+    // inclusive output box or null; device volumes, returns when done
+    void deskew_rotate(const pk_vol* in, double angle, double dz, double xy, bool reverse, bool objective,
+                       double x_step_thresh, const int64_t* bbox, pk_vol* out);
+    // bwdist(~bw) > E - 1 of bw = im > 0 with cleared faces (RLdecon_large_in_memory.m:150-155)
+    void edge_erosion_mask(const pk_vol* im, int E, pk_vol* mask);
     void decon_batch(int method, const pk_batch_item* items, int n, const float* psf_f, const int64_t fd[3],
                      const float* psf_b, const int64_t bd[3], int numit, const pk_opts* o);
     // step engine

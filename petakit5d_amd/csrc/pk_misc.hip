@@ -3,6 +3,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "pk_engine.hpp"
 #include "pk_misc.hpp"
 
 namespace pk {
@@ -135,6 +141,40 @@ __global__ void __launch_bounds__(256) k_epilogue(EpiArgs a) {
     }
 }
 
+// ---- debug metric (decon_lucy_function.m:145): sum((double(J) - double(I)).^2)
+// as gridDim.x block partials (grid-stride, then a fixed-order tree), reduced
+// by one block in a fixed order: deterministic.
+__global__ void k_sqdiff_partial(const float* __restrict__ J, const float* __restrict__ I, int64_t n,
+                                 double* __restrict__ partials) {
+    __shared__ double sh[256];
+    double a = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double d = (double)J[i] - (double)I[i];
+        a += d * d;
+    }
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partials, int n, double* __restrict__ out) {
+    __shared__ double sh[256];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) a += partials[i];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
 // ---- edge-erosion mask (decon_mask_edge_erosion.m)
 __global__ void k_mask_init(const uint16_t* __restrict__ raw16, const float* __restrict__ rawf, int64_t n,
                             uint8_t* __restrict__ m, unsigned long long* __restrict__ count) {
@@ -165,10 +205,77 @@ __global__ void k_mask_faces(uint8_t* __restrict__ m, int64_t n1, int64_t n2, in
     }
 }
 
-// In-plane erosion with a disk of radius r (pixels with dy^2+dx^2 <= r^2);
-// out-of-image neighbours count as set (binary erosion padding).
+// Neighbourhood of strel('disk', r) with MATLAB's default N = 4, the element
+// decon_mask_edge_erosion.m:22-27 erodes with.  r < 3: the exact disk
+// dx^2 + dy^2 <= r^2.  r >= 3: the radial decomposition into four periodic
+// lines documented for strel (Adams 1993; Jones & Soille 1996): offsets
+// (1,0), (1,1), (0,1), (-1,1), each repeated floor(k / |v|) times either side,
+// k = 2r / (cot(pi/8) + 1 / sin(pi/8)), then -- the sum falling short of r --
+// a horizontal and a vertical line of length 2(r - e - 1) + 1 when that is
+// >= 3 (e: the sum's row extent).  The element is convex and symmetric, so it
+// is kept as the half-width of each row.
+DiskSE disk_se(int r) {
+    DiskSE se{};
+    if (r < 0 || r > kDiskMaxR) fail(PK_ERR_INVALID_ARG, "EdgeErosion - 1 must lie in [0, " +
+                                                             std::to_string(kDiskMaxR) + "]");
+    if (r < 3) {
+        se.R = r;
+        for (int dy = -r; dy <= r; ++dy) {
+            int h = 0;
+            while ((h + 1) * (h + 1) + dy * dy <= r * r) ++h;
+            se.w[dy + r] = (int16_t)h;
+        }
+        return se;
+    }
+    static const int v[4][2] = {{1, 0}, {1, 1}, {0, 1}, {-1, 1}};
+    const double t = 3.14159265358979323846 / 8.0;
+    const double k = 2.0 * r / (1.0 / std::tan(t) + 1.0 / std::sin(t));
+    // Minkowski sum of the lines, row by row: every row of every partial sum
+    // is one run [lo, hi] (a line's copies of a run overlap or touch), on a
+    // (2r+1)-row table (the sum never reaches past r)
+    const int G = 2 * r + 1;
+    const int kNone = 1 << 30;
+    std::vector<int> lo(G, kNone), hi(G, -kNone), nlo(G), nhi(G);
+    lo[r] = hi[r] = 0;
+    for (int q = 0; q < 4; ++q) {
+        const int rp = (int)std::floor(k / std::sqrt((double)(v[q][0] * v[q][0] + v[q][1] * v[q][1])));
+        std::fill(nlo.begin(), nlo.end(), kNone);
+        std::fill(nhi.begin(), nhi.end(), -kNone);
+        for (int y = 0; y < G; ++y) {
+            if (lo[y] > hi[y]) continue;
+            for (int i = -rp; i <= rp; ++i) {
+                const int yy = y + i * v[q][0];
+                if (yy < 0 || yy >= G) continue;
+                nlo[yy] = std::min(nlo[yy], lo[y] + i * v[q][1]);
+                nhi[yy] = std::max(nhi[yy], hi[y] + i * v[q][1]);
+            }
+        }
+        lo.swap(nlo);
+        hi.swap(nhi);
+    }
+    std::vector<int> w(G, -1);  // the full sum is symmetric in dx: lo = -hi
+    for (int y = 0; y < G; ++y)
+        if (lo[y] <= hi[y]) w[y] = hi[y];
+    int e = 0;
+    for (int y = 0; y < G; ++y)
+        if (w[y] >= 0) e = std::max(e, std::abs(y - r));
+    const int len = 2 * (r - e - 1) + 1;
+    const int h = len >= 3 ? (len - 1) / 2 : 0;
+    se.R = e + h;
+    for (int dy = -se.R; dy <= se.R; ++dy) {
+        int best = -1;
+        for (int sy = dy - h; sy <= dy + h; ++sy)
+            if (sy >= -e && sy <= e) best = std::max(best, w[sy + r]);
+        se.w[dy + se.R] = (int16_t)(best + h);
+    }
+    return se;
+}
+
+// In-plane erosion of every slice by the element se (row dy in [-R, R] spans
+// |dx| <= w[dy + R]); out-of-image neighbours count as set (binary erosion
+// padding, once around the whole decomposed element).
 __global__ void k_mask_erode_disk(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n1,
-                                  int64_t n2, int64_t n3, int r, const unsigned long long* __restrict__ count) {
+                                  int64_t n2, int64_t n3, DiskSE se, const unsigned long long* __restrict__ count) {
     const int64_t n = n1 * n2 * n3;
     const bool full = (*count == (unsigned long long)n);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -179,15 +286,73 @@ __global__ void k_mask_erode_disk(const uint8_t* __restrict__ src, uint8_t* __re
         }
         const int64_t y = i % n1, x = (i / n1) % n2, z = i / (n1 * n2);
         uint8_t v = 1;
-        for (int dx = -r; dx <= r && v; ++dx)
-            for (int dy = -r; dy <= r && v; ++dy) {
-                if (dx * dx + dy * dy > r * r) continue;
-                const int64_t yy = y + dy, xx = x + dx;
-                if (yy < 0 || yy >= n1 || xx < 0 || xx >= n2) continue;
-                v = src[yy + n1 * (xx + n2 * z)];
-            }
+        for (int dy = -se.R; dy <= se.R && v; ++dy) {
+            const int64_t yy = y + dy;
+            if (yy < 0 || yy >= n1) continue;
+            const int hw = se.w[dy + se.R];
+            const int64_t x0 = x - hw < 0 ? 0 : x - hw, x1 = x + hw >= n2 ? n2 - 1 : x + hw;
+            for (int64_t xx = x0; xx <= x1 && v; ++xx) v = src[yy + n1 * (xx + n2 * z)];
+        }
         dst[i] = v;
     }
+}
+
+// ---- whole-volume edge erosion of the large-volume driver
+// (RLdecon_large_in_memory.m:150-155): bw = im > 0 with the six faces cleared,
+// kept where bwdist(~bw) > E - 1, i.e. where no false voxel lies within
+// Euclidean distance E - 1.  The squared distance to the nearest false voxel
+// separates over the axes (d^2 = min over false voxels of dy^2 + dx^2 + dz^2),
+// and only distances up to R = E - 1 matter, so three passes of a bounded
+// search give it exactly: pass 1 along dim 1 from the mask, passes 2 and 3
+// along dims 2 and 3 over the previous pass's squared distances, each taking
+// min(d_prev(q + k) + k^2) over |k| <= R; values above R^2 saturate at 65535.
+__global__ void k_edt_mask_pass1(const uint16_t* __restrict__ raw16, const float* __restrict__ rawf, int64_t n1,
+                                 int64_t n2, int64_t n3, int R, uint16_t* __restrict__ d) {
+    const int64_t n = n1 * n2 * n3;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int R2 = R * R;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t y = i % n1, x = (i / n1) % n2, z = i / (n1 * n2);
+        const int64_t row = i - y;
+        auto bw = [&](int64_t yy) -> bool {  // im > 0, faces cleared
+            if (yy == 0 || yy == n1 - 1 || x == 0 || x == n2 - 1 || z == 0 || z == n3 - 1) return false;
+            return raw16 ? raw16[row + yy] > 0 : rawf[row + yy] > 0.f;
+        };
+        int best = 65535;
+        for (int k = 0; k <= R && best == 65535; ++k) {  // nearest false voxel along the row
+            if ((y - k >= 0 && !bw(y - k)) || (y + k < n1 && !bw(y + k))) best = k * k;
+        }
+        d[i] = (uint16_t)(best <= R2 ? best : 65535);
+    }
+}
+
+__global__ void k_edt_mask_pass(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int64_t n1, int64_t n2,
+                                int64_t n3, int axis, int R) {
+    const int64_t n = n1 * n2 * n3;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int R2 = R * R;
+    const int64_t step = axis == 2 ? n1 : n1 * n2;
+    const int64_t len = axis == 2 ? n2 : n3;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t q = axis == 2 ? (i / n1) % n2 : i / (n1 * n2);
+        int best = 65535;
+        for (int k = -R; k <= R; ++k) {
+            const int64_t qq = q + k;
+            if (qq < 0 || qq >= len) continue;
+            const int v = src[i + k * step];
+            if (v == 65535) continue;
+            const int c = v + k * k;
+            if (c < best) best = c;
+        }
+        dst[i] = (uint16_t)(best <= R2 ? best : 65535);
+    }
+}
+
+__global__ void k_edt_mask_final(const uint16_t* __restrict__ d, uint8_t* __restrict__ mask, int64_t n, int R) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int R2 = R * R;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        mask[i] = (uint8_t)(d[i] == 65535 || (int)d[i] > R2);
 }
 
 // Erosion along dim 3 with a (2h+1)-long line (decon_mask_edge_erosion.m:25-27).

@@ -114,6 +114,10 @@ __global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v);
 __global__ void k_scatter_psf(const float* __restrict__ q, int p1, int p2, int p3, float* __restrict__ dst,
                               int64_t n1, int64_t n2, int64_t n3);
 __global__ void k_lambda(const double* __restrict__ partials, int nblk, float eps, float* __restrict__ lambda);
+// debug metric: block partials of sum((J - I)^2) in double (blockDim 256), then their fixed-order sum
+__global__ void k_sqdiff_partial(const float* __restrict__ J, const float* __restrict__ I, int64_t n,
+                                 double* __restrict__ partials);
+__global__ void k_sum_partials(const double* __restrict__ partials, int n, double* __restrict__ out);
 constexpr int kEpiChunk = 1024;  // output voxels per epilogue work item (4 per thread)
 __global__ void k_epilogue(EpiArgs a);
 // grid for k_epilogue: one block per work item, capped (the kernel strides)
@@ -128,8 +132,21 @@ __global__ void k_mask_init(const uint16_t* __restrict__ raw16, const float* __r
                             uint8_t* __restrict__ m, unsigned long long* __restrict__ count);
 __global__ void k_mask_faces(uint8_t* __restrict__ m, int64_t n1, int64_t n2, int64_t n3, int is3d,
                              const unsigned long long* __restrict__ count, int E);
+// strel('disk', r) as the half-width of each row dy in [-R, R]: |dx| <= w[dy + R]
+constexpr int kDiskMaxR = 255;
+struct DiskSE {
+    int R;
+    int16_t w[2 * kDiskMaxR + 1];
+};
+DiskSE disk_se(int r);  // host: the element decon_mask_edge_erosion.m:22-27 uses (default N = 4)
 __global__ void k_mask_erode_disk(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n1,
-                                  int64_t n2, int64_t n3, int r, const unsigned long long* __restrict__ count);
+                                  int64_t n2, int64_t n3, DiskSE se, const unsigned long long* __restrict__ count);
+// bwdist(~bw) > R of bw = im > 0 with cleared faces (RLdecon_large_in_memory.m:150-155), R <= 255
+__global__ void k_edt_mask_pass1(const uint16_t* __restrict__ raw16, const float* __restrict__ rawf, int64_t n1,
+                                 int64_t n2, int64_t n3, int R, uint16_t* __restrict__ d);
+__global__ void k_edt_mask_pass(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int64_t n1, int64_t n2,
+                                int64_t n3, int axis, int R);
+__global__ void k_edt_mask_final(const uint16_t* __restrict__ d, uint8_t* __restrict__ mask, int64_t n, int R);
 __global__ void k_mask_erode_z(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n1, int64_t n2,
                                int64_t n3, int h, const unsigned long long* __restrict__ count);
 

@@ -184,8 +184,6 @@ def _run(method: int, I, psfs, NUMIT: int, kwargs: dict):
     o = _parse(kwargs)
     if not o["useGPU"]:
         raise ValueError("useGPU=false: this engine implements the GPU branch only")
-    if o["debug"]:
-        raise NotImplementedError("debug=true (per-iteration TIFF dumps) is not supported by the GPU engine")
     ctx = o["ctx"] or _lib.default_context()
     vol, kind = _input_vol(I)
     orig = vol.dims
@@ -206,6 +204,11 @@ def _run(method: int, I, psfs, NUMIT: int, kwargs: dict):
     err = np.zeros((max(int(NUMIT), 0), 4), dtype=np.float64, order="F")
     errp = ctypes.c_void_p(err.ctypes.data) if err.size else None
     arrs = [_psf_array(p) for p in psfs]
+    hook = None
+    if o["debug"]:
+        opts.flags |= PK_FLAG_DEBUG
+        hook = _debug_hook(str(o["debug_folder"]), orig, int(NUMIT))
+        ctx.check(L.pk_set_iter_hook(ctx.handle, int(o["saveStep"]), hook, None))
     if method == 0:
         (pf, fd), = arrs
         st = L.pk_decon_lucy(ctx.handle, ctypes.byref(vol.c), ctypes.c_void_p(pf.ctypes.data), i64x3(fd),
@@ -215,11 +218,55 @@ def _run(method: int, I, psfs, NUMIT: int, kwargs: dict):
         st = L.pk_decon_lucy_omw(ctx.handle, ctypes.byref(vol.c), ctypes.c_void_p(pf.ctypes.data), i64x3(fd),
                                  ctypes.c_void_p(pb.ctypes.data), i64x3(bd), int(NUMIT), ctypes.byref(opts),
                                  ctypes.byref(out.c), errp)
+    if hook is not None:
+        L.pk_set_iter_hook(ctx.handle, 1, None, None)
     ctx.check(st)
+    if o["debug"]:
+        err = _debug_trim(err, method, int(NUMIT), int(o["saveStep"]))
     J = out.obj
     if len(np.shape(I)) < 3 and kind == "numpy":
         J = J.reshape(J.shape[: len(np.shape(I))], order="F")
     return J, err
+
+
+PK_FLAG_DEBUG = 16
+_HOOK_T = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                           ctypes.POINTER(ctypes.c_int64))
+
+
+def _debug_hook(folder, orig, numit):
+    """The iteration hook of debug=true: writes uint16(J) of every saveStep-th
+    iteration as folder/Iter_%04d.tif (decon_lucy_function.m:159-167), cropped
+    to the caller's size when the engine padded the volume."""
+    import os
+
+    from .tiff_io import writetiff
+    if numit >= 1:
+        os.makedirs(folder, exist_ok=True)  # mkdir(debug_folder) at k == 1
+
+    def cb(_user, k, data, dims):
+        d = (int(dims[0]), int(dims[1]), int(dims[2]))
+        a = np.ctypeslib.as_array(data, shape=(d[0] * d[1] * d[2],)).reshape(d, order="F")
+        a = a[:orig[0], :orig[1], :orig[2]].astype(np.float64)
+        r = np.sign(a) * np.floor(np.abs(a) + 0.5)  # uint16(): round half away from zero, saturate
+        writetiff(np.asfortranarray(np.clip(np.nan_to_num(r), 0, 65535).astype(np.uint16)),
+                  os.path.join(folder, f"Iter_{int(k):04d}.tif"))
+    return _HOOK_T(cb)
+
+
+def _debug_trim(err, method, numit, estep):
+    """The reference's last debug step: decon_lucy_function keeps rows
+    1..istp, istp = k / estep of the last k > 2 estep that is a multiple of
+    estep (MATLAB raises an undefined-variable error when there is none,
+    decon_lucy_function.m:170-172); decon_lucy_omw_function keeps rows
+    1..floor(NUMIT / estep) (decon_lucy_omw_function.m:141-143)."""
+    if method == 1:
+        return np.asfortranarray(err[:numit // estep])
+    ks = [k for k in range(1, numit + 1) if k % estep == 0 and k > 2 * estep]
+    if not ks:
+        raise RuntimeError("Unrecognized function or variable 'istp' (decon_lucy_function.m:171: debug needs "
+                           "NUMIT >= 3 * saveStep)")
+    return np.asfortranarray(err[:ks[-1] // estep])
 
 
 def decon_lucy_function(I, PSF, NUMIT, **kwargs: Any):

@@ -1,0 +1,91 @@
+"""deskewRotateFrame3D (microscopeDataProcessing/deskew_rotate/deskewRotateFrame3D.m:1-260):
+deskew and rotation of a stage-scan volume in one resampling, the reference's
+default after decon (RLdecon.m:22 DSRCombined = true, RLdecon.m:489-495).
+Runs on the device through libpkdecon.so's pk_deskew_rotate_frame3d:
+skewed-space interpolation to a finer slice step when the slice-to-slice
+shift exceeds xStepThresh, then the row-preserving bilinear warp of the
+reference's fast path (volume_deskew_rotate_warp_mex.cpp); objective scans
+take imwarp's trilinear form.  Linear interpolation, no resampling."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import PK_MEM_DEVICE, PK_SINGLE, PK_UINT16, PkVol, i64x3
+
+__all__ = ["deskewRotateFrame3D", "deskew_rotate_out_size"]
+
+
+def deskew_rotate_out_size(shape, angle, dz, xyPixelSize, reverse=False, objectiveScan=False, xStepThresh=2.0):
+    """outSize [ny nx' nz'] of deskewRotateFrame3D (deskewRotateFrame3D.m:61-69)."""
+    d = tuple(int(n) for n in shape) + (1,) * (3 - len(shape))
+    out = (ctypes.c_int64 * 3)()
+    rc = _lib.load().pk_deskew_rotate_out_dims(i64x3(d), ctypes.c_double(angle), ctypes.c_double(dz),
+                                               ctypes.c_double(xyPixelSize), int(bool(reverse)),
+                                               int(bool(objectiveScan)), ctypes.c_double(xStepThresh), out)
+    if rc != 0:
+        raise _lib.PkDeconError(rc, "deskewRotateFrame3D: invalid size or geometry")
+    return tuple(int(v) for v in out)
+
+
+def deskewRotateFrame3D(vol, angle, dz, xyPixelSize, reverse=False, Crop=True, bbox=None, objectiveScan=False,
+                        xStepThresh=2.0, resampleFactor=None, gpuProcess=False, save16bit=False,
+                        interpMethod="linear", ctx=None):
+    """volout = deskewRotateFrame3D(vol, angle, dz, xyPixelSize, 'reverse', reverse,
+    'Crop', Crop, 'bbox', bbox, 'objectiveScan', objectiveScan, 'xStepThresh',
+    xStepThresh, 'resampleFactor', resampleFactor, 'save16bit', save16bit,
+    'interpMethod', interpMethod).
+
+    vol: uint16 or single, numpy (column-major result) or a CUDA tensor (CUDA
+    result).  The output is single, or uint16 for a uint16 input with
+    save16bit (the reference's mex does the same).  'Crop' is accepted and,
+    as in the reference, has no effect; gpuProcess only moves the reference's
+    data to its GPU and changes nothing here.  resampleFactor other than
+    [] / ones and 'cubic' take the reference's imwarp form, which this engine
+    does not serve (NotImplementedError)."""
+    import torch
+    if str(interpMethod).lower() != "linear":
+        raise NotImplementedError("deskewRotateFrame3D: only 'linear' interpolation is served")
+    if resampleFactor is not None and np.size(resampleFactor) and np.any(np.asarray(resampleFactor) != 1):
+        raise NotImplementedError("deskewRotateFrame3D: resampling is not served")
+    on_dev = isinstance(vol, torch.Tensor) and vol.is_cuda
+    if on_dev:
+        t = vol
+    else:
+        a = np.asarray(vol.detach().cpu().numpy() if isinstance(vol, torch.Tensor) else vol)
+        if a.dtype not in (np.uint16, np.float32):
+            a = a.astype(np.float32)
+        a = np.asfortranarray(a.reshape(a.shape + (1,) * (3 - a.ndim)))
+        t = torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).cuda().permute(2, 1, 0)
+    if t.dtype not in (torch.uint16, torch.float32):
+        t = t.to(torch.float32)
+    d = tuple(int(n) for n in t.shape)
+    if t.stride() != (1, d[0], d[0] * d[1]):
+        t = t.permute(2, 1, 0).contiguous().permute(2, 1, 0)
+    out_size = deskew_rotate_out_size(d, angle, dz, xyPixelSize, reverse, objectiveScan, xStepThresh)
+    box = None
+    od = out_size
+    if bbox is not None and np.size(bbox):
+        b = [int(v) for v in np.asarray(bbox).reshape(-1)]
+        if len(b) != 6:
+            raise ValueError("bbox must be [s1 s2 s3 e1 e2 e3]")
+        box = (ctypes.c_int64 * 6)(*b)
+        od = (b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1)
+    u16 = t.dtype == torch.uint16 and bool(save16bit)
+    out = torch.empty((od[2], od[1], od[0]), dtype=torch.uint16 if u16 else torch.float32,
+                      device=t.device).permute(2, 1, 0)
+    vin = PkVol(ctypes.c_void_p(t.data_ptr()), PK_UINT16 if t.dtype == torch.uint16 else PK_SINGLE, PK_MEM_DEVICE,
+                (ctypes.c_int64 * 3)(*d))
+    vout = PkVol(ctypes.c_void_p(out.data_ptr()), PK_UINT16 if u16 else PK_SINGLE, PK_MEM_DEVICE,
+                 (ctypes.c_int64 * 3)(*od))
+    ctx = ctx or _lib.default_context()
+    torch.cuda.current_stream(t.device).synchronize()
+    ctx.check(_lib.load().pk_deskew_rotate_frame3d(ctx.handle, ctypes.byref(vin), ctypes.c_double(angle),
+                                                   ctypes.c_double(dz), ctypes.c_double(xyPixelSize),
+                                                   int(bool(reverse)), int(bool(objectiveScan)),
+                                                   ctypes.c_double(xStepThresh), box, ctypes.byref(vout)))
+    if on_dev:
+        return out
+    return np.asfortranarray(out.permute(2, 1, 0).cpu().numpy().transpose(2, 1, 0))

@@ -120,7 +120,7 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
                    dampFactor=1.0, scaleFactor=None, deconOffset=0.0, deconBbox=None, EdgeErosion=0,
                    psfgenPath=None, psfFsn=None, psfGen=False, dzPSF=None, dz=None, Deskew=False, SkewAngle=32.45,
                    xyPixelSize=0.108, Reverse=False, Crop=False, objectiveScan=False, Rotate=False,
-                   DSRCombined=True, ctx=None):
+                   DSRCombined=True, xStepThresh=2.0, Resample=None, ctx=None):
     """deconvolved = RLdecon(..., 'rawdata', rawdata, ...) for one volume
     (rawdata an array, or a .tif / .zarr path; psf an array or a .tif path).
 
@@ -144,10 +144,13 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     SkewAngle, zAniso, Reverse, 'Crop', true) with zAniso = dz_ratio dz /
     xyPixelSize (:107, 483-486), or of the deconvolved volume itself for
     objective scans (:494-497); uint16 when save16bit (:464-468, 498-501).
-    objectiveScan turns Deskew off (:147-149); Rotate of a stage scan needs
-    DSRCombined=False (the combined deskewRotateFrame3D, :488-493, is not this
-    engine's).  Other methods ('original' deconvlucy, 'cudagen') are not this
-    engine's either."""
+    Deskew and Rotate of a stage scan with DSRCombined (the default) run
+    deskewRotateFrame3D(deconvolved, SkewAngle, dz, xyPixelSize, 'reverse',
+    Reverse, 'Crop', true, 'resample', Resample, 'xStepThresh', xStepThresh,
+    'save16bit', save16bit) instead and return ds = None (:488-493).
+    objectiveScan turns Deskew off (:147-149); Rotate of a stage scan without
+    Deskew produces nothing in the reference and is an error here.  Other
+    methods ('original' deconvlucy, 'cudagen') are not this engine's."""
     if RLMethod not in ("simplified", "omw"):
         raise ValueError(f"RLMethod '{RLMethod}': this engine serves 'simplified' and 'omw'")
     SkewAngle = abs(float(SkewAngle))  # RLdecon.m:101
@@ -156,9 +159,9 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     if Deskew or Rotate:
         if dz is None:
             raise ValueError("Deskew / Rotate need dz")
-        if Rotate and not objectiveScan and (DSRCombined or not Deskew):
-            raise NotImplementedError("Rotate of a stage scan runs deskewRotateFrame3D (DSRCombined) in the "
-                                      "reference; pass Deskew=True, DSRCombined=False for deskew then rotate")
+        if Rotate and not objectiveScan and not Deskew:
+            raise ValueError("Rotate of a stage scan needs Deskew (RLdecon.m:437-501 produce no rotated output "
+                             "without it)")
         J = RLdecon_volume(rawdata, psf, nIter, RLMethod, Background, wienerAlpha, OTFCumThresh, hannWinBounds,
                            skewed, flipZstack, save16bit, dampFactor, scaleFactor, deconOffset, deconBbox,
                            EdgeErosion, psfgenPath, psfFsn, psfGen, dzPSF, dz, SkewAngle=SkewAngle,
@@ -166,6 +169,12 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
         from .deskew import deskewFrame3D
         from .rotate import rotateFrame3D
         zAniso = psfgen_dz_ratio(SkewAngle, objectiveScan) * dz / xyPixelSize
+        if Deskew and Rotate and DSRCombined:  # RLdecon.m:488-493
+            from .deskew_rotate import deskewRotateFrame3D
+            dsr = deskewRotateFrame3D(J, SkewAngle, dz, xyPixelSize, reverse=Reverse, Crop=True,
+                                      objectiveScan=objectiveScan, resampleFactor=Resample, xStepThresh=xStepThresh,
+                                      save16bit=save16bit, ctx=ctx)
+            return J, None, (_to_uint16(dsr) if save16bit else dsr)
         ds = deskewFrame3D(J, SkewAngle, dz, xyPixelSize, Reverse, Crop=Crop, ctx=ctx) if Deskew else None
         dsr = None
         if Rotate:

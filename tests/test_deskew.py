@@ -68,3 +68,103 @@ def test_geometry_constants():
     dx = math.cos(32.45 * math.pi / 180) * 0.3 / 0.108
     nx_out, xstep, xshift = D.geometry((4, 10, 11), 32.45, 0.3, 0.108, reverse=True)
     assert (xstep, xshift) == (-dx, dx + math.ceil(10 * dx)) and nx_out == math.ceil(10 * dx) + 10
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# deskewRotateFrame3D (deskewRotateFrame3D.m:50-253): the oracle's geometry and resampling against closed forms.
+@pytest.mark.parametrize("angle,dz,xy", [(32.45, 0.3, 0.108), (32.45, 0.2, 0.108), (31.5, 0.5, 0.1)])
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("shape", [(40, 64, 30), (16, 100, 60)])
+def test_dsr_out_size_matches_oracle(angle, dz, xy, reverse, shape):
+    from petakit5d_amd.deskew_rotate import deskew_rotate_out_size
+    _, out, _, _, _ = D.dsr_geometry(shape, angle, dz, xy, reverse)
+    assert deskew_rotate_out_size(shape, angle, dz, xy, reverse) == out
+
+
+def test_dsr_geometry_by_hand():
+    """dx = 0.3 cos(32.45 deg) / 0.108 = 2.344 > 2: |dx| / 2 < 1.5, so dz' =
+    dz / ceil(dz / (0.108 * 2 / cos)) = 0.15 (step 0.5, 2 * 255 + 1 slices);
+    outSize = round([ny, 511 cos + 255 zAniso / sin, 511 sin - 4]) with
+    zAniso = sin * 0.3 / 0.108."""
+    t = math.radians(32.45)
+    tm, out, xs, step, _ = D.dsr_geometry((512, 512, 256), 32.45, 0.3, 0.108)
+    za = math.sin(t) * 0.3 / 0.108
+    assert out == (512, round(511 * math.cos(t) + 255 * za / math.sin(t)), round(511 * math.sin(t) - 4))
+    assert abs(xs - math.cos(t) * 0.3 / 0.108) < 1e-12 and step == 0.5
+    assert D.row_structured(tm)
+    _, _, _, step2, _ = D.dsr_geometry((512, 512, 256), 32.45, 0.2, 0.108)
+    assert step2 == 1.0  # dx = 1.56 <= 2: no interpolation
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("xstep,step", [(2.344, 0.5), (3.1, 1 / 3), (2.7, 0.25)])
+def test_skewed_interp_follows_the_skew(reverse, xstep, step):
+    """Slices of a stage scan are shifted copies: slice z (1-based) holds
+    G(x + s (z - 1) xstep), s = +1 (-1 reversed), the deskew convention of
+    deskewFrame3D.m (x' = x + xstep (z - 1)).  With G linear, every
+    interpolated slice at zout must equal G(x + s (zout - 1) xstep) wherever
+    its samples stay inside the slices (linear interpolation is exact there)."""
+    ny, nx, nz = 3, 60, 7
+    s = -1 if reverse else 1
+    G = lambda u: 0.75 * u + 2.0  # noqa: E731
+    x = np.arange(nx, dtype=np.float64)
+    vol = np.stack([np.broadcast_to(G(x + s * z * xstep), (ny, nx)) for z in range(nz)], axis=2).astype(np.float32)
+    out = D.skewed_interp(vol, xstep, step, reverse)
+    assert out.shape[2] == round((nz - 1) / step) + 1
+    m = int(np.ceil(xstep)) + 1
+    for zi in range(out.shape[2]):
+        zout = 1 + zi * step
+        want = G(x + s * (zout - 1) * xstep)
+        np.testing.assert_allclose(out[:, m:nx - m, zi], np.broadcast_to(want[m:nx - m], (ny, nx - 2 * m)),
+                                   rtol=2e-6, atol=2e-5)
+
+
+def test_dsr_objective_scan_identity_and_stretch():
+    """Objective scan at angle 0: no shear, no rotation.  dz = xyPixelSize is the
+    identity; dz = 2 xyPixelSize doubles z, output slice Z (1-based) sampling
+    input slice (Z - (2 nz + 1) / 2) / 2 + (nz + 1) / 2."""
+    g = np.random.default_rng(2)
+    v = g.random((6, 7, 5)).astype(np.float32)
+    np.testing.assert_array_equal(D.deskew_rotate(v, 0.0, 0.1, 0.1, objectiveScan=True), v)
+    r = D.deskew_rotate(v, 0.0, 0.2, 0.1, objectiveScan=True)
+    assert r.shape == (6, 7, 10)
+    for Z in range(1, 11):
+        zs = (Z - 5.5) / 2 + 3  # 1-based source slice
+        z0 = math.floor(zs)
+        f = zs - z0
+        lo = v[:, :, z0 - 1] if 1 <= z0 <= 5 else 0 * v[:, :, 0]
+        hi = v[:, :, z0] if 1 <= z0 + 1 <= 5 else 0 * v[:, :, 0]
+        np.testing.assert_allclose(r[:, :, Z - 1], (1 - f) * lo + f * hi, rtol=1e-6, atol=1e-6)
+
+
+def test_dsr_equals_deskew_then_rotate_on_smooth_data():
+    """deskewRotateFrame3D is deskewFrame3D followed by rotateFrame3D in one
+    resampling (deskewRotateFrame3D.m:1-6): on a smooth volume the combined
+    result matches the two-step one (independently pinned oracles, same output
+    box) up to the extra interpolation, and a wrong sign, centre or scale
+    would not.  Forward (not reversed) scans: the reference's default fast path
+    builds its 0-based matrix from the 1-based shear xshift = -dx
+    (deskewRotateFrame3D.m:131-133, 212-218 corrects only the reversed case),
+    so its output samples the deskewed volume dx columns further along x than
+    the imwarp form of :210 -- the oracle keeps that, and the two-step result
+    is taken of the deskewed volume moved by dx."""
+    from scipy import ndimage as ndi
+    from petakit5d_amd.rotate import rotate_transform
+    ny, nx, nz = 8, 96, 48
+    g = np.random.default_rng(3)
+    obj = np.zeros((ny, nx, nz))
+    obj[4, g.integers(20, 76, 12), g.integers(10, 38, 12)] = 1000.0
+    vol = ndi.gaussian_filter(obj, (1.0, 3.0, 3.0)).astype(np.float32)
+    angle, dz, xy = 32.45, 0.2, 0.108  # dx = 1.56: no skewed-space interpolation
+    dx = math.cos(math.radians(angle)) * dz / xy
+    za = math.sin(math.radians(angle)) * dz / xy
+    for reverse in (False, True):
+        dsr = D.deskew_rotate(vol, angle, dz, xy, reverse=reverse)
+        ds = D.deskew(vol, angle, dz, xy, reverse=reverse).astype(np.float32)
+        T, od = rotate_transform(ds.shape, angle, za, reverse, Crop=False, ObjectiveScan=True, outSize=dsr.shape)
+        shifted = ds if reverse else ndi.shift(ds, (0, -dx, 0), order=1, mode="constant")
+        two = D.general_warp(shifted, T, od)
+        err = np.linalg.norm(dsr - two) / np.linalg.norm(two)
+        assert err < 0.05, (reverse, err)
+        if not reverse:  # the unshifted two-step result is far off: the offset is real
+            assert np.linalg.norm(dsr - D.general_warp(ds, T, od)) / np.linalg.norm(two) > 0.2

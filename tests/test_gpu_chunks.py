@@ -25,7 +25,8 @@ def test_large_in_memory_equals_per_batch_calls(oracle, method, numit):
     im = _vol(shape, 5)
     psf = synth.gaussian_psf((9, 9, 11), 1.2, 3.0, 10.0)
     psf_b = np.ascontiguousarray(psf[::-1, ::-1, ::-1])
-    J = pk.decon_large_in_memory(im, psf, psf_b, numit, method=method, batchSize=bs, Background=100, save16bit=False)
+    J = pk.decon_large_in_memory(im, psf, psf_b, numit, method=method, batchSize=bs, EdgeErosion=0, Background=100,
+                                 save16bit=False)
     assert J.shape == shape and J.dtype == np.float32
     border = matlab_round((np.array(psf.shape) + 10) / 2)
     batch, region, local = XR_zarrChunkCoordinatesExtraction(shape, batchSize=bs, BlockSize=bs, sameBatchSize=False,
@@ -86,10 +87,44 @@ def test_decon_zarr_equals_in_memory(tmp_path, compressor):
     psf_b = np.ascontiguousarray(psf[::-1, ::-1, ::-1])
     src = str(tmp_path / "in.zarr")
     pk.write_zarr(src, im, chunks=(32, 32, 16), compressor=compressor)
-    R = pk.decon_large_in_memory(im, psf, psf_b, 2, batchSize=(64, 64, 32), Background=100)
+    R = pk.decon_large_in_memory(im, psf, psf_b, 2, batchSize=(64, 64, 32), EdgeErosion=0, Background=100)
     for g in (16, 1):
         dst = str(tmp_path / f"out{g}.zarr")
         # batchSize (40, 40, 20) rounds up to whole 32 x 32 x 16 chunks: (64, 64, 32)
         assert pk.decon_zarr(src, dst, psf, psf_b, 2, batchSize=(40, 40, 20), groupSize=g, Background=100) is None
         np.testing.assert_array_equal(pk.read_zarr(dst), R)
         assert json.load(open(f"{dst}/.zarray"))["compressor"]["id"] == compressor
+
+
+@pytest.mark.parametrize("E", [8, 3])
+def test_large_in_memory_edge_erosion(oracle, E, tmp_path):
+    """The driver's default EdgeErosion = 8 (RLdecon_large_in_memory.m:25,
+    141-162, 271-274): the assembled result is zeroed where bwdist(~bw) <=
+    E - 1, bw = im > 0 with cleared faces; the mask is checked against
+    scipy's exact Euclidean distance transform, the volume against the same
+    run without erosion.  An ErodeMaskfile replaces the computed mask."""
+    import torch
+    from scipy import ndimage as ndi
+    import petakit5d_amd as pk
+    from petakit5d_amd.chunks import edge_erosion_mask
+    im = _vol((96, 80, 40), 12)
+    im[:30, :25, :] = 0   # an irregular I > 0 region
+    im[60:64, 40:44, 10:30] = 0
+    psf = synth.gaussian_psf((7, 7, 9), 1.0, 2.5)
+    psf_b = np.ascontiguousarray(psf[::-1, ::-1, ::-1])
+    bw = im > 0
+    bw[[0, -1], :, :] = False
+    bw[:, [0, -1], :] = False
+    bw[:, :, [0, -1]] = False
+    want = ndi.distance_transform_edt(bw) > E - 1
+    m = edge_erosion_mask(im, E)
+    np.testing.assert_array_equal(m, want)
+    dev = torch.from_numpy(np.ascontiguousarray(im.transpose(2, 1, 0))).cuda().permute(2, 1, 0)
+    np.testing.assert_array_equal(edge_erosion_mask(dev, E).cpu().numpy(), want)
+    J0 = pk.decon_large_in_memory(im, psf, psf_b, 2, batchSize=(48, 48, 24), EdgeErosion=0, Background=100)
+    JE = pk.decon_large_in_memory(im, psf, psf_b, 2, batchSize=(48, 48, 24), EdgeErosion=E, Background=100)
+    np.testing.assert_array_equal(JE, np.where(want, J0, 0))
+    fn = str(tmp_path / "mask.tif")
+    pk.writetiff(want[:, ::-1, :].astype(np.uint8).copy(order="F"), fn)
+    JM = pk.decon_large_in_memory(im, psf, psf_b, 2, batchSize=(48, 48, 24), ErodeMaskfile=fn, Background=100)
+    np.testing.assert_array_equal(JM, np.where(want[:, ::-1, :], J0, 0))

@@ -108,6 +108,8 @@ def test_config1_biggs15(oracle):
     dict(save16bit=False, scaleFactor=2.5),
     dict(save16bit=True, scaleFactor=3.0, deconOffset=7.0),
     dict(save16bit=False, EdgeErosion=2),
+    dict(save16bit=False, EdgeErosion=4),   # strel('disk', 3): the periodic-line decomposition (a 5x5 square)
+    dict(save16bit=True, EdgeErosion=8),    # strel('disk', 7), XR_RLdeconFrame3D.m:41's default
 ], ids=lambda k: "-".join(f"{a}={b}" for a, b in k.items()))
 def test_epilogue_options_vs_oracle(oracle, kw):
     import petakit5d_amd as pk
@@ -346,6 +348,7 @@ EPILOGUE_PASS = 8  # PK_FLAG_EPILOGUE_PASS
     dict(save16bit=False, dampFactor=1.5),
     dict(save16bit=True, scaleFactor=3.0, deconOffset=7.0),
     dict(save16bit=False, EdgeErosion=2, deconOffset=1.0),
+    dict(save16bit=True, EdgeErosion=8),
 ], ids=lambda k: "-".join(f"{a}={b}" for a, b in k.items()).replace(" ", ""))
 def test_fused_epilogue_matches_epilogue_pass(shape, method, numit, kw):
     """The last update writes the output through the epilogue itself; the
@@ -364,3 +367,59 @@ def test_fused_epilogue_matches_epilogue_pass(shape, method, numit, kw):
         B, _, _ = pk.decon_lucy_function(I, psf, numit, Background=100, flags=EPILOGUE_PASS, **kw)
     assert A.shape == B.shape and A.dtype == B.dtype
     np.testing.assert_array_equal(A, B)
+
+
+@pytest.mark.parametrize("method,numit,step", [("rl", 6, 1), ("omw", 7, 2), ("rl", 7, 2)])
+def test_debug_err_mat_and_iteration_files(oracle, tmp_path, method, numit, step):
+    """debug=true (decon_lucy_function.m:144-172, decon_lucy_omw_function.m:
+    114-143): err_mat(k, 2) is the mean squared difference between the k-th
+    estimate and the background-subtracted input -- checked against the
+    oracle's estimates after k iterations --, column 3 its ratio to row 1,
+    column 4 the reference's stopping criterion (worked by hand below for
+    saveStep 2), trimmed as the reference trims; every saveStep-th estimate is
+    written as Iter_%04d.tif (uint16)."""
+    import os
+    import petakit5d_amd as pk
+    I, psf = _case((48, 40, 24), (9, 9, 11), seed=23)
+    pb = _flip(psf)
+    folder = str(tmp_path / "dbg")
+    kw = dict(Background=100, save16bit=False, debug=True, debug_folder=folder, saveStep=step)
+    if method == "rl":
+        J, err, _ = pk.decon_lucy_function(I, psf, numit, **kw)
+    else:
+        J, err = pk.decon_lucy_omw_function(I, psf, pb, numit, **kw)
+    Ibg = np.maximum(I.astype(np.float32) - np.float32(100), 0).astype(np.float64)
+    mse = []
+    for k in range(1, numit + 1):
+        if method == "rl":
+            Rk, _, _ = oracle.decon_lucy_function(I, psf, k, Background=100, save16bit=False)
+        else:
+            Rk, _ = oracle.decon_lucy_omw_function(I, psf, pb, k, Background=100, save16bit=False)
+        mse.append(float(np.mean((Rk.astype(np.float64) - Ibg) ** 2)))
+        if k % step == 0:
+            f = os.path.join(folder, f"Iter_{k:04d}.tif")
+            T = pk.readtiff(f)
+            assert T.dtype == np.uint16
+            assert np.abs(T.astype(np.int64) - np.clip(np.floor(Rk.astype(np.float64) + 0.5), 0, 65535)).max() <= 1
+    rows = numit // step if method == "omw" else max(k for k in range(1, numit + 1)
+                                                       if k % step == 0 and k > 2 * step) // step
+    assert err.shape == (rows, 4)
+    np.testing.assert_array_equal(err[:, 0], np.arange(1, rows + 1))
+    np.testing.assert_allclose(err[:, 1], mse[:rows], rtol=1e-4)
+    np.testing.assert_allclose(err[:, 2], err[:, 1] / err[0, 1], rtol=1e-12)
+    if step == 1:  # rows 1, 2: 1; row k > 2: min(|e3(k) - e3(k-1)| / 10, |e3(k) + e3(k-2) - 2 e3(k-1)| / 2)
+        e3 = err[:, 2]
+        want = [1.0, 1.0] + [min(abs(e3[k] - e3[k - 1]) / 10, abs(e3[k] + e3[k - 2] - 2 * e3[k - 1]) / 2)
+                             for k in range(2, rows)]
+        np.testing.assert_allclose(err[:, 3], want, rtol=1e-12)
+    else:  # saveStep 2, NUMIT 7: rows 1..3 come from iterations 1..3, all before 3 * saveStep: criterion 1
+        np.testing.assert_array_equal(err[:, 3], np.ones(rows))
+
+
+def test_debug_rl_needs_three_save_steps(tmp_path):
+    """decon_lucy_function.m:170-172 reads istp, which only an iteration k > 2
+    saveStep that is a multiple of saveStep defines."""
+    import petakit5d_amd as pk
+    I, psf = _case((32, 32, 16), (7, 7, 9), seed=24)
+    with pytest.raises(RuntimeError, match="istp"):
+        pk.decon_lucy_function(I, psf, 2, Background=100, debug=True, debug_folder=str(tmp_path / "d"))
