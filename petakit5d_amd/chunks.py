@@ -182,5 +182,10 @@ def decon_large_in_memory(im, PSF_f, PSF_b, NUMIT, method="omw", batchSize=(512,
             import torch
             mask = torch.from_numpy(np.ascontiguousarray(mask.transpose(2, 1, 0))).to(out.device).permute(2, 1, 0)
     if mask is not None:  # im .* cast(im_bw_erode, class(im))
-        out[~mask] = 0
+        if _is_torch(out):
+            import torch
+            tgt = out.view(torch.int16) if out.dtype == torch.uint16 else out  # same bits; uint16 lacks masked_fill
+            tgt.masked_fill_(~mask, 0)
+        else:
+            out[~mask] = 0
     return out

@@ -300,16 +300,30 @@ def write_zarr(path, data, chunks=(256, 256, 256), compressor="zlib", level=1, d
 
 
 def decon_zarr(in_path, out_path, PSF_f, PSF_b, NUMIT, method="omw", batchSize=(512, 512, 512), chunks=None,
-               compressor=None, groupSize=16, **kwargs):
-    """Deconvolve the store at in_path into a store at out_path, streaming:
-    batches of batchSize (rounded up to whole output chunks) with a border of
-    round((size(psf) + 10) / 2) (RLdecon_large_in_memory.m:171-175, as
-    RLdecon_for_zarr_block.m:101-171 takes them from the store), read
-    groupSize at a time, each group deconvolved by ONE batched engine call,
-    and each batch's region written straight into the output store.  Only one
-    group's batches are held in host memory.  The output has the input's chunk
-    shape unless chunks is given, and the input's compressor (blosc or zlib;
-    zlib for other inputs) unless compressor is given.  Returns None."""
+               compressor=None, groupSize=20, flagPath=None, Overwrite=False, **kwargs):
+    """Deconvolve the store at in_path into a store at out_path as the Zarr
+    large-volume driver does (RLdecon_large_in_place.m:172-266 with
+    RLdecon_for_zarr_block.m:70-177), streaming: batches of batchSize
+    (rounded up to whole output chunks) with a border of round((size(psf) +
+    10) / 2), all of the same size (sameBatchSize = true, clipped at the
+    volume's start), sorted by voxel count and then by box (the reference's
+    sortrows, :179-182), taken groupSize at a time (taskSize 20); each group
+    is deconvolved by ONE batched engine call and each batch's region is
+    written straight into the output store.  The next group's batches are
+    read from the store while the current group runs on the device, and the
+    previous group's regions are written meanwhile.
+
+    flagPath (default out_path + '.zarr_flag', the reference's zarr_flag
+    folder): a group that completes writes blocks_<first>_<last>.mat (1-based
+    batch numbers in the sorted order, RLdecon_large_in_place.m:220-221,
+    RLdecon_for_zarr_block.m:173-177); a later run skips groups whose flag
+    file exists unless Overwrite (:79-86), so an interrupted run resumes.
+    The output has the input's chunk shape unless chunks is given, and the
+    input's compressor (blosc or zlib; zlib for other inputs) unless
+    compressor is given.  Returns None."""
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+
     from .batch import decon_batch
     from .chunks import XR_zarrChunkCoordinatesExtraction, _vec3, matlab_round
     m, _, _ = _meta(in_path)
@@ -324,17 +338,57 @@ def decon_zarr(in_path, out_path, PSF_f, PSF_b, NUMIT, method="omw", batchSize=(
     bs = np.array([-(-int(b) // c) * c for b, c in zip(bs, ochunks)])  # whole output chunks per batch
     psf_shape = np.shape(PSF_f)
     border = matlab_round((np.asarray(psf_shape, dtype=np.float64) + 10) / 2)
-    batch, region, _ = XR_zarrChunkCoordinatesExtraction(shape, batchSize=bs, BlockSize=bs, sameBatchSize=False,
-                                                         BorderSize=border)
-    meta_out = None
-    for g0 in range(0, len(batch), max(1, int(groupSize))):
-        bb, rr = batch[g0:g0 + groupSize], region[g0:g0 + groupSize]
-        ims = [read_zarr(in_path, b) for b in bb]
-        outs = decon_batch(ims, PSF_f, PSF_b, NUMIT, method=method, **kwargs)
-        if meta_out is None:
-            meta_out = _create_store(out_path, shape, outs[0].dtype, ochunks, compressor, 1, ".", "F", 0)
-        for b, r, o in zip(bb, rr, outs):
+    batch, region, _ = zarr_batches(shape, bs, border)
+    flag_dir = flagPath or (str(out_path).rstrip("/") + ".zarr_flag")
+    os.makedirs(flag_dir, exist_ok=True)
+    odt = np.uint16 if kwargs.get("save16bit", True) else np.float32
+    meta_out = _meta(out_path)[0] if os.path.exists(os.path.join(out_path, ".zarray")) else \
+        _create_store(out_path, shape, odt, ochunks, compressor, 1, ".", "F", 0)
+    gsz = max(1, int(groupSize))
+    groups = []
+    for g0 in range(0, len(batch), gsz):
+        flag = os.path.join(flag_dir, f"blocks_{g0 + 1}_{min(g0 + gsz, len(batch))}.mat")
+        if os.path.exists(flag):
+            if not Overwrite:
+                continue  # done by an earlier run
+            os.remove(flag)
+        groups.append((g0, flag))
+
+    def read_group(g0):
+        return [read_zarr(in_path, b) for b in batch[g0:g0 + gsz]]
+
+    def write_group(g0, outs, flag, t0):
+        for b, r, o in zip(batch[g0:g0 + gsz], region[g0:g0 + gsz], outs):
             lo = r[:3] - b[:3]
             hi = lo + (r[3:] - r[:3]) + 1
             _write_chunks(out_path, meta_out, np.asarray(o)[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]],
                           origin=tuple(int(v) - 1 for v in r[:3]))
+        from scipy.io import savemat
+        tmp = flag[:-4] + f"_{os.getpid()}.mat"
+        savemat(tmp, {"t": time.perf_counter() - t0})
+        os.replace(tmp, flag)
+
+    with ThreadPoolExecutor(max_workers=1) as reader, ThreadPoolExecutor(max_workers=1) as writer:
+        pending = reader.submit(read_group, groups[0][0]) if groups else None
+        writes = []
+        for gi, (g0, flag) in enumerate(groups):
+            t0 = time.perf_counter()
+            ims = pending.result()
+            pending = reader.submit(read_group, groups[gi + 1][0]) if gi + 1 < len(groups) else None
+            outs = decon_batch(ims, PSF_f, PSF_b, NUMIT, method=method, **kwargs)
+            writes.append(writer.submit(write_group, g0, outs, flag, t0))
+        for w in writes:
+            w.result()
+
+
+def zarr_batches(shape, batchSize, border):
+    """Batch, region and local boxes of the Zarr driver: sameBatchSize = true,
+    BlockSize = batchSize, sorted by batch voxel count, then by the box
+    itself (sortrows([prod(size), box]), RLdecon_large_in_place.m:174-182)."""
+    from .chunks import XR_zarrChunkCoordinatesExtraction
+    batch, region, local = XR_zarrChunkCoordinatesExtraction(shape, batchSize=batchSize, BlockSize=batchSize,
+                                                             sameBatchSize=True, BorderSize=border)
+    vox = np.prod(batch[:, 3:] - batch[:, :3] + 1, axis=1)
+    keys = np.concatenate([vox[:, None], batch], axis=1)
+    order = np.lexsort(keys.T[::-1])  # sortrows: first column first
+    return batch[order], region[order], local[order]

@@ -73,12 +73,34 @@ def test_large_in_memory_device_volume_matches_host():
     np.testing.assert_array_equal(Jd.cpu().numpy(), Jh)
 
 
+def _zarr_reference(im, psf, psf_b, numit, bs, ochunks):
+    """The Zarr driver's result assembled from one decon call per batch on the
+    driver's boxes (same-size batches, sorted by size, RLdecon_large_in_place.m:
+    172-182)."""
+    import petakit5d_amd as pk
+    from petakit5d_amd.chunks import matlab_round
+    from petakit5d_amd.zarr_io import zarr_batches
+    bs = np.array([-(-int(x) // c) * c for x, c in zip(bs, ochunks)])
+    border = matlab_round((np.array(psf.shape) + 10) / 2)
+    batch, region, _ = zarr_batches(im.shape, bs, border)
+    out = np.zeros(im.shape, np.uint16, order="F")
+    for bb, r in zip(batch, region):
+        c = np.asfortranarray(im[bb[0] - 1:bb[3], bb[1] - 1:bb[4], bb[2] - 1:bb[5]])
+        R, _ = pk.decon_lucy_omw_function(c, pk.crop_psf_to_data(psf, c.shape), pk.crop_psf_to_data(psf_b, c.shape),
+                                          numit, Background=100, pad=True)
+        lo = r[:3] - bb[:3]
+        hi = lo + (r[3:] - r[:3]) + 1
+        out[r[0] - 1:r[3], r[1] - 1:r[4], r[2] - 1:r[5]] = R[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+    return out, len(batch)
+
+
 @pytest.mark.parametrize("compressor", ["zlib", "blosc"])
-def test_decon_zarr_equals_in_memory(tmp_path, compressor):
-    """A Zarr store deconvolved batch group by batch group (read, one batched
-    call per group, regions written straight into the output store) equals
-    the in-memory driver on the same array, bitwise, whatever the group size;
-    blosc stores (the reference's default codec) are written back as blosc."""
+def test_decon_zarr_equals_per_batch_calls(tmp_path, compressor):
+    """A Zarr store deconvolved group by group (same-size batches sorted by
+    size, reads of the next group overlapping the device work, regions written
+    straight into the output store) equals one decon call per batch on the same
+    boxes, bitwise, whatever the group size; blosc stores (the reference's
+    default codec) are written back as blosc."""
     import json
     import petakit5d_amd as pk
     print(f"[decon_zarr] {compressor}", flush=True)
@@ -87,13 +109,39 @@ def test_decon_zarr_equals_in_memory(tmp_path, compressor):
     psf_b = np.ascontiguousarray(psf[::-1, ::-1, ::-1])
     src = str(tmp_path / "in.zarr")
     pk.write_zarr(src, im, chunks=(32, 32, 16), compressor=compressor)
-    R = pk.decon_large_in_memory(im, psf, psf_b, 2, batchSize=(64, 64, 32), EdgeErosion=0, Background=100)
-    for g in (16, 1):
+    R, _ = _zarr_reference(im, psf, psf_b, 2, (40, 40, 20), (32, 32, 16))
+    for g in (16, 1, 3):
         dst = str(tmp_path / f"out{g}.zarr")
         # batchSize (40, 40, 20) rounds up to whole 32 x 32 x 16 chunks: (64, 64, 32)
         assert pk.decon_zarr(src, dst, psf, psf_b, 2, batchSize=(40, 40, 20), groupSize=g, Background=100) is None
         np.testing.assert_array_equal(pk.read_zarr(dst), R)
         assert json.load(open(f"{dst}/.zarray"))["compressor"]["id"] == compressor
+
+
+def test_decon_zarr_flag_files_resume(tmp_path):
+    """Groups that finished leave blocks_<first>_<last>.mat flag files
+    (RLdecon_for_zarr_block.m:173-177); a second run skips them (:79-86), so
+    a region erased from the output stays erased until Overwrite reruns it."""
+    import os
+    import glob
+    import petakit5d_amd as pk
+    im = _vol((80, 72, 36), 9)
+    psf = synth.gaussian_psf((7, 7, 9), 1.0, 2.5)
+    psf_b = np.ascontiguousarray(psf[::-1, ::-1, ::-1])
+    src, dst = str(tmp_path / "in.zarr"), str(tmp_path / "out.zarr")
+    pk.write_zarr(src, im, chunks=(32, 32, 16))
+    R, nb = _zarr_reference(im, psf, psf_b, 2, (40, 40, 20), (32, 32, 16))
+    pk.decon_zarr(src, dst, psf, psf_b, 2, batchSize=(40, 40, 20), groupSize=3, Background=100)
+    flags = sorted(os.path.basename(f) for f in glob.glob(dst + ".zarr_flag/*.mat"))
+    want = sorted(f"blocks_{a}_{min(a + 2, nb)}.mat" for a in range(1, nb + 1, 3))
+    assert flags == want
+    np.testing.assert_array_equal(pk.read_zarr(dst), R)
+    victim = sorted(glob.glob(dst + "/0.0.0"))[0]
+    os.remove(victim)  # an absent chunk reads as the fill value 0
+    pk.decon_zarr(src, dst, psf, psf_b, 2, batchSize=(40, 40, 20), groupSize=3, Background=100)
+    assert not pk.read_zarr(dst)[:32, :32, :16].any()
+    pk.decon_zarr(src, dst, psf, psf_b, 2, batchSize=(40, 40, 20), groupSize=3, Background=100, Overwrite=True)
+    np.testing.assert_array_equal(pk.read_zarr(dst), R)
 
 
 @pytest.mark.parametrize("E", [8, 3])
