@@ -365,6 +365,16 @@ def _mround(x):
     return float(np.sign(x) * np.floor(np.abs(x) + 0.5))
 
 
+def _mmax(x, y):
+    """MATLAB max(x, y) of scalars: NaN is skipped."""
+    return y if np.isnan(x) else max(x, y)
+
+
+def _mmin(x, y):
+    """MATLAB min(x, y) of scalars: NaN is skipped."""
+    return y if np.isnan(x) else min(x, y)
+
+
 def wb_backprojector(psf, bp_type="traditional", alpha=0.001, beta=1.0, n=10, resFlag=1, iRes=(0, 0, 0)):
     """Returns (PSF_bp, OTF_bp, info) like BackProjector(PSF_fp, bp_type, alpha, beta, n, resFlag, iRes)."""
     p = np.asarray(psf, np.float32)
@@ -391,13 +401,13 @@ def wb_backprojector(psf, bp_type="traditional", alpha=0.001, beta=1.0, n=10, re
         raise ValueError("Processing terminated, please set resFlag as 0, 1, or 2")
     if resFlag == 2:
         t = 1.0 / res / (1.0 / S)
-        lo = [int(max(_mround(Sc[a] - t[a]), 1)) for a in range(3)]
-        hi = [int(min(_mround(Sc[a] + t[a]), S[a])) for a in range(3)]
+        lo = [int(_mmax(_mround(Sc[a] - t[a]), 1)) for a in range(3)]
+        hi = [int(_mmin(_mround(Sc[a] + t[a]), S[a])) for a in range(3)]
     else:
         px = (1.0 / S).astype(np.float32)
         t = ((np.float32(1) / res) / px).astype(np.float32)
-        lo = [int(max(_mround(float(np.float32(Sc[a]) - t[a])), 1)) for a in range(3)]
-        hi = [int(min(_mround(float(np.float32(Sc[a]) + t[a])), S[a])) for a in range(3)]
+        lo = [int(_mmax(_mround(float(np.float32(Sc[a]) - t[a])), 1)) for a in range(3)]
+        hi = [int(_mmin(_mround(float(np.float32(Sc[a]) + t[a])), S[a])) for a in range(3)]
     lx = on.max(axis=(1, 2))  # max over z then over y (:117-121)
     ly = on.max(axis=(0, 2))
     lz = on.max(axis=(0, 1))

@@ -656,14 +656,16 @@ void wb_backprojector(const float* psf_in, const int64_t d[3], int bp_type, doub
             td[a] = (double)t[a];
         }
     }
-    // 1-based index of an OTF profile sample: max(round(Sc - t), 1) / min(round(Sc + t), S)
+    // 1-based index of an OTF profile sample: max(round(Sc - t), 1) / min(round(Sc + t), S).
+    // MATLAB's max and min skip NaN, so a NaN cutoff (a step-like profile has
+    // no FWHM) takes index 1 / S.
     auto lo_idx = [&](int a) {
         const double x = res_flag == 2 ? mround(Sc[a] - td[a]) : mround((double)((float)Sc[a] - t[a]));
-        return (int64_t)std::max(x, 1.0);
+        return std::isnan(x) ? (int64_t)1 : (int64_t)std::max(x, 1.0);
     };
     auto hi_idx = [&](int a) {
         const double x = res_flag == 2 ? mround(Sc[a] + td[a]) : mround((double)((float)Sc[a] + t[a]));
-        return (int64_t)std::min(x, (double)d[a]);
+        return std::isnan(x) ? d[a] : (int64_t)std::min(x, (double)d[a]);
     };
     // cutoff gains of the traditional back projector (:117-137)
     std::vector<double> lx((size_t)d[0], -1e300), ly((size_t)d[1], -1e300), lz((size_t)d[2], -1e300);

@@ -21,6 +21,34 @@ struct mxArray_tag {
 static jmp_buf g_err;
 static char g_errid[128];
 
+/* MATLAB's memory manager frees the mxArrays and mxMalloc blocks a MEX call
+ * leaves behind; this registry frees them at exit, so a leak checker run on
+ * the harness reports only memory the gateway or the library itself lose. */
+static void **g_blocks;
+static size_t g_nblocks, g_capblocks;
+static void *mx_alloc(size_t n) {
+    void *p = calloc(1, n ? n : 1);
+    if (g_nblocks == g_capblocks) {
+        g_capblocks = g_capblocks ? 2 * g_capblocks : 256;
+        g_blocks = realloc(g_blocks, g_capblocks * sizeof(void *));
+    }
+    g_blocks[g_nblocks++] = p;
+    return p;
+}
+static void mx_release(void *p) {
+    if (!p) return;
+    for (size_t i = 0; i < g_nblocks; ++i)
+        if (g_blocks[i] == p) { g_blocks[i] = g_blocks[--g_nblocks]; free(p); return; }
+    free(p);
+}
+static void mx_release_all(void) {
+    for (size_t i = 0; i < g_nblocks; ++i) free(g_blocks[i]);
+    free(g_blocks);
+    g_blocks = NULL;
+    g_nblocks = g_capblocks = 0;
+}
+static char *mx_strdup(const char *s) { char *d = mx_alloc(strlen(s) + 1); strcpy(d, s); return d; }
+
 static size_t esz(mxClassID c) { return c == mxDOUBLE_CLASS ? 8 : c == mxSINGLE_CLASS ? 4 : c == mxUINT16_CLASS ? 2 : 1; }
 mwSize mxGetNumberOfDimensions(const mxArray *a) { return a->nd; }
 const mwSize *mxGetDimensions(const mxArray *a) { return a->dims; }
@@ -38,19 +66,19 @@ mxArray *mxGetField(const mxArray *a, size_t i, const char *name) {
 double mxGetScalar(const mxArray *a) { return a->cls == mxDOUBLE_CLASS ? ((double *)a->data)[0] : 0.0; }
 double *mxGetPr(const mxArray *a) { return (double *)a->data; }
 void *mxGetData(const mxArray *a) { return a->data; }
-char *mxArrayToString(const mxArray *a) { return strdup(a->str); }
-void *mxMalloc(size_t n) { return malloc(n); }
-void mxFree(void *p) { free(p); }
+char *mxArrayToString(const mxArray *a) { return mx_strdup(a->str); }
+void *mxMalloc(size_t n) { return mx_alloc(n); }
+void mxFree(void *p) { mx_release(p); }
 mxArray *mxCreateNumericArray(mwSize nd, const mwSize *dims, mxClassID c, mxComplexity cx) {
-    mxArray *a = calloc(1, sizeof(*a));
+    mxArray *a = mx_alloc(sizeof(*a));
     a->cls = c; a->nd = nd;
     for (mwSize i = 0; i < nd; ++i) a->dims[i] = dims[i];
-    a->data = calloc(mxGetNumberOfElements(a) ? mxGetNumberOfElements(a) : 1, esz(c) * (cx == mxCOMPLEX ? 2 : 1));
+    a->data = mx_alloc((mxGetNumberOfElements(a) ? mxGetNumberOfElements(a) : 1) * esz(c) * (cx == mxCOMPLEX ? 2 : 1));
     return a;
 }
 mxArray *mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity cx) { mwSize d[2] = {m, n}; return mxCreateNumericArray(2, d, mxDOUBLE_CLASS, cx); }
 mxArray *mxCreateDoubleScalar(double v) { mxArray *a = mxCreateDoubleMatrix(1, 1, mxREAL); ((double *)a->data)[0] = v; return a; }
-void mxDestroyArray(mxArray *a) { if (a) { free(a->data); free(a); } }
+void mxDestroyArray(mxArray *a) { if (a) { mx_release(a->data); mx_release(a->str); mx_release(a); } }
 void mexErrMsgIdAndTxt(const char *id, const char *fmt, ...) {
     (void)fmt;
     snprintf(g_errid, sizeof(g_errid), "%s", id);
@@ -60,7 +88,7 @@ int mexAtExit(void (*fn)(void)) { (void)fn; return 0; }
 
 void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]);
 
-static mxArray *str(const char *s) { mxArray *a = calloc(1, sizeof(*a)); a->cls = mxCHAR_CLASS; a->nd = 2; a->dims[0] = 1; a->dims[1] = strlen(s); a->str = strdup(s); return a; }
+static mxArray *str(const char *s) { mxArray *a = mx_alloc(sizeof(*a)); a->cls = mxCHAR_CLASS; a->nd = 2; a->dims[0] = 1; a->dims[1] = strlen(s); a->str = mx_strdup(s); return a; }
 static mxArray *dbl(double v) { return mxCreateDoubleScalar(v); }
 
 static int call(int nlhs, mxArray **plhs, int nrhs, const mxArray **prhs) {
@@ -89,7 +117,7 @@ static void spill(const char *dir, const char *name, const void *p, size_t bytes
 }
 
 static mxArray *opts_struct(double background, double save16bit) {
-    mxArray *s = calloc(1, sizeof(*s));
+    mxArray *s = mx_alloc(sizeof(*s));
     s->cls = mxSTRUCT_CLASS; s->nd = 2; s->dims[0] = s->dims[1] = 1;
     s->fname[0] = "Background"; s->fval[0] = dbl(background);
     s->fname[1] = "save16bit"; s->fval[1] = dbl(save16bit);
@@ -132,6 +160,7 @@ static int files_mode(int argc, char **argv) {
 }
 
 int main(int argc, char **argv) {
+    atexit(mx_release_all);
     if (argc > 1 && !strcmp(argv[1], "files")) return files_mode(argc, argv);
     mxArray *out[4] = {0};
     const mxArray *a1[] = {str("fgfn"), dbl(101), dbl(1), dbl(0), dbl(7)};
@@ -144,7 +173,7 @@ int main(int argc, char **argv) {
     mxArray *I = mxCreateNumericArray(3, d, mxDOUBLE_CLASS, mxREAL);
     mwSize pdm[3] = {3, 3, 3};
     mxArray *P = mxCreateNumericArray(3, pdm, mxSINGLE_CLASS, mxREAL);
-    mxArray *s = calloc(1, sizeof(*s)); s->cls = mxSTRUCT_CLASS; s->nd = 2; s->dims[0] = s->dims[1] = 1;
+    mxArray *s = mx_alloc(sizeof(*s)); s->cls = mxSTRUCT_CLASS; s->nd = 2; s->dims[0] = s->dims[1] = 1;
     const mxArray *a4[] = {str("rl"), I, P, dbl(2), s};
     call(1, out, 5, a4);  /* double input -> unsupportedType */
     mxArray *I16 = mxCreateNumericArray(3, d, mxUINT16_CLASS, mxREAL);

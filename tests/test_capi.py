@@ -100,6 +100,31 @@ def test_wb_backprojector_library_vs_oracle(bp_type):
             assert np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300) < 1e-6
 
 
+@pytest.mark.parametrize("bp_type", ["traditional", "wiener", "butterworth"])
+def test_wb_backprojector_step_like_profile(bp_type):
+    """A z profile that never falls back below half maximum has no FWHM
+    (fwhm returns NaN, BackProjector.m:474-509): the cutoff sample indices
+    are then max(round(NaN), 1) = 1 and min(round(NaN), Sz) = Sz, MATLAB's max
+    and min skipping NaN (BackProjector.m:127-129).  Library and oracle agree,
+    NaN for NaN where the Butterworth mask carries the NaN cutoff."""
+    import bp_oracle
+    import petakit5d_amd as pk
+    from petakit5d_amd import synth
+    pz = np.array([0.1, 0.3, 0.7, 1.0, 0.9, 0.8, 0.7], np.float32)
+    psf = (synth.gaussian_psf((24, 26, 3), 1.1, 1.0, 0.0)[:, :, 1:2] * pz[None, None, :]).astype(np.float32)
+    P, O, info = pk.BackProjector(psf, bp_type, 0.005, 1, 10, 1, info=True)
+    Pr, Or, ir = bp_oracle.wb_backprojector(psf, bp_type, 0.005, 1, 10, 1)
+    assert np.isnan(ir["FWHM"][2]) and np.isnan(info["FWHM"][2])
+    assert np.isfinite(ir["beta_fp"]) and abs(info["beta_fp"] - ir["beta_fp"]) < 1e-12
+    for a, b in ((P, Pr), (O, Or)):
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        f = ~np.isnan(b)
+        if f.any():
+            assert np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300) < 1e-6
+    if bp_type == "butterworth":
+        assert np.isnan(P).all()
+
+
 def test_wb_backprojector_errors():
     import petakit5d_amd as pk
     p = np.ones((9, 9, 9), np.float32)
