@@ -20,6 +20,8 @@
 #include <chrono>
 #include <cstring>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <deque>
 #include <map>
 #include <memory>
@@ -77,6 +79,10 @@ struct Server {
     bool stop = false;
     uint64_t next_id = 1;
     int max_active = 64;
+    // failure injection for tests of the failure path (PK_SERVE_INJECT_FAILURE="id:k",
+    // read at start): request `id` fails on the host before its iteration k + 1
+    uint64_t inject_id = 0;
+    int inject_k = 0;
     pk_serve_stats stats{};
 };
 
@@ -246,6 +252,22 @@ static void scheduler(pk_ctx_impl* ctx) {
             // ---- one iteration for every request in flight.  Nothing waits for
             // the device here: the step-table double buffer keeps the host at
             // most two steps ahead, and finished requests retire by event.
+            if (S.inject_id) {
+                // a request whose host-side work fails mid-flight: it alone fails, its
+                // memory and OTF pins are released once its queued steps have run
+                std::vector<std::shared_ptr<ServeReq>> keep;
+                for (auto& r : S.active) {
+                    if (r->id == S.inject_id && r->c.k == S.inject_k) {
+                        quiesce(ctx);
+                        failed.emplace_back(r, Error{PK_ERR_INVALID_ARG, "injected failure (PK_SERVE_INJECT_FAILURE)"});
+                        r->mem.release();
+                        unpin(ctx, *r);
+                    } else {
+                        keep.push_back(r);
+                    }
+                }
+                S.active.swap(keep);
+            }
             if (!S.active.empty()) {
                 std::vector<std::shared_ptr<ServeReq>> ending;
                 try {
@@ -367,6 +389,14 @@ void serve_start(pk_ctx_impl* ctx, int max_active) {
     if (ctx->srv) fail(PK_ERR_BUSY, "server already running");
     ctx->srv = std::make_unique<Server>();
     ctx->srv->max_active = max_active > 0 ? max_active : 64;
+    if (const char* inj = std::getenv("PK_SERVE_INJECT_FAILURE")) {
+        unsigned long long id = 0;
+        int k = 0;
+        if (std::sscanf(inj, "%llu:%d", &id, &k) == 2 && k >= 0) {
+            ctx->srv->inject_id = id;
+            ctx->srv->inject_k = k;
+        }
+    }
     // many request shapes and channels: keep their OTFs (the HBM budget still bounds them)
     ctx->otf_max_entries = std::max<size_t>(ctx->otf_max_entries, 256);
     ctx->srv->th = std::thread(scheduler, ctx);
