@@ -24,7 +24,8 @@ Other workloads (--workload): rl128 (Biggs RL 15 it. on 128x128x64), batched
 (64 random-size chunks, OMW-2 and RL-10 in batched launches), served
 (continuous-batching server, Poisson arrivals, p50/p99 vs the plain path),
 large (2048x2048x512, OMW-2 and Biggs-10), deskew (deskewFrame3D of a
-512x512x256 decon result), rotate (rotateFrame3D of the deskewed volume).
+512x512x256 decon result), rotate (rotateFrame3D of the deskewed volume),
+probes (the passes' access patterns as plain copies: HBM ceilings).
 """
 import argparse
 import ctypes
@@ -38,8 +39,9 @@ import time
 import numpy as np
 
 # Hardware queues per process, read when the HIP runtime initialises: batched
-# and served steps run eight chunk-set streams, and HIP's default of 4 queues
-# leaves them two.  Raised to 12 when lower (a setting above 12 is kept).
+# and served steps run up to eight chunk-set streams (four by default) beside
+# the engine's and the copy streams, and HIP's default of 4 queues leaves them
+# two.  Raised to 12 when lower (a setting above 12 is kept).
 try:
     _q = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
 except ValueError:
@@ -58,7 +60,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="omw", choices=["omw", "rl128", "batched", "served", "large", "copy",
-                                                                  "deskew", "rotate"])
+                                                                  "deskew", "rotate", "probes"])
     ap.add_argument("--shape", type=int, nargs=3, default=None)
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--duration", type=float, default=60.0, help="served: seconds of Poisson arrivals")
@@ -683,6 +685,48 @@ def wl_copy(args, torch, pk, L):
     return res
 
 
+def wl_probes(args, torch, pk, L):
+    """Copy probes (pk_probe_copy): the pitched half spectrum of a transform
+    copied with each pass's access pattern and no arithmetic -- the HBM
+    ceilings the passes are compared with.  Sequential 16-byte copy, the
+    dim-2 column pattern (y passes) and the dim-3 column pattern (z_conv) at
+    several tile widths T and values per thread E, XCD-aware block order (and
+    the linear order beside it), and the dim-3 pattern with a second input
+    read (z_conv's three streams), at 512x512x256 and 2048x2048x512.  `value` is the dim-3 pattern at the
+    shape z_conv runs with at 512x512x256 (T = 16, E = 16)."""
+    ctx = pk.Context(0)
+    reps = max(5, args.steps)
+    out = {}
+    cases = {(512, 512, 256): [(0, 0, 0), (1, 8, 16), (1, 16, 16), (1, 32, 16), (1, 16, 32), (1, 64, 32),
+                               (2, 8, 16), (2, 16, 16), (2, 32, 16), (2, 64, 16), (2, 16, 8), (2, 32, 8),
+                               (3, 16, 16), (3, 32, 16)],
+             (2048, 2048, 512): [(0, 0, 0), (1, 8, 32), (1, 16, 32), (2, 8, 16), (2, 16, 16), (2, 32, 16),
+                                 (2, 16, 32), (2, 32, 32), (2, 64, 32), (3, 16, 8), (3, 16, 16), (3, 32, 16),
+                                 (3, 32, 32)]}
+    names = {0: "sequential", 1: "dim2_columns", 2: "dim3_columns", 3: "dim3_columns_two_inputs"}
+    for dims, cs in cases.items():
+        key = "x".join(map(str, dims))
+        rec = {}
+        for pat, T, E in cs:
+            progress(f"probe {key} {names[pat]} T={T} E={E}")
+            ms, nb = ctx.probe_copy(pat, dims, T or 16, E or 16, True, reps)
+            rec[names[pat] if pat == 0 else f"{names[pat]}_T{T}_E{E}"] = {"ms": round(ms, 4),
+                                                                          "GB_s": round(nb / ms / 1e6, 1)}
+            ms2, _ = ctx.probe_copy(pat, dims, T or 16, E or 16, False, reps) if pat else (None, None)
+            if ms2:
+                rec[f"{names[pat]}_T{T}_E{E}"]["GB_s_linear_order"] = round(nb / ms2 / 1e6, 1)
+        out[key] = rec
+    ctx.close()
+    v = out["512x512x256"]["dim3_columns_T16_E16"]["GB_s"]
+    res = base_result("strided-pattern copy GB/s (dim-3 columns, T=16, E=16, 512x512x256)", v, "GB/s", args,
+                      out["512x512x256"]["dim3_columns_T16_E16"]["ms"], {
+                          "workload": "copy of the pitched half spectrum with each pass's access pattern, no "
+                                      "arithmetic (read + written bytes of the valid columns)"})
+    res["probes"] = out
+    res["dtype"] = "f32"
+    return res
+
+
 def wl_deskew(args, torch, pk, L):
     """deskewFrame3D of a decon result (single, 512x512x256, the default
     workload's size) at the reference defaults SkewAngle 32.45, dz 0.3,
@@ -811,7 +855,7 @@ def main():
 
     L = _lib.load()
     res = {"omw": wl_omw, "rl128": wl_rl128, "batched": wl_batched, "served": wl_served, "large": wl_large,
-           "copy": wl_copy, "deskew": wl_deskew, "rotate": wl_rotate}[
+           "copy": wl_copy, "deskew": wl_deskew, "rotate": wl_rotate, "probes": wl_probes}[
         args.workload](args, torch, pk, L)
     print(json.dumps(res), flush=True)
 

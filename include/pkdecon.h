@@ -317,6 +317,20 @@ pk_status pk_profile_enable(pk_ctx *ctx, int32_t enable);
 /* Copies up to max entries into out; returns the number of entries (-1: null ctx). */
 int32_t pk_profile_read(pk_ctx *ctx, pk_prof_entry *out, int32_t max);
 
+/* Copy probes (diagnostics; no counterpart in the reference): copies the
+ * pitched half spectrum of a dims-sized transform into a second device
+ * buffer with one pass's access pattern and no arithmetic -- pattern 0:
+ * sequential 16-byte accesses; 1: the dim-2 column passes (tiles of `tile`
+ * adjacent columns, column stride = row pitch); 2: the dim-3 column pass
+ * (column stride = plane pitch); 3: pattern 2 with a second input read at the
+ * same points (the OTF convolution's traffic).  Each thread moves vals_per_thread points
+ * (4, 8, 16 or 32; the column length must be a multiple) of one column;
+ * xcd_order = 1 takes the passes' XCD-aware block order.  *ms = mean time of
+ * reps copies (HIP events on the context's stream), *bytes = bytes read plus
+ * written per copy (valid columns only).  Uses the context's workspace. */
+pk_status pk_probe_copy(pk_ctx *ctx, int32_t pattern, const int64_t dims[3], int32_t tile, int32_t vals_per_thread,
+                        int32_t xcd_order, int32_t reps, double *ms, double *bytes);
+
 /* findGoodFactorNumber(givenNum, directionFlag, allowOddSeven) (utils/findGoodFactorNumber.m:1-26).
  * max_prime = 7 is the reference rule; 5 gives the 2/3/5-smooth sizes the GPU transforms need. */
 int64_t pk_find_good_factor_number(int64_t n, int32_t direction, int32_t allow_odd_seven, int32_t max_prime);
@@ -351,6 +365,19 @@ pk_status pk_deskew_rotate_out_dims(const int64_t dims[3], double angle, double 
 pk_status pk_deskew_rotate_frame3d(pk_ctx *ctx, const pk_vol *in, double angle, double dz, double xy_pixel_size,
                                    int32_t reverse, int32_t objective_scan, double x_step_thresh, const int64_t bbox[6],
                                    pk_vol *out);
+/* The same with interpMethod 'cubic' (deskewRotateFrame3D.m:73 leaves the fast
+ * path, :98-110 and :209-210): for a stage scan whose step exceeds
+ * x_step_thresh the skewed-space interpolation runs first (linear; its result
+ * rounded to uint16 values when in and out are uint16, as the reference's
+ * interpolation returns uint16 with save16bit), then imwarp(vol_1,
+ * affine3d(ds_S*(T1*S*R*T2)), 'cubic', 'FillValues', 0, 'OutputView', RA) as
+ * pk_affine_warp3d_cubic computes it; bbox is RA's output window (:203-205).
+ * imwarp returns the class of vol_1, so a caller mirroring the reference
+ * passes a uint16 out for a uint16 volume that is not interpolated, or that
+ * is interpolated with save16bit. */
+pk_status pk_deskew_rotate_frame3d_cubic(pk_ctx *ctx, const pk_vol *in, double angle, double dz, double xy_pixel_size,
+                                         int32_t reverse, int32_t objective_scan, double x_step_thresh,
+                                         const int64_t bbox[6], pk_vol *out);
 
 /* Affine resampling as imwarp(vol, affine3d(T), 'linear', 'FillValues', 0,
  * 'OutputView', imref3d(out_dims, 1, 1, 1)) does for rotateFrame3D
@@ -362,6 +389,13 @@ pk_status pk_deskew_rotate_frame3d(pk_ctx *ctx, const pk_vol *in, double angle, 
  * column-major single; returns when done. */
 pk_status pk_affine_warp3d(pk_ctx *ctx, const float *d_vol, const int64_t dims[3], const double inv_tform[16],
                            const int64_t out_dims[3], float *d_out);
+/* The same resampling with imwarp's 'cubic' method (deskewFrame3D.m:70,
+ * rotateFrame3D.m:96 with 'Interp', 'cubic'): the input padded with one voxel
+ * of zeros, cubic convolution (Keys, a = -0.5) over the padded grid with Keys'
+ * boundary rule f(-1) = 3 f(0) - 3 f(1) + f(2) for the taps past its ends,
+ * fill value 0 outside the padded grid.  Every input dimension >= 2. */
+pk_status pk_affine_warp3d_cubic(pk_ctx *ctx, const float *d_vol, const int64_t dims[3], const double inv_tform[16],
+                                 const int64_t out_dims[3], float *d_out);
 
 /* TIFF LZW strip codec (host code; no context, no device).  Decodes one LZW
  * strip as libtiff's codec does for the reference reader's

@@ -1,4 +1,6 @@
-"""numpy restatement of deskewFrame3D (linear interpolation).
+"""numpy restatement of deskewFrame3D (linear interpolation), of
+deskewRotateFrame3D (linear fast path; 'cubic' through imwarp's form), and of
+imwarp's cubic resampling (cubic_warp).
 
 TEST INFRASTRUCTURE ONLY: the checker for the engine's deskew.  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg may import it.
@@ -143,6 +145,90 @@ def general_warp(vol, T, out):
     coords = np.stack([pts[:, 1] - 1, pts[:, 0] - 1, pts[:, 2] - 1])
     r = ndi.map_coordinates(v, coords, order=1, mode="grid-constant", cval=0.0, prefilter=False)
     return r.reshape(out).astype(np.float32)
+
+
+def keys_weights(t):
+    """Cubic convolution weights (Keys 1981, a = -0.5) of the samples at
+    offsets -1, 0, +1, +2 from floor(x), t = x - floor(x), from the kernel
+    W(s) = 1.5|s|^3 - 2.5|s|^2 + 1 (|s| <= 1), -0.5|s|^3 + 2.5|s|^2 - 4|s| + 2
+    (1 < |s| < 2), 0 otherwise."""
+    def W(s):
+        s = abs(s)
+        if s <= 1:
+            return 1.5 * s ** 3 - 2.5 * s ** 2 + 1
+        if s < 2:
+            return -0.5 * s ** 3 + 2.5 * s ** 2 - 4 * s + 2
+        return 0.0
+    return [W(1 + t), W(t), W(1 - t), W(2 - t)]
+
+
+def padded_extended(vol):
+    """The grid imwarp's cubic resampling reads: vol padded with one zero
+    voxel per side (1-based indices 0 .. n + 1), then extended by one more per
+    side with Keys' boundary rule f(-1) = 3 f(0) - 3 f(1) + f(2) and
+    f(n + 2) = 3 f(n + 1) - 3 f(n) + f(n - 1), axis by axis.  Returned array
+    index e holds 1-based index e - 1 (e = 0 .. n + 3 for -1 .. n + 2)."""
+    g = np.pad(np.asarray(vol, np.float64), 1)
+    for ax in range(3):
+        g = np.moveaxis(g, ax, 0)
+        lo = 3 * g[0] - 3 * g[1] + g[2]
+        hi = 3 * g[-1] - 3 * g[-2] + g[-3]
+        g = np.concatenate([lo[None], g, hi[None]], axis=0)
+        g = np.moveaxis(g, 0, ax)
+    return g
+
+
+def cubic_warp(vol, inv, out, origin=(0, 0, 0)):
+    """imwarp(vol, tform, 'cubic', 'FillValues', 0, 'OutputView', RA) with
+    inv = inv(T) (row-vector convention) and RA's window starting at 1-based
+    world offset origin + 1 ([x y z] = [col row page]): tricubic convolution
+    over padded_extended(vol) for points inside the padded grid [0, n + 1] on
+    every axis, 0 elsewhere.  FP64 throughout."""
+    v = np.asarray(vol, np.float64)
+    n = np.array(v.shape)
+    g = padded_extended(v)
+    Y, X, Z = np.meshgrid(np.arange(1, out[0] + 1) + origin[1], np.arange(1, out[1] + 1) + origin[0],
+                          np.arange(1, out[2] + 1) + origin[2], indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel(), np.ones(X.size)], axis=1) @ inv
+    c = np.stack([pts[:, 1], pts[:, 0], pts[:, 2]], axis=1)  # (dim1, dim2, dim3), 1-based
+    inside = np.all((c >= 0) & (c <= n + 1), axis=1)
+    c = c[inside]
+    fl = np.floor(c).astype(np.int64)
+    t = c - fl
+    w = [np.array([keys_weights(q) for q in t[:, a]]).reshape(-1, 4) for a in range(3)]  # (M, 4) per axis
+    acc = np.zeros(len(c))
+    for dz in range(4):
+        for dx in range(4):
+            for dy in range(4):
+                wt = w[0][:, dy] * w[1][:, dx] * w[2][:, dz]
+                live = wt != 0.0  # the taps past the extension at x = n + 1 exactly carry weight 0
+                # 1-based tap fl - 1 + d sits at index fl + d of the extended grid
+                i0 = np.where(live, fl[:, 0] + dy, 0)
+                i1 = np.where(live, fl[:, 1] + dx, 0)
+                i2 = np.where(live, fl[:, 2] + dz, 0)
+                acc += np.where(live, wt * g[i0, i1, i2], 0.0)
+    res = np.zeros(X.size)
+    res[inside] = acc
+    return res.reshape(out)
+
+
+def deskew_rotate_cubic(vol, angle, dz, xy, reverse=False, objectiveScan=False, xStepThresh=2.0, bbox=None,
+                        round_interp=False):
+    """deskewRotateFrame3D(vol, ..., 'interpMethod', 'cubic')
+    (deskewRotateFrame3D.m:73, 98-110, 203-210): skewed-space interpolation
+    when the step is large (round_interp: its result rounded as uint16, the
+    reference's uint16 output with save16bit), then cubic_warp of the whole
+    forward transform over RA (the bbox window or outSize).  FP64 result."""
+    tmat, out, xstep, stepsize, T = dsr_geometry(vol.shape, angle, dz, xy, reverse, objectiveScan, xStepThresh)
+    v = np.asarray(vol, np.float32)
+    if stepsize != 1:
+        v = skewed_interp(v, xstep, stepsize, reverse)
+        if round_interp:
+            v = np.clip(np.floor(v.astype(np.float64) + 0.5), 0, 65535).astype(np.float32)
+    if bbox is None:
+        bbox = [1, 1, 1, out[0], out[1], out[2]]
+    od = (bbox[3] - bbox[0] + 1, bbox[4] - bbox[1] + 1, bbox[5] - bbox[2] + 1)
+    return cubic_warp(v, np.linalg.inv(T), od, origin=(bbox[1] - 1, bbox[0] - 1, bbox[2] - 1))
 
 
 def skewed_interp(vol, xstep, stepsize, reverse=False):

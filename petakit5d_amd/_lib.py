@@ -66,6 +66,9 @@ EXPORTS = [
     "pk_fft_c2r",
     "pk_profile_enable",
     "pk_profile_read",
+    "pk_probe_copy",
+    "pk_affine_warp3d_cubic",
+    "pk_deskew_rotate_frame3d_cubic",
     "pk_find_good_factor_number",
     "pk_lzw_decode",
     "pk_lzw_encode",
@@ -191,6 +194,8 @@ def load():
         L.pk_profile_enable.argtypes = [vp, ctypes.c_int32]
         L.pk_profile_read.argtypes = [vp, ctypes.POINTER(PkProfEntry), ctypes.c_int32]
         L.pk_profile_read.restype = ctypes.c_int32
+        L.pk_probe_copy.argtypes = [vp, ctypes.c_int32, i64p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.pk_find_good_factor_number.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.pk_find_good_factor_number.restype = ctypes.c_int64
         for name in EXPORTS:
@@ -253,6 +258,14 @@ class Context:
         n = self._L.pk_profile_read(self.handle, buf, n)
         return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms), float(buf[i].bytes_per_launch))
                 for i in range(n)}
+
+    def probe_copy(self, pattern: int, dims, tile: int = 16, vals: int = 16, xcd: bool = True, reps: int = 20):
+        """(ms per copy, bytes per copy) of pk_probe_copy: pattern 0 sequential,
+        1 dim-2 columns, 2 dim-3 columns of a dims-sized transform's spectrum."""
+        ms, nb = ctypes.c_double(), ctypes.c_double()
+        self.check(self._L.pk_probe_copy(self.handle, int(pattern), i64x3(tuple(int(x) for x in dims)), int(tile),
+                                         int(vals), 1 if xcd else 0, int(reps), ctypes.byref(ms), ctypes.byref(nb)))
+        return ms.value, nb.value
 
     def device_bytes(self) -> int:
         return int(self._L.pk_device_bytes(self.handle))

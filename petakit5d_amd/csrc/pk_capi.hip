@@ -25,6 +25,8 @@ struct Affine3 {
 };
 void launch_affine3d(const float* vol, float* out, const int64_t dims[3], const int64_t odims[3], const Affine3& m,
                      hipStream_t s);
+void launch_affine3d_cubic(const float* vol, float* out, const int64_t dims[3], const int64_t odims[3],
+                           const Affine3& m, hipStream_t s);
 }  // namespace pk
 
 namespace {
@@ -336,6 +338,14 @@ pk_status pk_deskew_rotate_frame3d(pk_ctx* c, const pk_vol* in, double angle, do
     });
 }
 
+pk_status pk_deskew_rotate_frame3d_cubic(pk_ctx* c, const pk_vol* in, double angle, double dz, double xy_pixel_size,
+                                         int32_t reverse, int32_t objective_scan, double x_step_thresh,
+                                         const int64_t bbox[6], pk_vol* out) {
+    return guarded(c, [&] {
+        c->deskew_rotate(in, angle, dz, xy_pixel_size, reverse != 0, objective_scan != 0, x_step_thresh, bbox, out, 1);
+    });
+}
+
 pk_status pk_set_iter_hook(pk_ctx* c, int32_t save_step, pk_iter_hook hook, void* user) {
     return guarded(c, [&] {
         if (save_step < 1) pk::fail(PK_ERR_INVALID_ARG, "saveStep must be >= 1");
@@ -365,12 +375,40 @@ pk_status pk_affine_warp3d(pk_ctx* c, const float* d_vol, const int64_t dims[3],
     });
 }
 
+pk_status pk_affine_warp3d_cubic(pk_ctx* c, const float* d_vol, const int64_t dims[3], const double inv_tform[16],
+                                 const int64_t out_dims[3], float* d_out) {
+    return guarded(c, [&] {
+        if (!d_vol || !d_out || !dims || !out_dims || !inv_tform) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        for (int i = 0; i < 3; ++i)
+            if (dims[i] < 2 || out_dims[i] < 1)
+                pk::fail(PK_ERR_INVALID_ARG, "cubic: input dimensions must be >= 2, output dimensions >= 1");
+        pk::Affine3 m;
+        for (int i = 0; i < 16; ++i) {
+            if (!std::isfinite(inv_tform[i])) pk::fail(PK_ERR_INVALID_ARG, "transform entries must be finite");
+            m.a[i / 4][i % 4] = inv_tform[i];
+        }
+        pk::launch_affine3d_cubic(d_vol, d_out, dims, out_dims, m, c->stream);
+        PK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 pk_status pk_profile_enable(pk_ctx* c, int32_t enable) {
     return guarded(c, [&] {
         PK_HIP(hipStreamSynchronize(c->stream));
         c->prof_flush();
         c->prof.clear();
         c->prof_on = enable != 0;
+    });
+}
+
+pk_status pk_probe_copy(pk_ctx* c, int32_t pattern, const int64_t dims[3], int32_t tile, int32_t vals_per_thread,
+                        int32_t xcd_order, int32_t reps, double* ms, double* bytes) {
+    return guarded(c, [&] {
+        if (!dims || !ms || !bytes) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        if (pattern < 0 || pattern > 3 || reps < 1) pk::fail(PK_ERR_INVALID_ARG, "pattern 0..3, reps >= 1");
+        for (int d = 0; d < 3; ++d)
+            if (dims[d] < 2) pk::fail(PK_ERR_INVALID_ARG, "probe dimensions must be >= 2");
+        *ms = c->probe_copy(pattern, dims, tile, vals_per_thread, xcd_order, reps, bytes);
     });
 }
 

@@ -173,4 +173,102 @@ void launch_affine3d(const float* vol, float* out, const int64_t dims[3], const 
     PK_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- cubic
+// imwarp(..., 'cubic', 'FillValues', 0): the input padded with one voxel of
+// the fill value 0 on every side, then cubic convolution (Keys, a = -0.5:
+// the kernel that reproduces quadratics) over that padded grid, whose two
+// missing neighbours past each end come from Keys' boundary rule
+// f(-1) = 3 f(0) - 3 f(1) + f(2); a point outside the padded grid takes the
+// fill value.  Per axis the (up to) four taps fold into weights on real
+// samples: a tap on the zero pad drops out, the extrapolated tap below the
+// grid (padded index -1, i.e. 1-based -1) adds -3 w to sample 1 and w to
+// sample 2, and symmetrically at the top.
+struct CubicTaps {
+    int64_t idx[4];  // 0-based samples
+    float w[4];
+    int n;
+};
+
+__device__ __forceinline__ CubicTaps cubic_taps(double x, int64_t n) {
+    // x: 1-based coordinate, within [0, n + 1]
+    const double f0 = floor(x);
+    const int64_t i = (int64_t)f0;
+    const float t = (float)(x - f0);
+    const float t2 = t * t, t3 = t2 * t;
+    const float w[4] = {-0.5f * t3 + t2 - 0.5f * t, 1.5f * t3 - 2.5f * t2 + 1.f, -1.5f * t3 + 2.f * t2 + 0.5f * t,
+                        0.5f * t3 - 0.5f * t2};
+    CubicTaps c{};
+    auto add = [&](int64_t j, float v) {  // j: 1-based real sample
+        if (j < 1 || j > n) return;        // the zero pad
+        for (int k = 0; k < c.n; ++k)
+            if (c.idx[k] == j - 1) {
+                c.w[k] += v;
+                return;
+            }
+        c.idx[c.n] = j - 1;
+        c.w[c.n] = v;
+        ++c.n;
+    };
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t j = i - 1 + k;
+        if (w[k] == 0.f) continue;  // also the taps past the extension at x = n + 1 exactly
+        if (j == -1) {               // 3 f(0) - 3 f(1) + f(2), f(0) the pad
+            add(1, -3.f * w[k]);
+            add(2, w[k]);
+        } else if (j == n + 2) {     // 3 f(n+1) - 3 f(n) + f(n-1), f(n+1) the pad
+            add(n, -3.f * w[k]);
+            add(n - 1, w[k]);
+        } else {
+            add(j, w[k]);
+        }
+    }
+    return c;
+}
+
+__global__ void __launch_bounds__(256) k_affine3d_cubic(const float* __restrict__ vol, float* __restrict__ out,
+                                                        int64_t ny, int64_t nx, int64_t nz, int64_t oy, int64_t ox,
+                                                        Affine3 m) {
+    const int64_t col = blockIdx.x;
+    const int64_t c = col % ox, p = col / ox;
+    const double u = (double)(c + 1), w = (double)(p + 1);
+    const double bx = u * m.a[0][0] + w * m.a[2][0] + m.a[3][0];
+    const double by = u * m.a[0][1] + w * m.a[2][1] + m.a[3][1];
+    const double bz = u * m.a[0][2] + w * m.a[2][2] + m.a[3][2];
+    float* __restrict__ o = out + col * oy;
+    for (int64_t r = threadIdx.x; r < oy; r += blockDim.x) {
+        const double v = (double)(r + 1);
+        const double x = bx + v * m.a[1][0], y = by + v * m.a[1][1], z = bz + v * m.a[1][2];
+        float val = 0.f;
+        if (x >= 0.0 && x <= (double)(nx + 1) && y >= 0.0 && y <= (double)(ny + 1) && z >= 0.0 &&
+            z <= (double)(nz + 1)) {
+            const CubicTaps tx = cubic_taps(x, nx), ty = cubic_taps(y, ny), tz = cubic_taps(z, nz);
+            for (int kz = 0; kz < tz.n; ++kz) {
+                float sx = 0.f;
+                for (int kx = 0; kx < tx.n; ++kx) {
+                    const float* __restrict__ rowp = vol + ny * (tx.idx[kx] + nx * tz.idx[kz]);
+                    float sy = 0.f;
+                    for (int ky = 0; ky < ty.n; ++ky) sy += ty.w[ky] * rowp[ty.idx[ky]];
+                    sx += tx.w[kx] * sy;
+                }
+                val += tz.w[kz] * sx;
+            }
+        }
+        o[r] = val;
+    }
+}
+
+void launch_affine3d_cubic(const float* vol, float* out, const int64_t dims[3], const int64_t odims[3],
+                           const Affine3& m, hipStream_t s) {
+    for (int d = 0; d < 3; ++d)
+        if (dims[d] < 2) fail(PK_ERR_INVALID_ARG, "cubic interpolation needs at least 2 voxels along every axis");
+    const int64_t cols = odims[1] * odims[2];
+    if (cols <= 0 || odims[0] <= 0) return;
+    const unsigned nt = odims[0] >= 256 ? 256u : (unsigned)((odims[0] + 63) / 64 * 64);
+    if (cols * (int64_t)nt > 0xFFFFFFFFll) fail(PK_ERR_UNSUPPORTED_SIZE, "output volume has too many columns");
+    hipLaunchKernelGGL(k_affine3d_cubic, dim3((unsigned)cols), dim3(nt), 0, s, vol, out, dims[0], dims[1], dims[2],
+                       odims[0], odims[1], m);
+    PK_HIP(hipGetLastError());
+}
+
 }  // namespace pk

@@ -36,6 +36,8 @@ struct Affine3 {
 };
 void launch_affine3d(const float* vol, float* out, const int64_t dims[3], const int64_t odims[3], const Affine3& m,
                      hipStream_t s);
+void launch_affine3d_cubic(const float* vol, float* out, const int64_t dims[3], const int64_t odims[3],
+                           const Affine3& m, hipStream_t s);
 
 struct DsrGeom {
     double tmat[4][4];  // backward map over 0-based (dim1, dim2, dim3), column-vector form
@@ -295,6 +297,13 @@ __global__ void k_u16_to_f32(const uint16_t* __restrict__ in, float* __restrict_
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = (float)in[i];
 }
 
+// in place: the value uint16() of the single value would hold (the skewed-space
+// interpolation's uint16 output, which the cubic imwarp then reads)
+__global__ void k_round_u16(float* __restrict__ v, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) v[i] = (float)u16_round(v[i]);
+}
+
 __global__ void k_f32_to_u16(const float* __restrict__ in, uint16_t* __restrict__ out, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -312,7 +321,7 @@ void dsr_output_size(const int64_t dims[3], double angle, double dz, double xy, 
 static unsigned blocks_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 65536); }
 
 void pk_ctx_impl::deskew_rotate(const pk_vol* in, double angle, double dz, double xy, bool reverse, bool objective,
-                                double x_step_thresh, const int64_t* bbox, pk_vol* out) {
+                                double x_step_thresh, const int64_t* bbox, pk_vol* out, int interp) {
     if (!in || !out || !in->data || !out->data) fail(PK_ERR_INVALID_ARG, "null volume");
     if (in->mem != PK_MEM_DEVICE || out->mem != PK_MEM_DEVICE) fail(PK_ERR_INVALID_ARG, "device volumes expected");
     if (in->dtype != PK_UINT16 && in->dtype != PK_SINGLE) fail(PK_ERR_UNSUPPORTED_TYPE, "input class uint16 or single");
@@ -334,15 +343,39 @@ void pk_ctx_impl::deskew_rotate(const pk_vol* in, double angle, double dz, doubl
     const int64_t ny = in->dims[0], nx = in->dims[1], nz = in->dims[2];
     const int64_t on = od[0] * od[1] * od[2];
     const bool u16in = in->dtype == PK_UINT16, u16out = out->dtype == PK_UINT16;
-    if (!g.rows) {  // the general imwarp form (:210), trilinear over zero padding
-        if (bbox) fail(PK_ERR_INVALID_ARG, "bbox is served for stage scans only");
+    if (!g.rows || interp == 1) {
+        // the general imwarp form (:210, and :227-250 for 'cubic': the skewed-space
+        // interpolation first when the step is large), linear or cubic over zero
+        // padding; a bbox is imref3d's output window (:203-205), an offset of the
+        // output grid
+        const bool skew = interp == 1 && g.stepsize != 1.0;
+        const int64_t sz = skew ? g.ntotal : nz;
         const float* src = static_cast<const float*>(in->data);
         size_t need = 0;
-        if (u16in) need += sizeof(float) * (size_t)(ny * nx * nz);
+        if (u16in || skew) need += (sizeof(float) * (size_t)(ny * nx * sz) + 255) / 256 * 256;
         if (u16out) need += sizeof(float) * (size_t)on + 256;
         ensure(ws.dsr, need);
         char* p = ws.dsr.as<char>();
-        if (u16in) {
+        if (skew) {
+            float* f = reinterpret_cast<float*>(p);
+            const dim3 grid((unsigned)((ny + 255) / 256), (unsigned)(nx * sz));
+            if (nx * sz > 0x7FFFFFFF) fail(PK_ERR_UNSUPPORTED_SIZE, "too many rows");
+            if (u16in)
+                hipLaunchKernelGGL(k_skewed_interp<uint16_t>, grid, dim3(256), 0, stream,
+                                   static_cast<const uint16_t*>(in->data), f, ny, nx, nz, g.xstep, g.stepsize,
+                                   reverse ? 1 : 0);
+            else
+                hipLaunchKernelGGL(k_skewed_interp<float>, grid, dim3(256), 0, stream,
+                                   static_cast<const float*>(in->data), f, ny, nx, nz, g.xstep, g.stepsize,
+                                   reverse ? 1 : 0);
+            // the reference's interpolation returns uint16 for a uint16 volume with save16bit
+            // (:105), and the cubic imwarp reads that
+            if (u16in && u16out)
+                hipLaunchKernelGGL(k_round_u16, dim3(blocks_for(ny * nx * sz)), dim3(256), 0, stream, f,
+                                   ny * nx * sz);
+            src = f;
+            p += (sizeof(float) * (size_t)(ny * nx * sz) + 255) / 256 * 256;
+        } else if (u16in) {
             float* f = reinterpret_cast<float*>(p);
             hipLaunchKernelGGL(k_u16_to_f32, dim3(blocks_for(ny * nx * nz)), dim3(256), 0, stream,
                                static_cast<const uint16_t*>(in->data), f, ny * nx * nz);
@@ -350,9 +383,17 @@ void pk_ctx_impl::deskew_rotate(const pk_vol* in, double angle, double dz, doubl
             p += (sizeof(float) * (size_t)(ny * nx * nz) + 255) / 256 * 256;
         }
         float* dst = u16out ? reinterpret_cast<float*>(p) : static_cast<float*>(out->data);
-        Affine3 m;
-        if (!invert4(g.fwd, m.a)) fail(PK_ERR_INVALID_ARG, "singular transform");
-        launch_affine3d(src, dst, in->dims, od, m, stream);
+        Affine3 m, mi;
+        if (!invert4(g.fwd, mi.a)) fail(PK_ERR_INVALID_ARG, "singular transform");
+        // output voxel (r, c, p) of the window is world point (c + b2 - 1, r + b1 - 1, p + b3 - 1)
+        const double off[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0},
+                                  {(double)(b[1] - 1), (double)(b[0] - 1), (double)(b[2] - 1), 1}};
+        matmul4(off, mi.a, m.a);
+        const int64_t sd[3] = {ny, nx, sz};
+        if (interp == 1)
+            launch_affine3d_cubic(src, dst, sd, od, m, stream);
+        else
+            launch_affine3d(src, dst, sd, od, m, stream);
         if (u16out)
             hipLaunchKernelGGL(k_f32_to_u16, dim3(blocks_for(on)), dim3(256), 0, stream, dst,
                                static_cast<uint16_t*>(out->data), on);

@@ -185,9 +185,12 @@ struct Tune {
     // 5.19 -> 4.64 ms
     int xcd = 1;
     int slab = -1;           // PK_SLAB_PLANES: planes per slab (0 = off), -1 = built-in
-    // PK_CHUNK_STREAMS: streams of a batched step (1..8); measured on 64 chunks
-    // (OMW-2 / Biggs-10 Gvoxel/s): 4 -> 17.8 / 3.28, 6 -> 18.0 / 3.38, 8 -> 18.3 / 3.42
-    int streams = 8;
+    // PK_CHUNK_STREAMS: streams of a batched step (1..8).  Measured on 64 chunks
+    // (OMW-2 / Biggs-10 Gvoxel/s, one box, runs interleaved, twelve hardware
+    // queues): 1 -> 16.65 / 3.08, 2 -> 17.97 / 3.35, 4 -> 18.35-18.39 / 3.42,
+    // 6 -> 17.78 / 3.34, 8 -> 17.92-18.06 / 3.34-3.35; the served mix at 4
+    // streams had 5.6 % more capacity than at 8
+    int streams = 4;
     // PK_TUNE_COL_CANDS: strided-axis shapes the autotuner times per axis, best-ranked first
     // (64 chunks, OMW-2 Gvoxel/s: 4 -> 17.5, 8 -> 17.0)
     int col_cands = 4;

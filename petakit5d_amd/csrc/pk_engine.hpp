@@ -259,7 +259,7 @@ struct pk_ctx_impl {
     void run_col(const VolPlan& vp, int axis, float2* data, int mode, const float2* otf, float scale, int64_t z0 = 0,
                  int64_t nz = 0);
     int64_t slab_planes(const VolPlan& vp) const;
-    int chunk_streams() const;  // streams a batched step may use (PK_CHUNK_STREAMS, default 8)
+    int chunk_streams() const;  // streams a batched step may use (PK_CHUNK_STREAMS, default 4)
     std::string autotune(int64_t n1, int64_t n2, int64_t n3);
     void set_shapes(int64_t n1, int64_t n2, int64_t n3, const std::array<int, 8>& v);
     bool apply_shapes(VolPlan& vp, const std::array<int, 8>& v);
@@ -302,12 +302,16 @@ struct pk_ctx_impl {
     const uint8_t* edge_mask(const uint16_t* raw16, const float* rawf, int64_t n1, int64_t n2, int64_t n3, int E,
                              uint8_t* m1, uint8_t* m2, unsigned long long* cnt);
     void reset();
-    // deskewRotateFrame3D, linear, no resampling (pk_dsr.hip); bbox: 1-based
+    // deskewRotateFrame3D, no resampling (pk_dsr.hip); bbox: 1-based
     // inclusive output box or null; device volumes, returns when done
+    // interp: 0 linear, 1 cubic (imwarp's form, pk_deskew.hip)
     void deskew_rotate(const pk_vol* in, double angle, double dz, double xy, bool reverse, bool objective,
-                       double x_step_thresh, const int64_t* bbox, pk_vol* out);
+                       double x_step_thresh, const int64_t* bbox, pk_vol* out, int interp = 0);
     // bwdist(~bw) > E - 1 of bw = im > 0 with cleared faces (RLdecon_large_in_memory.m:150-155)
     void edge_erosion_mask(const pk_vol* im, int E, pk_vol* mask);
+    // copy of a pitched half spectrum with one pass's access pattern (pk_probe.hip): ms per copy
+    // (mean of reps) and the bytes it moves
+    double probe_copy(int pattern, const int64_t dims[3], int tile, int vals, int xcd, int reps, double* bytes);
     void decon_batch(int method, const pk_batch_item* items, int n, const float* psf_f, const int64_t fd[3],
                      const float* psf_b, const int64_t bd[3], int numit, const pk_opts* o);
     // step engine

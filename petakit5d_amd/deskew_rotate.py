@@ -5,7 +5,10 @@ Runs on the device through libpkdecon.so's pk_deskew_rotate_frame3d:
 skewed-space interpolation to a finer slice step when the slice-to-slice
 shift exceeds xStepThresh, then the row-preserving bilinear warp of the
 reference's fast path (volume_deskew_rotate_warp_mex.cpp); objective scans
-take imwarp's trilinear form.  Linear interpolation, no resampling."""
+take imwarp's trilinear form.  interpMethod 'cubic' leaves the fast path as
+the reference does: the skewed-space interpolation when the step is large,
+then imwarp's cubic convolution of the whole transform
+(pk_deskew_rotate_frame3d_cubic).  No resampling."""
 from __future__ import annotations
 
 import ctypes
@@ -43,11 +46,16 @@ def deskewRotateFrame3D(vol, angle, dz, xyPixelSize, reverse=False, Crop=True, b
     save16bit (the reference's mex does the same).  'Crop' is accepted and,
     as in the reference, has no effect; gpuProcess only moves the reference's
     data to its GPU and changes nothing here.  resampleFactor other than
-    [] / ones and 'cubic' take the reference's imwarp form, which this engine
-    does not serve (NotImplementedError)."""
+    [] / ones takes the reference's imwarp form with resampling, which this
+    engine does not serve (NotImplementedError).  With 'cubic' the output has
+    imwarp's class of what it resamples (deskewRotateFrame3D.m:98-110, 210):
+    uint16 for a uint16 volume that is not interpolated in skewed space, or
+    is with save16bit; single otherwise."""
+    import math
     import torch
-    if str(interpMethod).lower() != "linear":
-        raise NotImplementedError("deskewRotateFrame3D: only 'linear' interpolation is served")
+    interp = str(interpMethod).lower()
+    if interp not in ("linear", "cubic"):
+        raise ValueError("deskewRotateFrame3D: interpMethod must be 'linear' or 'cubic'")
     if resampleFactor is not None and np.size(resampleFactor) and np.any(np.asarray(resampleFactor) != 1):
         raise NotImplementedError("deskewRotateFrame3D: resampling is not served")
     on_dev = isinstance(vol, torch.Tensor) and vol.is_cuda
@@ -74,6 +82,9 @@ def deskewRotateFrame3D(vol, angle, dz, xyPixelSize, reverse=False, Crop=True, b
         box = (ctypes.c_int64 * 6)(*b)
         od = (b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1)
     u16 = t.dtype == torch.uint16 and bool(save16bit)
+    if interp == "cubic" and t.dtype == torch.uint16:
+        do_interp = not objectiveScan and abs(math.cos(angle * math.pi / 180) * dz / xyPixelSize) > xStepThresh
+        u16 = bool(save16bit) or not do_interp
     out = torch.empty((od[2], od[1], od[0]), dtype=torch.uint16 if u16 else torch.float32,
                       device=t.device).permute(2, 1, 0)
     vin = PkVol(ctypes.c_void_p(t.data_ptr()), PK_UINT16 if t.dtype == torch.uint16 else PK_SINGLE, PK_MEM_DEVICE,
@@ -82,10 +93,10 @@ def deskewRotateFrame3D(vol, angle, dz, xyPixelSize, reverse=False, Crop=True, b
                  (ctypes.c_int64 * 3)(*od))
     ctx = ctx or _lib.default_context()
     torch.cuda.current_stream(t.device).synchronize()
-    ctx.check(_lib.load().pk_deskew_rotate_frame3d(ctx.handle, ctypes.byref(vin), ctypes.c_double(angle),
-                                                   ctypes.c_double(dz), ctypes.c_double(xyPixelSize),
-                                                   int(bool(reverse)), int(bool(objectiveScan)),
-                                                   ctypes.c_double(xStepThresh), box, ctypes.byref(vout)))
+    fn = _lib.load().pk_deskew_rotate_frame3d_cubic if interp == "cubic" else _lib.load().pk_deskew_rotate_frame3d
+    ctx.check(fn(ctx.handle, ctypes.byref(vin), ctypes.c_double(angle), ctypes.c_double(dz),
+                 ctypes.c_double(xyPixelSize), int(bool(reverse)), int(bool(objectiveScan)),
+                 ctypes.c_double(xStepThresh), box, ctypes.byref(vout)))
     if on_dev:
         return out
     return np.asfortranarray(out.permute(2, 1, 0).cpu().numpy().transpose(2, 1, 0))

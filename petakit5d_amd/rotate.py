@@ -1,9 +1,9 @@
 """rotateFrame3D (pointDetection/llsmtools/transforms/rotateFrame3D.m:26-100):
 rotation of a deskewed (or objective-scan) volume about the y axis with the z
 anisotropy zxRatio, resampled on the device by libpkdecon.so's
-pk_affine_warp3d (imwarp 'linear', fill value 0).  The transform, the output
-size and the automatic crop follow the reference line by line; 'cubic' is not
-served."""
+pk_affine_warp3d (imwarp 'linear', fill value 0) or pk_affine_warp3d_cubic
+(imwarp 'cubic').  The transform, the output size and the automatic crop
+follow the reference line by line."""
 from __future__ import annotations
 
 import ctypes
@@ -82,11 +82,13 @@ def rotate_transform(shape, angle, zxRatio, reverse=False, Crop=True, resample=N
 def rotateFrame3D(vol, angle, zxRatio, reverse=False, Crop=True, resample=None, ObjectiveScan=False, outSize=None,
                   Interp="linear", ctx=None):
     """volout = rotateFrame3D(vol, angle, zxRatio, reverse, 'Crop', Crop,
-    'resample', resample, 'ObjectiveScan', ObjectiveScan, 'outSize', outSize):
-    numpy in -> column-major numpy out, CUDA tensor in -> CUDA tensor out."""
+    'resample', resample, 'ObjectiveScan', ObjectiveScan, 'outSize', outSize,
+    'Interp', Interp): numpy in -> column-major numpy out, CUDA tensor in ->
+    CUDA tensor out.  Interp 'linear' or 'cubic' (rotateFrame3D.m:40)."""
     import torch
-    if str(Interp).lower() != "linear":
-        raise NotImplementedError("rotateFrame3D: only 'linear' interpolation is served")
+    interp = str(Interp).lower()
+    if interp not in ("linear", "cubic"):
+        raise ValueError("rotateFrame3D: Interp must be 'linear' or 'cubic'")
     on_dev = isinstance(vol, torch.Tensor) and vol.is_cuda
     if on_dev:
         t = vol.to(torch.float32)
@@ -106,9 +108,9 @@ def rotateFrame3D(vol, angle, zxRatio, reverse=False, Crop=True, resample=None, 
     out = torch.empty((od[2], od[1], od[0]), dtype=torch.float32, device=t.device).permute(2, 1, 0)
     ctx = ctx or _lib.default_context()
     torch.cuda.current_stream(t.device).synchronize()
-    ctx.check(_lib.load().pk_affine_warp3d(ctx.handle, ctypes.c_void_p(t.data_ptr()), i64x3(d),
-                                           inv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), i64x3(od),
-                                           ctypes.c_void_p(out.data_ptr())))
+    fn = _lib.load().pk_affine_warp3d_cubic if interp == "cubic" else _lib.load().pk_affine_warp3d
+    ctx.check(fn(ctx.handle, ctypes.c_void_p(t.data_ptr()), i64x3(d), inv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                 i64x3(od), ctypes.c_void_p(out.data_ptr())))
     if on_dev:
         return out
     return np.asfortranarray(out.permute(2, 1, 0).cpu().numpy().transpose(2, 1, 0))

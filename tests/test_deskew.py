@@ -168,3 +168,89 @@ def test_dsr_equals_deskew_then_rotate_on_smooth_data():
         assert err < 0.05, (reverse, err)
         if not reverse:  # the unshifted two-step result is far off: the offset is real
             assert np.linalg.norm(dsr - D.general_warp(ds, T, od)) / np.linalg.norm(two) > 0.2
+
+
+# ---------------------------------------------------------------- 'cubic'
+# imwarp's 'cubic' restated as cubic convolution (Keys, a = -0.5) over the
+# volume padded with one zero voxel, Keys' boundary rule past the padded grid,
+# fill value 0 outside it.  Pinned by closed forms: the kernel's textbook
+# weights, exact reproduction of quadratics where every tap is a sample,
+# integer shifts that copy samples bit for bit, and the hand-derived edge
+# values of the boundary rule.
+
+def test_keys_weights_closed_forms():
+    assert D.keys_weights(0.0) == [0.0, 1.0, 0.0, 0.0]
+    assert D.keys_weights(0.5) == [-1 / 16, 9 / 16, 9 / 16, -1 / 16]  # the half-sample filter (-1, 9, 9, -1) / 16
+    for t in np.linspace(0, 0.999, 37):
+        w = D.keys_weights(float(t))
+        assert abs(sum(w) - 1) < 1e-14
+        for a, b, c in ((1.0, 2.0, 3.0), (-0.5, 0.3, 1.7)):
+            f = [a + b * x + c * x * x for x in (-1, 0, 1, 2)]
+            assert abs(sum(wk * fk for wk, fk in zip(w, f)) - (a + b * t + c * t * t)) < 1e-12
+
+
+def test_cubic_warp_integer_shift_and_quadratic():
+    v = np.random.default_rng(0).random((5, 6, 7))
+    inv = np.eye(4)
+    inv[3, :3] = [2, -1, 3]  # reads x + 2, y - 1, z + 3
+    r = D.cubic_warp(v, inv, (5, 6, 7))
+    exp = np.zeros_like(v)
+    exp[1:, :4, :4] = v[:4, 2:, 3:]
+    assert np.array_equal(r, exp)
+    # a quadratic, rotated about y: exact wherever all 64 taps are samples
+    Y, X, Z = np.meshgrid(np.arange(1, 21), np.arange(1, 23), np.arange(1, 19), indexing="ij")
+    quad = lambda x, y, z: 0.3 + 0.02 * x - 0.05 * y + 0.01 * z + 0.003 * x * x - 0.002 * x * z + 0.004 * y * z \
+        + 0.001 * y * y  # noqa: E731
+    th = 0.3
+    R = np.array([[math.cos(th), 0, -math.sin(th), 0], [0, 1, 0, 0], [math.sin(th), 0, math.cos(th), 0],
+                  [0, 0, 0, 1]])
+    T1, T2 = np.eye(4), np.eye(4)
+    T1[3, :3] = -np.array([11.5, 10.5, 9.5])
+    T2[3, :3] = [11.5, 10.5, 9.5]
+    inv = np.linalg.inv(T1 @ R @ T2)
+    r = D.cubic_warp(quad(X, Y, Z), inv, (20, 22, 18)).ravel()
+    p = np.stack([X.ravel(), Y.ravel(), Z.ravel(), np.ones(X.size)], 1) @ inv
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    inner = (x >= 2) & (x < 21) & (y >= 2) & (y < 19) & (z >= 2) & (z < 17)
+    assert inner.sum() > 4000
+    assert np.abs(r[inner] - quad(x, y, z)[inner]).max() < 1e-12
+
+
+def test_cubic_warp_edge_closed_forms():
+    """Along x at integer y, z: half a sample outside the data (x = 0.5) the
+    taps are f(-1) = -3 v1 + v2 (boundary rule), f(0) = 0 (pad), v1, v2, so the
+    value is 0.75 v1 - 0.125 v2; at x = n + 0.5 it is 0.75 vn - 0.125 v(n-1);
+    on the pad (x = 0, n + 1) it is 0, and beyond it the fill value 0."""
+    v = np.random.default_rng(1).random((4, 7, 3)) + 1
+    for xq, want in ((0.5, 0.75 * v[:, 0] - 0.125 * v[:, 1]), (7.5, 0.75 * v[:, 6] - 0.125 * v[:, 5]),
+                     (0.0, 0 * v[:, 0]), (8.0, 0 * v[:, 0]), (-0.25, 0 * v[:, 0]), (8.3, 0 * v[:, 0])):
+        inv = np.eye(4)
+        inv[0, 0] = 0.0
+        inv[3, 0] = xq  # every output column reads x = xq
+        r = D.cubic_warp(v, inv, (4, 1, 3))
+        np.testing.assert_allclose(r[:, 0, :], want, rtol=0, atol=1e-14)
+
+
+def test_dsr_cubic_matches_linear_on_smooth_data():
+    """The cubic and the linear deskew-rotate (independent resampling paths:
+    the general imwarp form vs the row warp) agree on a smooth volume up to
+    interpolation error, for a forward scan with skewed-space interpolation
+    and a reversed one without; the cubic form carries the reference's dx
+    offset only on its fast path, so forward scans are compared shifted."""
+    from scipy import ndimage as ndi
+    ny, nx, nz = 6, 80, 40
+    g = np.random.default_rng(5)
+    obj = np.zeros((ny, nx, nz))
+    obj[3, g.integers(20, 60, 10), g.integers(8, 32, 10)] = 1000.0
+    vol = ndi.gaussian_filter(obj, (1.0, 3.0, 3.0)).astype(np.float32)
+    for dz, reverse in ((0.5, False), (0.2, True)):
+        lin = D.deskew_rotate(vol, 32.45, dz, 0.108, reverse=reverse)
+        cub = D.deskew_rotate_cubic(vol, 32.45, dz, 0.108, reverse=reverse)
+        assert cub.shape == lin.shape
+        if not reverse:  # the fast path's forward-scan offset: dx of the (interpolated) slices along x
+            _, _, xstep, stepsize, _ = D.dsr_geometry(vol.shape, 32.45, dz, 0.108, reverse)
+            ref = D.deskew_rotate_cubic(ndi.shift(vol.astype(np.float64), (0, -xstep * stepsize, 0), order=3,
+                                                  mode="constant"), 32.45, dz, 0.108, reverse=reverse)
+            cub = ref
+        err = np.linalg.norm(cub - lin) / np.linalg.norm(lin)
+        assert err < 0.08, (dz, reverse, err)
