@@ -733,7 +733,23 @@ std::string pk_ctx_impl::autotune(int64_t n1, int64_t n2, int64_t n3) {
             first = false;
         }
     }
-    const float order_z = 2 * tf[0] + tc[1], order_y = 2 * tf[1] + tc[0];
+    // The pass order is decided on the two chains as the iteration runs them
+    // (other-axis forward, convolution, other-axis inverse, back to back): the
+    // sum of the kernels timed alone misjudged it (512x512x256: the dim-3
+    // convolution timed 0.198 ms alone and runs 0.174 ms after the dim-2
+    // forward pass).
+    auto chain = [&](int cax) {
+        VolPlan t = vp;
+        t.y = cax == 3 ? fwd[0] : conv[0];
+        t.z = cax == 3 ? conv[1] : fwd[1];
+        const int oax = cax == 3 ? 2 : 3;
+        return timed([&] {
+            run_col(t, oax, A, CM_FWD, nullptr, 1.f);
+            run_col(t, cax, A, CM_CONV, H, scale);
+            run_col(t, oax, A, CM_INV, nullptr, 1.f);
+        });
+    };
+    const float order_z = chain(3), order_y = chain(2);
     if (order_y < order_z * (1.f - kTuneMargin)) {  // the built-in order convolves along dim 3
         vp.conv_axis = 2;
         vp.y = conv[0];

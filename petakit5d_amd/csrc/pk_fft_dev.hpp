@@ -1067,15 +1067,26 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
     } else if constexpr (MODE == CM_FWD) {
         fft_run<E, false, false, MIXED>(v, pl, p, lds, c, T, act);
     } else {
-        // OTF values prefetched before the forward transform
+        // E <= 20: the OTF values are prefetched before the forward transform,
+        // which hides their latency behind it.  E >= 24: they are read after it,
+        // as each is used -- held through the transform they cost 2E registers,
+        // which spilled the E = 32 kernels (256 VGPRs) and left the others at two
+        // waves per SIMD.  Measured: 2048x2048x512's z_conv (E = 32) 3.62 ->
+        // 4.39 TB/s read late; 512x512x256's (E = 16) 0.174 -> 0.197 ms read late.
+        constexpr bool late = E >= 24;
         const float2* __restrict__ h0 = a.otf + (valid ? off0 : 0);
-        float2 h[E];
+        float2 h[late ? 1 : E];
+        if constexpr (!late) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) h[e] = valid ? h0[e * step] : make_float2(0.f, 0.f);
+            for (int e = 0; e < E; ++e) h[e] = valid ? h0[e * step] : make_float2(0.f, 0.f);
+        }
         fft_run<E, false, false, MIXED>(v, pl, p, lds, c, T, act);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const float2 hs = make_float2(h[e].x * a.scale, h[e].y * a.scale);
+            float2 he;
+            if constexpr (late) he = valid ? h0[e * step] : make_float2(0.f, 0.f);
+            else he = h[e];
+            const float2 hs = make_float2(he.x * a.scale, he.y * a.scale);
             v[e] = (MODE == CM_CONVJ) ? cmulc(v[e], hs) : cmul(v[e], hs);
         }
         fft_run<E, true, false, MIXED>(v, pl, p, lds, c, T, act);

@@ -69,3 +69,31 @@ def test_ninety_degrees_swaps_x_and_z():
     out = _warp_ref(vol, T, od)
     exp = np.transpose(vol, (0, 2, 1))[:, :, ::-1]
     np.testing.assert_allclose(out, exp, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("angle", [32.45, -20.0, 57.3])
+def test_transform_closed_forms_general_angle(angle):
+    """At a general angle: T maps the input centre ((n+1)/2 per axis) onto the
+    output centre; every corner of the input box lands inside the uncropped
+    output box; reverse=True is the x-mirror of the forward map on both sides
+    (R(-theta) = M R(theta) M with M the x mirror, which commutes with the z
+    scaling): T_rev(p) = mirror_out(T_fwd(mirror_in(p)))."""
+    shape, zx = (16, 90, 40), 1.7
+    ny, nx, nz = shape
+    T, od = rotate_transform(shape, angle, zx, Crop=False, ObjectiveScan=True)
+    cin = np.array([(nx + 1) / 2, (ny + 1) / 2, (nz + 1) / 2, 1.0])
+    cout = np.array([(od[1] + 1) / 2, (od[0] + 1) / 2, (od[2] + 1) / 2])
+    np.testing.assert_allclose((cin @ T)[:3], cout, rtol=0, atol=1e-9)
+    corners = np.array([[x, y, z, 1.0] for x in (0.5, nx + 0.5) for y in (0.5, ny + 0.5) for z in (0.5, nz + 0.5)])
+    q = corners @ T
+    lim = np.array([od[1], od[0], od[2]]) + 0.5
+    assert np.all(q[:, :3] >= 0.5 - 1.0) and np.all(q[:, :3] <= lim + 1.0)  # within rounding of the box
+    Tr, odr = rotate_transform(shape, angle, zx, reverse=True, Crop=False, ObjectiveScan=True, outSize=od)
+    assert odr == od
+    g = np.random.default_rng(1)
+    p = np.c_[g.uniform(1, nx, 50), g.uniform(1, ny, 50), g.uniform(1, nz, 50), np.ones(50)]
+    pm = p.copy()
+    pm[:, 0] = nx + 1 - p[:, 0]
+    fwd = pm @ T
+    fwd[:, 0] = od[1] + 1 - fwd[:, 0]
+    np.testing.assert_allclose(p @ Tr, fwd, rtol=0, atol=1e-9)
