@@ -104,11 +104,11 @@ def blosc_decode(raw: bytes) -> bytes:
     if flags & 0x2:  # plain copy
         return bytes(raw[16:16 + nbytes])
     if flags & 0x4:
-        raise NotImplementedError("blosc bit shuffle is not available in this build")
+        raise NotImplementedError("blosc bit shuffle is not supported by petakit5d_amd")
     code = flags >> 5
     if code not in (1, 3):
         name = {0: "blosclz", 2: "snappy", 4: "zstd"}.get(code, str(code))
-        raise NotImplementedError(f"blosc '{name}' streams are not available in this build (lz4, lz4hc, zlib are)")
+        raise NotImplementedError(f"blosc '{name}' streams are not supported by petakit5d_amd (lz4, lz4hc, zlib are)")
     if nbytes == 0:
         return b""
     if blocksize == 0:
@@ -143,7 +143,7 @@ def blosc_decode(raw: bytes) -> bytes:
 def blosc_encode(data: bytes, typesize: int, shuffle: int = 1, blocksize: int = 1 << 18) -> bytes:
     """Bytes -> one blosc 1 chunk with lz4 streams (byte shuffle when shuffle=1)."""
     if shuffle not in (0, 1):
-        raise NotImplementedError("blosc bit shuffle is not available in this build")
+        raise NotImplementedError("blosc bit shuffle is not supported by petakit5d_amd")
     L = _lz4()
     nbytes = len(data)
     typesize = max(1, int(typesize))
@@ -196,7 +196,7 @@ def _decode(raw, comp):
         return gzip.decompress(raw)
     if cid == "blosc":
         return blosc_decode(raw)
-    raise NotImplementedError(f"chunk compressor '{cid}' is not available in this build (none, zlib, gzip, blosc are)")
+    raise NotImplementedError(f"chunk compressor '{cid}' is not supported by petakit5d_amd (none, zlib, gzip, blosc are)")
 
 
 def _chunk_path(path, idx, sep):
