@@ -3,12 +3,12 @@ connected component, peak centring and axial resampling, as psf_gen_new.m does
 (decon/psf_gen_new.m:1-140).  Host code (numpy / scipy), run once per PSF.
 
 Toolbox steps are restated with scipy: medfilt3 (symmetric border padding) ->
-ndimage.median_filter(mode='reflect'); bwareaopen / bwlabeln with
-26-connectivity -> ndimage.label with a full 3x3x3 structure; imclose with
-strel('sphere', 3) -> binary dilation (zero border) then erosion (border
-padded with true, as imerode pads binary images).  Their exact tie and
-label-order rules are MATLAB's own; without MATLAB the result is pinned by
-the tests' closed forms and invariants only."""
+ndimage.median_filter(mode='reflect'), medfilt2 (zero padding) ->
+mode='constant'; bwareaopen / bwlabeln with 26-connectivity (8 in 2-D) ->
+ndimage.label with a full 3x3(x3) structure; imclose with strel('sphere', 3)
+or strel('disk', 3) (the decomposed disk, a 5 x 5 square) -> binary dilation
+(zero border) then erosion (border padded with true, as imerode pads binary
+images)."""
 from __future__ import annotations
 
 import numpy as np
@@ -34,10 +34,48 @@ def _sphere(r):
     return (x * x + y * y + z * z) <= r * r
 
 
-def _disk(r):
+def _medfilt3(a):
+    """medfilt3(a, [3 3 3]): 3x3x3 median, symmetric border padding (medfilt3's default)."""
+    return ndimage.median_filter(a, size=3, mode="reflect")
+
+
+def _medfilt2(a):
+    """medfilt2(a, [3 3]): 3x3 median, zero border padding (medfilt2's default)."""
+    return ndimage.median_filter(a, size=3, mode="constant", cval=0.0)
+
+
+def _strel_disk(r):
+    """Neighbourhood of strel('disk', r) with the default N = 4: the exact
+    disk x^2 + y^2 <= r^2 for r < 3; otherwise the Minkowski sum of the four
+    periodic lines along (1,0), (1,1), (0,1), (-1,1), each with 2 rp + 1
+    points, rp = floor(k / |v|), k = 2r / (cot(pi/8) + 1/sin(pi/8)) (Adams'
+    radial decomposition, as strel documents it), plus a horizontal and a
+    vertical line of length 2 (r - e - 1) + 1 when that is at least 3, e the
+    row extent of the sum (the compensation for its undersize).  r = 3 gives
+    a 5 x 5 square, r = 5 the documented 9 x 9 element."""
     g = np.arange(-r, r + 1)
-    y, x = np.meshgrid(g, g, indexing="ij")
-    return (x * x + y * y) <= r * r
+    if r < 3:
+        y, x = np.meshgrid(g, g, indexing="ij")
+        return (x * x + y * y) <= r * r
+    k = 2 * r / (1 / np.tan(np.pi / 8) + 1 / np.sin(np.pi / 8))
+    G = 2 * r + 1
+    nh = np.zeros((G, G), bool)
+    nh[r, r] = True
+    for vy, vx in ((1, 0), (1, 1), (0, 1), (-1, 1)):
+        rp = int(np.floor(k / np.hypot(vy, vx)))
+        line = np.zeros((2 * rp * abs(vy) + 1, 2 * rp * abs(vx) + 1), bool)
+        for i in range(-rp, rp + 1):
+            line[rp * abs(vy) + i * vy, rp * abs(vx) + i * vx] = True
+        nh = ndimage.binary_dilation(nh, structure=line)
+    rows = np.nonzero(nh.any(axis=1))[0]
+    e = int(rows.max() - r)
+    n = 2 * (r - e - 1) + 1
+    if n >= 3:
+        nh = ndimage.binary_dilation(nh, structure=np.ones((1, n), bool))
+        nh = ndimage.binary_dilation(nh, structure=np.ones((n, 1), bool))
+    ys, xs = np.nonzero(nh)
+    R = int(max(np.abs(ys - r).max(), np.abs(xs - r).max()))
+    return nh[r - R:r + R + 1, r - R:r + R + 1]
 
 
 def _component_mask(score, peak_map, min_size, conn_full, close_se):
@@ -77,11 +115,11 @@ def psf_gen_new(psf, dz_psf, dz_data, medFactor=1.5, PSFGenMethod="masked"):
             if method == "median":
                 psf_raw = psf_raw - np.float32(medFactor) * np.float32(np.median(fl[fl > 0]))
                 psf_raw = np.maximum(psf_raw, 0)
-                med = ndimage.median_filter(psf_raw.astype(np.float64), size=3, mode="reflect")
+                med = _medfilt3(psf_raw.astype(np.float64))
                 bw = _component_mask(med, med, 300, True, _sphere(3))
                 psf_raw = psf_raw * bw
             elif method == "masked":
-                med = ndimage.median_filter(psf, size=3, mode="reflect")
+                med = _medfilt3(psf)
                 edge = med[ends, :, :]
                 a = np.sqrt(np.abs(edge - np.float32(100))).max(axis=0, keepdims=True) * np.float32(3) + \
                     edge.mean(axis=0, keepdims=True, dtype=np.float32)
@@ -94,17 +132,16 @@ def psf_gen_new(psf, dz_psf, dz_data, medFactor=1.5, PSFGenMethod="masked"):
     else:  # 2-D PSF (psf_gen_new.m:58-86)
         p2 = psf_raw[:, :, 0]
         if method == "median":
-            p2 = p2 - np.float32(medFactor) * np.float32(np.median(p2[p2 > 0]))
-            p2 = np.maximum(p2, 0)
-            med = ndimage.median_filter(p2, size=3, mode="reflect")
-            bw = _component_mask(med, med, 30, True, _disk(3))
-            p2 = p2 * bw
+            # psf_gen_new.m:71 filters with medfilt(psf_raw, [3, 3]), a function MATLAB does not
+            # define (medfilt2 / medfilt3 exist), so the reference stops here with an error
+            raise ValueError("psf_gen_new: the 2-D 'median' method calls medfilt, which is undefined in the "
+                             "reference (psf_gen_new.m:71); use 'masked'")
         elif method == "masked":
-            med = ndimage.median_filter(psf[:, :, 0], size=3, mode="reflect")
+            med = _medfilt2(psf[:, :, 0])
             edge = med[ends, :]
             a = np.sqrt(edge).max(axis=0, keepdims=True) * np.float32(3) + edge.mean(axis=0, keepdims=True,
                                                                                   dtype=np.float32)
-            bw = _component_mask(med - a, med, 50, True, _disk(3))
+            bw = _component_mask(med - a, med, 50, True, _strel_disk(3))
             bg = psf_raw[ends, :, :].mean(axis=(0, 2), dtype=np.float32)[None, :]
             p2 = p2 * bw - bg
             p2[p2 < 0] = 0
