@@ -120,7 +120,9 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
                    dampFactor=1.0, scaleFactor=None, deconOffset=0.0, deconBbox=None, EdgeErosion=0,
                    psfgenPath=None, psfFsn=None, psfGen=False, dzPSF=None, dz=None, Deskew=False, SkewAngle=32.45,
                    xyPixelSize=0.108, Reverse=False, Crop=False, objectiveScan=False, Rotate=False,
-                   DSRCombined=True, xStepThresh=2.0, Resample=None, ctx=None):
+                   DSRCombined=True, xStepThresh=2.0, Resample=None, nTapering=0, interpMethod="linear",
+                   fixIter=True, errThresh=None, ErodeMaskfile=None, debug=False, saveStep=5, debug_folder=None,
+                   useGPU=True, ctx=None):
     """deconvolved = RLdecon(..., 'rawdata', rawdata, ...) for one volume
     (rawdata an array, or a .tif / .zarr path; psf an array or a .tif path).
 
@@ -150,9 +152,26 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     'save16bit', save16bit) instead and return ds = None (:488-493).
     objectiveScan turns Deskew off (:147-149); Rotate of a stage scan without
     Deskew produces nothing in the reference and is an error here.  Other
-    methods ('original' deconvlucy, 'cudagen') are not this engine's."""
+    methods ('original' deconvlucy, 'cudagen') are not this engine's.
+
+    ErodeMaskfile (:364-413): with EdgeErosion > 0 and an existing Zarr mask
+    the decon runs without erosion and its result is multiplied by mask > 0.
+    debug / saveStep (:374-380) go to the decon functions, with saveStep =
+    min(saveStep, nIter) and the iteration files in debug_folder (the
+    reference's deconPath/<name>_debug/; a temporary folder when not given).
+    fixIter and errThresh are parsed and unused, as in the reference
+    (:84-85).  interpMethod: the reference validates it with @isnumeric
+    (:26), so only the default is usable and a method name is rejected.
+    nTapering > 0 (edgetaper of each slice, :352-357) is not served."""
     if RLMethod not in ("simplified", "omw"):
         raise ValueError(f"RLMethod '{RLMethod}': this engine serves 'simplified' and 'omw'")
+    if str(interpMethod) != "linear":
+        raise ValueError("interpMethod: RLdecon.m:26 validates it with @isnumeric, so a method name is rejected; "
+                         "call deskewFrame3D / rotateFrame3D / deskewRotateFrame3D for 'cubic'")
+    if nTapering and nTapering > 0:
+        raise NotImplementedError("nTapering > 0 (edgetaper of each slice, RLdecon.m:352-357) is not served")
+    if not useGPU:
+        raise ValueError("useGPU=false: this engine implements the GPU branch only")
     SkewAngle = abs(float(SkewAngle))  # RLdecon.m:101
     if objectiveScan:
         Deskew = False
@@ -165,7 +184,8 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
         J = RLdecon_volume(rawdata, psf, nIter, RLMethod, Background, wienerAlpha, OTFCumThresh, hannWinBounds,
                            skewed, flipZstack, save16bit, dampFactor, scaleFactor, deconOffset, deconBbox,
                            EdgeErosion, psfgenPath, psfFsn, psfGen, dzPSF, dz, SkewAngle=SkewAngle,
-                           objectiveScan=objectiveScan, ctx=ctx)
+                           objectiveScan=objectiveScan, ErodeMaskfile=ErodeMaskfile, debug=debug, saveStep=saveStep,
+                           debug_folder=debug_folder, ctx=ctx)
         from .deskew import deskewFrame3D
         from .rotate import rotateFrame3D
         zAniso = psfgen_dz_ratio(SkewAngle, objectiveScan) * dz / xyPixelSize
@@ -225,12 +245,31 @@ def RLdecon_volume(rawdata, psf, nIter, RLMethod="simplified", Background=0.0, w
     if scaleFactor is None:
         scaleFactor = 1.0
     psf = np.asfortranarray(psf / np.float32(psf.sum(dtype=np.float32)))
+    use_mask = bool(EdgeErosion and EdgeErosion > 0 and ErodeMaskfile and os.path.exists(str(ErodeMaskfile)))
     kw = dict(Background=Background, save16bit=save16bit, dampFactor=dampFactor, scaleFactor=scaleFactor,
-              deconOffset=deconOffset, EdgeErosion=EdgeErosion, pad=True, ctx=ctx)
+              deconOffset=deconOffset, EdgeErosion=0 if use_mask else EdgeErosion, pad=True, ctx=ctx)
+    if debug:
+        import tempfile
+        kw.update(debug=True, saveStep=min(int(saveStep), int(nIter)),
+                  debug_folder=debug_folder or tempfile.mkdtemp(prefix="RLdecon_debug_"))
     if deconBbox is not None and len(deconBbox) == 6:
         kw["bbox"] = list(deconBbox)
     if RLMethod == "simplified":
         J, _, _ = decon_lucy_function(rawdata, psf, int(nIter), **kw)
     else:
         J, _ = decon_lucy_omw_function(rawdata, psf, psf_b, int(nIter), **kw)
+    if use_mask:  # deconvolved .* cast(mask > 0, class(deconvolved)) (RLdecon.m:409-413)
+        from .zarr_io import read_zarr
+        keep = np.asarray(read_zarr(str(ErodeMaskfile))) > 0
+        if keep.shape != tuple(J.shape):
+            raise ValueError(f"ErodeMaskfile holds {keep.shape}, the deconvolved volume is {tuple(J.shape)}")
+        if _is_torch(J):
+            import torch
+            k = torch.from_numpy(np.ascontiguousarray(keep.transpose(2, 1, 0))).to(J.device).permute(2, 1, 0)
+            if J.dtype == torch.uint16:  # same bits as int16; uint16 has no masked_fill
+                J.view(torch.int16).masked_fill_(~k, 0)
+            else:
+                J = J * k.to(J.dtype)
+        else:
+            J = np.asfortranarray(J * keep.astype(J.dtype))
     return J

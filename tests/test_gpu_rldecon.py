@@ -174,3 +174,34 @@ def test_rotate_negative_angle_and_device_tensor():
     np.testing.assert_array_equal(Jt.cpu().numpy(), Jp)
     np.testing.assert_array_equal(dst.cpu().numpy(), dsp)
     np.testing.assert_array_equal(dsrt.cpu().numpy(), dsrp)
+
+
+def test_erode_maskfile_and_debug(tmp_path):
+    """ErodeMaskfile (RLdecon.m:364-413): with EdgeErosion > 0 and an existing
+    Zarr mask the decon runs without erosion and is multiplied by mask > 0, on
+    host and device inputs.  debug (:374-380) reaches the decon function with
+    saveStep = min(saveStep, nIter) and writes the iteration files."""
+    import torch
+    import petakit5d_amd as pk
+    from petakit5d_amd.zarr_io import write_zarr
+    raw, psf = _case()
+    mask = np.zeros(raw.shape, np.uint8)
+    mask[5:50, 4:40, 2:16] = 1
+    mf = str(tmp_path / "mask.zarr")
+    write_zarr(mf, mask, chunks=(32, 32, 16))
+    J = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=True, EdgeErosion=8, ErodeMaskfile=mf)
+    J0 = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=True, EdgeErosion=0)
+    np.testing.assert_array_equal(J, J0 * mask.astype(np.uint16))
+    rt = torch.from_numpy(np.ascontiguousarray(raw.transpose(2, 1, 0))).cuda().permute(2, 1, 0)
+    Jt = pk.RLdecon_volume(rt, psf, 3, Background=100, save16bit=True, EdgeErosion=8, ErodeMaskfile=mf)
+    assert Jt.is_cuda
+    np.testing.assert_array_equal(Jt.cpu().numpy(), J)
+    # without the mask file the edge erosion itself runs
+    Je = pk.RLdecon_volume(raw, psf, 3, Background=100, save16bit=True, EdgeErosion=8,
+                           ErodeMaskfile=str(tmp_path / "absent.zarr"))
+    assert not np.array_equal(Je, J0)
+    dbg = tmp_path / "dbg"
+    Jd = pk.RLdecon_volume(raw, psf, 6, Background=100, save16bit=True, debug=True, saveStep=2,
+                           debug_folder=str(dbg))
+    np.testing.assert_array_equal(Jd, pk.RLdecon_volume(raw, psf, 6, Background=100, save16bit=True))
+    assert sorted(os.listdir(dbg)) == ["Iter_0002.tif", "Iter_0004.tif", "Iter_0006.tif"]

@@ -45,3 +45,18 @@ def test_psfgen_dz_ratio():
     assert psfgen_dz_ratio(-32.45) == psfgen_dz_ratio(32.45)
     assert psfgen_dz_ratio(32.45, objectiveScan=True) == 1.0
     assert psfgen_dz_ratio(90) == 1.0 and psfgen_dz_ratio(0) == 0.0
+
+
+def test_option_rules_of_the_reference():
+    """interpMethod is validated with @isnumeric in RLdecon.m:26, so a method
+    name is rejected; nTapering > 0 (edgetaper, :352-357) is not served;
+    useGPU=false has no branch here.  All raise before any device work."""
+    import petakit5d_amd as pk
+    raw = np.ones((8, 8, 4), np.uint16)
+    psf = np.ones((3, 3, 3), np.float32)
+    with pytest.raises(ValueError, match="isnumeric"):
+        pk.RLdecon_volume(raw, psf, 2, interpMethod="cubic")
+    with pytest.raises(NotImplementedError, match="nTapering"):
+        pk.RLdecon_volume(raw, psf, 2, nTapering=5)
+    with pytest.raises(ValueError, match="useGPU"):
+        pk.RLdecon_volume(raw, psf, 2, useGPU=False)
