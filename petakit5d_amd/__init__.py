@@ -7,7 +7,7 @@ reference's MATLAB call signatures.
 from ._lib import Context, PkDeconError, default_context, load  # noqa: F401
 from .backprojector import BackProjector, omw_backprojector_generation  # noqa: F401
 from .batch import decon_batch  # noqa: F401
-from .chunks import XR_zarrChunkCoordinatesExtraction, decon_large_in_memory  # noqa: F401
+from .chunks import XR_zarrChunkCoordinatesExtraction, decon_large_in_memory, edge_erosion_mask  # noqa: F401
 from .psfgen import psf_gen_new  # noqa: F401
 from .rldecon import RLdecon_volume  # noqa: F401
 from .tiff_io import readtiff, writetiff  # noqa: F401
@@ -40,6 +40,7 @@ __all__ = [
     "decon_batch",
     "XR_zarrChunkCoordinatesExtraction",
     "decon_large_in_memory",
+    "edge_erosion_mask",
     "RLdecon_volume",
     "psf_gen_new",
     "read_zarr",

@@ -423,3 +423,38 @@ def test_debug_rl_needs_three_save_steps(tmp_path):
     I, psf = _case((32, 32, 16), (7, 7, 9), seed=24)
     with pytest.raises(RuntimeError, match="istp"):
         pk.decon_lucy_function(I, psf, 2, Background=100, debug=True, debug_folder=str(tmp_path / "d"))
+
+
+@pytest.mark.parametrize("shape,psf_shape,kind", [
+    ((15, 6, 8), (1, 1, 1), "delta"),     # a one-voxel PSF on a small volume of odd 15-point rows
+    ((72, 12, 10), (5, 5, 3), "zeros"),   # an all-zero image: I ./ max(CX, eps) keeps J at 0
+    ((45, 6, 1), (3, 3, 1), "ramp"),      # a single plane of odd rows
+])
+def test_small_and_degenerate_inputs_vs_oracle(oracle, shape, psf_shape, kind):
+    """Edge cases of the decon functions against the oracle, RL (Biggs, 5
+    iterations) and OMW (2 iterations), at row and column lengths whose
+    kernels the other tests run: a one-voxel PSF (OTF = 1, so J stays
+    max(I - bg, 0)), an all-zero image, and a single plane."""
+    import petakit5d_amd as pk
+    print(f"[edge] {shape} psf {psf_shape} {kind}", flush=True)
+    n = int(np.prod(shape))
+    if kind == "zeros":
+        I = np.zeros(shape, np.uint16)
+    else:
+        I = np.asfortranarray((np.arange(n).reshape(shape, order="F") * 37 % 500 + 120).astype(np.uint16))
+    psf = np.ones(psf_shape, np.float32) if kind == "delta" else synth.gaussian_psf(psf_shape, 0.8, 0.8)
+    pb = _flip(psf)
+    for method, it in (("rl", 5), ("omw", 2)):
+        if method == "rl":
+            J, _, _ = pk.decon_lucy_function(I, psf, it, Background=100, save16bit=False)
+            R, _, _ = oracle.decon_lucy_function(I, psf, it, Background=100, save16bit=False)
+        else:
+            J, _ = pk.decon_lucy_omw_function(I, psf, pb, it, Background=100, save16bit=False)
+            R, _ = oracle.decon_lucy_omw_function(I, psf, pb, it, Background=100, save16bit=False)
+        assert J.shape == shape and np.isfinite(J).all()
+        if kind == "zeros":
+            assert not J.any() and not R.any()
+        else:
+            assert rel_l2(J, R) <= TOL, (method, rel_l2(J, R))
+        if kind == "delta":  # OTF = 1: every iteration keeps J = max(I - bg, 0)
+            np.testing.assert_allclose(J, np.maximum(I.astype(np.float32) - 100, 0), rtol=1e-6, atol=1e-4)
