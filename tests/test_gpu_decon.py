@@ -436,6 +436,7 @@ def test_small_and_degenerate_inputs_vs_oracle(oracle, shape, psf_shape, kind):
     kernels the other tests run: a one-voxel PSF (OTF = 1, so J stays
     max(I - bg, 0)), an all-zero image, and a single plane."""
     import petakit5d_amd as pk
+    from petakit5d_amd import synth
     print(f"[edge] {shape} psf {psf_shape} {kind}", flush=True)
     n = int(np.prod(shape))
     if kind == "zeros":
