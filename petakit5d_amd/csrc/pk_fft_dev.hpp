@@ -450,6 +450,60 @@ struct XArgs {
     int64_t s0, s1, s2, pn2;
 };
 
+// Cache policy of the passes' streaming accesses (compile-time).  A
+// nontemporal access (the `nt` bit) tells the caches not to keep the line for
+// reuse: every pass touches each value once, and the next pass reads it back
+// only after the whole volume has gone through.
+//   PK_NT_COL: bit 0 strided-pass spectrum loads, bit 1 their stores, bit 2 OTF loads
+//   PK_NT_ROW: bit 0 row-pass spectrum loads, bit 1 their stores, bit 2 real
+//              operand loads, bit 3 real stores
+// Default: nontemporal stores in the strided passes.  Measured against all
+// accesses temporal (ms per call / Gvoxel/s): 512x512x256 OMW-2 2.578 ->
+// 2.546; 64 chunks OMW-2 18.33 -> 18.71, Biggs-10 3.39 -> 3.52;
+// 2048x2048x512 unchanged (23.45 -> 23.55, 4.17 -> 4.16).  Nontemporal loads
+// as well (7): 2.587 ms.  Stores in every pass (PK_NT_COL 2, PK_NT_ROW 10) or
+// every access (7, 15): 2.584 ms -- the row passes gain and lose in turn.
+#ifndef PK_NT_COL
+#define PK_NT_COL 2
+#endif
+#ifndef PK_NT_ROW
+#define PK_NT_ROW 0
+#endif
+typedef float pk_vf2 __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ float2 ld_f2(const float2* p) {
+    if constexpr (NT) {
+        const pk_vf2 v = __builtin_nontemporal_load(reinterpret_cast<const pk_vf2*>(p));
+        return make_float2(v.x, v.y);
+    } else {
+        return *p;
+    }
+}
+template <bool NT>
+__device__ __forceinline__ void st_f2(float2* p, float2 v) {
+    if constexpr (NT) {
+        pk_vf2 w;
+        w.x = v.x;
+        w.y = v.y;
+        __builtin_nontemporal_store(w, reinterpret_cast<pk_vf2*>(p));
+    } else {
+        *p = v;
+    }
+}
+template <bool NT>
+__device__ __forceinline__ float ld_f1(const float* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st_f1(float* p, float v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+constexpr bool kNtColLd = (PK_NT_COL & 1) != 0, kNtColSt = (PK_NT_COL & 2) != 0, kNtOtfLd = (PK_NT_COL & 4) != 0;
+constexpr bool kNtRowLd = (PK_NT_ROW & 1) != 0, kNtRowSt = (PK_NT_ROW & 2) != 0;
+constexpr bool kNtRealLd = (PK_NT_ROW & 4) != 0, kNtRealSt = (PK_NT_ROW & 8) != 0;
+
 __device__ __forceinline__ float fmax0(float a) { return a > 0.f ? a : 0.f; }  // max(a,0), NaN -> 0
 __device__ __forceinline__ float2 fmax0(float2 a) { return make_float2(fmax0(a.x), fmax0(a.y)); }
 
@@ -799,12 +853,12 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
     const int64_t r2 = rowc * L;  // real row start: float2 pairs (even rows) or floats (ODD)
     // real-space value(s) m of the row: a pair, or one value in .x (ODD)
     auto ldr = [&](const float* __restrict__ base, int m) -> float2 {
-        if constexpr (ODD) return make_float2(base[r2 + m], 0.f);
-        else return reinterpret_cast<const float2*>(base)[r2 + m];
+        if constexpr (ODD) return make_float2(ld_f1<kNtRealLd>(base + r2 + m), 0.f);
+        else return ld_f2<kNtRealLd>(reinterpret_cast<const float2*>(base) + r2 + m);
     };
     auto str = [&](float* __restrict__ base, int m, float2 o) {
-        if constexpr (ODD) base[r2 + m] = o.x;
-        else reinterpret_cast<float2*>(base)[r2 + m] = o;
+        if constexpr (ODD) st_f1<kNtRealSt>(base + r2 + m, o.x);
+        else st_f2<kNtRealSt>(reinterpret_cast<float2*>(base) + r2 + m, o);
     };
 
     RawRow rr_src{0, false};
@@ -853,8 +907,8 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int m = p + e * P;
-            if constexpr (ODD) v[e] = m < H1 ? sp[m] : cconj(sp[L - m]);  // the Hermitian half supplies the rest
-            else v[e] = c2r_pre(sp[m], cconj(sp[L - m]), rt[e * P]);
+            if constexpr (ODD) v[e] = m < H1 ? ld_f2<kNtRowLd>(sp + m) : cconj(ld_f2<kNtRowLd>(sp + (L - m)));  // the Hermitian half supplies the rest
+            else v[e] = c2r_pre(ld_f2<kNtRowLd>(sp + m), cconj(ld_f2<kNtRowLd>(sp + (L - m))), rt[e * P]);
         }
         ct_fft<L, E, true>(v, p, lds, lbase, tw);
         if constexpr (kXrowLateOperands) load_operands();
@@ -919,7 +973,7 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
         if (valid) {
 #pragma unroll
             for (int e = 0; e < E; ++e)
-                if (p + e * P < H1) sp[p + e * P] = v[e];
+                if (p + e * P < H1) st_f2<kNtRowSt>(sp + (p + e * P), v[e]);
         }
     } else if (out_spec) {
         ct_fft<L, E, false>(v, p, lds, lbase, tw);
@@ -937,8 +991,8 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
         const float2 z0 = lds[lbase];
         if (valid) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) sp[p + e * P] = X[e];
-            if (p == 0) sp[L] = make_float2(z0.x - z0.y, 0.f);
+            for (int e = 0; e < E; ++e) st_f2<kNtRowSt>(sp + (p + e * P), X[e]);
+            if (p == 0) st_f2<kNtRowSt>(sp + L, make_float2(z0.x - z0.y, 0.f));
         }
     }
 
@@ -1061,7 +1115,7 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
     float2* __restrict__ ptr0 = a.data + (valid ? off0 : 0);
     float2 v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = valid ? ptr0[e * step] : make_float2(0.f, 0.f);
+    for (int e = 0; e < E; ++e) v[e] = valid ? ld_f2<kNtColLd>(ptr0 + e * step) : make_float2(0.f, 0.f);
     if constexpr (MODE == CM_INV) {
         fft_run<E, true, false, MIXED>(v, pl, p, lds, c, T, act);
     } else if constexpr (MODE == CM_FWD) {
@@ -1078,13 +1132,13 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
         float2 h[late ? 1 : E];
         if constexpr (!late) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) h[e] = valid ? h0[e * step] : make_float2(0.f, 0.f);
+            for (int e = 0; e < E; ++e) h[e] = valid ? ld_f2<kNtOtfLd>(h0 + e * step) : make_float2(0.f, 0.f);
         }
         fft_run<E, false, false, MIXED>(v, pl, p, lds, c, T, act);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             float2 he;
-            if constexpr (late) he = valid ? h0[e * step] : make_float2(0.f, 0.f);
+            if constexpr (late) he = valid ? ld_f2<kNtOtfLd>(h0 + e * step) : make_float2(0.f, 0.f);
             else he = h[e];
             const float2 hs = make_float2(he.x * a.scale, he.y * a.scale);
             v[e] = (MODE == CM_CONVJ) ? cmulc(v[e], hs) : cmul(v[e], hs);
@@ -1093,7 +1147,7 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
     }
     if (valid) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) ptr0[e * step] = v[e];
+        for (int e = 0; e < E; ++e) st_f2<kNtColSt>(ptr0 + e * step, v[e]);
     }
 }
 

@@ -15,8 +15,10 @@
 //     the second read is kept).
 // A column tile is moved as the passes move it: block = T columns x P
 // threads, thread (c, p) holds the E points at rows p + e P of its column in
-// registers, all loads before all stores.  The block-to-tile order can be the
-// passes' XCD-aware one.  Timed with HIP events on the context's stream.
+// registers, all loads before all stores, with the strided passes' cache
+// policy (PK_NT_COL: nontemporal stores by default).  The block-to-tile order
+// can be the passes' XCD-aware one.  The sequential copy is a plain copy
+// (temporal loads and stores).  Timed with HIP events on the context's stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,7 +60,7 @@ __global__ void __launch_bounds__(1024) k_probe_cols(const float2* __restrict__ 
         for (int e = 0; e < E; ++e) v[e] = make_float2(v[e].x + h[e].x, v[e].y + h[e].y);
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) dst[off0 + e * step] = v[e];
+    for (int e = 0; e < E; ++e) st_f2<kNtColSt>(dst + off0 + e * step, v[e]);
 }
 
 __global__ void k_probe_seq(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
