@@ -324,12 +324,18 @@ int32_t pk_profile_read(pk_ctx *ctx, pk_prof_entry *out, int32_t max);
  * adjacent columns, column stride = row pitch); 2: the dim-3 column pass
  * (column stride = plane pitch); 3: pattern 2 with a second input read at the
  * same points (the OTF convolution's traffic).  Each thread moves vals_per_thread points
- * (4, 8, 16 or 32; the column length must be a multiple) of one column;
- * xcd_order = 1 takes the passes' XCD-aware block order.  *ms = mean time of
- * reps copies (HIP events on the context's stream), *bytes = bytes read plus
+ * (4, 8, 16 or 32; the column length must be a multiple) of one column.
+ * Column patterns store with the strided passes' cache policy (nontemporal
+ * stores for tiles of 16 or more columns by default); the sequential copy is
+ * a plain one.  flags: bit 0 takes
+ * the passes' XCD-aware block order; bit 1 runs each copy cold -- a 1 GiB
+ * device buffer (four times the memory-side cache) is read before it,
+ * outside the timed interval -- instead of back to back, where a copy can
+ * read what the previous one left in that cache.  *ms = mean time of reps
+ * copies (HIP events on the context's stream), *bytes = bytes read plus
  * written per copy (valid columns only).  Uses the context's workspace. */
 pk_status pk_probe_copy(pk_ctx *ctx, int32_t pattern, const int64_t dims[3], int32_t tile, int32_t vals_per_thread,
-                        int32_t xcd_order, int32_t reps, double *ms, double *bytes);
+                        int32_t flags, int32_t reps, double *ms, double *bytes);
 
 /* findGoodFactorNumber(givenNum, directionFlag, allowOddSeven) (utils/findGoodFactorNumber.m:1-26).
  * max_prime = 7 is the reference rule; 5 gives the 2/3/5-smooth sizes the GPU transforms need. */

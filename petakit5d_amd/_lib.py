@@ -259,12 +259,16 @@ class Context:
         return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms), float(buf[i].bytes_per_launch))
                 for i in range(n)}
 
-    def probe_copy(self, pattern: int, dims, tile: int = 16, vals: int = 16, xcd: bool = True, reps: int = 20):
+    def probe_copy(self, pattern: int, dims, tile: int = 16, vals: int = 16, xcd: bool = True, reps: int = 20,
+                   cold: bool = False):
         """(ms per copy, bytes per copy) of pk_probe_copy: pattern 0 sequential,
-        1 dim-2 columns, 2 dim-3 columns of a dims-sized transform's spectrum."""
+        1 dim-2 columns, 2 dim-3 columns, 3 dim-3 columns with a second input
+        of a dims-sized transform's spectrum; cold: each copy after the
+        memory-side cache is flushed (1 GiB read), else back to back."""
         ms, nb = ctypes.c_double(), ctypes.c_double()
+        flags = (1 if xcd else 0) | (2 if cold else 0)
         self.check(self._L.pk_probe_copy(self.handle, int(pattern), i64x3(tuple(int(x) for x in dims)), int(tile),
-                                         int(vals), 1 if xcd else 0, int(reps), ctypes.byref(ms), ctypes.byref(nb)))
+                                         int(vals), flags, int(reps), ctypes.byref(ms), ctypes.byref(nb)))
         return ms.value, nb.value
 
     def device_bytes(self) -> int:

@@ -402,13 +402,14 @@ pk_status pk_profile_enable(pk_ctx* c, int32_t enable) {
 }
 
 pk_status pk_probe_copy(pk_ctx* c, int32_t pattern, const int64_t dims[3], int32_t tile, int32_t vals_per_thread,
-                        int32_t xcd_order, int32_t reps, double* ms, double* bytes) {
+                        int32_t flags, int32_t reps, double* ms, double* bytes) {
     return guarded(c, [&] {
         if (!dims || !ms || !bytes) pk::fail(PK_ERR_INVALID_ARG, "null argument");
         if (pattern < 0 || pattern > 3 || reps < 1) pk::fail(PK_ERR_INVALID_ARG, "pattern 0..3, reps >= 1");
         for (int d = 0; d < 3; ++d)
             if (dims[d] < 2) pk::fail(PK_ERR_INVALID_ARG, "probe dimensions must be >= 2");
-        *ms = c->probe_copy(pattern, dims, tile, vals_per_thread, xcd_order, reps, bytes);
+        if (flags & ~3) pk::fail(PK_ERR_INVALID_ARG, "probe flags: bit 0 XCD order, bit 1 cold");
+        *ms = c->probe_copy(pattern, dims, tile, vals_per_thread, flags, reps, bytes);
     });
 }
 

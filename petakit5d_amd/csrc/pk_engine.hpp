@@ -311,7 +311,8 @@ struct pk_ctx_impl {
     void edge_erosion_mask(const pk_vol* im, int E, pk_vol* mask);
     // copy of a pitched half spectrum with one pass's access pattern (pk_probe.hip): ms per copy
     // (mean of reps) and the bytes it moves
-    double probe_copy(int pattern, const int64_t dims[3], int tile, int vals, int xcd, int reps, double* bytes);
+    // flags: bit 0 XCD-aware order, bit 1 cold copies (memory-side cache flushed before each)
+    double probe_copy(int pattern, const int64_t dims[3], int tile, int vals, int flags, int reps, double* bytes);
     void decon_batch(int method, const pk_batch_item* items, int n, const float* psf_f, const int64_t fd[3],
                      const float* psf_b, const int64_t bd[3], int numit, const pk_opts* o);
     // step engine

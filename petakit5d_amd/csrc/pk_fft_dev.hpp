@@ -457,7 +457,8 @@ struct XArgs {
 //   PK_NT_COL: bit 0 strided-pass spectrum loads, bit 1 their stores, bit 2 OTF loads
 //   PK_NT_ROW: bit 0 row-pass spectrum loads, bit 1 their stores, bit 2 real
 //              operand loads, bit 3 real stores
-// Default: nontemporal stores in the strided passes.  Measured against all
+// Default: nontemporal stores in the strided passes (tiles of 16 or more
+// columns: whole 128-byte row segments).  Measured against all
 // accesses temporal (ms per call / Gvoxel/s): 512x512x256 OMW-2 2.578 ->
 // 2.546; 64 chunks OMW-2 18.33 -> 18.71, Biggs-10 3.39 -> 3.52;
 // 2048x2048x512 unchanged (23.45 -> 23.55, 4.17 -> 4.16).  Nontemporal loads
@@ -1146,8 +1147,16 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
         fft_run<E, true, false, MIXED>(v, pl, p, lds, c, T, act);
     }
     if (valid) {
+        // nontemporal stores only for whole 128-byte row segments: with
+        // 8-column tiles (64 bytes) they left the two halves of a line unmerged
+        // (WRITE_SIZE of the 2048-point dim-2 passes +16 %)
+        if (kNtColSt && T >= 16) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) st_f2<kNtColSt>(ptr0 + e * step, v[e]);
+            for (int e = 0; e < E; ++e) st_f2<true>(ptr0 + e * step, v[e]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) st_f2<false>(ptr0 + e * step, v[e]);
+        }
     }
 }
 

@@ -16,9 +16,12 @@
 // A column tile is moved as the passes move it: block = T columns x P
 // threads, thread (c, p) holds the E points at rows p + e P of its column in
 // registers, all loads before all stores, with the strided passes' cache
-// policy (PK_NT_COL: nontemporal stores by default).  The block-to-tile order
+// policy (PK_NT_COL: nontemporal stores for tiles of 16 or more columns).  The block-to-tile order
 // can be the passes' XCD-aware one.  The sequential copy is a plain copy
-// (temporal loads and stores).  Timed with HIP events on the context's stream.
+// (temporal loads and stores).  Timed with HIP events on the context's stream:
+// back to back (a copy may then read what the previous one left in the
+// memory-side cache), or cold, each copy after a read of another 1 GiB
+// buffer, as a pass meets its input after the passes before it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,8 +62,25 @@ __global__ void __launch_bounds__(1024) k_probe_cols(const float2* __restrict__ 
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = make_float2(v[e].x + h[e].x, v[e].y + h[e].y);
     }
+    if (kNtColSt && T >= 16) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) st_f2<kNtColSt>(dst + off0 + e * step, v[e]);
+        for (int e = 0; e < E; ++e) st_f2<true>(dst + off0 + e * step, v[e]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) st_f2<false>(dst + off0 + e * step, v[e]);
+    }
+}
+
+// Reads n float4 (the cold probes' cache flush: read-only, so no dirty lines
+// are left to write back during the timed copy); the sum is stored only under
+// a condition no finite input meets, which keeps the loads.
+__global__ void k_probe_flush(const float4* __restrict__ src, int64_t n, float* __restrict__ sink) {
+    float s = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float4 v = src[i];
+        s += v.x + v.y + v.z + v.w;
+    }
+    if (s == -1.2345e-38f) sink[0] = s;
 }
 
 __global__ void k_probe_seq(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
@@ -76,8 +96,10 @@ void launch_cols(dim3 grid, dim3 block, hipStream_t st, const float2* s, const f
 
 }  // namespace
 
-double pk_ctx_impl::probe_copy(int pattern, const int64_t dims[3], int tile, int vals, int xcd, int reps,
+double pk_ctx_impl::probe_copy(int pattern, const int64_t dims[3], int tile, int vals, int flags, int reps,
                                double* bytes) {
+    int xcd = flags & 1;
+    const bool cold = (flags & 2) != 0;
     const VolPlan& vp = plan(dims[0], dims[1], dims[2]);
     ensure(ws.A, sizeof(float2) * (size_t)vp.S);
     ensure(ws.tmp, sizeof(float2) * (size_t)vp.S);
@@ -136,18 +158,42 @@ double pk_ctx_impl::probe_copy(int pattern, const int64_t dims[3], int tile, int
         PK_HIP(hipGetLastError());
     };
     launch();  // warm-up
+    // cold copies: a buffer four times the memory-side cache is read before
+    // each copy (outside the timed interval)
+    constexpr size_t kFlushBytes = size_t(1) << 30;
+    struct DevBuf {
+        void* p = nullptr;
+        ~DevBuf() {
+            if (p) (void)hipFree(p);
+        }
+    } flush;
+    if (cold) {
+        PK_HIP(hipMalloc(&flush.p, kFlushBytes + 256));
+        PK_HIP(hipMemsetAsync(flush.p, 0, kFlushBytes + 256, stream));
+    }
     hipEvent_t e0, e1;
     PK_HIP(hipEventCreate(&e0));
     PK_HIP(hipEventCreate(&e1));
-    PK_HIP(hipEventRecord(e0, stream));
-    for (int r = 0; r < reps; ++r) launch();
-    PK_HIP(hipEventRecord(e1, stream));
-    PK_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    PK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    double total = 0.0;
+    const int rounds = cold ? reps : 1;
+    for (int k = 0; k < rounds; ++k) {
+        if (cold) {
+            hipLaunchKernelGGL(k_probe_flush, dim3(2048), dim3(256), 0, stream,
+                               static_cast<const float4*>(flush.p), (int64_t)(kFlushBytes / sizeof(float4)),
+                               reinterpret_cast<float*>(static_cast<char*>(flush.p) + kFlushBytes));
+            PK_HIP(hipGetLastError());
+        }
+        PK_HIP(hipEventRecord(e0, stream));
+        for (int r = 0; r < (cold ? 1 : reps); ++r) launch();
+        PK_HIP(hipEventRecord(e1, stream));
+        PK_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        total += ms;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return (double)ms / reps;
+    return total / reps;
 }
 
 }  // namespace pk
