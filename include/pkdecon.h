@@ -384,6 +384,27 @@ pk_status pk_deskew_rotate_frame3d(pk_ctx *ctx, const pk_vol *in, double angle, 
 pk_status pk_deskew_rotate_frame3d_cubic(pk_ctx *ctx, const pk_vol *in, double angle, double dz, double xy_pixel_size,
                                          int32_t reverse, int32_t objective_scan, double x_step_thresh,
                                          const int64_t bbox[6], pk_vol *out);
+/* deskewRotateFrame3D(vol, angle, dz, xyPixelSize, ..., 'resampleFactor',
+ * resample, 'interpMethod', linear (interp 0) or cubic (1))
+ * (deskewRotateFrame3D.m:72-73, 183-210): a resampleFactor other than
+ * [1 1 1] (x y z order) leaves the fast path; the skewed-space interpolation
+ * runs first when the step exceeds x_step_thresh (its result rounded to
+ * uint16 values when in and out are uint16), then imwarp(vol_1,
+ * affine3d(ds_S*(T1*S*R*T2)*(RT1*RS*RT2)), interpMethod, 'FillValues', 0,
+ * 'OutputView', RA) with RS = diag(1 ./ resample) about the box centre and RA
+ * of size round(outSize ./ resample([2 1 3])) (or the bbox window of it).
+ * resample NULL or [1 1 1]: pk_deskew_rotate_frame3d (interp 0) or
+ * pk_deskew_rotate_frame3d_cubic (interp 1).  imwarp returns the class of
+ * vol_1: a uint16 out for a uint16 volume that is not interpolated, or that is
+ * interpolated with save16bit.  pk_deskew_rotate_resample_out_dims gives the
+ * output size (host only). */
+pk_status pk_deskew_rotate_resample_out_dims(const int64_t dims[3], double angle, double dz, double xy_pixel_size,
+                                             int32_t reverse, int32_t objective_scan, double x_step_thresh,
+                                             const double resample[3], int64_t out_dims[3]);
+pk_status pk_deskew_rotate_frame3d_resample(pk_ctx *ctx, const pk_vol *in, double angle, double dz,
+                                            double xy_pixel_size, int32_t reverse, int32_t objective_scan,
+                                            double x_step_thresh, const double resample[3], int32_t interp,
+                                            const int64_t bbox[6], pk_vol *out);
 
 /* Affine resampling as imwarp(vol, affine3d(T), 'linear', 'FillValues', 0,
  * 'OutputView', imref3d(out_dims, 1, 1, 1)) does for rotateFrame3D

@@ -1,6 +1,7 @@
 """numpy restatement of deskewFrame3D (linear interpolation), of
-deskewRotateFrame3D (linear fast path; 'cubic' through imwarp's form), and of
-imwarp's cubic resampling (cubic_warp).
+deskewRotateFrame3D (linear fast path; 'cubic' and resampleFactor through
+imwarp's form, deskew_rotate_general), and of imwarp's cubic resampling
+(cubic_warp).
 
 TEST INFRASTRUCTURE ONLY: the checker for the engine's deskew.  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg may import it.
@@ -62,10 +63,14 @@ def _mround(x):
     return math.floor(abs(x) + 0.5) * (1 if x >= 0 else -1)
 
 
-def dsr_geometry(shape, angle, dz, xy, reverse=False, objectiveScan=False, xStepThresh=2.0):
-    """(tmat, outSize, xstep, stepsize) -- tmat the 4x4 backward matrix over
-    0-based (dim1, dim2, dim3) coordinates, outSize [ny nx' nz'], xstep the
-    original |dx| and stepsize the interpolated slice step (1: none)."""
+def dsr_geometry(shape, angle, dz, xy, reverse=False, objectiveScan=False, xStepThresh=2.0, resample=None):
+    """(tmat, outSize, xstep, stepsize, T) -- tmat the 4x4 backward matrix over
+    0-based (dim1, dim2, dim3) coordinates (the fast path's), outSize [ny nx'
+    nz'], xstep the original |dx|, stepsize the interpolated slice step (1:
+    none), T imwarp's forward matrix.  resample (x y z) other than ones
+    (deskewRotateFrame3D.m:183-196): T gains RT1 * RS * RT2 -- the output box
+    centre to the origin, RS = diag(1 ./ resample), the origin to the centre of
+    the box round(outSize ./ resample([2 1 3])), which becomes outSize."""
     ny, nx, nz = (int(n) for n in shape)
     theta = angle * math.pi / 180
     dx = math.cos(theta) * dz / xy
@@ -120,6 +125,15 @@ def dsr_geometry(shape, angle, dz, xy, reverse=False, objectiveScan=False, xStep
         dsS[3, 0] -= dx
     M = dsS @ ((T1 + off) @ S @ R @ (T2 - off))
     tmat = np.linalg.inv(M.T)[[1, 0, 2, 3]][:, [1, 0, 2, 3]]
+    if resample is not None and np.any(np.asarray(resample, dtype=np.float64) != 1):
+        rs = np.asarray(resample, dtype=np.float64).reshape(3)
+        RT1 = np.eye(4)
+        RT1[3, :3] = -(np.array(out, dtype=np.float64)[[1, 0, 2]] + 1) / 2
+        RS = np.diag([1 / rs[0], 1 / rs[1], 1 / rs[2], 1.0])
+        out = [_mround(out[0] / rs[1]), _mround(out[1] / rs[0]), _mround(out[2] / rs[2])]
+        RT2 = np.eye(4)
+        RT2[3, :3] = (np.array(out, dtype=np.float64)[[1, 0, 2]] + 1) / 2
+        T = T @ (RT1 @ RS @ RT2)
     return tmat, tuple(out), xstep0, stepsize, T
 
 
@@ -132,15 +146,16 @@ def row_structured(tmat, tol=1e-9):
             and abs(t[1][0]) < tol and abs(t[1][1]) < tol and abs(t[2][0]) < tol)
 
 
-def general_warp(vol, T, out):
+def general_warp(vol, T, out, origin=(0, 0, 0)):
     """imwarp(vol, affine3d(T), 'linear', 'FillValues', 0, 'OutputView',
     imref3d(out, 1, 1, 1)): output voxel [x y z] (1-based, x = dim 2) reads
-    the input at [x y z 1] * inv(T), trilinear over one voxel of zero padding."""
+    the input at [x y z 1] * inv(T), trilinear over one voxel of zero padding.
+    origin (x, y, z): the output window's offset (a bbox window of RA)."""
     from scipy import ndimage as ndi
     v = np.asarray(vol, dtype=np.float64)
     inv = np.linalg.inv(T)
-    Y, X, Z = np.meshgrid(np.arange(1, out[0] + 1), np.arange(1, out[1] + 1), np.arange(1, out[2] + 1),
-                          indexing="ij")
+    Y, X, Z = np.meshgrid(np.arange(1, out[0] + 1) + origin[1], np.arange(1, out[1] + 1) + origin[0],
+                          np.arange(1, out[2] + 1) + origin[2], indexing="ij")
     pts = np.stack([X.ravel(), Y.ravel(), Z.ravel(), np.ones(X.size)], axis=1) @ inv
     coords = np.stack([pts[:, 1] - 1, pts[:, 0] - 1, pts[:, 2] - 1])
     r = ndi.map_coordinates(v, coords, order=1, mode="grid-constant", cval=0.0, prefilter=False)
@@ -212,14 +227,17 @@ def cubic_warp(vol, inv, out, origin=(0, 0, 0)):
     return res.reshape(out)
 
 
-def deskew_rotate_cubic(vol, angle, dz, xy, reverse=False, objectiveScan=False, xStepThresh=2.0, bbox=None,
-                        round_interp=False):
-    """deskewRotateFrame3D(vol, ..., 'interpMethod', 'cubic')
-    (deskewRotateFrame3D.m:73, 98-110, 203-210): skewed-space interpolation
-    when the step is large (round_interp: its result rounded as uint16, the
-    reference's uint16 output with save16bit), then cubic_warp of the whole
-    forward transform over RA (the bbox window or outSize).  FP64 result."""
-    tmat, out, xstep, stepsize, T = dsr_geometry(vol.shape, angle, dz, xy, reverse, objectiveScan, xStepThresh)
+def deskew_rotate_general(vol, angle, dz, xy, reverse=False, objectiveScan=False, xStepThresh=2.0, bbox=None,
+                          resample=None, interp="linear", round_interp=False):
+    """deskewRotateFrame3D's imwarp form (deskewRotateFrame3D.m:73, 98-110,
+    183-210), taken for 'cubic' or a resampleFactor other than ones:
+    skewed-space interpolation when the step is large (round_interp: its
+    result rounded as uint16, the reference's uint16 output with save16bit),
+    then imwarp of the whole forward transform (with the resampling) over RA,
+    the bbox window or outSize; trilinear (general_warp) or cubic
+    (cubic_warp).  Linear: single-precision result; cubic: FP64."""
+    tmat, out, xstep, stepsize, T = dsr_geometry(vol.shape, angle, dz, xy, reverse, objectiveScan, xStepThresh,
+                                                 resample)
     v = np.asarray(vol, np.float32)
     if stepsize != 1:
         v = skewed_interp(v, xstep, stepsize, reverse)
@@ -228,7 +246,18 @@ def deskew_rotate_cubic(vol, angle, dz, xy, reverse=False, objectiveScan=False, 
     if bbox is None:
         bbox = [1, 1, 1, out[0], out[1], out[2]]
     od = (bbox[3] - bbox[0] + 1, bbox[4] - bbox[1] + 1, bbox[5] - bbox[2] + 1)
-    return cubic_warp(v, np.linalg.inv(T), od, origin=(bbox[1] - 1, bbox[0] - 1, bbox[2] - 1))
+    origin = (bbox[1] - 1, bbox[0] - 1, bbox[2] - 1)
+    if interp == "cubic":
+        return cubic_warp(v, np.linalg.inv(T), od, origin=origin)
+    return general_warp(v, T, od, origin=origin)
+
+
+def deskew_rotate_cubic(vol, angle, dz, xy, reverse=False, objectiveScan=False, xStepThresh=2.0, bbox=None,
+                        round_interp=False, resample=None):
+    """deskewRotateFrame3D(vol, ..., 'interpMethod', 'cubic'): deskew_rotate_general
+    with cubic resampling.  FP64 result."""
+    return deskew_rotate_general(vol, angle, dz, xy, reverse, objectiveScan, xStepThresh, bbox, resample, "cubic",
+                                 round_interp)
 
 
 def skewed_interp(vol, xstep, stepsize, reverse=False):

@@ -69,6 +69,8 @@ EXPORTS = [
     "pk_probe_copy",
     "pk_affine_warp3d_cubic",
     "pk_deskew_rotate_frame3d_cubic",
+    "pk_deskew_rotate_resample_out_dims",
+    "pk_deskew_rotate_frame3d_resample",
     "pk_find_good_factor_number",
     "pk_lzw_decode",
     "pk_lzw_encode",

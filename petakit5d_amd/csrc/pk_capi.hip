@@ -346,6 +346,30 @@ pk_status pk_deskew_rotate_frame3d_cubic(pk_ctx* c, const pk_vol* in, double ang
     });
 }
 
+pk_status pk_deskew_rotate_resample_out_dims(const int64_t dims[3], double angle, double dz, double xy_pixel_size,
+                                             int32_t reverse, int32_t objective_scan, double x_step_thresh,
+                                             const double resample[3], int64_t out_dims[3]) {
+    return host_guarded(nullptr, [&] {
+        if (!dims || !out_dims) pk::fail(PK_ERR_INVALID_ARG, "null argument");
+        for (int i = 0; i < 3; ++i)
+            if (dims[i] < 2) pk::fail(PK_ERR_INVALID_ARG, "every axis needs at least 2 voxels");
+        pk::dsr_output_size(dims, angle, dz, xy_pixel_size, reverse != 0, objective_scan != 0, x_step_thresh, out_dims,
+                            resample);
+        if (out_dims[1] < 1 || out_dims[2] < 1) pk::fail(PK_ERR_INVALID_ARG, "empty output box");
+    });
+}
+
+pk_status pk_deskew_rotate_frame3d_resample(pk_ctx* c, const pk_vol* in, double angle, double dz, double xy_pixel_size,
+                                            int32_t reverse, int32_t objective_scan, double x_step_thresh,
+                                            const double resample[3], int32_t interp, const int64_t bbox[6],
+                                            pk_vol* out) {
+    return guarded(c, [&] {
+        if (interp != 0 && interp != 1) pk::fail(PK_ERR_INVALID_ARG, "interp: 0 linear, 1 cubic");
+        c->deskew_rotate(in, angle, dz, xy_pixel_size, reverse != 0, objective_scan != 0, x_step_thresh, bbox, out,
+                         interp, resample);
+    });
+}
+
 pk_status pk_set_iter_hook(pk_ctx* c, int32_t save_step, pk_iter_hook hook, void* user) {
     return guarded(c, [&] {
         if (save_step < 1) pk::fail(PK_ERR_INVALID_ARG, "saveStep must be >= 1");

@@ -47,6 +47,7 @@ struct DsrGeom {
     double stepsize;    // interpolated slice step in input slices (1: no interpolation)
     int64_t ntotal;     // slices after interpolation
     bool rows;          // the map keeps dim-1 rows
+    bool resampled;     // a resampleFactor other than [1 1 1]: imwarp's form only (:73)
 };
 
 static double mround(double x) { return x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5); }
@@ -89,7 +90,7 @@ static bool invert4(const double m[4][4], double r[4][4]) {
 }
 
 DsrGeom dsr_geometry(const int64_t dims[3], double angle, double dz, double xy, bool reverse, bool objective,
-                     double x_step_thresh) {
+                     double x_step_thresh, const double* rs = nullptr) {
     if (!(xy > 0) || !(dz > 0)) fail(PK_ERR_INVALID_ARG, "dz and xyPixelSize must be positive");
     const int64_t ny = dims[0], nx = dims[1];
     int64_t nz = dims[2];
@@ -161,6 +162,26 @@ DsrGeom dsr_geometry(const int64_t dims[3], double angle, double dz, double xy, 
         matmul4(ds, t, M);
     };
     chain(0, 0, S_ds, g.fwd);  // imwarp's form (:210)
+    if (rs && (rs[0] != 1.0 || rs[1] != 1.0 || rs[2] != 1.0)) {
+        // resampling after deskew and rotation (:183-196): about the output
+        // box's centre, RS = diag(1 ./ rs) in imwarp's x y z order, into
+        // round(outSize ./ rs([2 1 3])); the forward matrix gains RT1 * RS * RT2
+        for (int d = 0; d < 3; ++d)
+            if (!(rs[d] > 0) || !std::isfinite(rs[d])) fail(PK_ERR_INVALID_ARG, "resampleFactor must be positive");
+        const int64_t no[3] = {(int64_t)mround((double)g.out[0] / rs[1]), (int64_t)mround((double)g.out[1] / rs[0]),
+                               (int64_t)mround((double)g.out[2] / rs[2])};
+        if (no[0] < 1 || no[1] < 1 || no[2] < 1) fail(PK_ERR_INVALID_ARG, "resampleFactor leaves an empty output");
+        const double RT1[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {-ox, -oy, -oz, 1}};
+        const double RS[4][4] = {{1 / rs[0], 0, 0, 0}, {0, 1 / rs[1], 0, 0}, {0, 0, 1 / rs[2], 0}, {0, 0, 0, 1}};
+        const double RT2[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0},
+                                  {((double)no[1] + 1) / 2, ((double)no[0] + 1) / 2, ((double)no[2] + 1) / 2, 1}};
+        double r[4][4];
+        matmul4(RT1, RS, r);
+        matmul4(r, RT2, r);
+        matmul4(g.fwd, r, g.fwd);
+        for (int d = 0; d < 3; ++d) g.out[d] = no[d];
+        g.resampled = true;
+    }
     double ds2[4][4];
     std::memcpy(ds2, S_ds, sizeof(ds2));
     if (!objective && reverse) ds2[3][0] -= dx;  // :215-217
@@ -313,29 +334,31 @@ __global__ void k_f32_to_u16(const float* __restrict__ in, uint16_t* __restrict_
 #pragma clang fp contract(on)
 
 void dsr_output_size(const int64_t dims[3], double angle, double dz, double xy, bool reverse, bool objective,
-                     double x_step_thresh, int64_t out[3]) {
-    const DsrGeom g = dsr_geometry(dims, angle, dz, xy, reverse, objective, x_step_thresh);
+                     double x_step_thresh, int64_t out[3], const double* rs) {
+    const DsrGeom g = dsr_geometry(dims, angle, dz, xy, reverse, objective, x_step_thresh, rs);
     for (int d = 0; d < 3; ++d) out[d] = g.out[d];
 }
 
 static unsigned blocks_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 65536); }
 
 void pk_ctx_impl::deskew_rotate(const pk_vol* in, double angle, double dz, double xy, bool reverse, bool objective,
-                                double x_step_thresh, const int64_t* bbox, pk_vol* out, int interp) {
+                                double x_step_thresh, const int64_t* bbox, pk_vol* out, int interp, const double* rs) {
     if (!in || !out || !in->data || !out->data) fail(PK_ERR_INVALID_ARG, "null volume");
     if (in->mem != PK_MEM_DEVICE || out->mem != PK_MEM_DEVICE) fail(PK_ERR_INVALID_ARG, "device volumes expected");
     if (in->dtype != PK_UINT16 && in->dtype != PK_SINGLE) fail(PK_ERR_UNSUPPORTED_TYPE, "input class uint16 or single");
     if (out->dtype != PK_UINT16 && out->dtype != PK_SINGLE) fail(PK_ERR_UNSUPPORTED_TYPE, "output class uint16 or single");
     for (int d = 0; d < 3; ++d)
         if (in->dims[d] < 2) fail(PK_ERR_INVALID_ARG, "every axis needs at least 2 voxels");
-    const DsrGeom g = dsr_geometry(in->dims, angle, dz, xy, reverse, objective, x_step_thresh);
+    const DsrGeom g = dsr_geometry(in->dims, angle, dz, xy, reverse, objective, x_step_thresh, rs);
     int64_t b[6] = {1, 1, 1, g.out[0], g.out[1], g.out[2]};
     if (bbox) {
         std::memcpy(b, bbox, sizeof(b));
         for (int d = 0; d < 3; ++d)
             if (b[d] < 1 || b[d + 3] < b[d]) fail(PK_ERR_INVALID_ARG, "bbox: 1-based start <= end expected");
-        if (b[3] > in->dims[0]) fail(PK_ERR_INVALID_ARG, "bbox: dim-1 range beyond the volume's rows");
     }
+    const bool general = !g.rows || interp == 1 || g.resampled;
+    // the row-by-row warp maps output rows to input rows one to one
+    if (!general && b[3] > in->dims[0]) fail(PK_ERR_INVALID_ARG, "bbox: dim-1 range beyond the volume's rows");
     const int64_t od[3] = {b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1};
     for (int d = 0; d < 3; ++d)
         if (out->dims[d] != od[d]) fail(PK_ERR_INVALID_ARG, "output dims must equal the output box");
@@ -343,12 +366,12 @@ void pk_ctx_impl::deskew_rotate(const pk_vol* in, double angle, double dz, doubl
     const int64_t ny = in->dims[0], nx = in->dims[1], nz = in->dims[2];
     const int64_t on = od[0] * od[1] * od[2];
     const bool u16in = in->dtype == PK_UINT16, u16out = out->dtype == PK_UINT16;
-    if (!g.rows || interp == 1) {
-        // the general imwarp form (:210, and :227-250 for 'cubic': the skewed-space
-        // interpolation first when the step is large), linear or cubic over zero
-        // padding; a bbox is imref3d's output window (:203-205), an offset of the
-        // output grid
-        const bool skew = interp == 1 && g.stepsize != 1.0;
+    if (general) {
+        // the general imwarp form (:210; for 'cubic' or a resampling factor,
+        // :73 and :98-110, the skewed-space interpolation first when the step
+        // is large), linear or cubic over zero padding; a bbox is imref3d's
+        // output window (:203-205), an offset of the output grid
+        const bool skew = (interp == 1 || g.resampled) && g.stepsize != 1.0;
         const int64_t sz = skew ? g.ntotal : nz;
         const float* src = static_cast<const float*>(in->data);
         size_t need = 0;

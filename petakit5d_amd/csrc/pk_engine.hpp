@@ -127,9 +127,10 @@ DeconGeom decon_check(int method, const pk_vol* I, const float* psf_f, const int
 EpiArgs epi_args(const struct VolPlan& vp, const pk_opts* o, const DeconGeom& g, const float* I,
                  const uint8_t* mask, const uint16_t* raw16, const float* rawf, void* dout);
 
-// outSize [ny nx' nz'] of deskewRotateFrame3D (deskewRotateFrame3D.m:61-69)
+// outSize [ny nx' nz'] of deskewRotateFrame3D (deskewRotateFrame3D.m:61-69;
+// with resampleFactor rs (x y z order, null = none) round(outSize ./ rs([2 1 3])), :185-191)
 void dsr_output_size(const int64_t dims[3], double angle, double dz, double xy, bool reverse, bool objective,
-                     double x_step_thresh, int64_t out[3]);
+                     double x_step_thresh, int64_t out[3], const double* rs = nullptr);
 
 // err_mat of the reference's debug branch from the per-iteration MSE (NUMIT x 4, column-major)
 void debug_err_mat(const std::vector<double>& mse, int estep, double* err_mat);
@@ -302,11 +303,12 @@ struct pk_ctx_impl {
     const uint8_t* edge_mask(const uint16_t* raw16, const float* rawf, int64_t n1, int64_t n2, int64_t n3, int E,
                              uint8_t* m1, uint8_t* m2, unsigned long long* cnt);
     void reset();
-    // deskewRotateFrame3D, no resampling (pk_dsr.hip); bbox: 1-based
-    // inclusive output box or null; device volumes, returns when done
-    // interp: 0 linear, 1 cubic (imwarp's form, pk_deskew.hip)
+    // deskewRotateFrame3D (pk_dsr.hip); bbox: 1-based inclusive output box
+    // or null; device volumes, returns when done; interp: 0 linear, 1 cubic
+    // (imwarp's form, pk_deskew.hip); rs: resampleFactor (x y z) or null
     void deskew_rotate(const pk_vol* in, double angle, double dz, double xy, bool reverse, bool objective,
-                       double x_step_thresh, const int64_t* bbox, pk_vol* out, int interp = 0);
+                       double x_step_thresh, const int64_t* bbox, pk_vol* out, int interp = 0,
+                       const double* rs = nullptr);
     // bwdist(~bw) > E - 1 of bw = im > 0 with cleared faces (RLdecon_large_in_memory.m:150-155)
     void edge_erosion_mask(const pk_vol* im, int E, pk_vol* mask);
     // copy of a pitched half spectrum with one pass's access pattern (pk_probe.hip): ms per copy

@@ -254,3 +254,47 @@ def test_dsr_cubic_matches_linear_on_smooth_data():
             cub = ref
         err = np.linalg.norm(cub - lin) / np.linalg.norm(lin)
         assert err < 0.08, (dz, reverse, err)
+
+
+# ---------------------------------------------------------------- resampleFactor
+# deskewRotateFrame3D.m:183-196: T * (RT1 * RS * RT2) into round(outSize ./ rs([2 1 3])).
+
+@pytest.mark.parametrize("rs", [(1.0, 1.0, 1.0), (1.5, 0.5, 2.0), (2.0, 2.0, 2.0), (0.8, 1.0, 1.3)])
+@pytest.mark.parametrize("reverse", [False, True])
+def test_dsr_resample_geometry_closed_forms(rs, reverse):
+    """Ones leave the geometry alone; otherwise the output size is
+    round(outSize ./ rs([2 1 3])) (the C-ABI's host-only size agrees), the
+    new box centre maps to the input point the old centre mapped to, and a
+    unit step along output axis k covers rs(k) steps of the unresampled
+    output (the inverse map's rows scale by rs)."""
+    from petakit5d_amd.deskew_rotate import deskew_rotate_out_size
+    shape = (40, 64, 30)
+    _, out0, _, _, T0 = D.dsr_geometry(shape, 32.45, 0.3, 0.108, reverse)
+    _, out1, _, _, T1 = D.dsr_geometry(shape, 32.45, 0.3, 0.108, reverse, resample=rs)
+    assert deskew_rotate_out_size(shape, 32.45, 0.3, 0.108, reverse, resampleFactor=rs) == out1
+    if rs == (1.0, 1.0, 1.0):
+        assert out1 == out0 and np.array_equal(T1, T0)
+        return
+    assert out1 == (D._mround(out0[0] / rs[1]), D._mround(out0[1] / rs[0]), D._mround(out0[2] / rs[2]))
+    c0 = np.array([(out0[1] + 1) / 2, (out0[0] + 1) / 2, (out0[2] + 1) / 2, 1.0])
+    c1 = np.array([(out1[1] + 1) / 2, (out1[0] + 1) / 2, (out1[2] + 1) / 2, 1.0])
+    i0, i1 = np.linalg.inv(T0), np.linalg.inv(T1)
+    np.testing.assert_allclose(c1 @ i1, c0 @ i0, atol=1e-9)
+    for k in range(3):
+        e = np.zeros(4)
+        e[k] = 1.0
+        np.testing.assert_allclose(e @ i1, rs[k] * (e @ i0), atol=1e-12)
+
+
+def test_dsr_resample_y_by_two_is_the_row_pair_mean():
+    """Rows pass through the rotation about y untouched, so with rs = [1 2 1]
+    (ny even) output row r sits at y = 2 r - 1/2 of the unresampled output:
+    trilinear interpolation gives the mean of unresampled rows 2r - 1 and 2r
+    exactly (both share the x / z weights)."""
+    g = np.random.default_rng(11)
+    vol = g.random((12, 40, 20)).astype(np.float32) * 1000
+    for interp_dz in (0.2, 0.5):  # without and with skewed-space interpolation
+        full = D.deskew_rotate_general(vol, 32.45, interp_dz, 0.108, resample=None).astype(np.float64)
+        half = D.deskew_rotate_general(vol, 32.45, interp_dz, 0.108, resample=(1.0, 2.0, 1.0)).astype(np.float64)
+        assert half.shape == (6,) + full.shape[1:]
+        np.testing.assert_allclose(half, 0.5 * (full[0::2] + full[1::2]), rtol=1e-5, atol=1e-3)
