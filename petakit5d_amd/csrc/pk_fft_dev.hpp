@@ -457,15 +457,18 @@ struct XArgs {
 //   PK_NT_COL: bit 0 strided-pass spectrum loads, bit 1 their stores, bit 2 OTF loads
 //   PK_NT_ROW: bit 0 row-pass spectrum loads, bit 1 their stores, bit 2 real
 //              operand loads, bit 3 real stores
-// Default: nontemporal stores in the strided passes (tiles of 16 or more
-// columns: whole 128-byte row segments).  Measured against all
-// accesses temporal (ms per call / Gvoxel/s): 512x512x256 OMW-2 2.578 ->
-// 2.546; 64 chunks OMW-2 18.33 -> 18.71, Biggs-10 3.39 -> 3.52;
-// 2048x2048x512 unchanged (23.45 -> 23.55, 4.17 -> 4.16).  Nontemporal loads
-// as well (7): 2.587 ms.  Stores in every pass (PK_NT_COL 2, PK_NT_ROW 10) or
-// every access (7, 15): 2.584 ms -- the row passes gain and lose in turn.
+// Default (PK_NT_COL 6): nontemporal stores in the strided passes (tiles of
+// 16 or more columns: whole 128-byte row segments) and nontemporal OTF loads
+// (each OTF is read once per iteration and is as large as the spectrum, which
+// the next pass reads back).  Measured against all accesses temporal (ms per
+// call / Gvoxel/s): 512x512x256 OMW-2 2.578 -> 2.546 with the stores, -> 2.488
+// with the OTF loads too (z_conv 0.168 -> 0.153 ms); 64 chunks OMW-2 18.33 ->
+// 18.71 / Biggs-10 3.39 -> 3.52 with the stores, then 0.5-0.7 % lower with the
+// OTF loads (interleaved runs); 2048x2048x512 23.45 -> 23.72 -> 23.84.  Not
+// kept: spectrum loads as well (2.587 ms), row-pass operand loads (2.518),
+// stores in every pass (2.584), every access (2.583).
 #ifndef PK_NT_COL
-#define PK_NT_COL 2
+#define PK_NT_COL 6
 #endif
 #ifndef PK_NT_ROW
 #define PK_NT_ROW 0

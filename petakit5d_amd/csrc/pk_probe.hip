@@ -16,7 +16,8 @@
 // A column tile is moved as the passes move it: block = T columns x P
 // threads, thread (c, p) holds the E points at rows p + e P of its column in
 // registers, all loads before all stores, with the strided passes' cache
-// policy (PK_NT_COL: nontemporal stores for tiles of 16 or more columns).  The block-to-tile order
+// policy (PK_NT_COL: nontemporal stores for tiles of 16 or more columns, the
+// second input loaded as the convolution loads its OTF).  The block-to-tile order
 // can be the passes' XCD-aware one.  The sequential copy is a plain copy
 // (temporal loads and stores).  Timed with HIP events on the context's stream:
 // back to back (a copy may then read what the previous one left in the
@@ -58,7 +59,7 @@ __global__ void __launch_bounds__(1024) k_probe_cols(const float2* __restrict__ 
     if (src2) {
         float2 h[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) h[e] = src2[off0 + e * step];
+        for (int e = 0; e < E; ++e) h[e] = ld_f2<kNtOtfLd>(src2 + off0 + e * step);
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = make_float2(v[e].x + h[e].x, v[e].y + h[e].y);
     }
