@@ -372,6 +372,10 @@ def wl_rl128(args, torch, pk, L):
     iters = args.iters or 15
     ctx = pk.Context(0)
     It = col_major_cuda(I, torch)
+    tune = None
+    if not args.no_autotune:  # launch shapes measured for the size before the timed region (shared by the batch)
+        t = time.perf_counter()
+        tune = {"report": ctx.autotune(I.shape), "s": round(time.perf_counter() - t, 3)}
     call = Caller(ctx, L, "rl", It, psf, None, iters)
     ms, lat, prof = timed(call, args.steps, args.warmup, torch, ctx, events_in_timed=args.events_in_timed)
     nvox = float(I.size)
@@ -379,6 +383,8 @@ def wl_rl128(args, torch, pk, L):
         "workload": "Biggs RL 15 iterations, 128x128x64 uint16, 200 emitters, 31x31x41 PSF, background 100, Poisson; "
                     "one step = one decon call", "volume": list(I.shape), "iterations": iters})
     res["roofline"], res["passes"], _ = roofline(prof, args.steps)
+    if tune:
+        res["autotune"] = tune
     if not args.no_plain:
         pms, _, _ = timed(Caller(ctx, L, "rl", It, psf, None, iters, flags=PLAIN), max(2, args.steps // 3), 1, torch)
         res["plain_gpu"] = {"value": round(nvox / 1e6 / (pms / 1e3), 2), "ms_per_step": round(pms, 3)}
