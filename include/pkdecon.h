@@ -394,7 +394,9 @@ pk_status pk_deskew_rotate_frame3d_cubic(pk_ctx *ctx, const pk_vol *in, double a
  * 'OutputView', RA) with RS = diag(1 ./ resample) about the box centre and RA
  * of size round(outSize ./ resample([2 1 3])) (or the bbox window of it).
  * resample NULL or [1 1 1]: pk_deskew_rotate_frame3d (interp 0) or
- * pk_deskew_rotate_frame3d_cubic (interp 1).  imwarp returns the class of
+ * pk_deskew_rotate_frame3d_cubic (interp 1), unless interp + 2 is passed:
+ * that takes the imwarp form all the same, as the reference does for
+ * 'gpuProcess' true (:73).  imwarp returns the class of
  * vol_1: a uint16 out for a uint16 volume that is not interpolated, or that is
  * interpolated with save16bit.  pk_deskew_rotate_resample_out_dims gives the
  * output size (host only). */

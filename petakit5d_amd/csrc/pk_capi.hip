@@ -364,9 +364,9 @@ pk_status pk_deskew_rotate_frame3d_resample(pk_ctx* c, const pk_vol* in, double 
                                             const double resample[3], int32_t interp, const int64_t bbox[6],
                                             pk_vol* out) {
     return guarded(c, [&] {
-        if (interp != 0 && interp != 1) pk::fail(PK_ERR_INVALID_ARG, "interp: 0 linear, 1 cubic");
+        if (interp < 0 || interp > 3) pk::fail(PK_ERR_INVALID_ARG, "interp: 0 linear, 1 cubic, + 2 for the imwarp form");
         c->deskew_rotate(in, angle, dz, xy_pixel_size, reverse != 0, objective_scan != 0, x_step_thresh, bbox, out,
-                         interp, resample);
+                         interp & 1, resample, (interp & 2) != 0);
     });
 }
 

@@ -342,7 +342,8 @@ void dsr_output_size(const int64_t dims[3], double angle, double dz, double xy, 
 static unsigned blocks_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 65536); }
 
 void pk_ctx_impl::deskew_rotate(const pk_vol* in, double angle, double dz, double xy, bool reverse, bool objective,
-                                double x_step_thresh, const int64_t* bbox, pk_vol* out, int interp, const double* rs) {
+                                double x_step_thresh, const int64_t* bbox, pk_vol* out, int interp, const double* rs,
+                                bool imwarp_form) {
     if (!in || !out || !in->data || !out->data) fail(PK_ERR_INVALID_ARG, "null volume");
     if (in->mem != PK_MEM_DEVICE || out->mem != PK_MEM_DEVICE) fail(PK_ERR_INVALID_ARG, "device volumes expected");
     if (in->dtype != PK_UINT16 && in->dtype != PK_SINGLE) fail(PK_ERR_UNSUPPORTED_TYPE, "input class uint16 or single");
@@ -356,7 +357,10 @@ void pk_ctx_impl::deskew_rotate(const pk_vol* in, double angle, double dz, doubl
         for (int d = 0; d < 3; ++d)
             if (b[d] < 1 || b[d + 3] < b[d]) fail(PK_ERR_INVALID_ARG, "bbox: 1-based start <= end expected");
     }
-    const bool general = !g.rows || interp == 1 || g.resampled;
+    // the fast path unless the reference leaves it (:72-73: gpuProcess, a
+    // resampling factor, a method other than linear) or the map does not keep rows
+    const bool leave_fast = imwarp_form || interp == 1 || g.resampled;
+    const bool general = !g.rows || leave_fast;
     // the row-by-row warp maps output rows to input rows one to one
     if (!general && b[3] > in->dims[0]) fail(PK_ERR_INVALID_ARG, "bbox: dim-1 range beyond the volume's rows");
     const int64_t od[3] = {b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1};
@@ -367,11 +371,11 @@ void pk_ctx_impl::deskew_rotate(const pk_vol* in, double angle, double dz, doubl
     const int64_t on = od[0] * od[1] * od[2];
     const bool u16in = in->dtype == PK_UINT16, u16out = out->dtype == PK_UINT16;
     if (general) {
-        // the general imwarp form (:210; for 'cubic' or a resampling factor,
-        // :73 and :98-110, the skewed-space interpolation first when the step
-        // is large), linear or cubic over zero padding; a bbox is imref3d's
-        // output window (:203-205), an offset of the output grid
-        const bool skew = (interp == 1 || g.resampled) && g.stepsize != 1.0;
+        // the general imwarp form (:210; off the fast path, :73 and :98-110,
+        // the skewed-space interpolation first when the step is large), linear
+        // or cubic over zero padding; a bbox is imref3d's output window
+        // (:203-205), an offset of the output grid
+        const bool skew = leave_fast && g.stepsize != 1.0;
         const int64_t sz = skew ? g.ntotal : nz;
         const float* src = static_cast<const float*>(in->data);
         size_t need = 0;

@@ -305,10 +305,12 @@ struct pk_ctx_impl {
     void reset();
     // deskewRotateFrame3D (pk_dsr.hip); bbox: 1-based inclusive output box
     // or null; device volumes, returns when done; interp: 0 linear, 1 cubic
-    // (imwarp's form, pk_deskew.hip); rs: resampleFactor (x y z) or null
+    // (imwarp's form, pk_deskew.hip); rs: resampleFactor (x y z) or null;
+    // imwarp_form: the imwarp form even for linear without resampling (the
+    // reference's gpuProcess = true)
     void deskew_rotate(const pk_vol* in, double angle, double dz, double xy, bool reverse, bool objective,
                        double x_step_thresh, const int64_t* bbox, pk_vol* out, int interp = 0,
-                       const double* rs = nullptr);
+                       const double* rs = nullptr, bool imwarp_form = false);
     // bwdist(~bw) > E - 1 of bw = im > 0 with cleared faces (RLdecon_large_in_memory.m:150-155)
     void edge_erosion_mask(const pk_vol* im, int E, pk_vol* mask);
     // copy of a pitched half spectrum with one pass's access pattern (pk_probe.hip): ms per copy

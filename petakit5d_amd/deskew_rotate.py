@@ -10,7 +10,8 @@ the reference does: the skewed-space interpolation when the step is large,
 then imwarp's cubic convolution of the whole transform
 (pk_deskew_rotate_frame3d_cubic).  A resampleFactor other than ones takes the
 same imwarp form with the resampling folded into the transform
-(pk_deskew_rotate_frame3d_resample), linear or cubic."""
+(pk_deskew_rotate_frame3d_resample), linear or cubic, and so does gpuProcess
+(the reference runs imwarp on a gpuArray then, deskewRotateFrame3D.m:73)."""
 from __future__ import annotations
 
 import ctypes
@@ -69,11 +70,13 @@ def deskewRotateFrame3D(vol, angle, dz, xyPixelSize, reverse=False, Crop=True, b
     vol: uint16 or single, numpy (column-major result) or a CUDA tensor (CUDA
     result).  The output is single, or uint16 for a uint16 input with
     save16bit (the reference's mex does the same).  'Crop' is accepted and,
-    as in the reference, has no effect; gpuProcess only moves the reference's
-    data to its GPU and changes nothing here.  resampleFactor (x y z; [] or
-    ones: none) other than ones takes the reference's imwarp form with the
-    resampling folded in (:183-210).  With 'cubic' or a resampling factor
-    the output has imwarp's class of what it resamples (:98-110, 210):
+    as in the reference, has no effect.  gpuProcess leaves the fast path as
+    the reference does (:73): skewed-space interpolation when the step is
+    large, then imwarp's trilinear form (:210).  resampleFactor (x y z; [] or
+    ones: none) other than ones takes the same imwarp form with the
+    resampling folded in (:183-210).  With 'cubic', gpuProcess or a
+    resampling factor the output has imwarp's class of what it resamples
+    (:98-110, 210):
     uint16 for a uint16 volume that is not interpolated in skewed space, or
     is with save16bit; single otherwise."""
     import math
@@ -106,7 +109,8 @@ def deskewRotateFrame3D(vol, angle, dz, xyPixelSize, reverse=False, Crop=True, b
         box = (ctypes.c_int64 * 6)(*b)
         od = (b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1)
     u16 = t.dtype == torch.uint16 and bool(save16bit)
-    if (interp == "cubic" or rs is not None) and t.dtype == torch.uint16:
+    imwarp_form = interp == "cubic" or rs is not None or bool(gpuProcess)
+    if imwarp_form and t.dtype == torch.uint16:
         do_interp = not objectiveScan and abs(math.cos(angle * math.pi / 180) * dz / xyPixelSize) > xStepThresh
         u16 = bool(save16bit) or not do_interp
     out = torch.empty((od[2], od[1], od[0]), dtype=torch.uint16 if u16 else torch.float32,
@@ -120,9 +124,10 @@ def deskewRotateFrame3D(vol, angle, dz, xyPixelSize, reverse=False, Crop=True, b
     L = _lib.load()
     args = (ctypes.c_double(angle), ctypes.c_double(dz), ctypes.c_double(xyPixelSize), int(bool(reverse)),
             int(bool(objectiveScan)), ctypes.c_double(xStepThresh))
-    if rs is not None:
-        ctx.check(L.pk_deskew_rotate_frame3d_resample(ctx.handle, ctypes.byref(vin), *args, rs,
-                                                      1 if interp == "cubic" else 0, box, ctypes.byref(vout)))
+    if rs is not None or gpuProcess:
+        flag = (1 if interp == "cubic" else 0) + (2 if gpuProcess else 0)
+        ctx.check(L.pk_deskew_rotate_frame3d_resample(ctx.handle, ctypes.byref(vin), *args, rs, flag, box,
+                                                      ctypes.byref(vout)))
     else:
         fn = L.pk_deskew_rotate_frame3d_cubic if interp == "cubic" else L.pk_deskew_rotate_frame3d
         ctx.check(fn(ctx.handle, ctypes.byref(vin), *args, box, ctypes.byref(vout)))

@@ -4,7 +4,7 @@ reference's imwarp form (oracle/deskew_oracle.py: deskew_rotate_general;
 deskewRotateFrame3D.m:72-73, 98-110, 183-210): linear and cubic, forward and
 reversed, with and without skewed-space interpolation, uint16 with
 save16bit, a bbox window, and ones falling back to the fast path bit for
-bit."""
+bit; gpuProcess = true taking the same imwarp form without resampling."""
 import numpy as np
 import pytest
 
@@ -64,3 +64,21 @@ def test_dsr_resample_uint16_bbox_and_ones():
     np.testing.assert_array_equal(a, b)
     with pytest.raises(ValueError):
         pk.deskewRotateFrame3D(vf, 32.45, 0.5, 0.108, resampleFactor=[2, 2])
+
+
+@pytest.mark.parametrize("dz,reverse", [(0.5, False), (0.5, True), (0.2, False)])
+def test_dsr_gpu_process_takes_the_imwarp_form(dz, reverse):
+    """gpuProcess = true leaves the fast path in the reference
+    (deskewRotateFrame3D.m:73): the skewed-space interpolation when the step
+    is large, then imwarp's trilinear form over outSize -- which differs from
+    the fast path's row warp by the fast path's offset."""
+    import petakit5d_amd as pk
+    print(f"[dsr gpuProcess] dz={dz} reverse={reverse}", flush=True)
+    v = _smooth((12, 60, 24), 24)
+    got = pk.deskewRotateFrame3D(v, 32.45, dz, 0.108, reverse=reverse, gpuProcess=True)
+    ref = D.deskew_rotate_general(v, 32.45, dz, 0.108, reverse=reverse)
+    assert got.shape == ref.shape and got.dtype == np.float32
+    assert _relmax(got, ref) < 1e-5
+    v16 = np.asfortranarray(v.astype(np.uint16))
+    g16 = pk.deskewRotateFrame3D(v16, 32.45, dz, 0.108, reverse=reverse, gpuProcess=True)
+    assert g16.dtype == (np.float32 if dz == 0.5 else np.uint16)  # imwarp keeps the class of what it resamples
