@@ -20,9 +20,9 @@ except ImportError:  # pragma: no cover - scipy is present in this image
     _ndi = None
 
 
-def rng(config_id: int, stream: int = 0) -> np.random.Generator:
-    """Independent seeded sub-stream `stream` of workload `config_id`."""
-    return np.random.default_rng([0x5EED0000 + int(config_id), int(stream)])
+def rng(seed_id: int, stream: int = 0) -> np.random.Generator:
+    """Independent seeded sub-stream `stream` of the workload seeded by `seed_id`."""
+    return np.random.default_rng([0x5EED0000 + int(seed_id), int(stream)])
 
 
 def gaussian_psf(size, sigma_xy: float, sigma_z: float, shear_deg: float = 0.0) -> np.ndarray:
@@ -125,9 +125,9 @@ def poisson_u16(lam: np.ndarray, g: np.random.Generator) -> np.ndarray:
     return np.asfortranarray(np.clip(counts, 0, 65535).astype(np.uint16))
 
 
-def emitter_volume(shape, psf, n_emitters=200, amp_range=(500, 3000), background=100.0, config_id=1):
+def emitter_volume(shape, psf, n_emitters=200, amp_range=(500, 3000), background=100.0, seed_id=1):
     """Point emitters convolved with the PSF, plus a flat background, Poisson noise."""
-    g0, g1 = rng(config_id, 0), rng(config_id, 1)
+    g0, g1 = rng(seed_id, 0), rng(seed_id, 1)
     obj = point_emitters(shape, n_emitters, amp_range, g0)
     img = _fft_convolve_circular(obj, psf) + background
     return poisson_u16(img, g1)
@@ -141,7 +141,7 @@ def _gpu_available() -> bool:
         return False
 
 
-def _blob_volume_torch(shape, psf, n_blobs, config_id, bg_mean, dtype=None, as_tensor=False):
+def _blob_volume_torch(shape, psf, n_blobs, seed_id, bg_mean, dtype=None, as_tensor=False):
     """GPU rendering of blob_volume: blobs, background and PSF blur in one
     Fourier-domain product (float64, or float32 for very large volumes),
     Poisson noise from a seeded torch generator.  Same model as the host
@@ -150,7 +150,7 @@ def _blob_volume_torch(shape, psf, n_blobs, config_id, bg_mean, dtype=None, as_t
     import torch
     dev = torch.device("cuda")
     fdt = dtype or torch.float64
-    g0, g1, g2 = rng(config_id, 0), rng(config_id, 1), rng(config_id, 2)
+    g0, g1, g2 = rng(seed_id, 0), rng(seed_id, 1), rng(seed_id, 2)
     idx, b, w, centers = _blob_params(shape, n_blobs, g0)
     n1, n2, n3 = shape
     # torch tensors are indexed [z, y, x] = MATLAB (x=dim1 contiguous) reversed
@@ -195,44 +195,44 @@ def _blob_volume_torch(shape, psf, n_blobs, config_id, bg_mean, dtype=None, as_t
     return np.asfortranarray(counts.astype(np.uint16).transpose(2, 1, 0))
 
 
-def blob_volume(shape, psf, n_blobs=20000, config_id=2, bg_mean=100.0, device=None):
+def blob_volume(shape, psf, n_blobs=20000, seed_id=2, bg_mean=100.0, device=None):
     """Gaussian blobs + smooth sinusoidal background, blurred by the PSF,
     Poisson noise.  device='cuda' renders on the GPU (much faster for large
     volumes); the default picks the GPU when one is visible."""
     if device is None:
         device = "cuda" if _gpu_available() else "cpu"
     if device == "cuda":
-        return _blob_volume_torch(tuple(int(s) for s in shape), psf, n_blobs, config_id, bg_mean)
-    g0, g1, g2 = rng(config_id, 0), rng(config_id, 1), rng(config_id, 2)
+        return _blob_volume_torch(tuple(int(s) for s in shape), psf, n_blobs, seed_id, bg_mean)
+    g0, g1, g2 = rng(seed_id, 0), rng(seed_id, 1), rng(seed_id, 2)
     obj = gaussian_blobs(shape, n_blobs, g0) + smooth_background(shape, g1, bg_mean)
     img = _fft_convolve_circular(obj, psf)
     return poisson_u16(img, g2)
 
 
-def config1():
+def emitter_case():
     """128x128x64 uint16, 200 emitters (amp U[500,3000]), PSF 31x31x41
     (sigma_xy 1.2, sigma_z 3.0), background 100, Poisson."""
     psf = gaussian_psf((31, 31, 41), 1.2, 3.0)
     return emitter_volume((128, 128, 64), psf, 200, (500, 3000), 100.0, 1), psf
 
 
-def config2_psf():
+def sheared_psf():
     """64x64x101 Gaussian PSF, sigma_xy 1.0, sigma_z 2.5, sheared 31.5 deg along dim 2."""
     return gaussian_psf((64, 64, 101), 1.0, 2.5, 31.5)
 
 
-def config2(shape=(512, 512, 256), n_blobs=20000, device=None):
+def blob_case(shape=(512, 512, 256), n_blobs=20000, device=None):
     """512x512x256 uint16, 20k blobs, smooth background (mean 100), blurred by
-    the config-2 PSF, Poisson."""
-    psf = config2_psf()
+    the sheared 64x64x101 PSF, Poisson."""
+    psf = sheared_psf()
     return blob_volume(shape, psf, n_blobs, 2, device=device), psf
 
 
-def config5(shape=(2048, 2048, 512), n_blobs=500000):
-    """2048x2048x512 uint16 (2^31 voxels), 500k blobs, the config-2 PSF;
+def large_blob_case(shape=(2048, 2048, 512), n_blobs=500000):
+    """2048x2048x512 uint16 (2^31 voxels), 500k blobs, the sheared 64x64x101 PSF;
     rendered on the GPU in float32 and returned as a column-major CUDA
     tensor (it never leaves HBM)."""
     import torch
-    psf = config2_psf()
+    psf = sheared_psf()
     t = _blob_volume_torch(tuple(shape), psf, n_blobs, 5, 100.0, dtype=torch.float32, as_tensor=True)
     return t, psf

@@ -39,12 +39,12 @@ def test_backprojector_vs_oracle(bo, size, sig, shear, skewed):
     assert rel_l2(omw, romw) < 1e-5
 
 
-def test_config2_psf_backprojector_vs_oracle(bo):
-    """The 64x64x101 sheared config-2 PSF (413,696 voxels): OTF mask equal,
+def test_sheared_64x64x101_psf_backprojector_vs_oracle(bo):
+    """The 64x64x101 sheared PSF of the 512x512x256 workload (413,696 voxels): OTF mask equal,
     b_omw and OTF_bp_omw within 1e-5 of the oracle, DC gain 1 / (1 + alpha)."""
     import petakit5d_amd as pk
-    psf = synth.config2_psf()
-    print("[bp config2] 64x64x101", flush=True)
+    psf = synth.sheared_psf()
+    print("[bp sheared psf] 64x64x101", flush=True)
     b, omw, absc, mask = pk.omw_backprojector_generation(psf, 0.005, None)
     rb, romw, rabsc, rmask, rsk = bo.omw_backprojector_generation(psf, 0.005, None)
     assert b.shape == (64, 64, 101) and np.isfinite(b).all()

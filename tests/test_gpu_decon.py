@@ -88,16 +88,16 @@ def test_omw_wiener_butterworth_backprojector_vs_oracle(oracle):
     assert e32 <= TOL
 
 
-def test_config1_biggs15(oracle):
+def test_biggs15_128x128x64(oracle):
     """128x128x64, 200 emitters, PSF 31x31x41, Biggs RL 15 iterations."""
     import petakit5d_amd as pk
     from petakit5d_amd import synth
-    I, psf = synth.config1()
+    I, psf = synth.emitter_case()
     J, _, _ = pk.decon_lucy_function(I, psf, 15, Background=100, save16bit=False)
     R32, _, _ = oracle.decon_lucy_function(I, psf, 15, Background=100, save16bit=False)
     R64, _, _ = oracle.decon_lucy_function(I, psf, 15, Background=100, save16bit=False, precision="f64")
     e32, e64 = rel_l2(J, R32), rel_l2(J, R64)
-    print(f"[config1] rel-L2 vs CPU FP32 {e32:.3e}  vs FP64 {e64:.3e}")
+    print(f"[biggs15 128x128x64] rel-L2 vs CPU FP32 {e32:.3e}  vs FP64 {e64:.3e}")
     assert e32 <= TOL
 
 

@@ -1,18 +1,19 @@
-"""The five benchmark workloads on the device, each checked against the
-oracle (the CPU single-precision restatement of decon_lucy_function.m /
-decon_lucy_omw_function.m, useGPU=false) at its real size.
+"""The benchmark workloads on the device at their real sizes, each checked
+against the oracle (the CPU single-precision restatement of
+decon_lucy_function.m / decon_lucy_omw_function.m, useGPU=false):
 
-  config 2  OMW-2, 512x512x256 blob volume, 64x64x101 PSF sheared 31.5 deg,
-            Wiener-Butterworth and omw_backprojector_generation back projectors
-  config 3  64 chunks (128-384 x 128-384 x 64-256, padded to 2/3/5-smooth)
-            through pk_decon_batch, OMW-2 and Biggs-10: every chunk bitwise
-            equal to its single call, four chunks against the oracle
-  config 4  a served OMW-2 / Biggs-15 mix at 96x96x64..1024x1024x256 with two
-            64x64x101 PSFs: every result bitwise equal to its standalone call,
-            the two smallest against the oracle
-  config 5  2048x2048x512 (2^31 voxels, 64-bit indexing): delta PSF gives
-            J = I, Biggs-2 (lambda = 0) keeps the flux sum(J) = sum(I), and
-            OMW-2 on a 256^3 subvolume against the oracle
+- OMW-2 on a 512x512x256 blob volume with the 64x64x101 PSF sheared 31.5
+  deg, with the Wiener-Butterworth and the omw_backprojector_generation back
+  projectors;
+- 64 chunks (128-384 x 128-384 x 64-256, padded to 2/3/5-smooth) through
+  pk_decon_batch, OMW-2 and Biggs-10: every chunk bitwise equal to its single
+  call, four chunks against the oracle;
+- a served OMW-2 / Biggs-15 mix at 96x96x64..1024x1024x256 with two 64x64x101
+  PSFs: every result bitwise equal to its standalone call, the two smallest
+  against the oracle;
+- 2048x2048x512 (2^31 voxels, 64-bit indexing): a delta PSF gives J = I,
+  Biggs-2 (lambda = 0) keeps the flux sum(J) = sum(I), and OMW-2 on a 256^3
+  subvolume matches the oracle.
 
 Bar: rel-L2 <= 1e-4 against the oracle's FP32 path; the FP64 oracle's error
 is printed beside it (decon_lucy_omw_function.m:105-112,
@@ -65,26 +66,26 @@ def _to_host(t):
     return np.asfortranarray(t.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0))
 
 
-# ------------------------------------------------------------------------------------------- config 2
+# ------------------------------------------------------------------------------- OMW-2, 512x512x256
 @pytest.fixture(scope="module")
-def cfg2():
+def vol512():
     from petakit5d_amd import synth
     t = time.perf_counter()
-    I, psf = synth.config2()
-    _log(f"[cfg2] 512x512x256 blob volume rendered in {time.perf_counter() - t:.1f} s")
+    I, psf = synth.blob_case()
+    _log(f"[omw2 512] 512x512x256 blob volume rendered in {time.perf_counter() - t:.1f} s")
     return I, psf
 
 
 @pytest.mark.parametrize("bp", ["wiener-butterworth", "omw"])
-def test_config2_full_size_vs_oracle(oracle, cfg2, bp):
+def test_omw2_512x512x256_vs_oracle(oracle, vol512, bp):
     import petakit5d_amd as pk
-    I, psf = cfg2
+    I, psf = vol512
     assert I.shape == (512, 512, 256) and psf.shape == (64, 64, 101)
     if bp == "omw":
         pb, _, _, _ = pk.omw_backprojector_generation(psf, 0.005, None, OTFCumThresh=0.9, hannWinBounds=[0.8, 1])
     else:
         pb = pk.BackProjector(psf, "wiener-butterworth", 0.005, 1, 10)[0].astype(np.float32)
-    _log(f"[cfg2] OMW-2 512x512x256, back projector {bp}")
+    _log(f"[omw2 512] OMW-2 512x512x256, back projector {bp}")
     J, _ = pk.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False)
     R32, _ = oracle.decon_lucy_omw_function(I, psf, pb, 2, Background=100, save16bit=False)
     e32 = rel_l2(J, R32)
@@ -97,7 +98,7 @@ def test_config2_full_size_vs_oracle(oracle, cfg2, bp):
     np.testing.assert_array_equal(J16, np.clip(np.floor(J.astype(np.float64) + 0.5), 0, 65535).astype(np.uint16))
 
 
-# ------------------------------------------------------------------------------------------- config 3
+# ------------------------------------------------------------------------------- 64 chunks, batched
 def _chunk_sizes(n, seed):
     from petakit5d_amd import synth
     g = synth.rng(3, seed)
@@ -105,14 +106,14 @@ def _chunk_sizes(n, seed):
 
 
 @pytest.fixture(scope="module")
-def cfg3():
-    """64 chunks cropped from one blob volume rendered on the device (config 2's
+def chunks64():
+    """64 chunks cropped from one blob volume rendered on the device (the 512x512x256 workload's
     generator at the same density), the shared 64x64x101 PSF and its
     Wiener-Butterworth back projector."""
     import petakit5d_amd as pk
     from petakit5d_amd import synth
     t = time.perf_counter()
-    psf = synth.config2_psf()
+    psf = synth.sheared_psf()
     psf_b = pk.BackProjector(psf, "wiener-butterworth", 0.005, 1, 10)[0].astype(np.float32)
     sizes = _chunk_sizes(64, 0)
     pool = synth._blob_volume_torch((384, 384, 256), psf, 11250, 3, 100.0, as_tensor=True)
@@ -123,7 +124,7 @@ def cfg3():
         chunks.append(pool[o[0]:o[0] + s[0], o[1]:o[1] + s[1], o[2]:o[2] + s[2]].permute(2, 1, 0).contiguous()
                       .permute(2, 1, 0))
     del pool
-    _log(f"[cfg3] 64 chunks, {sum(np.prod(s) for s in sizes) / 1e6:.0f} M voxels, rendered in "
+    _log(f"[batch 64] 64 chunks, {sum(np.prod(s) for s in sizes) / 1e6:.0f} M voxels, rendered in "
          f"{time.perf_counter() - t:.1f} s")
     return chunks, sizes, psf, psf_b
 
@@ -144,11 +145,11 @@ def _picks(sizes):
 
 
 @pytest.mark.parametrize("method,numit", [("omw", 2), ("rl", 10)])
-def test_config3_batch_vs_single_calls_and_oracle(oracle, cfg3, method, numit):
+def test_64_chunk_batch_vs_single_calls_and_oracle(oracle, chunks64, method, numit):
     import torch
     import petakit5d_amd as pk
-    chunks, sizes, psf, psf_b = cfg3
-    _log(f"[cfg3] batched {method}-{numit} over 64 chunks")
+    chunks, sizes, psf, psf_b = chunks64
+    _log(f"[batch 64] batched {method}-{numit} over 64 chunks")
     ctx = pk.Context(0)
     outs = pk.decon_batch(chunks, psf, psf_b, numit, method=method, Background=100, save16bit=False, ctx=ctx)
     for i, (c, J) in enumerate(zip(chunks, outs)):
@@ -174,9 +175,9 @@ def test_config3_batch_vs_single_calls_and_oracle(oracle, cfg3, method, numit):
     ctx.close()
 
 
-# ------------------------------------------------------------------------------------------- config 4
-def test_config4_served_mix_vs_standalone_and_oracle(oracle):
-    """A served mix at config-4 sizes (log-uniform 96x96x64..1024x1024x256), two
+# ------------------------------------------------------------------------------- served mix
+def test_served_mix_vs_standalone_and_oracle(oracle):
+    """A served mix at sizes log-uniform in 96x96x64..1024x1024x256), two
     channels with their own 64x64x101 PSFs, 70 % OMW-2 / 30 % Biggs-15,
     200 GB budget, host uint16 in and float out."""
     import petakit5d_amd as pk
@@ -195,7 +196,7 @@ def test_config4_served_mix_vs_standalone_and_oracle(oracle):
     jobs[1] = ((120, 100, 70), "rl", 15, 1)
     vols = [_to_host(pool[:s[0], :s[1], :s[2]]) for s, _, _, _ in jobs]
     del pool
-    _log(f"[cfg4] {len(jobs)} requests: {[(j[0], j[1]) for j in jobs]}")
+    _log(f"[served mix] {len(jobs)} requests: {[(j[0], j[1]) for j in jobs]}")
     ctx = pk.Context(0, 200 << 30)
     srv = pk.Server(ctx, max_active=64)
     reqs = [srv.submit(v, psfs[ch], bps[ch], it, method=m, Background=100, save16bit=False)
@@ -220,32 +221,32 @@ def test_config4_served_mix_vs_standalone_and_oracle(oracle):
     ctx.close()
 
 
-# ------------------------------------------------------------------------------------------- config 5
+# ------------------------------------------------------------------------------- 2048x2048x512
 @pytest.fixture(scope="module")
-def cfg5():
+def vol2048():
     import torch
     from petakit5d_amd import synth
     t = time.perf_counter()
-    It, psf = synth.config5()
+    It, psf = synth.large_blob_case()
     torch.cuda.synchronize()
-    _log(f"[cfg5] 2048x2048x512 blob volume rendered on the device in {time.perf_counter() - t:.1f} s")
+    _log(f"[2048] 2048x2048x512 blob volume rendered on the device in {time.perf_counter() - t:.1f} s")
     yield It, psf
     del It
     torch.cuda.empty_cache()
 
 
-def test_config5_delta_psf_returns_input(cfg5):
+def test_2048x2048x512_delta_psf_returns_input(vol2048):
     """A delta PSF has an all-ones OTF, so OMW-2 must return J = I (minus the
     background) on all 2^31 voxels: a wrapped 32-bit index anywhere in the
     transforms, the fused passes or the epilogue breaks it."""
     import torch
     import petakit5d_amd as pk
-    It, _ = cfg5
+    It, _ = vol2048
     assert It.numel() == 2 ** 31
     delta = np.zeros((64, 64, 101), np.float32)
     delta[32, 32, 50] = 1.0
     ctx = pk.Context(0)
-    _log("[cfg5] OMW-2 with a delta PSF, 2048x2048x512")
+    _log("[2048] OMW-2 with a delta PSF, 2048x2048x512")
     J, _ = pk.decon_lucy_omw_function(It, delta, delta, 2, Background=0, save16bit=False, ctx=ctx)
     ctx.close()
     ref = It.to(torch.float32)
@@ -259,15 +260,15 @@ def test_config5_delta_psf_returns_input(cfg5):
     assert err <= 3e-6
 
 
-def test_config5_biggs2_keeps_flux(cfg5):
+def test_2048x2048x512_biggs2_keeps_flux(vol2048):
     """Biggs RL with NUMIT = 2 is plain RL (lambda = 0, decon_lucy_function.m:
     120-132); with a unit-sum PSF and periodic transforms every update keeps
     sum(J) = sum(max(I - Background, 0)), over all 2^31 voxels."""
     import torch
     import petakit5d_amd as pk
-    It, psf = cfg5
+    It, psf = vol2048
     ctx = pk.Context(0)
-    _log("[cfg5] Biggs-2 flux, 2048x2048x512")
+    _log("[2048] Biggs-2 flux, 2048x2048x512")
     J, _, _ = pk.decon_lucy_function(It, psf, 2, Background=100, save16bit=False, ctx=ctx)
     ctx.close()
     sj = float(J.sum(dtype=torch.float64))
@@ -278,13 +279,13 @@ def test_config5_biggs2_keeps_flux(cfg5):
     assert abs(sj / si - 1.0) <= 1e-4
 
 
-def test_config5_subvolume_vs_oracle(oracle, cfg5):
+def test_2048x2048x512_subvolume_vs_oracle(oracle, vol2048):
     """OMW-2 (Wiener-Butterworth back projector) on the 256^3 corner of the
     2048x2048x512 volume, standalone on the device and in the oracle; then the
     whole volume on the device, whose corner must differ from the standalone
     subvolume only near the corner's faces (circular borders)."""
     import petakit5d_amd as pk
-    It, psf = cfg5
+    It, psf = vol2048
     pb = pk.BackProjector(psf, "wiener-butterworth", 0.005, 1, 10)[0].astype(np.float32)
     sub = _to_host(It[:256, :256, :256])
     ctx = pk.Context(0)
@@ -292,7 +293,7 @@ def test_config5_subvolume_vs_oracle(oracle, cfg5):
     R32, _ = oracle.decon_lucy_omw_function(sub, psf, pb, 2, Background=100, save16bit=False)
     R64, _ = oracle.decon_lucy_omw_function(sub, psf, pb, 2, Background=100, save16bit=False, precision="f64")
     e32 = rel_l2(J, R32)
-    _log(f"[cfg5] 256^3 subvolume OMW-2: rel-L2 vs CPU FP32 {e32:.3e}  vs FP64 {rel_l2(J, R64):.3e}  "
+    _log(f"[2048] 256^3 subvolume OMW-2: rel-L2 vs CPU FP32 {e32:.3e}  vs FP64 {rel_l2(J, R64):.3e}  "
          f"(CPU FP32 vs FP64 {rel_l2(R32, R64):.3e})")
     assert e32 <= TOL
     Jfull, _ = pk.decon_lucy_omw_function(It, psf, pb, 2, Background=100, save16bit=False, ctx=ctx)
