@@ -246,7 +246,7 @@ def progress(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-# BASELINE.json's metric; the default run's line carries it with its parts
+# the headline metric; the default run's line carries it with its parts
 HEADLINE_METRIC = "deconvolved Mvoxel/s (OMW, 2 it) + p99 latency at fixed load vs plain GPU path"
 
 
