@@ -64,6 +64,7 @@ def parse():
     ap.add_argument("--shape", type=int, nargs=3, default=None)
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--duration", type=float, default=60.0, help="served: seconds of Poisson arrivals")
+    ap.add_argument("--max-active", type=int, default=32, help="served: requests in flight at most")
     ap.add_argument("--plain-seconds", type=float, default=15.0,
                     help="served: seconds of the same arrival trace replayed on the plain path")
     ap.add_argument("--no-batched", action="store_true", help="default workload: skip the batched section (64 chunks)")
@@ -538,7 +539,7 @@ def served_section(args, torch, pk, half_load=False):
         return vols[i % nshape], meth, (2 if meth == "omw" else 15), ch
 
     ctx = pk.Context(0, 200 << 30)
-    srv = pk.Server(ctx, max_active=64)
+    srv = pk.Server(ctx, max_active=args.max_active)
 
     fl = 0 if args.no_autotune else 4  # PK_FLAG_AUTOTUNE: each size is measured the first time it arrives (warm-up)
 
@@ -550,9 +551,10 @@ def served_section(args, torch, pk, half_load=False):
     for r in warm:
         r.result()
     del warm  # results consumed: their pinned staging returns to the host allocator's cache
-    # saturation capacity: 144 requests submitted at once; the server admits 64 and the
-    # rest join as others finish, so the device stays saturated until the queue drains
-    # (with 48, all admitted at once, the last long requests ran nearly alone)
+    # saturation capacity: 144 requests submitted at once; the server admits
+    # max_active of them and the rest join as others finish, so the device stays
+    # saturated until the queue drains (with 48 requests in all, every one admitted
+    # at once, the last long requests ran nearly alone)
     jobs = [job(i) for i in range(144)]
     t0 = time.perf_counter()
     reqs = [submit(*j) for j in jobs]
@@ -628,7 +630,7 @@ def served_section(args, torch, pk, half_load=False):
                                            "p99": round(float(np.percentile(lat_h, 99)), 2)}}
     srv.stop()
     res["capacity"] = {"req_per_s": round(cap_rps, 2), "Mvoxel_s": round(cap_vox / 1e6 / cap_s, 1),
-                       "how": "144 requests submitted at once, at most 64 in flight"}
+                       "how": f"144 requests submitted at once, at most {args.max_active} in flight"}
     if not args.no_plain:
         # the same trace on the plain path: one unfused call per request, FIFO, one worker
         lat_p = []

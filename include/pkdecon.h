@@ -220,6 +220,7 @@ typedef struct {
     double voxels;            /* voxels of completed requests */
 } pk_serve_stats;
 
+/* max_active: requests in flight at most (<= 0: 32); the rest wait in FIFO order. */
 pk_status pk_serve_start(pk_ctx *ctx, int32_t max_active);
 /* Stops after the requests in flight finish; queued requests are cancelled. */
 pk_status pk_serve_stop(pk_ctx *ctx);

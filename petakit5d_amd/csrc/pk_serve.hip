@@ -78,7 +78,11 @@ struct Server {
     }
     bool stop = false;
     uint64_t next_id = 1;
-    int max_active = 64;
+    // requests in flight at most.  Capacity of the served mix (144 requests
+    // at once, Gvoxel/s, one box, two runs each): 16 -> 3.93-3.97, 24 ->
+    // 3.79-3.93, 32 -> 3.89-3.92, 48 -> 3.77-3.81, 64 -> 3.72-3.78, 128 ->
+    // 3.61-3.71, with the same p99 at 0.7x load
+    int max_active = 32;
     // failure injection for tests of the failure path (PK_SERVE_INJECT_FAILURE="id:k",
     // read at start): request `id` fails on the host before its iteration k + 1
     uint64_t inject_id = 0;
@@ -388,7 +392,7 @@ static void scheduler(pk_ctx_impl* ctx) {
 void serve_start(pk_ctx_impl* ctx, int max_active) {
     if (ctx->srv) fail(PK_ERR_BUSY, "server already running");
     ctx->srv = std::make_unique<Server>();
-    ctx->srv->max_active = max_active > 0 ? max_active : 64;
+    ctx->srv->max_active = max_active > 0 ? max_active : 32;
     if (const char* inj = std::getenv("PK_SERVE_INJECT_FAILURE")) {
         unsigned long long id = 0;
         int k = 0;

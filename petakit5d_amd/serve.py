@@ -143,7 +143,7 @@ class Server:
     """Continuous-batching decon server on one device.  The context's HBM
     budget (pk_ctx_create) bounds the device memory of requests in flight."""
 
-    def __init__(self, ctx=None, max_active: int = 64):
+    def __init__(self, ctx=None, max_active: int = 32):
         self.ctx = ctx or _lib.default_context()
         self.L = _bind(_lib.load())
         self.stopped = False
