@@ -463,8 +463,9 @@ struct XArgs {
 // the next pass reads back).  Measured against all accesses temporal (ms per
 // call / Gvoxel/s): 512x512x256 OMW-2 2.578 -> 2.546 with the stores, -> 2.488
 // with the OTF loads too (z_conv 0.168 -> 0.153 ms); 64 chunks OMW-2 18.33 ->
-// 18.71 / Biggs-10 3.39 -> 3.52 with the stores, then 0.5-0.7 % lower with the
-// OTF loads (interleaved runs); 2048x2048x512 23.45 -> 23.72 -> 23.84.  Not
+// 18.71 / Biggs-10 3.39 -> 3.52 with the stores (the batched kernels keep
+// plain OTF loads: 0.5-0.7 % lower with nontemporal ones, interleaved runs);
+// 2048x2048x512 23.45 -> 23.72 -> 23.84.  Not
 // kept: spectrum loads as well (2.587 ms), row-pass operand loads (2.518),
 // stores in every pass (2.584), every access (2.583).
 #ifndef PK_NT_COL
@@ -1101,7 +1102,9 @@ struct ColChunk {
 
 // MIXED: the plan may hold stages whose radix does not divide E (compiled as
 // kernels of their own, so the all-register plans keep their register budget).
-template <int E, int MODE, bool MIXED = false>
+// OTF_NT: load the OTF nontemporally (single volumes; the batched kernels load
+// it plainly: 64 chunks ran 0.5-0.7 % faster that way, interleaved runs)
+template <int E, int MODE, bool MIXED = false, bool OTF_NT = kNtOtfLd>
 __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl, float2* lds, int64_t bx,
                                              int64_t by) {
     const int t = threadIdx.x;
@@ -1136,13 +1139,13 @@ __device__ __forceinline__ void colpass_body(const ColArgs& a, const FftPlan& pl
         float2 h[late ? 1 : E];
         if constexpr (!late) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) h[e] = valid ? ld_f2<kNtOtfLd>(h0 + e * step) : make_float2(0.f, 0.f);
+            for (int e = 0; e < E; ++e) h[e] = valid ? ld_f2<OTF_NT>(h0 + e * step) : make_float2(0.f, 0.f);
         }
         fft_run<E, false, false, MIXED>(v, pl, p, lds, c, T, act);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             float2 he;
-            if constexpr (late) he = valid ? ld_f2<kNtOtfLd>(h0 + e * step) : make_float2(0.f, 0.f);
+            if constexpr (late) he = valid ? ld_f2<OTF_NT>(h0 + e * step) : make_float2(0.f, 0.f);
             else he = h[e];
             const float2 hs = make_float2(he.x * a.scale, he.y * a.scale);
             v[e] = (MODE == CM_CONVJ) ? cmulc(v[e], hs) : cmul(v[e], hs);
@@ -1187,7 +1190,7 @@ __global__ void __launch_bounds__(NT) k_colpass_batch(const ColChunk* __restrict
     const ColArgs a = ch[c].a;
     const FftPlan& pl = ch[c].pl;  // read in place: a local copy indexed by stage would live in scratch
     const int64_t lb = (int64_t)blockIdx.x - starts[c];
-    colpass_body<E, MODE, MIXED>(a, pl, lds, lb % ch[c].gx, lb / ch[c].gx);
+    colpass_body<E, MODE, MIXED, false>(a, pl, lds, lb % ch[c].gx, lb / ch[c].gx);
 }
 
 }  // namespace pk
