@@ -853,6 +853,7 @@ void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a, int64_t z0, int64_t nz) {
         adv(a.raw16);
         adv(a.rawf);
         adv(a.I);
+        adv(a.I16);
         adv(a.J);
         adv(a.Jin);
         adv(a.J3);
@@ -871,6 +872,8 @@ void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a, int64_t z0, int64_t nz) {
         launch_xrow(vp.packed ? vp.x.fp.L : -vp.x.fp.L, vp.x.fp.E, dim3((unsigned)nblk), vp.x.lds, stream, a,
                     vp.x.fp.tw))
         PK_HIP(hipGetLastError());
+    else if (xm_u16(a.mode))
+        fail(PK_ERR_INVALID_ARG, "uint16-operand row modes need the compile-time row kernel");
     else
         PK_HIP(launch_xpass(vp.x.fp, dim3((unsigned)nblk), vp.x.block, vp.x.lds, stream, a));
     if (prof_on) {
@@ -881,9 +884,11 @@ void pk_ctx_impl::run_x(const VolPlan& vp, XArgs a, int64_t z0, int64_t nz) {
         switch (a.mode) {
             case XM_R2C: name = "x_r2c"; b = 4 * N + 8 * S; break;
             case XM_C2R: name = "x_c2r"; b = 8 * S + 4 * N; break;
-            case XM_PREP_U16: name = "x_prep"; b = 2 * N + (a.J ? 8 : 4) * N + 8 * S; break;
+            case XM_PREP_U16: name = "x_prep"; b = 2 * N + (a.I16 ? 2 : 4) * N + (a.J ? 4 : 0) * N + 8 * S; break;
             case XM_PREP_F32: name = "x_prep"; b = 4 * N + (a.J ? 8 : 4) * N + 8 * S; break;
             case XM_RATIO: name = "x_ratio"; b = 16 * S + 4 * N; break;
+            case XM_RATIO_U16: name = "x_ratio"; b = 16 * S + 2 * N; break;
+            case XM_OMW_UPDATE_U16: name = "x_update_r2c"; b = 8 * S + 6 * N + 8 * S; break;
             case XM_OMW_UPDATE: name = a.r2c_out ? "x_update_r2c" : "x_update"; b = 8 * S + 8 * N + (a.r2c_out ? 8 * S : 0); break;
             case XM_BIGGS_Y: name = "x_biggs_y"; b = 12 * N + 8 * S; break;
             case XM_BIGGS_UPDATE: name = "x_biggs_update"; b = 8 * S + 12 * N + (a.acc ? 4 * N : 0) + (a.r2c_out ? 8 * S : 0); break;
@@ -968,6 +973,11 @@ static int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 
 // dim-1 step, dim-2 forward) runs slab by slab so its working set stays in the
 // device's memory-side cache; 0 = whole volume.  A slab holds whole x-pass
 // blocks (nz * n2 a multiple of the rows per block).  PK_SLAB_PLANES overrides.
+bool pk_ctx_impl::x_compile_time(const VolPlan& vp) const {
+    return vp.x.block.x == (unsigned)kMaxThreads && !tune().generic_x &&
+           xrow_available(vp.packed ? vp.x.fp.L : -vp.x.fp.L, vp.x.fp.E);
+}
+
 int64_t pk_ctx_impl::slab_planes(const VolPlan& vp) const {
     int64_t k = tune().slab >= 0 ? tune().slab : 0;
     if (k <= 0 || k >= vp.n3 || vp.conv_axis != 3) return 0;
@@ -1107,7 +1117,7 @@ void pk_ctx_impl::reset() {
     plans.clear();
     tw.clear();
     rtw.clear();
-    ws.I.release(); ws.J.release(); ws.A.release(); ws.J3.release(); ws.Y.release(); ws.J4.release();
+    ws.I.release(); ws.I16.release(); ws.J.release(); ws.A.release(); ws.J3.release(); ws.Y.release(); ws.J4.release();
     ws.partials.release(); ws.lambda.release(); ws.raw.release(); ws.out.release(); ws.mask.release();
     ws.mask2.release(); ws.count.release(); ws.tmp.release(); ws.batch.release(); ws.dsr.release(); ws.dbg.release();
     block_cache_clear();
@@ -1240,7 +1250,7 @@ static XArgs prep_args(int method, float2* A, float* I, float* J, const uint16_t
 void pk_ctx_impl::omw_passes(const VolPlan& vp, const XArgs& pa, float* I, float* J, float2* A, const float2* Hf,
                              const float2* Hb, int numit, const EpiArgs& e, bool fuse_epi, int64_t kz,
                              const std::vector<uint64_t>* loop_key,
-                             const std::function<void(int, const float*)>* after) {
+                             const std::function<void(int, const float*)>* after, uint16_t* I16) {
     const int64_t step = kz > 0 ? kz : vp.n3;
     const float scale = (float)(1.0 / (double)vp.N);  // ifftn's 1/N
     auto slabs = [&](auto&& body) {
@@ -1262,9 +1272,10 @@ void pk_ctx_impl::omw_passes(const VolPlan& vp, const XArgs& pa, float* I, float
             slabs([&](int64_t z0, int64_t nz) {
                 y_inv(z0, nz);
                 XArgs r{};
-                r.mode = XM_RATIO;
+                r.mode = I16 ? XM_RATIO_U16 : XM_RATIO;
                 r.spec = A;
                 r.I = I;
+                r.I16 = I16;
                 run_x(vp, r, z0, nz);
                 y_fwd(z0, nz);
             });
@@ -1272,10 +1283,11 @@ void pk_ctx_impl::omw_passes(const VolPlan& vp, const XArgs& pa, float* I, float
             slabs([&](int64_t z0, int64_t nz) {
                 y_inv(z0, nz);
                 XArgs u{};
-                u.mode = XM_OMW_UPDATE;
+                u.mode = (k == 1 && I16) ? XM_OMW_UPDATE_U16 : XM_OMW_UPDATE;  // J_1 = I, from I16
                 u.spec = A;
                 u.J = J;
                 u.Jin = k == 1 ? I : nullptr;
+                u.I16 = I16;
                 u.r2c_out = k < numit;
                 if (!(fuse_epi && k == numit)) run_x(vp, u, z0, nz);
                 if (k < numit) y_fwd(z0, nz);
@@ -1336,6 +1348,18 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
     float* I = ws.I.as<float>();
     float* J = ws.J.as<float>();
     float2* A = ws.A.as<float2>();
+    // uint16 input: I = sat(raw - bg16) holds integers below 2^16, so the prep
+    // writes it as uint16 (2 bytes per voxel instead of 4) and the ratio and the
+    // first OMW update read it so, with the same bits.  Float I where something
+    // else reads it: the damp epilogue, the debug metric, OMW with one iteration
+    // (its fused last update), no iteration (J = I) and the unfused path; and
+    // only where the row passes have compile-time kernels (the uint16 modes).
+    uint16_t* I16 = nullptr;
+    if (raw16 && numit >= (method == 1 ? 2 : 1) && !(o->flags & (PK_FLAG_UNFUSED | PK_FLAG_DEBUG)) &&
+        !(o->damp_factor > 1) && x_compile_time(vp)) {
+        ensure(ws.I16, sizeof(uint16_t) * (size_t)N);
+        I16 = ws.I16.as<uint16_t>();
+    }
 
     // epilogue (decon_lucy_function.m:177-203)
     const uint8_t* mask = nullptr;
@@ -1407,11 +1431,12 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
         const std::vector<uint64_t> loop_key{
             (uint64_t)method, (uint64_t)numit, u64(&vp), (uint64_t)vp.N, u64(Hf), u64(Hb), u64(ws.I.p), u64(ws.J.p),
             u64(ws.A.p), u64(ws.J3.p), u64(ws.Y.p), u64(ws.J4.p), u64(ws.lambda.p), u64(ws.partials.p),
-            (uint64_t)kz, u64(stream), (uint64_t)fuse_epi};
-        const XArgs pa = prep_args(method, A, I, J, raw16, rawf, o->background);
+            (uint64_t)kz, u64(stream), (uint64_t)fuse_epi, u64(I16)};
+        XArgs pa = prep_args(method, A, I, J, raw16, rawf, o->background);
+        pa.I16 = I16;
 
         if (method == 1) {
-            omw_passes(vp, pa, I, J, A, Hf, Hb, numit, e, fuse_epi, kz, &loop_key, dbg ? &dbg_after : nullptr);
+            omw_passes(vp, pa, I, J, A, Hf, Hb, numit, e, fuse_epi, kz, &loop_key, dbg ? &dbg_after : nullptr, I16);
         } else {
             run_x(vp, pa);
             float* J2 = J;
@@ -1446,9 +1471,10 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
                 slabs([&](int64_t z0, int64_t nz) {
                     y_inv(z0, nz);
                     XArgs r{};
-                    r.mode = XM_RATIO;
+                    r.mode = I16 ? XM_RATIO_U16 : XM_RATIO;
                     r.spec = A;
                     r.I = I;
+                    r.I16 = I16;
                     run_x(vp, r, z0, nz);
                     y_fwd(z0, nz);
                 });

@@ -163,6 +163,7 @@ struct HostBuf {
 
 struct Workspace {
     DevBuf I, J, A, J3, Y, J4, partials, lambda, raw, out, mask, mask2, count, tmp;
+    DevBuf I16;    // I as uint16 for uint16 input (decon: XM_*_U16 row modes)
     DevBuf batch;  // batched decon: per-chunk arena
     DevBuf dsr;    // deskew-rotate: skewed-space interpolated volume / conversions
     DevBuf dbg;    // debug metrics: block partials of the squared differences and their sum
@@ -294,7 +295,9 @@ struct pk_ctx_impl {
     void omw_passes(const VolPlan& vp, const XArgs& prep, float* I, float* J, float2* A, const float2* Hf,
                     const float2* Hb, int numit, const EpiArgs& e, bool fuse_epi, int64_t kz,
                     const std::vector<uint64_t>* loop_key,
-                    const std::function<void(int, const float*)>* after = nullptr);
+                    const std::function<void(int, const float*)>* after = nullptr, uint16_t* I16 = nullptr);
+    // the plan's row passes run the compile-time kernel (the XM_*_U16 modes need it)
+    bool x_compile_time(const VolPlan& vp) const;
     // debug ('debug', true): per-iteration MSE and the iteration hook
     int32_t dbg_step = 1;
     pk_iter_hook dbg_hook = nullptr;

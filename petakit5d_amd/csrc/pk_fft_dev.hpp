@@ -411,9 +411,20 @@ enum XMode : int {
     // the new estimate goes through the epilogue into ep.out16 / ep.outf
     XM_OMW_FINAL = 8,    // mode 5 without the estimate store or the forward transform
     XM_BIGGS_FINAL = 9,  // mode 7 without Jn, J4, the dot partials or the forward transform
+    // uint16 input (compile-time row kernels only): I = sat(raw - bg16) holds
+    // integers below 2^16, kept as uint16 (I16), half the bytes of float I
+    XM_RATIO_U16 = 10,       // mode 4 reading I from I16
+    XM_OMW_UPDATE_U16 = 11,  // mode 5 reading the estimate from I16 (the first update: J_1 = I)
 };
-// the iteration mode a fused final mode extends
-constexpr int xm_base(int m) { return m == XM_OMW_FINAL ? XM_OMW_UPDATE : m == XM_BIGGS_FINAL ? XM_BIGGS_UPDATE : m; }
+// the iteration mode a fused final or uint16-operand mode extends
+constexpr int xm_base(int m) {
+    return m == XM_OMW_FINAL || m == XM_OMW_UPDATE_U16 ? XM_OMW_UPDATE
+           : m == XM_BIGGS_FINAL                      ? XM_BIGGS_UPDATE
+           : m == XM_RATIO_U16                        ? XM_RATIO
+                                                      : m;
+}
+constexpr bool xm_final(int m) { return m == XM_OMW_FINAL || m == XM_BIGGS_FINAL; }
+constexpr bool xm_u16(int m) { return m == XM_RATIO_U16 || m == XM_OMW_UPDATE_U16; }
 
 struct XArgs {
     int n1;            // real row length
@@ -436,6 +447,7 @@ struct XArgs {
     const uint16_t* raw16;
     const float* rawf;
     float* I;          // background-subtracted input
+    uint16_t* I16;     // uint16 input: I as uint16, written by the prep instead of I when set
     float* J;          // estimate (OMW J, Biggs J2 for mode 6); modes 2/3 store it only when non-null
     const float* Jin;  // mode 5: read the estimate from here instead of J when non-null (first iteration: I)
     const float* J3;   // Biggs x_{k-2}
@@ -521,6 +533,10 @@ __device__ __forceinline__ float2 ld_real(const float* __restrict__ base, int64_
 __device__ __forceinline__ void st_real(float* __restrict__ base, int64_t xrow, int m, bool packed, float2 v) {
     if (packed) reinterpret_cast<float2*>(base)[(xrow >> 1) + m] = v;
     else base[xrow + m] = v.x;
+}
+__device__ __forceinline__ void st_u16(uint16_t* __restrict__ base, int64_t xrow, int m, bool packed, float2 v) {
+    if (packed) reinterpret_cast<uint32_t*>(base)[(xrow >> 1) + m] = (uint32_t)v.x | ((uint32_t)v.y << 16);
+    else base[xrow + m] = (uint16_t)v.x;
 }
 __device__ __forceinline__ float2 ld_u16(const uint16_t* __restrict__ base, int64_t xrow, int m, bool packed) {
     if (packed) {
@@ -616,7 +632,7 @@ __device__ __forceinline__ float2 x_elem(float2 v, float2 opA, float2 opB, float
 template <int E, int MODE_>
 __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, float2* lds, int64_t blk) {
     constexpr int MODE = xm_base(MODE_);
-    constexpr bool EPI = MODE_ != MODE;  // fused epilogue
+    constexpr bool EPI = xm_final(MODE_);  // fused epilogue
     constexpr bool IN = (MODE == XM_C2R || MODE == XM_RATIO || MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
     constexpr bool OUT_ALWAYS = (MODE == XM_R2C || MODE == XM_PREP_U16 || MODE == XM_PREP_F32 ||
                                  MODE == XM_RATIO || MODE == XM_BIGGS_Y);
@@ -708,7 +724,8 @@ __device__ __forceinline__ void xpass_body(const XArgs& a, const FftPlan& pl, fl
                 o = x_elem<MODE>(v[e], opA[e], opB[e], a.bg, a.eps, lam);
             }
             if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
-                st_real(a.I, xrow, m, packed, o);
+                if (MODE == XM_PREP_U16 && a.I16) st_u16(a.I16, xrow, m, packed, o);
+                else st_real(a.I, xrow, m, packed, o);
                 if (a.J) st_real(a.J, xrow, m, packed, o);
             } else if constexpr (MODE == XM_OMW_UPDATE) {
                 if constexpr (EPI) {
@@ -824,7 +841,7 @@ __host__ __device__ constexpr int xrow_lds_rows(int P) {
 template <int L, int E, int MODE_, bool ODD = false>
 __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restrict__ tw, float2* lds, int64_t blk) {
     constexpr int MODE = xm_base(MODE_);
-    constexpr bool EPI = MODE_ != MODE;  // fused epilogue
+    constexpr bool EPI = xm_final(MODE_);  // fused epilogue
     constexpr bool IN = (MODE == XM_C2R || MODE == XM_RATIO || MODE == XM_OMW_UPDATE || MODE == XM_BIGGS_UPDATE);
     constexpr bool OUT_ALWAYS = (MODE == XM_R2C || MODE == XM_PREP_U16 || MODE == XM_PREP_F32 ||
                                  MODE == XM_RATIO || MODE == XM_BIGGS_Y);
@@ -865,6 +882,19 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
         if constexpr (ODD) st_f1<kNtRealSt>(base + r2 + m, o.x);
         else st_f2<kNtRealSt>(reinterpret_cast<float2*>(base) + r2 + m, o);
     };
+    // the same for I16 (integers in [0, 65535]: exact either way)
+    auto ld16 = [&](const uint16_t* __restrict__ base, int m) -> float2 {
+        if constexpr (ODD) {
+            return make_float2((float)base[r2 + m], 0.f);
+        } else {
+            const uint32_t w = reinterpret_cast<const uint32_t*>(base)[r2 + m];
+            return make_float2((float)(w & 0xffffu), (float)(w >> 16));
+        }
+    };
+    auto st16 = [&](uint16_t* __restrict__ base, int m, float2 o) {
+        if constexpr (ODD) base[r2 + m] = (uint16_t)o.x;
+        else reinterpret_cast<uint32_t*>(base)[r2 + m] = (uint32_t)o.x | ((uint32_t)o.y << 16);
+    };
 
     RawRow rr_src{0, false};
     if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
@@ -891,8 +921,14 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
             if constexpr (MODE == XM_PREP_F32)
                 opA[e] = a.s0 ? ld_raw_padded(a.rawf, a, rr_src, ODD ? (int64_t)m : 2 * (int64_t)m, !ODD)
                               : ldr(a.rawf, m);
-            if constexpr (MODE == XM_RATIO) opA[e] = ldr(a.I, m);
-            if constexpr (MODE == XM_OMW_UPDATE) opA[e] = ldr(a.Jin ? a.Jin : a.J, m);
+            if constexpr (MODE == XM_RATIO) {
+                if constexpr (xm_u16(MODE_)) opA[e] = ld16(a.I16, m);
+                else opA[e] = ldr(a.I, m);
+            }
+            if constexpr (MODE == XM_OMW_UPDATE) {
+                if constexpr (xm_u16(MODE_)) opA[e] = ld16(a.I16, m);
+                else opA[e] = ldr(a.Jin ? a.Jin : a.J, m);
+            }
             if constexpr (MODE == XM_BIGGS_Y) {
                 opA[e] = ldr(a.J, m);
                 opB[e] = ldr(a.J3, m);
@@ -937,7 +973,8 @@ __device__ __forceinline__ void xrow_body(const XArgs& a, const float2* __restri
         }
         if (valid) {
             if constexpr (MODE == XM_PREP_U16 || MODE == XM_PREP_F32) {
-                str(a.I, m, o);
+                if (MODE == XM_PREP_U16 && a.I16) st16(a.I16, m, o);
+                else str(a.I, m, o);
                 if (a.J) str(a.J, m, o);
             } else if constexpr (MODE == XM_OMW_UPDATE) {
                 if constexpr (EPI) {

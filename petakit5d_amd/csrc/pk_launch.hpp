@@ -71,6 +71,10 @@ inline void launch_modes_xrow(dim3 grid, size_t lds, hipStream_t s, const XArgs&
         case XM_BIGGS_Y: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_Y, ODD>), grid, b, lds, s, a, tw); break;
         case XM_BIGGS_UPDATE: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_UPDATE, ODD>), grid, b, lds, s, a, tw); break;
         case XM_OMW_FINAL: hipLaunchKernelGGL((k_xrow<L, E, XM_OMW_FINAL, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_RATIO_U16: hipLaunchKernelGGL((k_xrow<L, E, XM_RATIO_U16, ODD>), grid, b, lds, s, a, tw); break;
+        case XM_OMW_UPDATE_U16:
+            hipLaunchKernelGGL((k_xrow<L, E, XM_OMW_UPDATE_U16, ODD>), grid, b, lds, s, a, tw);
+            break;
         case XM_BIGGS_FINAL: hipLaunchKernelGGL((k_xrow<L, E, XM_BIGGS_FINAL, ODD>), grid, b, lds, s, a, tw); break;
         default: break;
     }
