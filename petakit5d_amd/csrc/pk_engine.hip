@@ -1117,7 +1117,7 @@ void pk_ctx_impl::reset() {
     plans.clear();
     tw.clear();
     rtw.clear();
-    ws.I.release(); ws.I16.release(); ws.J.release(); ws.A.release(); ws.J3.release(); ws.Y.release(); ws.J4.release();
+    ws.I.release(); ws.J.release(); ws.A.release(); ws.J3.release(); ws.Y.release(); ws.J4.release();
     ws.partials.release(); ws.lambda.release(); ws.raw.release(); ws.out.release(); ws.mask.release();
     ws.mask2.release(); ws.count.release(); ws.tmp.release(); ws.batch.release(); ws.dsr.release(); ws.dbg.release();
     block_cache_clear();
@@ -1354,12 +1354,12 @@ void pk_ctx_impl::decon(int method, const pk_vol* Iv, const float* psf_f, const 
     // else reads it: the damp epilogue, the debug metric, OMW with one iteration
     // (its fused last update), no iteration (J = I) and the unfused path; and
     // only where the row passes have compile-time kernels (the uint16 modes).
+    // Float I is then neither written nor read, so the uint16 copy takes the
+    // first half of its buffer.
     uint16_t* I16 = nullptr;
     if (raw16 && numit >= (method == 1 ? 2 : 1) && !(o->flags & (PK_FLAG_UNFUSED | PK_FLAG_DEBUG)) &&
-        !(o->damp_factor > 1) && x_compile_time(vp)) {
-        ensure(ws.I16, sizeof(uint16_t) * (size_t)N);
-        I16 = ws.I16.as<uint16_t>();
-    }
+        !(o->damp_factor > 1) && x_compile_time(vp))
+        I16 = static_cast<uint16_t*>(ws.I.p);
 
     // epilogue (decon_lucy_function.m:177-203)
     const uint8_t* mask = nullptr;

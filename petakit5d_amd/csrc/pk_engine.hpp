@@ -163,7 +163,6 @@ struct HostBuf {
 
 struct Workspace {
     DevBuf I, J, A, J3, Y, J4, partials, lambda, raw, out, mask, mask2, count, tmp;
-    DevBuf I16;    // I as uint16 for uint16 input (decon: XM_*_U16 row modes)
     DevBuf batch;  // batched decon: per-chunk arena
     DevBuf dsr;    // deskew-rotate: skewed-space interpolated volume / conversions
     DevBuf dbg;    // debug metrics: block partials of the squared differences and their sum
