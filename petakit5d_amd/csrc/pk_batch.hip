@@ -282,6 +282,18 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
         g.block = std::max(g.block, vp.x.block.x);
         g.lds = std::max(g.lds, vp.x.lds);
     };
+    // uint16 chunks keep I as uint16 in the first half of their float I buffer
+    // (I = sat(raw - bg16) holds integers below 2^16): the ratios and OMW's
+    // first update read it so, with the same bits, through compile-time row
+    // kernels.  Float I where other code reads it: the damp epilogue, OMW with
+    // one iteration (its last update reads J = I) and no iteration.
+    auto u16_est = [&](const Chunk& c) -> uint16_t* {
+        const VolPlan& vp = *c.vp;
+        const int ctk = vp.packed ? vp.x.fp.L : -vp.x.fp.L;
+        const bool ok = c.u16 && c.numit >= (c.method == 1 ? 2 : 1) && !(c.o.damp_factor > 1) &&
+                        vp.x.block.x == 256 && xrow_available(ctk, vp.x.fp.E);
+        return ok ? reinterpret_cast<uint16_t*>(c.I) : nullptr;
+    };
     auto add_col = [&](Pass<ColChunk>& ps, const Chunk& c, int axis, int mode, const float2* otf) {
         const VolPlan& vp = *c.vp;
         const AxisLaunch& ax = axis == 2 ? vp.y : vp.z;
@@ -370,6 +382,8 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
             a.bg = c.u16 ? (float)std::min(65535.0, std::max(0.0, std::round(c.o.background))) : (float)c.o.background;
             a.I = c.I;
             a.J = c.J;
+            a.I16 = u16_est(c);
+            if (a.I16 && c.method == 1) a.J = nullptr;  // OMW: the first update reads I (J_1 = I)
             add_x(prep[si], c, a);
             if (c.method == 0) {
                 hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)std::min<int64_t>((c.vp->N + 255) / 256, 65536)), dim3(256),
@@ -395,15 +409,19 @@ void pk_ctx_impl::run_step(std::vector<Chunk*>& act) {
         add_col(zc1[si], c, cax, CM_CONV, c.Hf);
         add_col(yi[si], c, oax, CM_INV, nullptr);
         {
-            XArgs a = base_x(c, XM_RATIO);
+            uint16_t* I16 = u16_est(c);
+            XArgs a = base_x(c, I16 ? XM_RATIO_U16 : XM_RATIO);
             a.I = c.I;
+            a.I16 = I16;
             add_x(ratio[si], c, a);
         }
         add_col(zc2[si], c, cax, c.method == 1 ? CM_CONV : CM_CONVJ, c.method == 1 ? c.Hb : c.Hf);
         const bool last = kn == c.numit;  // the last update writes the output through the epilogue
         if (c.method == 1) {
-            XArgs a = base_x(c, last ? XM_OMW_FINAL : XM_OMW_UPDATE);
+            uint16_t* I16 = kn == 1 ? u16_est(c) : nullptr;  // J_1 = I, read as uint16
+            XArgs a = base_x(c, last ? XM_OMW_FINAL : I16 ? XM_OMW_UPDATE_U16 : XM_OMW_UPDATE);
             a.J = c.J;
+            a.I16 = I16;
             a.r2c_out = !last;
             if (last) a.ep = chunk_epi(c);
             add_x(update[si], c, a);

@@ -90,6 +90,12 @@ inline void launch_modes_xrowb(dim3 grid, size_t lds, hipStream_t s, int mode, c
         case XM_PREP_F32: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_PREP_F32, ODD>), grid, b, lds, s, ch, starts, n); break;
         case XM_RATIO: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_RATIO, ODD>), grid, b, lds, s, ch, starts, n); break;
         case XM_OMW_UPDATE: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_OMW_UPDATE, ODD>), grid, b, lds, s, ch, starts, n); break;
+        case XM_RATIO_U16:
+            hipLaunchKernelGGL((k_xrow_batch<L, E, XM_RATIO_U16, ODD>), grid, b, lds, s, ch, starts, n);
+            break;
+        case XM_OMW_UPDATE_U16:
+            hipLaunchKernelGGL((k_xrow_batch<L, E, XM_OMW_UPDATE_U16, ODD>), grid, b, lds, s, ch, starts, n);
+            break;
         case XM_BIGGS_Y: hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_Y, ODD>), grid, b, lds, s, ch, starts, n); break;
         case XM_BIGGS_UPDATE:
             hipLaunchKernelGGL((k_xrow_batch<L, E, XM_BIGGS_UPDATE, ODD>), grid, b, lds, s, ch, starts, n);
