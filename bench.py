@@ -452,7 +452,9 @@ def batched_section(args, torch, pk, psf, psf_b, steps):
     del pool
     ctx = pk.Context(0)
     nvox = float(sum(np.prod(s) for s in sizes))
-    out = {"chunks": 64, "voxels": nvox}
+    from petakit5d_amd.decon import findGoodFactorNumber
+    npad = float(sum(np.prod([findGoodFactorNumber(n, 1, False, maxPrime=5) for n in s]) for s in sizes))
+    out = {"chunks": 64, "voxels": nvox, "padded_voxels": npad}
     for method, iters in (("omw", 2), ("rl", 10)):
         progress(f"batched {method}-{iters}: 64 chunks")
         fl = 0 if args.no_autotune else 4  # PK_FLAG_AUTOTUNE: shapes measured per chunk size in the warm-up batch
@@ -465,6 +467,8 @@ def batched_section(args, torch, pk, psf, psf_b, steps):
         model = float(sum(chunk_model_bytes(s, iters, method) for s in sizes))
         roof = nvox / 1e6 / (model / (HBM_PEAK_GBPS * 1e9))
         rec = {"Mvoxel_s": round(mvs, 1), "ms_per_batch": round(ms, 2),
+               # the same batch counted in transform-size (2/3/5-smooth padded) voxels
+               "padded_Mvoxel_s": round(npad / 1e6 / (ms / 1e3), 1),
                "roofline_Mvoxel_s": round(roof, 1), "roofline_frac": round(mvs / roof, 4),
                "model_bytes_per_batch": model,
                "launches_per_batch": int(sum(v[0] for v in prof.values()) / nb),
